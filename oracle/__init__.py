@@ -1,0 +1,295 @@
+"""ctypes front end of the oracle (a CPU restatement of monolish's reference
+CPU path; see oracle.h for what each function restates and how it is pinned).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+SUCCESS, SIZE_ERROR, MAXITER, BREAKDOWN, RESIDUAL_NAN, NOT_IMPL = 0, -1, -2, -3, -4, -10
+PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI = 0, 1, 2
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+class _COO(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("nnz", C.c_int64),
+                ("alloc", C.c_int64), ("row", C.POINTER(C.c_int32)),
+                ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = C.c_void_p
+_I64 = C.c_int64
+
+
+def _declare(L):
+    L.or_mt_seed.argtypes = [_P, C.c_uint32]
+    L.or_mt_next.argtypes = [_P]
+    L.or_mt_next.restype = C.c_uint32
+    L.or_random_vector_d.argtypes = [_P, _I64, C.c_double, C.c_double, C.c_uint32]
+    L.or_random_vector_s.argtypes = [_P, _I64, C.c_float, C.c_float, C.c_uint32]
+    L.or_coo_free.argtypes = [C.POINTER(_COO)]
+    L.or_laplacian_2d5p.argtypes = [C.POINTER(_COO), C.c_int, C.c_int]
+    L.or_input_mm.argtypes = [C.POINTER(_COO), C.c_char_p, C.c_int]
+    L.or_input_mm.restype = C.c_int
+    L.or_coo_sort_arrays.argtypes = [_I64, _P, _P, _P, C.c_int]
+    L.or_coo_sort_arrays.restype = _I64
+    L.or_crs_from_coo.argtypes = [C.POINTER(_COO), _P, _P, _P]
+    L.or_num_threads.restype = C.c_int
+    for s, T in (("d", C.c_double), ("s", C.c_float)):
+        getattr(L, f"or_crs_diag_{s}").argtypes = [_I64, _I64, _P, _P, _P, _P]
+        getattr(L, f"or_spmv_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
+        getattr(L, f"or_spmv_t_{s}").argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
+        for f in ("dot",):
+            getattr(L, f"or_{f}_{s}").argtypes = [_I64, _P, _P]
+            getattr(L, f"or_{f}_{s}").restype = T
+        getattr(L, f"or_nrm2_{s}").argtypes = [_I64, _P]
+        getattr(L, f"or_nrm2_{s}").restype = T
+        for f in ("axpy", "xpay"):
+            getattr(L, f"or_{f}_{s}").argtypes = [_I64, T, _P, _P]
+        getattr(L, f"or_axpyz_{s}").argtypes = [_I64, T, _P, _P, _P]
+        getattr(L, f"or_scal_{s}").argtypes = [_I64, T, _P]
+        getattr(L, f"or_copy_{s}").argtypes = [_I64, _P, _P]
+        for f in ("vadd", "vsub", "vmul"):
+            getattr(L, f"or_{f}_{s}").argtypes = [_I64, _P, _P, _P]
+        getattr(L, f"or_vreciprocal_{s}").argtypes = [_I64, _P, _P]
+        getattr(L, f"or_residual_l2_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
+        getattr(L, f"or_residual_l2_{s}").restype = T
+        for f in ("cg", "bicgstab", "jacobi"):
+            fn = getattr(L, f"or_{f}_{s}")
+            fn.argtypes = [_I64, _P, _P, _P, _P, _P, C.c_int, C.c_double, _I64, _I64,
+                           C.POINTER(_I64), C.POINTER(C.c_double), _P]
+            fn.restype = C.c_int
+
+
+def _sfx(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "d"
+    if dt == np.float32:
+        return "s"
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- random ----
+class MT19937:
+    """std::mt19937 restated (oracle.c)."""
+
+    def __init__(self, seed: int = 5489):
+        self._buf = C.create_string_buffer(624 * 4 + 8)
+        lib().or_mt_seed(self._buf, seed)
+
+    def __call__(self) -> int:
+        return lib().or_mt_next(self._buf)
+
+
+def random_vector(n: int, vmin: float, vmax: float, seed: int, dtype=np.float64) -> np.ndarray:
+    """vector<T>(N, min, max, seed) (src/utils/vector_constructor.cpp:98-113)."""
+    out = np.empty(n, dtype=dtype)
+    getattr(lib(), f"or_random_vector_{_sfx(dtype)}")(_ptr(out), n, vmin, vmax, seed)
+    return out
+
+
+# ------------------------------------------------------------------- COO ----
+def _take_coo(c: _COO):
+    nnz = c.nnz
+    row = np.ctypeslib.as_array(c.row, shape=(nnz,)).copy() if nnz else np.zeros(0, np.int32)
+    col = np.ctypeslib.as_array(c.col, shape=(nnz,)).copy() if nnz else np.zeros(0, np.int32)
+    val = np.ctypeslib.as_array(c.val, shape=(nnz,)).copy() if nnz else np.zeros(0, np.float64)
+    rows, cols = c.rows, c.cols
+    lib().or_coo_free(C.byref(c))
+    return rows, cols, row, col, val
+
+
+def laplacian_2d5p(m: int, n: int):
+    """util::laplacian_matrix_2D_5p(m, n) as COO (rows, cols, row, col, val)."""
+    c = _COO()
+    lib().or_laplacian_2d5p(C.byref(c), m, n)
+    return _take_coo(c)
+
+
+def input_mm(path: str, is_float: bool = False):
+    """COO::input_mm(path) as (rows, cols, row, col, val); raises on rejection."""
+    c = _COO()
+    rc = lib().or_input_mm(C.byref(c), path.encode(), int(is_float))
+    if rc != 0:
+        raise ValueError(f"or_input_mm({path}) rejected with {rc}")
+    return _take_coo(c)
+
+
+def coo_sort(row, col, val, merge: bool):
+    """COO::sort(merge) on triplets; returns sorted (row, col, val)."""
+    row = _c(row, np.int32).copy()
+    col = _c(col, np.int32).copy()
+    val = _c(val, np.float64).copy()
+    nnz = lib().or_coo_sort_arrays(len(row), _ptr(row), _ptr(col), _ptr(val), int(merge))
+    return row[:nnz], col[:nnz], val[:nnz]
+
+
+def crs_from_coo(rows, cols, row, col, val, dtype=np.float64):
+    """CRS(COO&) (src/utils/convert/convert_crs.cpp:9-43)."""
+    row = _c(row, np.int32)
+    col = _c(col, np.int32)
+    v = _c(val, np.float64)
+    c = _COO()
+    c.rows, c.cols, c.nnz, c.alloc = rows, cols, len(row), len(row)
+    c.row = row.ctypes.data_as(C.POINTER(C.c_int32))
+    c.col = col.ctypes.data_as(C.POINTER(C.c_int32))
+    c.val = v.ctypes.data_as(C.POINTER(C.c_double))
+    rp = np.zeros(rows + 1, np.int32)
+    ci = np.zeros(max(len(row), 1), np.int32)
+    vv = np.zeros(max(len(row), 1), np.float64)
+    lib().or_crs_from_coo(C.byref(c), _ptr(rp), _ptr(ci), _ptr(vv))
+    nnz = len(row)
+    return rp, ci[:nnz].copy(), vv[:nnz].astype(dtype)
+
+
+# ------------------------------------------------------------ BLAS / VML ----
+def crs_diag(rows, cols, rp, ci, val):
+    s = _sfx(val.dtype)
+    d = np.zeros(min(rows, cols), dtype=val.dtype)
+    getattr(lib(), f"or_crs_diag_{s}")(rows, cols, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)),
+                                       _ptr(val), _ptr(d))
+    return d
+
+
+def spmv(rp, ci, val, x):
+    rows = len(rp) - 1
+    y = np.zeros(rows, dtype=val.dtype)
+    x = _c(x, val.dtype)
+    getattr(lib(), f"or_spmv_{_sfx(val.dtype)}")(rows, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)),
+                                                 _ptr(val), _ptr(x), _ptr(y))
+    return y
+
+
+def spmv_t(rp, ci, val, x, cols):
+    rows = len(rp) - 1
+    y = np.zeros(cols, dtype=val.dtype)
+    x = _c(x, val.dtype)
+    getattr(lib(), f"or_spmv_t_{_sfx(val.dtype)}")(rows, cols, _ptr(_c(rp, np.int32)),
+                                                   _ptr(_c(ci, np.int32)), _ptr(val), _ptr(x), _ptr(y))
+    return y
+
+
+def dot(x, y):
+    x = np.ascontiguousarray(x)
+    y = _c(y, x.dtype)
+    return getattr(lib(), f"or_dot_{_sfx(x.dtype)}")(len(x), _ptr(x), _ptr(y))
+
+
+def nrm2(x):
+    x = np.ascontiguousarray(x)
+    return getattr(lib(), f"or_nrm2_{_sfx(x.dtype)}")(len(x), _ptr(x))
+
+
+def axpy(a, x, y):
+    y = y.copy()
+    getattr(lib(), f"or_axpy_{_sfx(y.dtype)}")(len(y), a, _ptr(_c(x, y.dtype)), _ptr(y))
+    return y
+
+
+def xpay(a, x, y):
+    y = y.copy()
+    getattr(lib(), f"or_xpay_{_sfx(y.dtype)}")(len(y), a, _ptr(_c(x, y.dtype)), _ptr(y))
+    return y
+
+
+def axpyz(a, x, y):
+    z = np.empty_like(y)
+    getattr(lib(), f"or_axpyz_{_sfx(y.dtype)}")(len(y), a, _ptr(_c(x, y.dtype)),
+                                                _ptr(_c(y, y.dtype)), _ptr(z))
+    return z
+
+
+def scal(a, x):
+    x = x.copy()
+    getattr(lib(), f"or_scal_{_sfx(x.dtype)}")(len(x), a, _ptr(x))
+    return x
+
+
+def _vv(name, a, b):
+    y = np.empty_like(a)
+    getattr(lib(), f"or_{name}_{_sfx(a.dtype)}")(len(a), _ptr(_c(a, a.dtype)), _ptr(_c(b, a.dtype)), _ptr(y))
+    return y
+
+
+def vadd(a, b):
+    return _vv("vadd", a, b)
+
+
+def vsub(a, b):
+    return _vv("vsub", a, b)
+
+
+def vmul(a, b):
+    return _vv("vmul", a, b)
+
+
+def vreciprocal(a):
+    y = np.empty_like(a)
+    getattr(lib(), f"or_vreciprocal_{_sfx(a.dtype)}")(len(a), _ptr(_c(a, a.dtype)), _ptr(y))
+    return y
+
+
+def residual_l2(rp, ci, val, x, b):
+    n = len(rp) - 1
+    return getattr(lib(), f"or_residual_l2_{_sfx(val.dtype)}")(
+        n, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)), _ptr(val),
+        _ptr(_c(x, val.dtype)), _ptr(_c(b, val.dtype)))
+
+
+# --------------------------------------------------------------- solvers ----
+def solve(method: str, rp, ci, val, x0, b, precond=PRECOND_JACOBI, tol=1e-8,
+          miniter=0, maxiter=2**62, history=False):
+    """Run equation::{CG,BiCGSTAB,Jacobi}::solve restated.  Returns
+    (status, x, final_iter, final_resid, rhistory or None)."""
+    n = len(rp) - 1
+    dt = val.dtype
+    x = _c(x0, dt).copy()
+    fi = C.c_int64(0)
+    fr = C.c_double(0.0)
+    hist = None
+    if history:
+        hist = np.full(int(min(maxiter, 1_000_000)), np.nan)
+    fn = getattr(lib(), f"or_{method}_{_sfx(dt)}")
+    rc = fn(n, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)), _ptr(val), _ptr(x),
+            _ptr(_c(b, dt)), precond, float(tol), int(miniter), int(maxiter),
+            C.byref(fi), C.byref(fr), _ptr(hist) if hist is not None else None)
+    if hist is not None:
+        hist = hist[: min(fi.value, len(hist))]
+    return rc, x, fi.value, fr.value, hist
+
+
+def num_threads() -> int:
+    return lib().or_num_threads()

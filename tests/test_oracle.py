@@ -1,0 +1,425 @@
+"""Pins the oracle (CPU restatement of monolish's reference path) against the
+reference's own known answers, the C++ standard library, and mathematics that
+fixes the answer independently of the oracle's code."""
+import json
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import GOLDEN, ans_check
+
+
+def _hex_d(v):
+    return struct.unpack("<d", bytes.fromhex(v)[::-1])[0]
+
+
+def _hex_f(v):
+    return struct.unpack("<f", bytes.fromhex(v)[::-1])[0]
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLDEN, "std_random.json")) as f:
+        return json.load(f)
+
+
+# ------------------------------------------------------------------ RNG ----
+def test_mt19937_standard_known_answer(oracle, golden):
+    # [rand.predef]: the 10000th invocation of a default-constructed
+    # std::mt19937 produces 4123659995.
+    g = oracle.MT19937()
+    v = 0
+    for _ in range(10000):
+        v = g()
+    assert v == 4123659995 == golden["mt19937_default_10000th"]
+
+
+def test_mt19937_seeded_stream(oracle, golden):
+    g = oracle.MT19937(1234)
+    assert [g() for _ in range(8)] == golden["mt19937_1234_first"]
+
+
+def test_uniform_double_matches_libstdcxx(oracle, golden):
+    for case in golden["uniform_double"]:
+        ref = np.array([_hex_d(v) for v in case["values"]])
+        got = oracle.random_vector(len(ref), case["min"], case["max"], case["seed"])
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), case["seed"]
+
+
+def test_uniform_float_matches_libstdcxx(oracle, golden):
+    for case in golden["uniform_float"]:
+        ref = np.array([_hex_f(v) for v in case["values"]], dtype=np.float32)
+        got = oracle.random_vector(len(ref), -1.0, 1.0, case["seed"], dtype=np.float32)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), case["seed"]
+
+
+def test_poisson_rhs_vector(oracle, golden):
+    v = oracle.random_vector(65536, -1.0, 1.0, 1001)
+    g = golden["rhs_65536_seed1001"]
+    assert v[-1] == _hex_d(g["last"])
+    s = 0.0
+    s2 = 0.0
+    for x in v.tolist():
+        s += x
+        s2 += x * x
+    assert s == g["sum"] and s2 == g["sumsq"]
+
+
+# ------------------------------------------------------------ COO / CRS ----
+def _dense_from_crs(rows, cols, rp, ci, val):
+    A = np.zeros((rows, cols))
+    for i in range(rows):
+        for j in range(rp[i], rp[i + 1]):
+            A[i, ci[j]] = val[j]
+    return A
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 4), (5, 5), (7, 2)])
+def test_laplacian_matches_stencil_definition(oracle, m, n):
+    rows, cols, r, c, v = oracle.laplacian_2d5p(m, n)
+    N = m * n
+    assert rows == cols == N
+    assert len(v) == 5 * m * n - 2 * m - 2 * n
+    rp, ci, val = oracle.crs_from_coo(rows, cols, r, c, v)
+    A = _dense_from_crs(N, N, rp, ci, val)
+    # the 5-point stencil written from its definition on an m x n grid
+    ref = np.zeros((N, N))
+    for gi in range(n):
+        for gj in range(m):
+            ii = gi * m + gj
+            ref[ii, ii] = 4.0
+            for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                a, b = gi + di, gj + dj
+                if 0 <= a < n and 0 <= b < m:
+                    ref[ii, a * m + b] = -1.0
+    assert np.array_equal(A, ref)
+    for i in range(N):  # ascending columns within each row
+        assert np.all(np.diff(ci[rp[i]:rp[i + 1]]) > 0)
+
+
+def test_laplacian_256_structure(oracle):
+    rows, cols, r, c, v = oracle.laplacian_2d5p(256, 256)
+    rp, ci, val = oracle.crs_from_coo(rows, cols, r, c, v)
+    assert rows == 65536 and len(val) == 326656
+    assert rp[0] == 0 and rp[-1] == 326656 and np.all(np.diff(rp) >= 3)
+    S = sp.csr_matrix((val, ci, rp), shape=(rows, cols))
+    assert (S - S.T).nnz == 0
+    rs = np.asarray(S.sum(axis=1)).ravel()
+    nb = np.diff(rp) - 1  # neighbours per row
+    assert np.array_equal(rs, 4.0 - nb)
+
+
+def _addr_coo():
+    # test/matrix/matrix_common.cpp: A = [[1,2,3],[4,0,5],[6,7,8]] built from
+    # triplet arrays in row-major order
+    row = np.array([0, 0, 0, 1, 1, 2, 2, 2], np.int32)
+    col = np.array([0, 1, 2, 0, 2, 0, 1, 2], np.int32)
+    val = np.array([1, 2, 3, 4, 5, 6, 7, 8], np.float64)
+    return row, col, val
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_reference_matvec_known_answers(oracle, dtype):
+    row, col, val = _addr_coo()
+    rp, ci, v = oracle.crs_from_coo(3, 3, row, col, val, dtype=dtype)
+    y = oracle.spmv(rp, ci, v, np.full(3, 10, dtype))
+    assert y.tolist() == [60, 90, 210]
+    rows, cols, r, c, vv = oracle.input_mm(os.path.join(GOLDEN, "test.mtx"), dtype == np.float32)
+    rp, ci, v = oracle.crs_from_coo(rows, cols, r, c, vv, dtype=dtype)
+    y = oracle.spmv(rp, ci, v, np.full(3, 10, dtype))
+    assert y.tolist() == [10, 0, 10]
+
+
+def test_reference_duplicate_merge_known_answer(oracle):
+    # matrix_common.cpp:185-209: expand to 4x4, insert (3,3)=1, then a
+    # duplicate (0,0)=3; sort(true) keeps 3.0 at (0,0) and nnz becomes 9
+    row, col, val = _addr_coo()
+    row = np.append(row, [3, 0]).astype(np.int32)
+    col = np.append(col, [3, 0]).astype(np.int32)
+    val = np.append(val, [1.0, 3.0])
+    r, c, v = oracle.coo_sort(row, col, val, merge=True)
+    assert len(v) == 9
+    at00 = [v[k] for k in range(len(v)) if r[k] == 0 and c[k] == 0]
+    assert at00 == [3.0]
+    keys = list(zip(r.tolist(), c.tolist()))
+    assert keys == sorted(keys)
+
+
+def test_coo_sort_is_a_sort(oracle):
+    rng = np.random.default_rng(7)
+    row = rng.integers(0, 40, 3000).astype(np.int32)
+    col = rng.integers(0, 40, 3000).astype(np.int32)
+    val = rng.standard_normal(3000)
+    r, c, v = oracle.coo_sort(row, col, val, merge=False)
+    key = r.astype(np.int64) * 64 + c
+    assert np.all(np.diff(key) >= 0)
+    assert sorted(zip(row.tolist(), col.tolist(), val.tolist())) == sorted(zip(r.tolist(), c.tolist(), v.tolist()))
+    r2, c2, v2 = oracle.coo_sort(row, col, val, merge=True)
+    assert len(set(zip(row.tolist(), col.tolist()))) == len(v2)
+    # merged value is one of the duplicates' values
+    groups = {}
+    for a, b, x in zip(row.tolist(), col.tolist(), val.tolist()):
+        groups.setdefault((a, b), set()).add(x)
+    for a, b, x in zip(r2.tolist(), c2.tolist(), v2.tolist()):
+        assert x in groups[(a, b)]
+
+
+def test_crs_from_coo_empty_rows(oracle):
+    # rows 0, 2 and 4 empty (row 4 is trailing): interior empty rows take the
+    # current offset, trailing rows keep 0 as the reference leaves them
+    row = np.array([1, 1, 3], np.int32)
+    col = np.array([0, 2, 1], np.int32)
+    val = np.array([1.0, 2.0, 3.0])
+    rp, ci, v = oracle.crs_from_coo(5, 3, row, col, val)
+    assert rp.tolist() == [0, 0, 2, 2, 3, 0]
+    y = oracle.spmv(rp, ci, v, np.array([1.0, 10.0, 100.0]))
+    assert y.tolist() == [0.0, 201.0, 0.0, 30.0, 0.0]
+
+
+def test_input_mm_rejects_non_general(oracle, tmp_path):
+    p = tmp_path / "sym.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real symmetric\n2 2 1\n1 1 1\n")
+    with pytest.raises(ValueError):
+        oracle.input_mm(str(p))
+
+
+def test_input_mm_comments_and_float_parse(oracle, tmp_path):
+    p = tmp_path / "c.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n% c1\n%c2\n2 3 2\n1 3 0.1\n2 1 -2.5e-3\n")
+    rows, cols, r, c, v = oracle.input_mm(str(p))
+    assert (rows, cols) == (2, 3) and r.tolist() == [0, 1] and c.tolist() == [2, 0]
+    assert v.tolist() == [0.1, -2.5e-3]
+    _, _, _, _, vf = oracle.input_mm(str(p), is_float=True)
+    assert vf.tolist() == [float(np.float32(0.1)), float(np.float32(-2.5e-3))]
+
+
+# ------------------------------------------------------------------ SpMV ----
+def _random_crs(rng, rows, cols, maxlen, sorted_cols=True, dtype=np.float64):
+    lens = rng.integers(0, maxlen + 1, rows)
+    rp = np.zeros(rows + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate([rng.choice(cols, size=l, replace=False) for l in lens]).astype(np.int32) \
+        if rp[-1] else np.zeros(0, np.int32)
+    if sorted_cols:
+        for i in range(rows):
+            ci[rp[i]:rp[i + 1]].sort()
+    val = rng.uniform(-1, 1, rp[-1]).astype(dtype)
+    return rp, ci, val
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_equals_dense_triple_loop(oracle, dtype):
+    # test/matrix/blas/matvec.hpp: dense answer loop; with ascending columns
+    # the skipped zero terms add exactly nothing, so the CRS sum must match
+    # the dense left-to-right sum bit for bit
+    rng = np.random.default_rng(11)
+    rp, ci, val = _random_crs(rng, 40, 57, 12, dtype=dtype)
+    x = rng.uniform(-1, 1, 57).astype(dtype)
+    A = _dense_from_crs(40, 57, rp, ci, val).astype(dtype)
+    ref = np.empty(40, dtype)
+    for i in range(40):
+        s = dtype(0)
+        for j in range(57):
+            s = dtype(s + dtype(A[i, j] * x[j]))
+        ref[i] = s
+    y = oracle.spmv(rp, ci, val, x)
+    assert np.array_equal(y, ref)
+
+
+def test_spmv_forward_error_bound(oracle):
+    rng = np.random.default_rng(12)
+    rp, ci, val = _random_crs(rng, 200, 300, 64, sorted_cols=False)
+    x = rng.uniform(-1, 1, 300)
+    y = oracle.spmv(rp, ci, val, x)
+    u = 2.0 ** -53
+    for i in range(200):
+        terms = [val[j] * x[ci[j]] for j in range(rp[i], rp[i + 1])]
+        k = len(terms)
+        exact = math.fsum(terms)
+        bound = (k * u / (1 - k * u)) * sum(abs(t) for t in terms) + 1e-300
+        assert abs(y[i] - exact) <= bound
+
+
+def test_spmv_transpose_matches_dense(oracle):
+    rng = np.random.default_rng(13)
+    rp, ci, val = _random_crs(rng, 30, 20, 8)
+    x = rng.integers(-4, 5, 30).astype(np.float64)
+    vals_int = np.round(val * 8)  # integer-valued: every sum is exact
+    y = oracle.spmv_t(rp, ci, vals_int, x, 20)
+    A = _dense_from_crs(30, 20, rp, ci, vals_int)
+    assert np.array_equal(y, A.T @ x)
+
+
+# ------------------------------------------------------------- BLAS-1 ----
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 255, 511, 512, 513, 1024, 4097, 131072 + 77])
+def test_dot_error_bound(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n)
+    y = rng.uniform(-1, 1, n)
+    d = oracle.dot(x, y)
+    terms = (x * y).tolist()
+    exact = math.fsum(terms)
+    u = 2.0 ** -53
+    depth = 1 + 8 + max(1, math.ceil(max(n, 1) / 512 / 256)) + 8  # additions on any path
+    bound = depth * u / (1 - depth * u) * sum(abs(t) for t in terms) + (n * 2 ** -1074)
+    assert abs(d - exact) <= bound
+    if n <= 2:
+        assert d == math.fsum(terms) if n < 2 else d == x[0] * y[0] + x[1] * y[1]
+
+
+def test_dot_integer_data_exact(oracle):
+    rng = np.random.default_rng(3)
+    x = rng.integers(-1000, 1000, 300001).astype(np.float64)
+    y = rng.integers(-1000, 1000, 300001).astype(np.float64)
+    assert oracle.dot(x, y) == float(np.dot(x.astype(np.int64), y.astype(np.int64)))
+    xf = rng.integers(-30, 30, 70001).astype(np.float32)
+    assert oracle.dot(xf, xf) == float(np.sum(xf.astype(np.int64) ** 2))
+
+
+def test_nrm2_is_sqrt_of_dot(oracle):
+    # src/blas/vector/nrm2.hpp: sqrt(dot(x,x)), no scaling: 3-4-5 and the
+    # overflow the unscaled form has for large entries
+    assert oracle.nrm2(np.array([3.0, 4.0])) == 5.0
+    assert oracle.nrm2(np.array([1e200, 1e200])) == math.inf
+    assert oracle.nrm2(np.array([3.0, 4.0], np.float32)) == 5.0
+
+
+def test_vector_ops_definitions(oracle):
+    x = np.array([1.0, -2.0, 0.5, 3.0])
+    y = np.array([10.0, 20.0, -30.0, 0.25])
+    a = 3.0
+    assert oracle.axpy(a, x, y).tolist() == [13.0, 14.0, -28.5, 9.25]      # y = a x + y
+    assert oracle.xpay(a, x, y).tolist() == [31.0, 58.0, -89.5, 3.75]      # y = x + a y
+    assert oracle.axpyz(a, x, y).tolist() == [13.0, 14.0, -28.5, 9.25]     # z = a x + y
+    assert oracle.scal(a, x).tolist() == [3.0, -6.0, 1.5, 9.0]
+    assert oracle.vsub(x, y).tolist() == [-9.0, -22.0, 30.5, 2.75]
+    assert oracle.vadd(x, y).tolist() == [11.0, 18.0, -29.5, 3.25]
+    assert oracle.vmul(x, y).tolist() == [10.0, -40.0, -15.0, 0.75]
+    r = oracle.vreciprocal(np.array([2.0, -4.0, 0.0, 3.0]))
+    assert r[:3].tolist() == [0.5, -0.25, math.inf] and r[3] == 1.0 / 3.0
+    rf = oracle.vreciprocal(np.array([3.0, 7.0], np.float32))
+    assert rf.tolist() == [float(np.float32(1) / np.float32(3)), float(np.float32(1) / np.float32(7))]
+
+
+def test_crs_diag_last_match_wins(oracle):
+    rp = np.array([0, 3, 4, 4], np.int32)
+    ci = np.array([0, 0, 2, 0], np.int32)
+    val = np.array([1.0, 9.0, 5.0, 7.0])
+    d = oracle.crs_diag(3, 3, rp, ci, val)
+    assert d.tolist() == [9.0, 0.0, 0.0]
+
+
+# --------------------------------------------------------------- solvers ----
+def _mtx_crs(oracle, name, dtype):
+    rows, cols, r, c, v = oracle.input_mm(os.path.join(GOLDEN, name), dtype == np.float32)
+    return oracle.crs_from_coo(rows, cols, r, c, v, dtype=dtype)
+
+
+@pytest.mark.parametrize("method,mtx", [("cg", "test.mtx"), ("bicgstab", "test.mtx"),
+                                        ("bicgstab", "test2.mtx"), ("jacobi", "test.mtx")])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-8), (np.float32, 1e-4)])
+@pytest.mark.parametrize("precond", [1, 2])
+def test_reference_solver_harness(oracle, method, mtx, dtype, tol, precond):
+    # test/equation/iterative/equation_kernel.hpp:4-56: b = A*1, x0 from
+    # vector(n, 0, 1, test_random_engine()) with test_random_engine =
+    # mt19937(1234), maxiter 10000; pass when x is within 10*tol of 1
+    rp, ci, val = _mtx_crs(oracle, mtx, dtype)
+    n = len(rp) - 1
+    ans = np.ones(n, dtype)
+    b = oracle.spmv(rp, ci, val, ans)
+    x0 = oracle.random_vector(n, 0.0, 1.0, 822569775, dtype=dtype)
+    rc, x, it, res, _ = oracle.solve(method, rp, ci, val, x0, b, precond=precond, tol=tol, maxiter=10000)
+    assert rc == 0
+    assert ans_check(x, ans, tol * 10)
+
+
+def _poisson(oracle, m, dtype=np.float64):
+    rows, cols, r, c, v = oracle.laplacian_2d5p(m, m)
+    return oracle.crs_from_coo(rows, cols, r, c, v, dtype=dtype)
+
+
+def test_cg_matches_direct_solve(oracle):
+    rp, ci, val = _poisson(oracle, 24)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 1001)
+    tol = 1e-10 * oracle.nrm2(b)
+    rc, x, it, res, hist = oracle.solve("cg", rp, ci, val, np.zeros(n), b, tol=tol, history=True)
+    assert rc == 0 and res < tol and it <= n
+    A = sp.csr_matrix((val, ci, rp), shape=(n, n))
+    xs = spla.spsolve(A.tocsc(), b)
+    kappa = 4.0 / (4 * math.sin(math.pi / 50) ** 2)  # upper bound for 24x24
+    assert np.linalg.norm(x - xs) <= kappa * 1e-10 * np.linalg.norm(xs) * 10
+    # the recursively updated residual tracks the true residual
+    true_res = oracle.residual_l2(rp, ci, val, x, b)
+    assert abs(true_res - res) <= 1e-3 * tol + 1e-14 * np.linalg.norm(b)
+    assert hist[-1] == res
+
+
+def test_cg_finite_termination(oracle):
+    # SPD diagonal matrix with 3 distinct eigenvalues: CG ends in 3 steps
+    n = 30
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.arange(n, dtype=np.int32)
+    val = np.array([1.0, 2.0, 5.0] * 10)
+    b = oracle.random_vector(n, -1.0, 1.0, 5)
+    rc, x, it, res, _ = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=1, tol=1e-12)
+    assert rc == 0 and it == 3
+    assert np.allclose(x, b / val, rtol=1e-13)
+
+
+def test_bicgstab_nonsymmetric_direct(oracle):
+    # 1D upwind convection-diffusion, nonsymmetric
+    n = 200
+    A = sp.diags([-1.0 - 0.8, 2.0 + 0.8, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+    A.sort_indices()
+    rp, ci, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+    b = oracle.random_vector(n, -1.0, 1.0, 9)
+    rc, x, it, res, _ = oracle.solve("bicgstab", rp, ci, val, np.zeros(n), b, tol=1e-12)
+    assert rc == 0
+    xs = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_jacobi_solver_direct_and_relative_stop(oracle):
+    n = 50
+    A = sp.diags([-1.0, 4.0, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+    rp, ci, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    b = oracle.random_vector(n, -1.0, 1.0, 3)
+    rc, x, it, res, hist = oracle.solve("jacobi", rp, ci, val, np.zeros(n), b, tol=1e-10, history=True)
+    assert rc == 0 and hist[-1] < 1e-10 and (it == 1 or hist[-2] >= 1e-10)
+    xs = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs)
+    # 1x1: one Jacobi step is the exact solve
+    rc, x, it, res, _ = oracle.solve("jacobi", np.array([0, 1], np.int32), np.array([0], np.int32),
+                                     np.array([4.0]), np.zeros(1), np.array([2.0]), tol=1e-12)
+    assert x.tolist() == [0.5] and it == 2 and rc == 0
+
+
+def test_solver_error_paths(oracle):
+    rp, ci, val = _poisson(oracle, 8)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 1)
+    # maxiter: MAXITER with final_iter == maxiter
+    rc, x, it, res, _ = oracle.solve("cg", rp, ci, val, np.zeros(n), b, tol=1e-30, maxiter=5)
+    assert rc == oracle.MAXITER and it == 5 and res > 0
+    # NaN in b: RESIDUAL_NAN at the first iteration
+    bn = b.copy()
+    bn[3] = np.nan
+    for m in ("cg", "bicgstab"):
+        rc, x, it, res, _ = oracle.solve(m, rp, ci, val, np.zeros(n), bn, tol=1e-8)
+        assert rc == oracle.RESIDUAL_NAN and it == 1, m
+    # r0 = 0: CG divides 0/0 (NaN); BiCGSTAB reports breakdown
+    xe = oracle.random_vector(n, -1.0, 1.0, 2)
+    be = oracle.spmv(rp, ci, val, xe)
+    rc, x, it, res, _ = oracle.solve("cg", rp, ci, val, xe, be, tol=1e-8)
+    assert rc == oracle.RESIDUAL_NAN and it == 1 and np.all(np.isnan(x))
+    rc, x, it, res, _ = oracle.solve("bicgstab", rp, ci, val, xe, be, tol=1e-8)
+    assert rc == oracle.BREAKDOWN and it == 1
+    # miniter forces iterations past convergence
+    rc, x, it, res, _ = oracle.solve("cg", rp, ci, val, np.zeros(n), b, tol=1.0, miniter=7)
+    assert rc == 0 and it == 7
