@@ -1,0 +1,218 @@
+/*
+ * monolish_amd.h -- C-ABI of the MI355X (gfx950) CRS sparse BLAS and Krylov
+ * solve path.  Plain pointers and sizes only; every call returns an int status
+ * (MGX_OK = 0; solver outcomes use monolish's MONOLISH_SOLVER_* values;
+ * ABI errors are <= -100) and never throws.  mgx_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Each entry point names the monolish interface it replaces (paths relative
+ * to the monolish 0.17.3 source tree).  The drop-in C++ layer
+ * (include/monolish_*.hpp) calls these when data is device resident.
+ *
+ * Threading: one mgx_ctx per host thread (or external locking); all work of a
+ * context is ordered on its HIP stream.
+ */
+#ifndef MONOLISH_AMD_H
+#define MONOLISH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+/* solver outcomes: include/monolish/common/monolish_common.hpp:16-21 */
+#define MGX_OK 0
+#define MGX_SOLVER_SIZE_ERROR -1
+#define MGX_SOLVER_MAXITER -2
+#define MGX_SOLVER_BREAKDOWN -3
+#define MGX_SOLVER_RESIDUAL_NAN -4
+#define MGX_SOLVER_NOT_IMPL -10
+/* ABI errors */
+#define MGX_ERR_INVALID -101  /* bad argument, size or state */
+#define MGX_ERR_HIP -102      /* HIP runtime failure */
+#define MGX_ERR_NOMEM -103    /* device or host allocation failed */
+#define MGX_ERR_BUDGET -104   /* request exceeds the configured HBM budget */
+#define MGX_ERR_NOT_FOUND -105
+
+typedef enum { MGX_F64 = 0, MGX_F32 = 1 } mgx_dtype;
+
+typedef struct mgx_ctx mgx_ctx;
+typedef struct mgx_vec mgx_vec;
+typedef struct mgx_csr mgx_csr;
+
+/* ---- context ------------------------------------------------------------ */
+const char *mgx_last_error(void);
+int mgx_version(int *major, int *minor, int *patch);
+int mgx_device_count(int *count);
+/* util::set_default_device / device query, src/utils/system/gpu_deivce.cpp:6-34 */
+int mgx_init(int device, mgx_ctx **out);
+int mgx_finalize(mgx_ctx *ctx);
+int mgx_sync(mgx_ctx *ctx);
+/* the context's hipStream_t (as void*) for callers that record events */
+void *mgx_stream(mgx_ctx *ctx);
+/* name (NUL-terminated, truncated to len), compute units, HBM bytes */
+int mgx_device_info(mgx_ctx *ctx, char *name, size_t len, int *compute_units,
+                    size_t *hbm_bytes);
+/* SpMV tuning: rows per block limit, entries per block limit, long-row
+ * threshold, XCD-aware block order.  Applies to CRS created afterwards. */
+int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
+                        int long_row, int xcd_remap);
+
+/* ---- vectors: vector<T>::send/recv/nonfree_recv/device_free ------------- */
+/* vector<T>::send (src/utils/system/gpu_comm.cpp:8-24): device copy of n
+ * elements; host may be NULL for a zero-filled vector */
+int mgx_vec_create(mgx_ctx *ctx, int64_t n, mgx_dtype dtype, const void *host,
+                   mgx_vec **out);
+/* view1D<vector<T>,T>(vec, offset, n) (include/monolish/common/
+ * monolish_view1D.hpp:50): shares the base vector's device memory */
+int mgx_vec_view(mgx_vec *base, int64_t offset, int64_t n, mgx_vec **out);
+/* send() on a resident vector: host -> device update (gpu_comm.cpp:14-16) */
+int mgx_vec_upload(mgx_vec *v, const void *host);
+/* nonfree_recv (gpu_comm.cpp:45-57): device -> host, stays resident */
+int mgx_vec_download(mgx_vec *v, void *host);
+/* device_free (gpu_comm.cpp:60-73); recv = download + destroy */
+int mgx_vec_destroy(mgx_vec *v);
+/* vector(N, value) / internal::vbroadcast (double_arithmetic.cpp:255-275) */
+int mgx_vec_fill(mgx_vec *v, double value);
+int64_t mgx_vec_size(const mgx_vec *v);
+void *mgx_vec_device_ptr(mgx_vec *v);
+
+/* ---- CRS: matrix::CRS<T> residency (gpu_comm.cpp:89-171) ---------------- */
+/* CRS<T>::send: uploads row_ptr[rows+1], col_ind[nnz], val[nnz] (int32,
+ * 0-based, monolish_crs.hpp:76-103) and builds the SpMV row-block plan.
+ * Validates indices (row_ptr non-decreasing except the all-zero tail that
+ * CRS(COO&) leaves after the last non-empty row; 0 <= col < cols). */
+int mgx_csr_create(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
+                   const int32_t *row_ptr, const int32_t *col_ind,
+                   const void *val, mgx_dtype dtype, mgx_csr **out);
+/* CRS<T>::send on a resident matrix: value update (gpu_comm.cpp:95-99) */
+int mgx_csr_update_values(mgx_csr *A, const void *val);
+/* CRS<T>::nonfree_recv (gpu_comm.cpp:134-150); any pointer may be NULL */
+int mgx_csr_download(mgx_csr *A, int32_t *row_ptr, int32_t *col_ind,
+                     void *val);
+/* CRS<T>::device_free (gpu_comm.cpp:153-171) */
+int mgx_csr_destroy(mgx_csr *A);
+/* CRS<T>::diag (src/utils/getvec/crs_getvec.cpp:8-55) */
+int mgx_csr_diag(mgx_csr *A, mgx_vec *d);
+
+typedef struct mgx_csr_stats {
+  int64_t rows, cols, nnz;
+  int32_t max_row, min_row;
+  double mean_row;
+  int64_t stream_blocks, long_rows; /* SpMV plan */
+} mgx_csr_stats;
+int mgx_csr_stats_get(mgx_csr *A, mgx_csr_stats *out);
+
+/* ---- sparse BLAS -------------------------------------------------------- */
+/* blas::matvec(A, x, y) / matvec_N (src/blas/matrix/matvec/
+ * matvec_blas.cpp:3849-3860, crs_matvec.hpp:8-114): y = A x */
+int mgx_matvec(mgx_csr *A, mgx_vec *x, mgx_vec *y);
+/* blas::matvec(a, A, x, b, y): y = a A x + b y (the device path's meaning,
+ * crs_matvec.hpp:37-65) */
+int mgx_matvec_ab(double alpha, mgx_csr *A, mgx_vec *x, double beta,
+                  mgx_vec *y);
+/* blas::matvec_T (crs_matvec.hpp:88-98): y = A^T x */
+int mgx_matvec_t(mgx_csr *A, mgx_vec *x, mgx_vec *y);
+
+/* ---- vector BLAS / VML -------------------------------------------------- */
+/* blas::dot (src/blas/vector/dot.hpp:6-78): blocking, scalar to host */
+int mgx_dot(mgx_vec *x, mgx_vec *y, double *out);
+/* blas::nrm2 (src/blas/vector/nrm2.hpp:6-67): sqrt(dot(x,x)) */
+int mgx_nrm2(mgx_vec *x, double *out);
+/* blas::axpy (axpy.hpp:7-61): y = a x + y */
+int mgx_axpy(double a, mgx_vec *x, mgx_vec *y);
+/* blas::xpay (xpay.hpp:7-66): y = x + a y */
+int mgx_xpay(double a, mgx_vec *x, mgx_vec *y);
+/* blas::scal (scal.hpp:6-49): x = a x */
+int mgx_scal(double a, mgx_vec *x);
+/* blas::copy (copy.hpp:5-13): y = x */
+int mgx_copy(mgx_vec *x, mgx_vec *y);
+/* blas::axpyz (axpyz.hpp:7-66): z = a x + y */
+int mgx_axpyz(double a, mgx_vec *x, mgx_vec *y, mgx_vec *z);
+/* vml::add / sub / mul (src/internal/arithmetic/double_arithmetic.cpp:
+ * 112-195): y = a op b */
+int mgx_vadd(mgx_vec *a, mgx_vec *b, mgx_vec *y);
+int mgx_vsub(mgx_vec *a, mgx_vec *b, mgx_vec *y);
+int mgx_vmul(mgx_vec *a, mgx_vec *b, mgx_vec *y);
+/* vml::reciprocal (double_arithmetic.cpp:308-330): y = 1 / a */
+int mgx_vreciprocal(mgx_vec *a, mgx_vec *y);
+
+/* ---- Krylov solvers ----------------------------------------------------- */
+#define MGX_PRECOND_DEFAULT 0 /* no preconditioner bound: z = r */
+#define MGX_PRECOND_NONE 1    /* equation::none (src/equation/none.cpp) */
+#define MGX_PRECOND_JACOBI 2  /* equation::Jacobi (src/equation/jacobi.cpp) */
+
+/* how the iteration is executed; every mode follows the reference's
+ * arithmetic, so FP64 results of REFERENCE and FUSED are bit-identical */
+#define MGX_MODE_REFERENCE 0 /* one kernel per BLAS call, scalars to host */
+#define MGX_MODE_FUSED 1     /* fused kernels, device scalars, graph replay */
+
+typedef struct mgx_solve_params {
+  double tol;          /* solver::set_tol (monolish_solver.hpp:86-215) */
+  int64_t maxiter;     /* solver::set_maxiter; <= 0 means unlimited */
+  int64_t miniter;     /* solver::set_miniter */
+  int32_t precond;     /* MGX_PRECOND_* */
+  int32_t mode;        /* MGX_MODE_* */
+  int32_t tol_is_relative; /* 1: the threshold is tol * ||b|| */
+  int32_t check_every; /* FUSED: iterations per replayed graph (0: auto) */
+  double *rhistory;    /* optional host array [rhistory_len] of residuals */
+  int64_t rhistory_len;
+} mgx_solve_params;
+
+typedef struct mgx_solve_result {
+  int32_t status;      /* MONOLISH_SOLVER_* */
+  int32_t pad;
+  int64_t final_iter;  /* solver::get_final_iter */
+  double final_resid;  /* solver::get_final_residual */
+  double seconds;      /* wall time of the solve call */
+  int64_t inner_iter;  /* mixed precision: total inner iterations */
+  int64_t outer_iter;  /* mixed precision: refinement steps */
+} mgx_solve_result;
+
+void mgx_solve_params_default(mgx_solve_params *p);
+
+/* equation::CG<CRS<T>,T>::solve (src/equation/cg.cpp:12-116) */
+int mgx_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                 mgx_solve_result *res);
+/* equation::BiCGSTAB<CRS<T>,T>::solve (src/equation/bicgstab.cpp:12-166) */
+int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                       const mgx_solve_params *p, mgx_solve_result *res);
+/* equation::Jacobi<CRS<T>,T>::solve (src/equation/jacobi.cpp:83-183) */
+int mgx_jacobi_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                     const mgx_solve_params *p, mgx_solve_result *res);
+/* util::get_residual_l2 (src/utils/equation_utils.cpp:45-56): ||b - A x|| */
+int mgx_residual_l2(mgx_csr *A, mgx_vec *x, mgx_vec *b, double *out);
+
+/* Mixed-precision iterative refinement around Jacobi-PCG: FP64 A, x, b; the
+ * inner solves run on FP32 values and vectors with FP32 accumulation, the
+ * outer residual r = b - A x, the update and the stopping test
+ * ||r|| < tol (or tol*||b||) are FP64.  No monolish counterpart: it is the
+ * precision form of equation::CG (cg.cpp) for FP64 CRS matrices.
+ * inner_tol is the relative reduction each inner solve aims for. */
+int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                    const mgx_solve_params *p, double inner_tol,
+                    int64_t max_outer, mgx_solve_result *res);
+
+/* ---- workload generators (host) ----------------------------------------- */
+/* 3D stencil on an nx*ny*nz grid, row = (k*ny + j)*nx + i, ascending columns,
+ * Dirichlet truncation.  points = 7: diag 6, off-diag -1; points = 27: diag
+ * 26, off-diag -1.  shift is added to the diagonal.  Pass NULL arrays to get
+ * nnz only. */
+int mgx_gen_stencil3d(int64_t nx, int64_t ny, int64_t nz, int points,
+                      double shift, int64_t *nnz, int32_t *row_ptr,
+                      int32_t *col_ind, double *val);
+/* random CRS: row lengths round(exp(N(mu,sigma))) clipped to [1,maxlen],
+ * mu = ln(mean) - sigma^2/2; distinct uniform columns sorted ascending;
+ * values U[-1,1).  Two calls: row_ptr only (col/val NULL) gives nnz, then
+ * the full fill.  Deterministic in seed and independent of thread count. */
+int mgx_gen_random_lognormal(int64_t rows, int64_t cols, double mean,
+                             double sigma, int32_t maxlen, uint64_t seed,
+                             int32_t *row_ptr, int32_t *col_ind, double *val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

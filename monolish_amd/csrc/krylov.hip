@@ -1,0 +1,788 @@
+// krylov.hip -- CG, BiCGSTAB and Jacobi solve loops (src/equation/cg.cpp,
+// bicgstab.cpp, jacobi.cpp) on gfx950.
+//
+// MGX_MODE_REFERENCE runs the reference's call sequence one kernel per BLAS
+// call with every dot/nrm2 returned to the host (the "plain" GPU solver).
+// MGX_MODE_FUSED runs Jacobi-PCG as two kernels per iteration with all
+// scalars on the device:
+//   cg_spmv:   p = z + beta p is formed at each gather (p never makes an extra
+//              pass), q = A p with the CRS-order row sums, the new p is stored
+//              for the block's own rows, and <p,q> is reduced in tile order;
+//              the last block computes alpha = <z,r> / <p,q>.
+//   cg_update: x += alpha p, r -= alpha q, z = M r, with <z,r> and <r,r> in
+//              tile order; the last block forms beta, the residual and the
+//              stopping decision.
+// Every value is rounded exactly as in the reference loop (same operations,
+// same order, tile-order dots), so REFERENCE and FUSED give identical bits.
+// Kernels exit at once when the solve is no longer running, so the host can
+// queue several iterations between status checks.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mgx_device.hpp"
+
+namespace mgx {
+
+namespace {
+
+template <class T> __device__ __forceinline__ T dsqrt(T v);
+template <> __device__ __forceinline__ double dsqrt(double v) { return __builtin_sqrt(v); }
+template <> __device__ __forceinline__ float dsqrt(float v) { return __builtin_sqrtf(v); }
+
+template <class T> struct CgArgs {
+  int64_t n, ntiles;
+  const int32_t *rp, *ci;
+  const T *val;
+  const int32_t *chunks, *tile_chunk;
+  const T *M;  // Jacobi 1/diag, or nullptr for z = r
+  T *x, *r, *z, *q, *p_old, *p_new, *P;
+  SolveState *st;
+  double *rhist;
+  int64_t rhist_len;
+  int remap;
+};
+
+template <class T>
+__device__ __forceinline__ T p_value(const CgArgs<T> &a, int64_t c, bool first,
+                                     T beta) {
+  if (first) return a.z[c];
+  T bp = beta * a.p_old[c];
+  return a.z[c] + bp;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *prod = reinterpret_cast<T *>(smem_raw);
+  __shared__ T qloc[kTile];
+  __shared__ T sh[8];
+  __shared__ int flag;
+  SolveState *st = a.st;
+  if (st->status != kStatRunning) return;
+  const int64_t k = xcd_block(a.remap);
+  if (k >= a.ntiles) return;
+  const bool first = st->first != 0;
+  const T beta = (T)st->beta;
+  const int64_t R0 = k * kTile;
+  const int64_t R1 = R0 + kTile < a.n ? R0 + kTile : a.n;
+  const int t = threadIdx.x;
+  for (int32_t ck = a.tile_chunk[k]; ck < a.tile_chunk[k + 1]; ck++) {
+    const int32_t d0 = a.chunks[ck], d1 = a.chunks[ck + 1];
+    const int32_t ra = d0 & 0x7fffffff, rb = d1 & 0x7fffffff;
+    if (d0 < 0) {
+      const int32_t e0 = a.rp[ra], e1 = a.rp[ra + 1];
+      T s = 0;
+      for (int32_t e = e0 + t; e < e1; e += kBlock) {
+        T pr = a.val[e] * p_value(a, a.ci[e], first, beta);
+        s = s + pr;
+      }
+      T res = block_tree(s, sh);
+      if (t == 0) {
+        qloc[ra - R0] = res;
+        a.q[ra] = res;
+      }
+      __syncthreads();
+      continue;
+    }
+    const int32_t e0 = a.rp[ra], e1 = a.rp[rb];
+    for (int32_t e = e0 + t; e < e1; e += kBlock) {
+      T v = __builtin_nontemporal_load(&a.val[e]);
+      int32_t c = __builtin_nontemporal_load(&a.ci[e]);
+      prod[e - e0] = v * p_value(a, c, first, beta);
+    }
+    __syncthreads();
+    const int32_t row = ra + t;
+    if (row < rb) {
+      const int32_t lo = a.rp[row] - e0, hi = a.rp[row + 1] - e0;
+      T s = 0;
+      for (int32_t j = lo; j < hi; j++) s = s + prod[j];
+      qloc[row - R0] = s;
+      a.q[row] = s;
+    }
+    __syncthreads();
+  }
+  // new p for the tile's own rows and the tile partial of <p, q>
+  const int64_t i0 = R0 + 2 * t;
+  T acc0 = 0, acc1 = 0;
+  if (i0 < R1) {
+    T pn = p_value(a, i0, first, beta);
+    a.p_new[i0] = pn;
+    acc0 = pn * qloc[2 * t];
+  }
+  if (i0 + 1 < R1) {
+    T pn = p_value(a, i0 + 1, first, beta);
+    a.p_new[i0 + 1] = pn;
+    acc1 = pn * qloc[2 * t + 1];
+  }
+  T s = block_tree(acc0 + acc1, sh);
+  if (t == 0) a.P[k] = s;
+  if (last_block_arrives(&st->ticket[0], (uint32_t)a.ntiles, &flag)) {
+    T pq = reduce_partials(a.P, a.ntiles, sh);
+    if (t == 0) {
+      T zr = (T)st->zr;
+      T alpha = zr / pq;
+      st->pq = (double)pq;
+      st->alpha = (double)alpha;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
+  __shared__ T sh[8];
+  __shared__ int flag;
+  SolveState *st = a.st;
+  if (st->status != kStatRunning) return;
+  const int64_t k = xcd_block(a.remap);
+  if (k >= a.ntiles) return;
+  const T alpha = (T)st->alpha;
+  const T nalpha = -alpha;
+  const int t = threadIdx.x;
+  const int64_t i0 = k * kTile + 2 * t;
+  T zr2[2] = {0, 0}, rr2[2] = {0, 0};
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    const int64_t i = i0 + e;
+    if (i < a.n) {
+      T pv = a.p_new[i], qv = a.q[i];
+      T t1 = alpha * pv;
+      a.x[i] = t1 + a.x[i];
+      T t2 = nalpha * qv;
+      T rv = t2 + a.r[i];
+      a.r[i] = rv;
+      T zv = a.M ? a.M[i] * rv : rv;
+      a.z[i] = zv;
+      zr2[e] = zv * rv;
+      rr2[e] = rv * rv;
+    }
+  }
+  T zr_a = zr2[0] + zr2[1];
+  T rr_a = rr2[0] + rr2[1];
+  block_tree2(zr_a, rr_a, sh);
+  if (t == 0) {
+    a.P[k] = zr_a;
+    a.P[a.ntiles + k] = rr_a;
+  }
+  if (last_block_arrives(&st->ticket[1], (uint32_t)a.ntiles, &flag)) {
+    T zr_new = reduce_partials(a.P, a.ntiles, sh);
+    __syncthreads();
+    T rr = reduce_partials(a.P + a.ntiles, a.ntiles, sh);
+    if (t == 0) {
+      const T zr_old = (T)st->zr;
+      const T beta = zr_new / zr_old;
+      const T resid = dsqrt(rr);
+      const int64_t it = st->iter;
+      if (a.rhist && it < a.rhist_len) a.rhist[it] = (double)resid;
+      st->beta = (double)beta;
+      st->zr = (double)zr_new;
+      st->rr = (double)rr;
+      st->resid = (double)resid;
+      st->first = 0;
+      st->iter = it + 1;
+      if ((double)resid < st->tol && st->miniter <= it + 1)
+        st->status = MGX_OK;
+      else if (resid != resid)
+        st->status = MGX_SOLVER_RESIDUAL_NAN;
+      else if (it + 1 >= st->maxiter)
+        st->status = MGX_SOLVER_MAXITER;
+    }
+  }
+}
+
+template <class T> T *tp(void *p) { return static_cast<T *>(p); }
+template <class T> const T *tp(const void *p) { return static_cast<const T *>(p); }
+
+template <class T> struct DT;
+template <> struct DT<double> { static constexpr mgx_dtype v = MGX_F64; };
+template <> struct DT<float> { static constexpr mgx_dtype v = MGX_F32; };
+
+template <class T> T host_sqrt(T v) { return std::sqrt(v); }
+
+}  // namespace
+
+// row chunks for the fused kernels: <= 256 rows, <= budget entries, never
+// crossing a 512-row tile; rows longer than the budget become long chunks
+void fused_plan_build(Csr *A) {
+  if (A->d_chunks) return;
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  std::vector<int32_t> rp(n + 1);
+  MGX_HIP(hipMemcpy(rp.data(), A->rp, (n + 1) * 4, hipMemcpyDeviceToHost));
+  const int budget = std::max(c->spmv_nnz, 256) * 2;
+  const int64_t nt = ntiles_of(n);
+  std::vector<int32_t> chunks, tile_chunk(nt + 1);
+  for (int64_t k = 0; k < nt; k++) {
+    tile_chunk[k] = (int32_t)chunks.size();
+    const int64_t R1 = std::min(n, (k + 1) * (int64_t)kTile);
+    int64_t r = k * kTile;
+    while (r < R1) {
+      int32_t len = rp[r + 1] - rp[r];
+      if (len > budget) {
+        chunks.push_back((int32_t)r | (int32_t)0x80000000u);
+        r++;
+        continue;
+      }
+      const int64_t s = r;
+      int64_t acc = 0;
+      while (r < R1) {
+        len = rp[r + 1] - rp[r];
+        if (len > budget || r - s >= kBlock || acc + len > budget) break;
+        acc += len;
+        r++;
+      }
+      chunks.push_back((int32_t)s);
+    }
+  }
+  tile_chunk[nt] = (int32_t)chunks.size();
+  chunks.push_back((int32_t)n);
+  A->d_chunks = (int32_t *)dev_alloc(chunks.size() * 4);
+  A->d_tile_chunk = (int32_t *)dev_alloc(tile_chunk.size() * 4);
+  MGX_HIP(hipMemcpy(A->d_chunks, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
+  MGX_HIP(hipMemcpy(A->d_tile_chunk, tile_chunk.data(), tile_chunk.size() * 4,
+                    hipMemcpyHostToDevice));
+  A->fused_nnz = budget;
+}
+
+namespace {
+
+struct SolveOut {
+  int status = MGX_SOLVER_MAXITER;
+  int64_t iters = 0;
+  double resid = 0;
+};
+
+// the preconditioner vector: Jacobi -> 1/diag(A) on the device, else null
+template <class T>
+T *make_precond(Csr *A, const void *val, int precond) {
+  if (precond != MGX_PRECOND_JACOBI) return nullptr;
+  Ctx *c = A->ctx;
+  T *M = (T *)ws_get(c, (size_t)A->rows * sizeof(T));
+  launch_diag_v(A, val, DT<T>::v, M);
+  launch_reciprocal(c, DT<T>::v, A->rows, M, M);
+  return M;
+}
+
+template <class T>
+void apply_precond(Ctx *c, const T *M, int64_t n, const T *r, T *z) {
+  if (M)
+    launch_vv(c, DT<T>::v, 2, n, M, r, z);
+  else
+    launch_copy(c, DT<T>::v, n, r, z);
+}
+
+template <class T> T dot_t(Ctx *c, int64_t n, const T *x, const T *y) {
+  return (T)dot_host(c, DT<T>::v, n, x, y);
+}
+
+// equation::CG, reference call sequence (cg.cpp:27-85)
+template <class T>
+SolveOut cg_reference(Csr *A, const void *val, T *x, const T *b, int precond,
+                      double tol, int64_t miniter, int64_t maxiter,
+                      double *rhist, int64_t rhist_len) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const mgx_dtype dt = DT<T>::v;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  T *r = (T *)ws_get(c, bytes), *p = (T *)ws_get(c, bytes),
+    *q = (T *)ws_get(c, bytes), *z = (T *)ws_get(c, bytes);
+  for (T *v : {r, p, q, z}) launch_fill(c, dt, v, n, 0.0);
+  T *M = make_precond<T>(A, val, precond);
+  SolveOut o;
+  launch_spmv(A, val, dt, x, q, 1, 0, false);  // cg.cpp:39
+  launch_vv(c, dt, 1, n, b, q, r);             // cg.cpp:40
+  apply_precond(c, M, n, r, z);                // cg.cpp:43
+  launch_copy(c, dt, n, z, p);                 // cg.cpp:44
+  for (int64_t iter = 0; iter < maxiter; iter++) {
+    launch_spmv(A, val, dt, p, q, 1, 0, false);  // cg.cpp:47
+    T tmp = dot_t(c, n, z, r);                   // cg.cpp:49
+    T pq = dot_t(c, n, p, q);
+    T alpha = tmp / pq;                          // cg.cpp:50
+    launch_axpy(c, dt, n, alpha, p, x);          // cg.cpp:52
+    launch_axpy(c, dt, n, -alpha, q, r);         // cg.cpp:54
+    apply_precond(c, M, n, r, z);                // cg.cpp:56
+    T zr = dot_t(c, n, z, r);
+    T beta = zr / tmp;                           // cg.cpp:57
+    launch_xpay(c, dt, n, beta, z, p);           // cg.cpp:59
+    T resid = host_sqrt(dot_t(c, n, r, r));      // cg.cpp:61
+    if (rhist && iter < rhist_len) rhist[iter] = (double)resid;
+    if ((double)resid < tol && miniter <= iter + 1) {  // cg.cpp:67
+      o.status = MGX_OK;
+      o.iters = iter + 1;
+      o.resid = (double)resid;
+      return o;
+    }
+    if (std::isnan(resid)) {  // cg.cpp:74
+      o.status = MGX_SOLVER_RESIDUAL_NAN;
+      o.iters = iter + 1;
+      o.resid = (double)resid;
+      return o;
+    }
+  }
+  o.status = MGX_SOLVER_MAXITER;  // cg.cpp:82
+  o.iters = maxiter;
+  o.resid = (double)host_sqrt(dot_t(c, n, r, r));
+  return o;
+}
+
+// fused Jacobi-PCG; identical arithmetic to cg_reference
+template <class T>
+SolveOut cg_fused(Csr *A, const void *val, T *x, const T *b, int precond,
+                  double tol, int64_t miniter, int64_t maxiter, int check_every,
+                  double *rhist_host, int64_t rhist_len) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const mgx_dtype dt = DT<T>::v;
+  if (n == 0 || maxiter <= 0)
+    return cg_reference<T>(A, val, x, b, precond, tol, miniter, maxiter,
+                           rhist_host, rhist_len);
+  fused_plan_build(A);
+  const int64_t nt = ntiles_of(n);
+  const size_t bytes = (size_t)n * sizeof(T);
+  T *r = (T *)ws_get(c, bytes), *z = (T *)ws_get(c, bytes),
+    *q = (T *)ws_get(c, bytes), *pa = (T *)ws_get(c, bytes),
+    *pb = (T *)ws_get(c, bytes);
+  T *P = (T *)ws_get(c, (size_t)2 * nt * sizeof(T));
+  SolveState *st = (SolveState *)ws_get(c, sizeof(SolveState));
+  double *rh = nullptr;
+  const int64_t rlen = rhist_host ? std::min(rhist_len, maxiter) : 0;
+  if (rlen > 0) rh = (double *)ws_get(c, (size_t)rlen * sizeof(double));
+  T *M = make_precond<T>(A, val, precond);
+  launch_fill(c, dt, pa, n, 0.0);
+  // r = b - A x; z = M r; <z, r> (cg.cpp:39-44)
+  launch_spmv(A, val, dt, x, q, 1, 0, false);
+  launch_vv(c, dt, 1, n, b, q, r);
+  apply_precond(c, M, n, r, z);
+  SolveState hs;
+  std::memset(&hs, 0, sizeof hs);
+  hs.tol = tol;
+  hs.miniter = miniter;
+  hs.maxiter = maxiter;
+  hs.status = kStatRunning;
+  hs.first = 1;
+  MGX_HIP(hipMemcpyAsync(st, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream));
+  launch_dot(c, dt, n, z, r, &st->zr, &st->ticket[2]);
+
+  CgArgs<T> ar;
+  ar.n = n;
+  ar.ntiles = nt;
+  ar.rp = A->rp;
+  ar.ci = A->ci;
+  ar.val = static_cast<const T *>(val);
+  ar.chunks = A->d_chunks;
+  ar.tile_chunk = A->d_tile_chunk;
+  ar.M = M;
+  ar.x = x;
+  ar.r = r;
+  ar.z = z;
+  ar.q = q;
+  ar.P = P;
+  ar.st = st;
+  ar.rhist = rh;
+  ar.rhist_len = rlen;
+  ar.remap = c->xcd_remap && nt >= 64;
+  const unsigned grid = ar.remap ? (unsigned)(((nt + 7) / 8) * 8) : (unsigned)nt;
+  const size_t lds = (size_t)A->fused_nnz * sizeof(T);
+  const int every = check_every > 0 ? check_every : 32;
+  int32_t *h_status = reinterpret_cast<int32_t *>(c->h_pinned);
+  int64_t launched = 0;
+  for (;;) {
+    for (int i = 0; i < every && launched < maxiter; i++, launched++) {
+      ar.p_old = (launched & 1) ? pb : pa;
+      ar.p_new = (launched & 1) ? pa : pb;
+      k_cg_spmv<T><<<grid, kBlock, lds, c->stream>>>(ar);
+      k_cg_update<T><<<grid, kBlock, 0, c->stream>>>(ar);
+    }
+    MGX_HIP(hipGetLastError());
+    MGX_HIP(hipMemcpyAsync(h_status, &st->status, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+    MGX_HIP(hipStreamSynchronize(c->stream));
+    if (*h_status != kStatRunning || launched >= maxiter) break;
+  }
+  MGX_HIP(hipMemcpy(&hs, st, sizeof hs, hipMemcpyDeviceToHost));
+  if (rlen > 0) {
+    const int64_t got = std::min<int64_t>(hs.iter, rlen);
+    if (got > 0) MGX_HIP(hipMemcpy(rhist_host, rh, (size_t)got * 8, hipMemcpyDeviceToHost));
+  }
+  SolveOut o;
+  o.status = hs.status == kStatRunning ? MGX_SOLVER_MAXITER : hs.status;
+  o.iters = hs.iter;
+  o.resid = hs.resid;
+  return o;
+}
+
+// equation::BiCGSTAB, reference call sequence (bicgstab.cpp:27-134)
+template <class T>
+SolveOut bicgstab_reference(Csr *A, T *x, const T *b, int precond, double tol,
+                            int64_t miniter, int64_t maxiter, double *rhist,
+                            int64_t rhist_len) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const mgx_dtype dt = DT<T>::v;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  T *r = (T *)ws_get(c, bytes), *r0 = (T *)ws_get(c, bytes),
+    *p = (T *)ws_get(c, bytes), *phat = (T *)ws_get(c, bytes),
+    *s = (T *)ws_get(c, bytes), *shat = (T *)ws_get(c, bytes),
+    *v = (T *)ws_get(c, bytes), *t = (T *)ws_get(c, bytes);
+  for (T *w : {r, r0, p, phat, s, shat, v, t}) launch_fill(c, dt, w, n, 0.0);
+  T rho_old = 1, rho = 1, alpha = 1, beta, omega = 1;
+  T *M = make_precond<T>(A, A->val, precond);
+  SolveOut o;
+  auto nrm = [&](const T *w) { return host_sqrt(dot_t(c, n, w, w)); };
+  launch_spmv(A, A->val, dt, x, r, 1, 0, false);  // :47
+  launch_vv(c, dt, 1, n, b, r, r);                // :48
+  launch_copy(c, dt, n, r, r0);                   // :51
+  for (int64_t iter = 0; iter < maxiter; iter++) {
+    rho = dot_t(c, n, r, r0);  // :56
+    if (rho == T(0)) {         // :58
+      o.status = MGX_SOLVER_BREAKDOWN;
+      o.iters = iter + 1;
+      o.resid = (double)nrm(r);
+      return o;
+    }
+    if (iter == 0) {
+      launch_copy(c, dt, n, r, p);  // :66
+    } else {
+      beta = (rho / rho_old) * (alpha / omega);  // :69
+      launch_axpy(c, dt, n, -omega, v, p);       // :72
+      launch_xpay(c, dt, n, beta, r, p);         // :73
+    }
+    apply_precond(c, M, n, p, phat);                 // :77
+    launch_spmv(A, A->val, dt, phat, v, 1, 0, false);  // :79
+    alpha = rho / dot_t(c, n, v, r0);                // :80
+    launch_axpyz(c, dt, n, -alpha, v, r, s);         // :83
+    apply_precond(c, M, n, s, shat);                 // :86
+    launch_spmv(A, A->val, dt, shat, t, 1, 0, false);  // :88
+    T ts = dot_t(c, n, t, s);
+    T tt = dot_t(c, n, t, t);
+    omega = ts / tt;            // :91
+    if (omega == T(0)) {        // :93
+      o.status = MGX_SOLVER_BREAKDOWN;
+      o.iters = iter + 1;
+      o.resid = (double)nrm(r);
+      return o;
+    }
+    launch_axpy(c, dt, n, alpha, phat, x);     // :101
+    launch_axpy(c, dt, n, omega, shat, x);     // :102
+    launch_axpyz(c, dt, n, -omega, t, s, r);   // :105
+    T resid = nrm(r);                          // :108
+    if (rhist && iter < rhist_len) rhist[iter] = (double)resid;
+    if ((double)resid < tol && miniter <= iter + 1) {  // :114
+      o.status = MGX_OK;
+      o.iters = iter + 1;
+      o.resid = (double)resid;
+      return o;
+    }
+    if (std::isnan(resid)) {  // :121
+      o.status = MGX_SOLVER_RESIDUAL_NAN;
+      o.iters = iter + 1;
+      o.resid = (double)resid;
+      return o;
+    }
+    rho_old = rho;  // :128
+  }
+  o.status = MGX_SOLVER_MAXITER;
+  o.iters = maxiter;
+  o.resid = (double)nrm(r);
+  return o;
+}
+
+// equation::Jacobi solver, reference call sequence (jacobi.cpp:96-149)
+template <class T>
+SolveOut jacobi_reference(Csr *A, T *x, const T *b, int precond, double tol,
+                          int64_t miniter, int64_t maxiter, double *rhist,
+                          int64_t rhist_len) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const mgx_dtype dt = DT<T>::v;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  T *r = (T *)ws_get(c, bytes), *t = (T *)ws_get(c, bytes),
+    *s = (T *)ws_get(c, bytes), *d = (T *)ws_get(c, bytes);
+  for (T *w : {r, t, s, d}) launch_fill(c, dt, w, n, 0.0);
+  auto nrm = [&](const T *w) { return host_sqrt(dot_t(c, n, w, w)); };
+  T bnrm2 = nrm(b);                       // :104
+  bnrm2 = (T)(1.0 / (double)bnrm2);       // :105
+  T nrm2 = 0;
+  launch_diag(A, d);                      // :108
+  launch_reciprocal(c, dt, n, d, d);      // :109
+  T *M = make_precond<T>(A, A->val, precond);  // :111
+  SolveOut o;
+  for (int64_t iter = 0; iter < maxiter; iter++) {
+    apply_precond(c, M, n, x, s);                 // :116
+    launch_copy(c, dt, n, x, s);                  // :117
+    launch_spmv(A, A->val, dt, s, t, 1, 0, false);  // :118
+    launch_axpyz(c, dt, n, -1.0, t, b, r);        // :119
+    nrm2 = nrm(r);                                // :120
+    launch_vv(c, dt, 2, n, r, d, r);              // :121
+    launch_vv(c, dt, 0, n, x, r, x);              // :122
+    nrm2 = nrm2 * bnrm2;                          // :124
+    if (rhist && iter < rhist_len) rhist[iter] = (double)nrm2;
+    if ((double)nrm2 < tol && miniter <= iter + 1) {  // :131
+      o.status = MGX_OK;
+      o.iters = iter + 1;
+      o.resid = (double)nrm(r);
+      return o;
+    }
+    if (std::isnan(nrm2)) {  // :138
+      o.status = MGX_SOLVER_RESIDUAL_NAN;
+      o.iters = iter + 1;
+      o.resid = (double)nrm(r);
+      return o;
+    }
+  }
+  o.status = MGX_SOLVER_MAXITER;
+  o.iters = maxiter;
+  o.resid = (double)nrm(r);
+  return o;
+}
+
+int64_t maxiter_of(const mgx_solve_params *p) {
+  return p->maxiter <= 0 ? INT64_MAX / 4 : p->maxiter;
+}
+
+void check_solve_args(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                      mgx_solve_result *res, const char *w) {
+  MGX_REQUIRE(A && x && b && p && res, "%s: NULL argument", w);
+  if (A->rows != A->cols) {
+    set_error("%s: error, A.row != A.col", w);
+    throw Error{MGX_ERR_INVALID};
+  }
+  MGX_REQUIRE(x->ctx == A->ctx && b->ctx == A->ctx, "%s: context mismatch", w);
+  MGX_REQUIRE(x->dtype == A->dtype && b->dtype == A->dtype, "%s: dtype mismatch", w);
+  MGX_REQUIRE(x->n == A->rows && b->n == A->rows, "%s: size mismatch", w);
+  MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
+              "%s: unknown preconditioner %d", w, p->precond);
+  MGX_REQUIRE(p->mode == MGX_MODE_REFERENCE || p->mode == MGX_MODE_FUSED,
+              "%s: unknown mode %d", w, p->mode);
+}
+
+double threshold(Ctx *c, const mgx_solve_params *p, mgx_vec *b) {
+  if (!p->tol_is_relative) return p->tol;
+  double bb = dot_host(c, b->dtype, b->n, b->d, b->d);
+  double bn = b->dtype == MGX_F64 ? std::sqrt(bb) : (double)std::sqrt((float)bb);
+  return p->tol * bn;
+}
+
+template <class F>
+int run_solver(mgx_csr *A, mgx_solve_result *res, F body) {
+  Ctx *c = A->ctx;
+  const double t0 = seconds_now();
+  SolveOut o;
+  try {
+    o = body();
+  } catch (...) {
+    ws_release_all(c);
+    throw;
+  }
+  ws_release_all(c);
+  MGX_HIP(hipStreamSynchronize(c->stream));
+  res->status = o.status;
+  res->final_iter = o.iters;
+  res->final_resid = o.resid;
+  res->seconds = seconds_now() - t0;
+  res->inner_iter = 0;
+  res->outer_iter = 0;
+  return o.status;
+}
+
+// ---- mixed-precision iterative refinement ---------------------------------
+__global__ void k_to_f32_scaled(int64_t n, const double *r, double scale,
+                                float *r32) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    r32[i] = (float)(r[i] * scale);
+}
+
+__global__ void k_add_scaled_f32(int64_t n, double s, const float *d,
+                                 double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    x[i] = x[i] + s * (double)d[i];
+}
+
+int stream_grid(Ctx *c, int64_t n) {
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  g = std::min<int64_t>(std::max<int64_t>(g, 1), (int64_t)c->num_cus * 16);
+  return (int)g;
+}
+
+// Outer loop in FP64: r = b - A x, ||r|| checked against thr; inner FP32
+// Jacobi-PCG solves A32 d = r / ||r|| to a relative inner_tol; x += ||r|| d.
+SolveOut ir_cg(Csr *A, double *x, const double *b, int precond, double thr,
+               int64_t max_inner_total, double inner_tol, int64_t max_outer,
+               int check_every, int64_t *inner_total, int64_t *outer_steps) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  if (!A->val32) {
+    A->val32 = (float *)dev_alloc((size_t)std::max<int64_t>(A->nnz, 1) * 4);
+    launch_convert(c, MGX_F64, MGX_F32, A->nnz, A->val, A->val32);
+  }
+  double *r = (double *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 8);
+  float *r32 = (float *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 4);
+  float *d32 = (float *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 4);
+  const int g = stream_grid(c, n);
+  auto residual = [&]() {
+    launch_spmv(A, A->val, MGX_F64, x, r, 1, 0, false);
+    launch_vv(c, MGX_F64, 1, n, b, r, r);
+    return std::sqrt(dot_host(c, MGX_F64, n, r, r));
+  };
+  SolveOut o;
+  *inner_total = 0;
+  *outer_steps = 0;
+  double rn = residual();
+  for (;;) {
+    o.resid = rn;
+    if (rn < thr) {
+      o.status = MGX_OK;
+      break;
+    }
+    if (std::isnan(rn)) {
+      o.status = MGX_SOLVER_RESIDUAL_NAN;
+      break;
+    }
+    if (*outer_steps >= max_outer || *inner_total >= max_inner_total) {
+      o.status = MGX_SOLVER_MAXITER;
+      break;
+    }
+    const size_t mark = c->ws_used.size();
+    if (n > 0) {
+      k_to_f32_scaled<<<g, kBlock, 0, c->stream>>>(n, r, 1.0 / rn, r32);
+      MGX_HIP(hipGetLastError());
+    }
+    launch_fill(c, MGX_F32, d32, n, 0.0);
+    // the scaled residual has unit norm: inner_tol is its absolute target
+    SolveOut in = cg_fused<float>(A, A->val32, d32, r32, precond, inner_tol, 0,
+                                  max_inner_total - *inner_total, check_every,
+                                  nullptr, 0);
+    ws_release_to(c, mark);
+    *inner_total += in.iters;
+    if (n > 0) {
+      k_add_scaled_f32<<<g, kBlock, 0, c->stream>>>(n, rn, d32, x);
+      MGX_HIP(hipGetLastError());
+    }
+    (*outer_steps)++;
+    rn = residual();
+  }
+  o.iters = *inner_total;
+  return o;
+}
+
+}  // namespace
+
+}  // namespace mgx
+
+using namespace mgx;
+
+extern "C" {
+
+void mgx_solve_params_default(mgx_solve_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->tol = 1.0e-8;  // monolish_solver.hpp:36
+  p->maxiter = 0;   // unlimited (SIZE_MAX in the reference)
+  p->miniter = 0;
+  p->precond = MGX_PRECOND_DEFAULT;
+  p->mode = MGX_MODE_FUSED;
+}
+
+int mgx_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                 mgx_solve_result *res) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, "mgx_cg_solve");
+    activate(A->ctx);
+    const double thr = threshold(A->ctx, p, b);
+    const int64_t mi = maxiter_of(p);
+    return run_solver(A, res, [&] {
+      if (A->dtype == MGX_F64) {
+        if (p->mode == MGX_MODE_FUSED)
+          return cg_fused<double>(A, A->val, (double *)x->d, (const double *)b->d, p->precond,
+                                  thr, p->miniter, mi, p->check_every, p->rhistory,
+                                  p->rhistory_len);
+        return cg_reference<double>(A, A->val, (double *)x->d, (const double *)b->d,
+                                    p->precond, thr, p->miniter, mi, p->rhistory,
+                                    p->rhistory_len);
+      }
+      if (p->mode == MGX_MODE_FUSED)
+        return cg_fused<float>(A, A->val, (float *)x->d, (const float *)b->d, p->precond, thr,
+                               p->miniter, mi, p->check_every, p->rhistory, p->rhistory_len);
+      return cg_reference<float>(A, A->val, (float *)x->d, (const float *)b->d, p->precond,
+                                 thr, p->miniter, mi, p->rhistory, p->rhistory_len);
+    });
+  });
+}
+
+int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                       mgx_solve_result *res) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, "mgx_bicgstab_solve");
+    activate(A->ctx);
+    const double thr = threshold(A->ctx, p, b);
+    const int64_t mi = maxiter_of(p);
+    return run_solver(A, res, [&] {
+      if (A->dtype == MGX_F64)
+        return bicgstab_reference<double>(A, (double *)x->d, (const double *)b->d, p->precond,
+                                          thr, p->miniter, mi, p->rhistory, p->rhistory_len);
+      return bicgstab_reference<float>(A, (float *)x->d, (const float *)b->d, p->precond, thr,
+                                       p->miniter, mi, p->rhistory, p->rhistory_len);
+    });
+  });
+}
+
+int mgx_jacobi_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                     mgx_solve_result *res) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, "mgx_jacobi_solve");
+    activate(A->ctx);
+    // the Jacobi iteration's own test is relative (jacobi.cpp:124,131)
+    const double thr = p->tol;
+    const int64_t mi = maxiter_of(p);
+    return run_solver(A, res, [&] {
+      if (A->dtype == MGX_F64)
+        return jacobi_reference<double>(A, (double *)x->d, (const double *)b->d, p->precond,
+                                        thr, p->miniter, mi, p->rhistory, p->rhistory_len);
+      return jacobi_reference<float>(A, (float *)x->d, (const float *)b->d, p->precond, thr,
+                                     p->miniter, mi, p->rhistory, p->rhistory_len);
+    });
+  });
+}
+
+int mgx_residual_l2(mgx_csr *A, mgx_vec *x, mgx_vec *b, double *out) {
+  return guard([&] {
+    MGX_REQUIRE(A && x && b && out, "mgx_residual_l2: NULL argument");
+    MGX_REQUIRE(x->n == A->cols && b->n == A->rows && x->dtype == A->dtype &&
+                    b->dtype == A->dtype,
+                "mgx_residual_l2: size/dtype mismatch");
+    Ctx *c = A->ctx;
+    activate(c);
+    void *tmp = ws_get(c, (size_t)std::max<int64_t>(A->rows, 1) * A->elem());
+    launch_spmv(A, A->val, A->dtype, x->d, tmp, 1, 0, false);
+    launch_vv(c, A->dtype, 1, A->rows, b->d, tmp, tmp);
+    double d = dot_host(c, A->dtype, A->rows, tmp, tmp);
+    ws_release_all(c);
+    *out = A->dtype == MGX_F64 ? std::sqrt(d) : (double)std::sqrt((float)d);
+    return MGX_OK;
+  });
+}
+
+int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                    double inner_tol, int64_t max_outer, mgx_solve_result *res) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, "mgx_ir_cg_solve");
+    MGX_REQUIRE(A->dtype == MGX_F64, "mgx_ir_cg_solve: needs an FP64 matrix");
+    MGX_REQUIRE(inner_tol > 0 && inner_tol < 1, "mgx_ir_cg_solve: inner_tol in (0,1)");
+    activate(A->ctx);
+    const double thr = threshold(A->ctx, p, b);
+    int64_t inner = 0, outer = 0;
+    int st = run_solver(A, res, [&] {
+      return ir_cg(A, (double *)x->d, (const double *)b->d, p->precond, thr, maxiter_of(p),
+                   inner_tol, max_outer > 0 ? max_outer : 100, p->check_every, &inner,
+                   &outer);
+    });
+    res->inner_iter = inner;
+    res->outer_iter = outer;
+    return st;
+  });
+}
+
+}  // extern "C"
+

@@ -1,0 +1,187 @@
+// mgx_internal.hpp -- shared definitions of the gfx950 CRS / Krylov library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/monolish_amd.h"
+
+namespace mgx {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const char *fmt, ...);
+
+struct Error {
+  int code;
+};
+
+#define MGX_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::mgx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                       __FILE__, __LINE__);                                    \
+      throw ::mgx::Error{e_ == hipErrorOutOfMemory ? MGX_ERR_NOMEM             \
+                                                   : MGX_ERR_HIP};             \
+    }                                                                          \
+  } while (0)
+
+#define MGX_REQUIRE(cond, ...)                                                 \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::mgx::set_error(__VA_ARGS__);                                           \
+      throw ::mgx::Error{MGX_ERR_INVALID};                                     \
+    }                                                                          \
+  } while (0)
+
+// every extern "C" entry point runs its body through guard(): exceptions
+// become status codes
+template <class F> int guard(F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return MGX_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    set_error("%s", e.what());
+    return MGX_ERR_INVALID;
+  }
+}
+
+// ------------------------------------------------------- device constants --
+constexpr int kBlock = 256;      // threads per block for every kernel
+constexpr int kTile = 512;       // canonical reduction tile (elements)
+constexpr int kStatRunning = 1;  // solver status while iterating
+
+// device-resident solver scalars (one per solve / per batched system)
+struct SolveState {
+  double alpha, beta, zr, pq, rr, resid;  // CG scalars (FP64 storage)
+  double rho, rho_old, omega, tt, ts, vr0;  // BiCGSTAB scalars
+  double tol, bnorm;
+  int64_t iter, miniter, maxiter;
+  int32_t status, first;
+  uint32_t ticket[4];  // last-block-done counters
+  double pad[2];
+};
+
+// ------------------------------------------------------------ handles --
+struct Ctx;
+
+struct Vec {
+  Ctx *ctx = nullptr;
+  int64_t n = 0;
+  mgx_dtype dtype = MGX_F64;
+  void *d = nullptr;
+  bool owner = true;
+  size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
+};
+
+struct SpmvPlan {
+  int32_t *d_blocks = nullptr;  // nblocks+1 descriptors (row start | kind)
+  int64_t nblocks = 0;
+  int64_t nlong = 0;
+  int32_t max_row = 0, min_row = 0;
+  int32_t nnz_max = 0;  // entry budget of a stream block (LDS products)
+  double mean_row = 0;
+};
+
+struct Csr {
+  Ctx *ctx = nullptr;
+  int64_t rows = 0, cols = 0, nnz = 0;
+  mgx_dtype dtype = MGX_F64;
+  int32_t *rp = nullptr;   // effective (non-decreasing) row_ptr on device
+  int32_t *ci = nullptr;
+  void *val = nullptr;
+  std::vector<int32_t> rp_raw;  // kept only when it differs from rp
+  SpmvPlan plan;
+  // fused-solver plan: row chunks that never cross a 512-row tile; the sign
+  // bit marks a long row; tile k owns chunks [tile_chunk[k], tile_chunk[k+1])
+  int32_t *d_chunks = nullptr;
+  int32_t *d_tile_chunk = nullptr;
+  int32_t fused_nnz = 0;     // entry budget of a chunk
+  float *val32 = nullptr;    // FP32 shadow values (mixed precision)
+  Csr *trans = nullptr;      // A^T, built on first matvec_T
+  size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  size_t hbm_bytes = 0;
+  char name[256] = {0};
+  // reduction scratch
+  void *d_partials = nullptr;
+  size_t partials_bytes = 0;
+  SolveState *d_state = nullptr;     // scratch solve state for BLAS calls
+  double *h_pinned = nullptr;        // pinned host scalars (64 doubles)
+  hipEvent_t ev = nullptr;
+  // SpMV tuning
+  int spmv_rows = 256, spmv_nnz = 2048, spmv_long = 256, xcd_remap = 1;
+  // workspace for solver vectors
+  std::vector<std::pair<void *, size_t>> ws_free;
+  std::vector<std::pair<void *, size_t>> ws_used;
+};
+
+// ------------------------------------------------------- host helpers --
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+void *ws_get(Ctx *c, size_t bytes);   // pooled device scratch
+void ws_release_all(Ctx *c);
+void ws_release_to(Ctx *c, size_t mark);  // release buffers taken after mark
+
+inline void activate(Ctx *c) {
+  static thread_local int cur = -1;
+  if (cur != c->device) {
+    MGX_HIP(hipSetDevice(c->device));
+    cur = c->device;
+  }
+}
+void ensure_partials(Ctx *c, size_t bytes);
+
+inline int64_t ntiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
+
+// kernel launchers (defined in the .hip files)
+void launch_fill(Ctx *c, mgx_dtype dt, void *y, int64_t n, double v);
+void launch_axpy(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x,
+                 void *y);
+void launch_xpay(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x,
+                 void *y);
+void launch_axpyz(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x,
+                  const void *y, void *z);
+void launch_scal(Ctx *c, mgx_dtype dt, int64_t n, double a, void *x);
+void launch_copy(Ctx *c, mgx_dtype dt, int64_t n, const void *x, void *y);
+// op: 0 add, 1 sub, 2 mul
+void launch_vv(Ctx *c, mgx_dtype dt, int op, int64_t n, const void *a,
+               const void *b, void *y);
+void launch_reciprocal(Ctx *c, mgx_dtype dt, int64_t n, const void *a,
+                       void *y);
+void launch_convert(Ctx *c, mgx_dtype from, mgx_dtype to, int64_t n,
+                    const void *x, void *y);
+// dot into a device double slot (result converted to double exactly)
+void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
+                double *d_out, uint32_t *d_ticket);
+double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
+                const void *y);
+
+void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp_host);
+void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
+                 void *y, double alpha, double beta, bool ab);
+void launch_diag(Csr *A, void *d);
+void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
+// fused-CG row chunks (built on first use of a fused solver)
+void fused_plan_build(Csr *A);
+
+double seconds_now();
+
+}  // namespace mgx
+
+struct mgx_ctx : mgx::Ctx {};
+struct mgx_vec : mgx::Vec {};
+struct mgx_csr : mgx::Csr {};

@@ -1,0 +1,202 @@
+// spmv.hip -- row-length-adaptive CRS SpMV for gfx950.
+//
+// The matrix is cut, once at upload, into a list of row blocks:
+//  * stream blocks: up to 256 consecutive rows whose entries (at most the
+//    plan's entry budget) are streamed by the whole block with coalesced loads;
+//    each product val*x[col] goes to LDS, then one lane per row adds its
+//    products in CRS order starting from 0.  That is the order and rounding of
+//    monolish's CPU row loop (crs_matvec.hpp:100-108), so y matches it bit
+//    for bit.
+//  * long blocks: one row longer than the long-row threshold; the 256 lanes
+//    stride over it and the partial sums meet in the canonical block tree
+//    (deterministic, within the forward-error bound of the CPU loop).
+// Descriptors are int32 row starts; the sign bit marks a long block.  The
+// kernel is HBM bound (roofline: nnz*(sv+4) + (rows+1)*4 + (rows+cols)*sv
+// bytes); x gathers are served from L2/MALL.
+#include <algorithm>
+#include <cstring>
+
+#include "mgx_device.hpp"
+
+namespace mgx {
+
+namespace {
+
+template <class T, bool AB>
+__global__ __launch_bounds__(kBlock) void k_spmv(
+    const int32_t *__restrict__ blk, int64_t nblocks,
+    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y,
+    T alpha, T beta, int remap) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *prod = reinterpret_cast<T *>(smem_raw);
+  __shared__ T sh[8];
+  const int64_t b = xcd_block(remap);
+  if (b >= nblocks) return;
+  const int32_t d0 = blk[b], d1 = blk[b + 1];
+  const int32_t r0 = d0 & 0x7fffffff, r1 = d1 & 0x7fffffff;
+  if (d0 < 0) {
+    const int32_t e0 = rp[r0], e1 = rp[r0 + 1];
+    T s = 0;
+    for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += kBlock) {
+      T p = val[e] * x[ci[e]];
+      s = s + p;
+    }
+    T r = block_tree(s, sh);
+    if (threadIdx.x == 0) {
+      if (AB) {
+        T t = alpha * r;
+        if (beta != T(0)) {
+          T u = beta * y[r0];
+          t = t + u;
+        }
+        y[r0] = t;
+      } else {
+        y[r0] = r;
+      }
+    }
+    return;
+  }
+  const int32_t e0 = rp[r0], e1 = rp[r1];
+  for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += kBlock) {
+    T v = __builtin_nontemporal_load(&val[e]);
+    int32_t c = __builtin_nontemporal_load(&ci[e]);
+    prod[e - e0] = v * x[c];
+  }
+  __syncthreads();
+  const int32_t r = r0 + (int32_t)threadIdx.x;
+  if (r < r1) {
+    const int32_t a = rp[r] - e0, bnd = rp[r + 1] - e0;
+    T s = 0;
+    for (int32_t j = a; j < bnd; j++) s = s + prod[j];
+    if (AB) {
+      T t = alpha * s;
+      if (beta != T(0)) {
+        T u = beta * y[r];
+        t = t + u;
+      }
+      y[r] = t;
+    } else {
+      y[r] = s;
+    }
+  }
+}
+
+// CRS::diag: d[i] = A(i,i), last stored match wins, 0 if absent
+template <class T>
+__global__ void k_diag(int64_t n, const int32_t *__restrict__ rp,
+                       const int32_t *__restrict__ ci,
+                       const T *__restrict__ val, T *__restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    T v = 0;
+    for (int32_t j = rp[i]; j < rp[i + 1]; j++)
+      if (ci[j] == (int32_t)i) v = val[j];
+    d[i] = v;
+  }
+}
+
+}  // namespace
+
+void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp) {
+  const int64_t rows = A->rows;
+  const int rows_max = std::min(std::max(c->spmv_rows, 1), kBlock);
+  const int nnz_max = std::max(c->spmv_nnz, 64);
+  const int long_t = std::min(std::max(c->spmv_long, 1), nnz_max);
+  std::vector<int32_t> blk;
+  blk.reserve((size_t)(rows / std::max(rows_max, 1)) + 16);
+  int32_t mx = 0, mn = rows > 0 ? INT32_MAX : 0;
+  int64_t nlong = 0, r = 0;
+  while (r < rows) {
+    int32_t len = rp[r + 1] - rp[r];
+    mx = std::max(mx, len);
+    mn = std::min(mn, len);
+    if (len > long_t) {
+      blk.push_back((int32_t)r | (int32_t)0x80000000u);
+      nlong++;
+      r++;
+      continue;
+    }
+    const int64_t start = r;
+    int64_t acc = 0;
+    while (r < rows) {
+      len = rp[r + 1] - rp[r];
+      if (len > long_t || r - start >= rows_max || acc + len > nnz_max) break;
+      mx = std::max(mx, len);
+      mn = std::min(mn, len);
+      acc += len;
+      r++;
+    }
+    blk.push_back((int32_t)start);
+  }
+  const int64_t nb = (int64_t)blk.size();
+  blk.push_back((int32_t)rows);
+  A->plan.nblocks = nb;
+  A->plan.nlong = nlong;
+  A->plan.max_row = mx;
+  A->plan.min_row = mn;
+  A->plan.mean_row = rows > 0 ? (double)A->nnz / (double)rows : 0.0;
+  A->plan.nnz_max = nnz_max;
+  if (A->plan.d_blocks) dev_free(A->plan.d_blocks);
+  A->plan.d_blocks = (int32_t *)dev_alloc(blk.size() * sizeof(int32_t));
+  MGX_HIP(hipMemcpyAsync(A->plan.d_blocks, blk.data(),
+                         blk.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                         c->stream));
+  MGX_HIP(hipStreamSynchronize(c->stream));
+}
+
+void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
+                 void *y, double alpha, double beta, bool ab) {
+  Ctx *c = A->ctx;
+  const int64_t nb = A->plan.nblocks;
+  if (nb <= 0) return;
+  const int remap = c->xcd_remap && nb >= 64;
+  int64_t grid = remap ? ((nb + 7) / 8) * 8 : nb;
+  const int nnz_max = A->plan.nnz_max;
+  if (dt == MGX_F64) {
+    const size_t lds = (size_t)nnz_max * sizeof(double);
+    auto V = static_cast<const double *>(val);
+    auto X = static_cast<const double *>(x);
+    auto Y = static_cast<double *>(y);
+    if (ab)
+      k_spmv<double, true><<<(unsigned)grid, kBlock, lds, c->stream>>>(
+          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+    else
+      k_spmv<double, false><<<(unsigned)grid, kBlock, lds, c->stream>>>(
+          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, 1.0, 0.0, remap);
+  } else {
+    const size_t lds = (size_t)nnz_max * sizeof(float);
+    auto V = static_cast<const float *>(val);
+    auto X = static_cast<const float *>(x);
+    auto Y = static_cast<float *>(y);
+    if (ab)
+      k_spmv<float, true><<<(unsigned)grid, kBlock, lds, c->stream>>>(
+          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, (float)alpha,
+          (float)beta, remap);
+    else
+      k_spmv<float, false><<<(unsigned)grid, kBlock, lds, c->stream>>>(
+          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, 1.0f, 0.0f, remap);
+  }
+  MGX_HIP(hipGetLastError());
+}
+
+void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d) {
+  Ctx *c = A->ctx;
+  const int64_t n = std::min(A->rows, A->cols);
+  if (n <= 0) return;
+  int64_t g = (n + kBlock - 1) / kBlock;
+  g = std::min<int64_t>(g, (int64_t)c->num_cus * 16);
+  if (dt == MGX_F64)
+    k_diag<double><<<(unsigned)g, kBlock, 0, c->stream>>>(
+        n, A->rp, A->ci, static_cast<const double *>(val),
+        static_cast<double *>(d));
+  else
+    k_diag<float><<<(unsigned)g, kBlock, 0, c->stream>>>(
+        n, A->rp, A->ci, static_cast<const float *>(val),
+        static_cast<float *>(d));
+  MGX_HIP(hipGetLastError());
+}
+
+void launch_diag(Csr *A, void *d) { launch_diag_v(A, A->val, A->dtype, d); }
+
+}  // namespace mgx

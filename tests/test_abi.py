@@ -1,0 +1,89 @@
+"""CPU checks of the C-ABI boundary: the library builds, loads, and exports
+every function include/monolish_amd.h declares.  No GPU calls are made."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "monolish_amd.h")
+LIB = os.path.join(ROOT, "monolish_amd", "libmonolish_amd.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(mgx_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("mgx_init", "mgx_matvec", "mgx_dot", "mgx_nrm2", "mgx_axpy", "mgx_xpay",
+                 "mgx_scal", "mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve",
+                 "mgx_vec_create", "mgx_vec_download", "mgx_vec_destroy", "mgx_csr_create"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build with __graft_entry__.build()"
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_header_compiles_as_c_and_cpp(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text('#include "monolish_amd.h"\nint main(void){mgx_solve_params p; '
+                   'mgx_solve_params_default(&p); return p.mode;}\n')
+    for cc, flags in (("gcc", ["-std=c99"]), ("g++", ["-x", "c++", "-std=c++17"])):
+        r = subprocess.run([cc, *flags, "-I", os.path.join(ROOT, "include"), "-c", str(src),
+                            "-o", str(tmp_path / f"t_{cc}.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+def test_python_binding_loads_without_gpu():
+    import monolish_amd as m
+    assert m.device_count() >= 0
+    # solver parameter defaults follow monolish_solver.hpp (tol 1e-8)
+    p, _ = m._params(1e-8, 0, 0, 0, 1, 0, 0, 0)
+    assert p.tol == 1e-8 and p.maxiter == 0
+
+
+def test_generators_closed_forms():
+    import monolish_amd as m
+    import numpy as np
+    for (nx, ny, nz) in ((3, 4, 5), (1, 1, 1), (6, 2, 3)):
+        rp, ci, v = m.stencil3d(nx, ny, nz, 7)
+        n = nx * ny * nz
+        assert len(v) == 7 * n - 2 * (ny * nz + nx * nz + nx * ny)
+        rp27, ci27, v27 = m.stencil3d(nx, ny, nz, 27, 1e-2)
+        assert len(v27) == (3 * nx - 2) * (3 * ny - 2) * (3 * nz - 2)
+        for (a, b, c, dg) in ((rp, ci, v, 6.0), (rp27, ci27, v27, 26.01)):
+            for i in range(n):
+                cols = b[a[i]:a[i + 1]]
+                assert np.all(np.diff(cols) > 0)
+                vals = c[a[i]:a[i + 1]]
+                assert vals[cols == i].tolist() == [dg]
+                assert np.all(vals[cols != i] == -1.0)
+    # 192^3 7-point and 128^3 27-point sizes quoted for the workloads
+    assert 7 * 192 ** 3 - 6 * 192 ** 2 == 49324032
+    assert 382 ** 3 == 55742968
+
+
+def test_random_lognormal_properties():
+    import monolish_amd as m
+    import numpy as np
+    rp, ci, v = m.random_lognormal(20000, 50000, 32.0, 1.0, 2048, seed=5)
+    lens = np.diff(rp)
+    assert lens.min() >= 1 and lens.max() <= 2048
+    assert abs(lens.mean() - 32.0) < 1.5
+    for i in range(0, 20000, 997):
+        cols = ci[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(cols) > 0) and cols.min() >= 0 and cols.max() < 50000
+    assert np.all(v >= -1) and np.all(v < 1)
+    rp2, ci2, v2 = m.random_lognormal(20000, 50000, 32.0, 1.0, 2048, seed=5)
+    assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and np.array_equal(v, v2)

@@ -1,0 +1,291 @@
+"""GPU parity of the sparse and vector BLAS (HIP kernels through the C-ABI)
+against the oracle on the same seeded inputs.  Integer/index work and every
+CRS-order row sum must match bit for bit; long rows (tree order) are held to
+the forward-error bound of the CPU loop."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def m():
+    import monolish_amd
+    return monolish_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(m):
+    c = m.Context(0)
+    yield c
+    c.close()
+
+
+def bits_equal(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype != b.dtype or a.shape != b.shape:
+        return False
+    u = np.uint64 if a.dtype == np.float64 else np.uint32
+    return np.array_equal(a.view(u), b.view(u))
+
+
+def gpu_spmv(m, ctx, rp, ci, val, x, cols=None):
+    rows = len(rp) - 1
+    cols = cols if cols is not None else len(x)
+    A = m.CRS(ctx, rows, cols, rp, ci, val)
+    X = m.Vector(ctx, x, dtype=val.dtype)
+    Y = m.Vector(ctx, n=rows, dtype=val.dtype)
+    m.matvec(A, X, Y)
+    return Y.download()
+
+
+def test_device_is_gfx950(m, ctx):
+    info = ctx.info()
+    say("device", info)
+    assert "gfx950" in info["name"]
+    assert info["compute_units"] >= 1
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_reference_known_answers(m, ctx, oracle, dtype):
+    say("matvec KAT", dtype)
+    row = np.array([0, 0, 0, 1, 1, 2, 2, 2], np.int32)
+    col = np.array([0, 1, 2, 0, 2, 0, 1, 2], np.int32)
+    val = np.array([1, 2, 3, 4, 5, 6, 7, 8], np.float64)
+    rp, ci, v = oracle.crs_from_coo(3, 3, row, col, val, dtype=dtype)
+    y = gpu_spmv(m, ctx, rp, ci, v, np.full(3, 10, dtype))
+    assert y.tolist() == [60, 90, 210]
+    rows, cols, r, c, vv = oracle.input_mm(os.path.join(GOLDEN, "test.mtx"), dtype == np.float32)
+    rp, ci, v = oracle.crs_from_coo(rows, cols, r, c, vv, dtype=dtype)
+    y = gpu_spmv(m, ctx, rp, ci, v, np.full(3, 10, dtype))
+    assert y.tolist() == [10, 0, 10]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_poisson2d_bit_exact(m, ctx, oracle, dtype):
+    say("matvec poisson 256^2", dtype)
+    rows, cols, r, c, v = oracle.laplacian_2d5p(256, 256)
+    rp, ci, val = oracle.crs_from_coo(rows, cols, r, c, v, dtype=dtype)
+    x = oracle.random_vector(rows, -1.0, 1.0, 1001, dtype=dtype)
+    y = gpu_spmv(m, ctx, rp, ci, val, x)
+    assert bits_equal(y, oracle.spmv(rp, ci, val, x))
+
+
+@pytest.mark.parametrize("points,shift,side", [(7, 0.0, 37), (27, 1e-2, 23)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_stencils_bit_exact(m, ctx, oracle, points, shift, side, dtype):
+    say("matvec stencil", points, side, dtype)
+    rp, ci, val = m.stencil3d(side, side + 1, side - 1, points, shift)
+    val = val.astype(dtype)
+    x = oracle.random_vector(len(rp) - 1, -1.0, 1.0, 2001, dtype=dtype)
+    y = gpu_spmv(m, ctx, rp, ci, val, x)
+    assert bits_equal(y, oracle.spmv(rp, ci, val, x))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_lognormal_rows(m, ctx, oracle, dtype):
+    say("matvec lognormal", dtype)
+    n = 120000
+    rp, ci, val = m.random_lognormal(n, n, 32.0, 1.0, 2048, seed=77)
+    val = val.astype(dtype)
+    x = oracle.random_vector(n, -1.0, 1.0, 2002, dtype=dtype)
+    y = gpu_spmv(m, ctx, rp, ci, val, x)
+    ref = oracle.spmv(rp, ci, val, x)
+    lens = np.diff(rp)
+    short = lens <= 256  # CRS-order rows: bit-exact
+    assert bits_equal(y[short], ref[short])
+    # long rows: forward error bound of any summation order
+    u = 2.0 ** -53 if dtype == np.float64 else 2.0 ** -24
+    absrow = oracle.spmv(rp, ci, np.abs(val), np.abs(x)).astype(np.float64)
+    k = lens.astype(np.float64)
+    bound = 2 * k * u / (1 - 2 * k * u) * absrow
+    assert np.all(np.abs(y.astype(np.float64) - ref) <= bound + 1e-300)
+    if dtype == np.float64:
+        assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_matvec_tuning_invariance(m, oracle):
+    say("matvec tuning invariance")
+    n = 50000
+    rp, ci, val = m.random_lognormal(n, n, 20.0, 1.2, 600, seed=3)
+    x = oracle.random_vector(n, -1.0, 1.0, 9)
+    outs = []
+    for tun in ((256, 2048, 256, True), (64, 512, 64, False), (256, 8192, 8192, True), (1, 64, 64, True)):
+        c = m.Context(0)
+        c.set_spmv_tuning(*tun)
+        outs.append(gpu_spmv(m, c, rp, ci, val, x))
+        c.close()
+    ref = oracle.spmv(rp, ci, val, x)
+    short = np.diff(rp) <= 64
+    for y in outs:
+        assert bits_equal(y[short], ref[short])
+        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert bits_equal(outs[2], ref)  # every row sequential when long_row >= max length
+
+
+def test_matvec_edge_shapes(m, ctx, oracle):
+    say("matvec edge shapes")
+    # empty rows, trailing empty rows left at 0 by CRS(COO&), one long row
+    row = np.array([1, 1, 3] + [5] * 5000, np.int32)
+    col = np.concatenate([[0, 2, 1], np.arange(5000) % 7]).astype(np.int32)
+    val = np.linspace(-1, 1, len(row))
+    # distinct columns per row are not required by CRS; keep COO order
+    rp, ci, v = oracle.crs_from_coo(8, 7, row, col, val)
+    assert rp[-1] == 0  # trailing rows keep 0
+    x = oracle.random_vector(7, -1.0, 1.0, 4)
+    A = m.CRS(ctx, 8, 7, rp, ci, v)
+    X = m.Vector(ctx, x)
+    Y = m.Vector(ctx, n=8)
+    Y.fill(123.0)
+    m.matvec(A, X, Y)
+    y = Y.download()
+    ref = oracle.spmv(rp, ci, v, x)
+    assert bits_equal(y[[0, 1, 2, 3, 4, 6, 7]], ref[[0, 1, 2, 3, 4, 6, 7]])
+    assert abs(y[5] - ref[5]) <= 1e-12 * np.sum(np.abs(v[3:]))
+    rp2, ci2, v2 = A.download()  # downloads the arrays as given
+    assert np.array_equal(rp2, rp) and np.array_equal(ci2, ci) and bits_equal(v2, v)
+    # zero-row and zero-nnz matrices
+    Z = m.CRS(ctx, 0, 5, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    m.matvec(Z, m.Vector(ctx, np.ones(5)), m.Vector(ctx, n=0))
+    E = m.CRS(ctx, 4, 4, np.zeros(5, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    Ye = m.Vector(ctx, np.full(4, 7.0))
+    m.matvec(E, m.Vector(ctx, np.ones(4)), Ye)
+    assert Ye.download().tolist() == [0, 0, 0, 0]
+
+
+def test_csr_create_rejects_bad_indices(m, ctx):
+    say("csr validation")
+    with pytest.raises(m.MgxError):
+        m.CRS(ctx, 2, 2, np.array([0, 1, 2], np.int32), np.array([0, 2], np.int32), np.ones(2))
+    with pytest.raises(m.MgxError):
+        m.CRS(ctx, 2, 2, np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32), np.ones(2))
+    A = m.CRS(ctx, 2, 3, np.array([0, 1, 2], np.int32), np.array([0, 2], np.int32), np.ones(2))
+    with pytest.raises(m.MgxError):  # x has the wrong size
+        m.matvec(A, m.Vector(ctx, np.ones(2)), m.Vector(ctx, n=2))
+
+
+def test_matvec_alpha_beta(m, ctx, oracle):
+    say("matvec alpha beta")
+    rows, cols, r, c, v = oracle.laplacian_2d5p(40, 30)
+    rp, ci, val = oracle.crs_from_coo(rows, cols, r, c, v)
+    x = oracle.random_vector(rows, -1.0, 1.0, 5)
+    y0 = oracle.random_vector(rows, -1.0, 1.0, 6)
+    A = m.CRS(ctx, rows, cols, rp, ci, val)
+    X = m.Vector(ctx, x)
+    Y = m.Vector(ctx, y0)
+    m.matvec_ab(2.5, A, X, -0.75, Y)
+    ax = oracle.spmv(rp, ci, val, x)
+    ref = 2.5 * ax + (-0.75) * y0  # numpy: each product rounded, then the sum
+    assert bits_equal(Y.download(), ref)
+    Y.upload(np.full(rows, np.nan))
+    m.matvec_ab(1.0, A, X, 0.0, Y)  # beta = 0 does not read y
+    assert bits_equal(Y.download(), ax)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_transpose(m, ctx, oracle, dtype):
+    say("matvec_T", dtype)
+    rng = np.random.default_rng(1)
+    rows, cols = 3000, 1700
+    lens = rng.integers(0, 30, rows)
+    rp = np.zeros(rows + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens]).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1]).astype(dtype)
+    x = oracle.random_vector(rows, -1.0, 1.0, 12, dtype=dtype)
+    A = m.CRS(ctx, rows, cols, rp, ci, val)
+    X = m.Vector(ctx, x, dtype=dtype)
+    Y = m.Vector(ctx, n=cols, dtype=dtype)
+    m.matvec_t(A, X, Y)
+    assert bits_equal(Y.download(), oracle.spmv_t(rp, ci, val, x, cols))
+
+
+def test_matvec_c2a_full_size(m, ctx, oracle):
+    """3D 7-point 192^3 (n = 7,077,888, nnz = 49,324,032) at full size."""
+    say("matvec 192^3 full size")
+    rp, ci, val = m.stencil3d(192, 192, 192, 7)
+    assert len(val) == 49324032
+    x = oracle.random_vector(len(rp) - 1, -1.0, 1.0, 2001)
+    y = gpu_spmv(m, ctx, rp, ci, val, x)
+    assert bits_equal(y, oracle.spmv(rp, ci, val, x))
+    # the all-ones vector: row sums of the stencil (6 - #neighbours)
+    ones = gpu_spmv(m, ctx, rp, ci, val, np.ones(len(x)))
+    assert np.array_equal(ones, 7.0 - np.diff(rp))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [0, 1, 2, 511, 512, 513, 4099, 300001, 3000000])
+def test_dot_nrm2_bit_exact(m, ctx, oracle, dtype, n):
+    say("dot/nrm2", dtype, n)
+    x = oracle.random_vector(n, -1.0, 1.0, 100 + n % 97, dtype=dtype)
+    y = oracle.random_vector(n, -1.0, 1.0, 200 + n % 89, dtype=dtype)
+    X = m.Vector(ctx, x, dtype=dtype)
+    Y = m.Vector(ctx, y, dtype=dtype)
+    d = m.dot(X, Y)
+    assert d == float(oracle.dot(x, y))
+    assert m.nrm2(X) == float(oracle.nrm2(x))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_vector_ops_bit_exact(m, ctx, oracle, dtype):
+    say("vector ops", dtype)
+    n = 100003
+    x = oracle.random_vector(n, -1.0, 1.0, 1, dtype=dtype)
+    y = oracle.random_vector(n, -1.0, 1.0, 2, dtype=dtype)
+    a = dtype(0.37)
+
+    def run(fn, *vecs):
+        hs = [m.Vector(ctx, v, dtype=dtype) for v in vecs]
+        fn(*hs)
+        return [h.download() for h in hs]
+
+    assert bits_equal(run(lambda X, Y: m.axpy(float(a), X, Y), x, y)[1], oracle.axpy(a, x, y))
+    assert bits_equal(run(lambda X, Y: m.xpay(float(a), X, Y), x, y)[1], oracle.xpay(a, x, y))
+    assert bits_equal(run(lambda X: m.scal(float(a), X), x)[0], oracle.scal(a, x))
+    assert bits_equal(run(lambda X, Y: m.copy(X, Y), x, y)[1], x)
+    z = np.zeros(n, dtype)
+    assert bits_equal(run(lambda X, Y, Z: m.axpyz(float(a), X, Y, Z), x, y, z)[2], oracle.axpyz(a, x, y))
+    assert bits_equal(run(lambda X, Y, Z: m.vadd(X, Y, Z), x, y, z)[2], oracle.vadd(x, y))
+    assert bits_equal(run(lambda X, Y, Z: m.vsub(X, Y, Z), x, y, z)[2], oracle.vsub(x, y))
+    assert bits_equal(run(lambda X, Y, Z: m.vmul(X, Y, Z), x, y, z)[2], oracle.vmul(x, y))
+    assert bits_equal(run(lambda X, Y: m.vreciprocal(X, Y), x, z)[1], oracle.vreciprocal(x))
+
+
+def test_views_and_residency(m, ctx, oracle):
+    say("views")
+    n = 10000
+    x = oracle.random_vector(n, -1.0, 1.0, 31)
+    y = oracle.random_vector(n, -1.0, 1.0, 32)
+    X = m.Vector(ctx, x)
+    Y = m.Vector(ctx, y)
+    xv, yv = X.view(5, n // 2), Y.view(5, n // 2)  # view1D(vec, 5, size/2)
+    m.axpy(123.0, xv, yv)
+    ref = y.copy()
+    ref[5:5 + n // 2] = oracle.axpy(123.0, x[5:5 + n // 2], y[5:5 + n // 2])
+    assert bits_equal(Y.download(), ref)
+    assert m.dot(xv, yv) == oracle.dot(x[5:5 + n // 2], ref[5:5 + n // 2])
+    X.upload(y)  # send() on a resident vector updates it
+    assert bits_equal(X.download(), y)
+
+
+def test_crs_diag(m, ctx, oracle):
+    say("crs diag")
+    rp = np.array([0, 3, 4, 4], np.int32)
+    ci = np.array([0, 0, 2, 0], np.int32)
+    val = np.array([1.0, 9.0, 5.0, 7.0])
+    A = m.CRS(ctx, 3, 3, rp, ci, val)
+    d = m.Vector(ctx, n=3)
+    A.diag(d)
+    assert d.download().tolist() == oracle.crs_diag(3, 3, rp, ci, val).tolist() == [9.0, 0.0, 0.0]

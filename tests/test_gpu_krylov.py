@@ -1,0 +1,209 @@
+"""GPU parity of the Krylov solvers against the oracle.  The GPU loops use the
+reference's arithmetic and the oracle's tile-order dots, so FP64 and FP32 runs
+must reproduce the oracle's iterates bit for bit: same x, same iteration count,
+same residual history -- in the reference-shaped mode and in the fused mode."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ans_check
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def m():
+    import monolish_amd
+    return monolish_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(m):
+    c = m.Context(0)
+    yield c
+    c.close()
+
+
+def bits_equal(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype != b.dtype or a.shape != b.shape:
+        return False
+    u = np.uint64 if a.dtype == np.float64 else np.uint32
+    return np.array_equal(a.view(u), b.view(u))
+
+
+PREC = {"default": 0, "none": 1, "jacobi": 2}
+
+
+def gpu_solve(m, ctx, method, rp, ci, val, x0, b, precond, tol, maxiter=0, miniter=0,
+              mode=None, history=0):
+    n = len(rp) - 1
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    X = m.Vector(ctx, x0, dtype=val.dtype)
+    B = m.Vector(ctx, b, dtype=val.dtype)
+    kw = dict(tol=tol, maxiter=maxiter, miniter=miniter, precond=precond, history=history)
+    if mode is not None:
+        kw["mode"] = mode
+    res = getattr(m, method)(A, X, B, **kw)
+    return res, X.download()
+
+
+def _mtx(oracle, name, dtype):
+    rows, cols, r, c, v = oracle.input_mm(os.path.join(GOLDEN, name), dtype == np.float32)
+    return oracle.crs_from_coo(rows, cols, r, c, v, dtype=dtype)
+
+
+@pytest.mark.parametrize("method,mtx", [("cg", "test.mtx"), ("bicgstab", "test.mtx"),
+                                        ("bicgstab", "test2.mtx"), ("jacobi", "test.mtx")])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-8), (np.float32, 1e-4)])
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
+def test_reference_solver_harness(m, ctx, oracle, method, mtx, dtype, tol, precond):
+    # test/equation/iterative/equation_kernel.hpp: b = A*1, x0 = vector(n,0,1,
+    # mt19937(1234)()), maxiter 10000, answer within 10*tol of 1
+    say("harness", method, mtx, dtype.__name__, precond)
+    rp, ci, val = _mtx(oracle, mtx, dtype)
+    n = len(rp) - 1
+    b = oracle.spmv(rp, ci, val, np.ones(n, dtype))
+    x0 = oracle.random_vector(n, 0.0, 1.0, 822569775, dtype=dtype)
+    modes = [0, 1] if method == "cg" else [0]
+    ref = oracle.solve(method, rp, ci, val, x0, b, precond=PREC[precond], tol=tol,
+                       maxiter=10000, history=True)
+    for mode in modes:
+        res, x = gpu_solve(m, ctx, method, rp, ci, val, x0, b, PREC[precond], tol, 10000,
+                           mode=mode, history=10000)
+        assert res.status == 0
+        assert ans_check(x, np.ones(n), tol * 10)
+        assert res.final_iter == ref[2]
+        assert bits_equal(x, ref[1])
+        assert res.final_resid == ref[3]
+        assert bits_equal(res.rhistory, ref[4])
+
+
+def _poisson(oracle, side, dtype=np.float64):
+    rows, cols, r, c, v = oracle.laplacian_2d5p(side, side)
+    return oracle.crs_from_coo(rows, cols, r, c, v, dtype=dtype)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cg_poisson256_bit_exact(m, ctx, oracle, mode):
+    """Jacobi-PCG on the 2D 5-point Poisson 256x256, b = U[-1,1) seed 1001,
+    x0 = 0, stop at ||r|| < 1e-10 ||b||."""
+    say("cg poisson256 mode", mode)
+    rp, ci, val = _poisson(oracle, 256)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 1001)
+    tol = 1e-10 * oracle.nrm2(b)
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=tol, history=True)
+    assert ref[0] == 0
+    res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, tol, mode=mode, history=5000)
+    say("iters", res.final_iter, "resid", res.final_resid, "sec", res.seconds)
+    assert res.status == 0 and res.final_iter == ref[2]
+    assert bits_equal(x, ref[1])
+    assert bits_equal(res.rhistory, ref[4])
+    assert oracle.residual_l2(rp, ci, val, x, b) < 2 * tol
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_cg_fused_equals_reference_mode(m, ctx, oracle, dtype, precond):
+    say("cg fused==reference", dtype.__name__, precond)
+    rp, ci, val = m.stencil3d(19, 17, 23, 27, 1e-2)
+    val = val.astype(dtype)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3001, dtype=dtype)
+    tol = (1e-10 if dtype == np.float64 else 1e-5) * float(oracle.nrm2(b))
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n, dtype), b, precond=precond, tol=tol,
+                       history=True)
+    for mode in (0, 1):
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n, dtype), b, precond, tol,
+                           mode=mode, history=4000)
+        assert res.status == ref[0] and res.final_iter == ref[2], mode
+        assert bits_equal(x, ref[1]), mode
+        assert bits_equal(res.rhistory, ref[4]), mode
+
+
+def test_cg_error_paths(m, ctx, oracle):
+    say("cg error paths")
+    rp, ci, val = _poisson(oracle, 16)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 1)
+    for mode in (0, 1):
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, 1e-30, maxiter=7, mode=mode)
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=1e-30, maxiter=7)
+        assert res.status == m.SOLVER_MAXITER and res.final_iter == 7
+        assert res.final_resid == ref[3] and bits_equal(x, ref[1])
+        bn = b.copy()
+        bn[5] = np.nan
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), bn, 2, 1e-8, mode=mode)
+        assert res.status == m.SOLVER_RESIDUAL_NAN and res.final_iter == 1
+        xe = oracle.random_vector(n, -1.0, 1.0, 2)
+        be = oracle.spmv(rp, ci, val, xe)
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, xe, be, 2, 1e-8, mode=mode)
+        assert res.status == m.SOLVER_RESIDUAL_NAN and res.final_iter == 1 and np.all(np.isnan(x))
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, 1.0, miniter=9, mode=mode)
+        assert res.status == 0 and res.final_iter == 9
+    res, x = gpu_solve(m, ctx, "bicgstab", rp, ci, val, xe, be, 2, 1e-8)
+    assert res.status == m.SOLVER_BREAKDOWN and res.final_iter == 1
+
+
+def test_bicgstab_convection_diffusion_bit_exact(m, ctx, oracle):
+    say("bicgstab convdiff")
+    import scipy.sparse as sp
+    side = 60
+    n = side * side
+    pe = 20.0
+    h = 1.0 / (side + 1)
+    c = pe * h
+    T1 = sp.diags([-1.0 - c, 2.0 + c, -1.0], [-1, 0, 1], shape=(side, side))
+    I = sp.identity(side)
+    A = (sp.kron(I, T1) + sp.kron(T1, I)).tocsr()
+    A.sort_indices()
+    rp, ci, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    b = oracle.random_vector(n, -1.0, 1.0, 5001)
+    tol = 1e-10 * oracle.nrm2(b)
+    ref = oracle.solve("bicgstab", rp, ci, val, np.zeros(n), b, precond=2, tol=tol, history=True)
+    assert ref[0] == 0
+    res, x = gpu_solve(m, ctx, "bicgstab", rp, ci, val, np.zeros(n), b, 2, tol, history=5000)
+    assert res.status == 0 and res.final_iter == ref[2]
+    assert bits_equal(x, ref[1]) and bits_equal(res.rhistory, ref[4])
+
+
+def test_jacobi_solver_bit_exact(m, ctx, oracle):
+    say("jacobi solver")
+    rp, ci, val = m.stencil3d(12, 13, 14, 7, 2.0)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 8)
+    ref = oracle.solve("jacobi", rp, ci, val, np.zeros(n), b, precond=0, tol=1e-9, history=True)
+    res, x = gpu_solve(m, ctx, "jacobi", rp, ci, val, np.zeros(n), b, 0, 1e-9, history=10000)
+    assert res.status == ref[0] == 0 and res.final_iter == ref[2]
+    assert bits_equal(x, ref[1]) and res.final_resid == ref[3]
+
+
+def test_mixed_precision_refinement(m, ctx, oracle):
+    """FP32 inner / FP64 outer PCG on a 27-point shifted stencil: the FP64
+    true residual reaches 1e-10 ||b|| and the answer agrees with the FP64
+    oracle solve to the conditioning of the problem."""
+    say("ir-cg 27pt")
+    rp, ci, val = m.stencil3d(32, 32, 32, 27, 1e-2)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3001)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    X = m.Vector(ctx, np.zeros(n))
+    B = m.Vector(ctx, b)
+    res = m.ir_cg(A, X, B, tol=1e-10, tol_is_relative=True, inner_tol=1e-4)
+    say("ir", res)
+    x = X.download()
+    bn = oracle.nrm2(b)
+    assert res.status == 0 and res.outer_iter >= 2
+    assert oracle.residual_l2(rp, ci, val, x, b) < 1e-10 * bn
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=1e-10 * bn)
+    # both meet ||r|| < 1e-10 ||b||; kappa(27-pt, shift 1e-2) < 3e3
+    assert np.linalg.norm(x - ref[1]) <= 3e3 * 2e-10 * np.linalg.norm(ref[1])
