@@ -1,0 +1,318 @@
+"""Benchmark of the CRS SpMV + Krylov path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+
+The timed step is one SpMV (blas::matvec) over the 3D 7-point Laplacian on a
+192^3 grid in FP64 (n = 7,077,888, nnz = 49,324,032), operands resident in HBM.
+``value`` is the SpMV's achieved algorithmic bandwidth in GB/s, measured with
+HIP events on the library's stream around exactly K launches.  The line also
+carries the roofline object, the CPU baseline (the oracle's CPU restatement of
+monolish's OpenMP row loop, timed on this host), the FP32 / random-matrix SpMV
+sweep, and the Krylov side of the metric: Jacobi-PCG solves/s on the 2D Poisson
+256x256 problem to a 1e-10 relative residual, plain FP64 GPU CG (one kernel per
+BLAS call, scalars returned to the host) against the fused solver, with p50/p99
+latency, plus mixed-precision refinement against FP64 CG on the 27-point
+stencil.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (the library's own stream)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float(0)
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def spmv_bytes(nnz: int, rows: int, cols: int, sv: int) -> int:
+    # val + col_ind + row_ptr + x (read once) + y (written)
+    return nnz * (sv + 4) + (rows + 1) * 4 + cols * sv + rows * sv
+
+
+def time_spmv(m, ev, ctx, A, X, Y, steps, warmup):
+    for _ in range(warmup):
+        m.matvec(A, X, Y)
+    ctx.sync()
+    a, b = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(a, ctx.stream)
+    for _ in range(steps):
+        m.matvec(A, X, Y)
+    ev.record(b, ctx.stream)
+    ms = ev.elapsed_ms(a, b)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    return ms / steps, wall
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return model
+
+
+def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup):
+    rows = len(rp) - 1
+    A = m.CRS(ctx, rows, len(x), rp, ci, val)
+    X = m.Vector(ctx, x, dtype=val.dtype)
+    Y = m.Vector(ctx, n=rows, dtype=val.dtype)
+    ms, wall = time_spmv(m, ev, ctx, A, X, Y, steps, warmup)
+    nbytes = spmv_bytes(len(val), rows, len(x), val.dtype.itemsize)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"{name}: {ms * 1e3:.1f} us/SpMV, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.3f} of 8 TB/s)")
+    A.free()
+    X.free()
+    Y.free()
+    return {"us_per_spmv": round(ms * 1e3, 2), "GB_s": round(gbs, 1),
+            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes, "nnz": int(len(val)),
+            "rows": rows}, ms, wall
+
+
+def bench_krylov(m, ctx, reps_fused, reps_plain):
+    import oracle
+    rows, cols, r, c, v = oracle.laplacian_2d5p(256, 256)
+    rp, ci, val = oracle.crs_from_coo(rows, cols, r, c, v)
+    n = rows
+    b = oracle.random_vector(n, -1.0, 1.0, 1001)
+    tol = 1e-10 * float(oracle.nrm2(b))
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    B = m.Vector(ctx, b)
+    X = m.Vector(ctx, n=n)
+    out = {"workload": "Jacobi-PCG, 2D 5-point Poisson 256x256 (n=65536, nnz=326656), "
+                       "b=U[-1,1) seed 1001, x0=0, ||r||<1e-10||b||"}
+
+    def run(mode, reps):
+        lat = []
+        res = None
+        for i in range(reps + 1):
+            X.fill(0.0)
+            ctx.sync()
+            t0 = time.perf_counter()
+            res = m.cg(A, X, B, tol=tol, precond=m.PRECOND_JACOBI, mode=mode)
+            dt = time.perf_counter() - t0
+            if i > 0:
+                lat.append(dt)
+        lat.sort()
+        p99 = lat[min(len(lat) - 1, int(np.ceil(0.99 * len(lat))) - 1)]
+        return res, {"solves_per_s": round(1.0 / statistics.mean(lat), 2),
+                     "p50_ms": round(1e3 * statistics.median(lat), 3), "p99_ms": round(1e3 * p99, 3),
+                     "iterations": res.final_iter, "final_resid": res.final_resid, "solves": len(lat)}
+
+    res_p, plain = run(m.MODE_REFERENCE, reps_plain)
+    xp = X.download()
+    res_f, fused = run(m.MODE_FUSED, reps_fused)
+    xf = X.download()
+    out["plain_fp64_gpu_cg"] = plain
+    out["fused_fp64_gpu_cg"] = fused
+    out["fused_vs_plain"] = round(fused["solves_per_s"] / plain["solves_per_s"], 2)
+    out["fused_bit_identical_to_plain"] = bool(np.array_equal(xp.view(np.uint64), xf.view(np.uint64)))
+    out["true_rel_residual"] = float(oracle.residual_l2(rp, ci, val, xf, b)) / float(oracle.nrm2(b))
+    # CPU: the oracle's restatement of the reference CG loop, same inputs
+    t0 = time.perf_counter()
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_JACOBI, tol=tol)
+    t_cpu = time.perf_counter() - t0
+    out["cpu_cg"] = {"solves_per_s": round(1.0 / t_cpu, 2), "iterations": ref[2],
+                     "threads": oracle.num_threads(),
+                     "gpu_bit_identical": bool(np.array_equal(ref[1].view(np.uint64), xf.view(np.uint64)))}
+    log(f"krylov C1: plain {plain['solves_per_s']}/s fused {fused['solves_per_s']}/s "
+        f"(x{out['fused_vs_plain']}), {fused['iterations']} it, p99 {fused['p99_ms']} ms")
+    return out
+
+
+def bench_mixed(m, ctx, side, reps):
+    import oracle
+    rp, ci, val = m.stencil3d(side, side, side, 27, 1e-2)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3001)
+    bn = float(oracle.nrm2(b))
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    B = m.Vector(ctx, b)
+    X = m.Vector(ctx, n=n)
+    res = {}
+    for name, fn in (("fp64_fused_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI)),
+                     ("fp64_plain_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI,
+                                                    mode=m.MODE_REFERENCE)),
+                     ("mixed_ir_cg", lambda: m.ir_cg(A, X, B, tol=1e-10, tol_is_relative=True,
+                                                     inner_tol=1e-4))):
+        ts = []
+        r = None
+        for i in range(reps + 1):
+            X.fill(0.0)
+            ctx.sync()
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        x = X.download()
+        true_rel = float(oracle.residual_l2(rp, ci, val, x, b)) / bn
+        res[name] = {"ms": round(1e3 * statistics.median(ts[1:]), 3), "iterations": r.final_iter,
+                     "outer": r.outer_iter, "true_rel_residual": true_rel, "status": r.status}
+        res[name + "_x"] = x
+    ref = res["fp64_fused_cg_x"]
+    for name in ("fp64_plain_cg", "mixed_ir_cg"):
+        xx = res.pop(name + "_x")
+        res[name]["rel_err_vs_fp64"] = float(np.linalg.norm(xx - ref) / np.linalg.norm(ref))
+    res.pop("fp64_fused_cg_x")
+    res["mixed_vs_plain_fp64"] = round(res["fp64_plain_cg"]["ms"] / res["mixed_ir_cg"]["ms"], 2)
+    res["mixed_vs_fused_fp64"] = round(res["fp64_fused_cg"]["ms"] / res["mixed_ir_cg"]["ms"], 2)
+    res["workload"] = (f"27-point stencil {side}^3 (n={n}, nnz={len(val)}), diag 26+1e-2, "
+                       f"b=U[-1,1) seed 3001, ||b-Ax|| < 1e-10||b||")
+    log(f"mixed C3 {side}^3: {res}")
+    return res
+
+
+def cpu_baseline_spmv(rp, ci, val, x, budget_s):
+    import oracle
+    oracle.spmv(rp, ci, val, x)  # warm-up
+    ts = []
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end and len(ts) < 200:
+        t0 = time.perf_counter()
+        oracle.spmv(rp, ci, val, x)
+        ts.append(time.perf_counter() - t0)
+    med = statistics.median(ts)
+    gbs = spmv_bytes(len(val), len(rp) - 1, len(x), 8) / med / 1e9
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"oracle (CPU restatement of the reference OpenMP CRS row loop) on the full "
+                      f"192^3 7-point FP64 matrix, median of {len(ts)} SpMVs "
+                      f"({med * 1e3:.1f} ms each), CPU: {cpu_info()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--quick", action="store_true", help="skip the sweep, Krylov and CPU legs")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run 5 vadd launches of known size (counter calibration) and 5 "
+                         "SpMVs on the 192^3 matrix, for rocprofv3 --pmc runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
+                                   "in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import monolish_amd as m
+
+    ev = HipEvents()
+    ctx = m.Context(0)
+    info = ctx.info()
+    log("device", info)
+
+    if args.traffic_probe:
+        n = 1 << 25  # vadd: reads 2*n*8 bytes, writes n*8 bytes per launch
+        a = m.Vector(ctx, np.ones(n))
+        b = m.Vector(ctx, np.ones(n))
+        y = m.Vector(ctx, n=n)
+        for _ in range(5):
+            m.vadd(a, b, y)
+        ctx.sync()
+        rp, ci, val = m.stencil3d(192, 192, 192, 7)
+        A = m.CRS(ctx, len(rp) - 1, len(rp) - 1, rp, ci, val)
+        X = m.Vector(ctx, np.ones(len(rp) - 1))
+        Y = m.Vector(ctx, n=len(rp) - 1)
+        for _ in range(5):
+            m.matvec(A, X, Y)
+        ctx.sync()
+        print(json.dumps({"vadd_read_bytes": 2 * n * 8, "vadd_write_bytes": n * 8,
+                          "spmv_alg_bytes": spmv_bytes(len(val), len(rp) - 1, len(rp) - 1, 8)}))
+        ctx.close()
+        return
+
+    # ---- headline: SpMV, 3D 7-point 192^3, FP64 ---------------------------
+    rp, ci, val = m.stencil3d(192, 192, 192, 7)
+    import oracle
+    x = oracle.random_vector(len(rp) - 1, -1.0, 1.0, 2001)
+    head, ms, wall = bench_spmv_matrix(m, ev, ctx, "C2a fp64", rp, ci, val, x, args.steps, args.warmup)
+    sweep = {"c2a_7pt_192_fp64": head}
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "spmv_c2a_fp64_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_spmv<double,false>", "bytes_per_launch": head["bytes"]}
+    line = {
+        "metric": "CG solves/s @1e-10 + p99 latency vs plain FP64 GPU CG; SpMV HBM GB/s",
+        "value": head["GB_s"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (generated matrices and seeded U[-1,1) vectors)",
+        "config": {"workload": "SpMV (blas::matvec) on the 3D 7-point Laplacian 192^3, FP64, "
+                               "CRS int32 indices, n=7077888, nnz=49324032",
+                   "timed_region_s": round(wall, 3)},
+        "roofline": roofline,
+    }
+    if not args.quick:
+        line["cpu_baseline"] = cpu_baseline_spmv(rp, ci, val, x, args.cpu_seconds)
+        r32, _, _ = bench_spmv_matrix(m, ev, ctx, "C2a fp32", rp, ci, val.astype(np.float32),
+                                      x.astype(np.float32), args.steps, args.warmup)
+        sweep["c2a_7pt_192_fp32"] = r32
+        del rp, ci, val, x
+        n2 = 4_000_000
+        rp2, ci2, v2 = m.random_lognormal(n2, n2, 32.0, 1.0, 2048, seed=2002)
+        x2 = oracle.random_vector(n2, -1.0, 1.0, 2002)
+        s64, _, _ = bench_spmv_matrix(m, ev, ctx, "C2b fp64", rp2, ci2, v2, x2, max(args.steps // 4, 10),
+                                      args.warmup)
+        s32, _, _ = bench_spmv_matrix(m, ev, ctx, "C2b fp32", rp2, ci2, v2.astype(np.float32),
+                                      x2.astype(np.float32), max(args.steps // 4, 10), args.warmup)
+        sweep["c2b_lognormal_4M_fp64"] = s64
+        sweep["c2b_lognormal_4M_fp32"] = s32
+        del rp2, ci2, v2, x2
+        line["spmv_sweep"] = sweep
+        line["krylov"] = bench_krylov(m, ctx, reps_fused=30, reps_plain=5)
+        line["mixed_precision"] = bench_mixed(m, ctx, 128, reps=2)
+    line["device"] = info["name"]
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

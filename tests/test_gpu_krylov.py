@@ -148,8 +148,11 @@ def test_cg_error_paths(m, ctx, oracle):
         be = oracle.spmv(rp, ci, val, xe)
         res, x = gpu_solve(m, ctx, "cg", rp, ci, val, xe, be, 2, 1e-8, mode=mode)
         assert res.status == m.SOLVER_RESIDUAL_NAN and res.final_iter == 1 and np.all(np.isnan(x))
-        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, 1.0, miniter=9, mode=mode)
-        assert res.status == 0 and res.final_iter == 9
+        # a tolerance met at once: miniter alone decides the iteration count
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, 1e6, miniter=9, mode=mode)
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=1e6, miniter=9)
+        assert res.status == 0 and res.final_iter == 9 == ref[2]
+        assert bits_equal(x, ref[1])
     res, x = gpu_solve(m, ctx, "bicgstab", rp, ci, val, xe, be, 2, 1e-8)
     assert res.status == m.SOLVER_BREAKDOWN and res.final_iter == 1
 
