@@ -1,80 +1,67 @@
 // blas1.hip -- vector BLAS / VML kernels for gfx950.
 //
-// Streaming kernels are HBM bound: grid-stride loops, 256 threads, every
-// multiply and add rounded separately (built with -ffp-contract=off) so the
-// results equal monolish's CPU loops bit for bit.  dot/nrm2 use the canonical
-// tile order (mgx_device.hpp) so they are deterministic and reproducible on
-// the host.
+// Element-wise kernels are HBM bound.  Each lane moves 16 bytes per operand
+// (2 doubles or 4 floats) with non-temporal loads and stores when all operands
+// are 16-byte aligned (a scalar variant handles views at odd offsets), one
+// pass, no grid-stride loop.  Every multiply and add rounds separately (built
+// with -ffp-contract=off), so results equal monolish's CPU loops bit for bit.
+//
+// dot/nrm2 follow the canonical tile order (mgx_device.hpp) with a fixed grid
+// of 256 blocks: block b owns the tiles k = b, b+256, b+512, ..., which are
+// exactly the partials stage-2 lane b adds in order, so each block adds its
+// tile partials itself and publishes a single value; the last block runs the
+// final 256-lane tree.
 #include "mgx_device.hpp"
 
 namespace mgx {
 
 namespace {
 
-inline int grid_for(Ctx *c, int64_t n, int per_thread = 4) {
-  int64_t g = (n + (int64_t)kBlock * per_thread - 1) / ((int64_t)kBlock * per_thread);
-  int64_t cap = (int64_t)c->num_cus * 16;
-  if (g > cap) g = cap;
-  if (g < 1) g = 1;
-  return (int)g;
-}
+enum EwOp { EW_FILL, EW_AXPY, EW_XPAY, EW_SCAL, EW_ADD, EW_SUB, EW_MUL, EW_RECIP };
 
-template <class T> __global__ void k_fill(T *y, int64_t n, T v) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock)
-    y[i] = v;
-}
-
-template <class T>
-__global__ void k_axpy(int64_t n, T a, const T *__restrict__ x,
-                       T *__restrict__ y) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
-    T t = a * x[i];
-    y[i] = t + y[i];
+template <int OP, class T> __device__ __forceinline__ T ew(T a, T x, T y) {
+  if (OP == EW_FILL) return a;
+  if (OP == EW_AXPY) {  // y = a x + y
+    T t = a * x;
+    return t + y;
   }
-}
-
-template <class T>
-__global__ void k_xpay(int64_t n, T a, const T *__restrict__ x,
-                       T *__restrict__ y) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
-    T t = a * y[i];
-    y[i] = x[i] + t;
+  if (OP == EW_XPAY) {  // y = x + a y
+    T t = a * y;
+    return x + t;
   }
+  if (OP == EW_SCAL) return a * x;
+  if (OP == EW_ADD) return x + y;
+  if (OP == EW_SUB) return x - y;
+  if (OP == EW_MUL) return x * y;
+  return T(1) / x;  // EW_RECIP
 }
 
-// z may alias x or y (element-wise, each element read before written)
-template <class T>
-__global__ void k_axpyz(int64_t n, T a, const T *x, const T *y, T *z) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
-    T t = a * x[i];
-    z[i] = t + y[i];
+template <int OP> constexpr bool reads_x() { return OP != EW_FILL; }
+template <int OP> constexpr bool reads_y() {
+  return OP == EW_AXPY || OP == EW_XPAY || OP == EW_ADD || OP == EW_SUB || OP == EW_MUL;
+}
+
+// z[i] = op(a, x[i], y[i]); W elements per lane (W > 1 requires 16-byte
+// aligned x, y, z); z may alias x or y
+template <class T, int OP, int W>
+__global__ __launch_bounds__(kBlock) void k_ew(int64_t n, T a, const T *x,
+                                               const T *y, T *z) {
+  const int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * W;
+  if (W > 1 && i + W <= n) {
+    typedef T V __attribute__((ext_vector_type(W)));
+    V xv = {}, yv = {}, r;
+    if (reads_x<OP>()) xv = __builtin_nontemporal_load(reinterpret_cast<const V *>(x + i));
+    if (reads_y<OP>()) yv = __builtin_nontemporal_load(reinterpret_cast<const V *>(y + i));
+#pragma unroll
+    for (int j = 0; j < W; j++) r[j] = ew<OP>(a, (T)xv[j], (T)yv[j]);
+    __builtin_nontemporal_store(r, reinterpret_cast<V *>(z + i));
+  } else {
+    for (int64_t j = i; j < n && j < i + W; j++) {
+      T xs = reads_x<OP>() ? x[j] : T(0);
+      T ys = reads_y<OP>() ? y[j] : T(0);
+      z[j] = ew<OP>(a, xs, ys);
+    }
   }
-}
-
-template <class T> __global__ void k_scal(int64_t n, T a, T *x) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock)
-    x[i] = a * x[i];
-}
-
-template <class T, int OP>
-__global__ void k_vv(int64_t n, const T *a, const T *b, T *y) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
-    T u = a[i], v = b[i];
-    y[i] = OP == 0 ? u + v : (OP == 1 ? u - v : u * v);
-  }
-}
-
-template <class T>
-__global__ void k_reciprocal(int64_t n, const T *a, T *y) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock)
-    y[i] = T(1) / a[i];
 }
 
 template <class S, class D>
@@ -84,33 +71,56 @@ __global__ void k_convert(int64_t n, const S *x, D *y) {
     y[i] = (D)x[i];
 }
 
-// canonical dot: one partial per 512-element tile, last block sums them
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t n,
-                                                const T *__restrict__ x,
-                                                const T *__restrict__ y,
-                                                T *__restrict__ P,
-                                                int64_t ntiles,
+constexpr int kDotBlocks = 256;  // = stage-2 lanes
+constexpr int kDotChunk = 32;    // tiles per block between LDS flushes
+
+// canonical dot; S holds one value per block
+template <class T, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *x,
+                                                const T *y, T *S,
                                                 uint32_t *ticket,
                                                 double *out) {
+  __shared__ T wsum[kDotChunk][4];
   __shared__ T sh[8];
   __shared__ int flag;
-  for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
-    const int64_t i0 = k * kTile + 2 * threadIdx.x;
-    T p0 = 0, p1 = 0;
-    if (i0 + 1 < n) {
-      p0 = x[i0] * y[i0];
-      p1 = x[i0 + 1] * y[i0 + 1];
-    } else if (i0 < n) {
-      p0 = x[i0] * y[i0];
+  const int64_t ntiles = ntiles_of(n);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  T acc = 0;
+  for (int64_t k0 = blockIdx.x; k0 < ntiles; k0 += (int64_t)kDotBlocks * kDotChunk) {
+    const int64_t left = (ntiles - k0 + kDotBlocks - 1) / kDotBlocks;
+    const int cnt = left < kDotChunk ? (int)left : kDotChunk;
+#pragma unroll 4
+    for (int j = 0; j < cnt; j++) {
+      const int64_t i0 = (k0 + (int64_t)kDotBlocks * j) * kTile + 2 * threadIdx.x;
+      T p0 = 0, p1 = 0;
+      if (VEC && i0 + 1 < n) {
+        typedef T V2 __attribute__((ext_vector_type(2)));
+        V2 xv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(x + i0));
+        V2 yv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(y + i0));
+        p0 = xv[0] * yv[0];
+        p1 = xv[1] * yv[1];
+      } else {
+        if (i0 < n) p0 = x[i0] * y[i0];
+        if (i0 + 1 < n) p1 = x[i0 + 1] * y[i0 + 1];
+      }
+      T w = wave_tree(p0 + p1);
+      if (lane == 0) wsum[j][wv] = w;
     }
-    T a = p0 + p1;
-    T s = block_tree(a, sh);
-    if (threadIdx.x == 0) P[k] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < cnt; j++) {
+        T lo = wsum[j][0] + wsum[j][1];
+        T hi = wsum[j][2] + wsum[j][3];
+        T p = lo + hi;
+        acc = acc + p;
+      }
+    }
     __syncthreads();
   }
-  if (last_block_arrives(ticket, gridDim.x, &flag)) {
-    T r = reduce_partials(P, ntiles, sh);
+  if (threadIdx.x == 0) publish_partial(S, blockIdx.x, acc);
+  if (last_block_arrives(ticket, kDotBlocks, &flag)) {
+    T v = load_partial(S, threadIdx.x);
+    T r = block_tree(v, sh);
     if (threadIdx.x == 0) *out = (double)r;
   }
 }
@@ -120,54 +130,57 @@ template <class T> const T *tp(const void *p) {
   return static_cast<const T *>(p);
 }
 
+inline bool aligned16(const void *p) {
+  return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+template <class T, int OP>
+void ew_launch_t(Ctx *c, int64_t n, T a, const void *x, const void *y, void *z) {
+  constexpr int W = 16 / sizeof(T);
+  const bool vec = aligned16(x) && aligned16(y) && aligned16(z);
+  if (vec) {
+    const int64_t g = (n + (int64_t)kBlock * W - 1) / ((int64_t)kBlock * W);
+    k_ew<T, OP, W><<<(unsigned)g, kBlock, 0, c->stream>>>(n, a, tp<T>(x), tp<T>(y), tp<T>(z));
+  } else {
+    const int64_t g = (n + kBlock - 1) / kBlock;
+    k_ew<T, OP, 1><<<(unsigned)g, kBlock, 0, c->stream>>>(n, a, tp<T>(x), tp<T>(y), tp<T>(z));
+  }
+  MGX_HIP(hipGetLastError());
+}
+
+template <int OP>
+void ew_launch(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x, const void *y,
+               void *z) {
+  if (n <= 0) return;
+  if (dt == MGX_F64)
+    ew_launch_t<double, OP>(c, n, a, x, y, z);
+  else
+    ew_launch_t<float, OP>(c, n, (float)a, x, y, z);
+}
+
 }  // namespace
 
 void launch_fill(Ctx *c, mgx_dtype dt, void *y, int64_t n, double v) {
-  if (n <= 0) return;
-  if (dt == MGX_F64)
-    k_fill<double><<<grid_for(c, n), kBlock, 0, c->stream>>>(tp<double>(y), n, v);
-  else
-    k_fill<float><<<grid_for(c, n), kBlock, 0, c->stream>>>(tp<float>(y), n, (float)v);
-  MGX_HIP(hipGetLastError());
+  ew_launch<EW_FILL>(c, dt, n, v, nullptr, nullptr, y);
 }
 
 void launch_axpy(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x,
                  void *y) {
-  if (n <= 0) return;
-  if (dt == MGX_F64)
-    k_axpy<double><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, a, tp<double>(x), tp<double>(y));
-  else
-    k_axpy<float><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, (float)a, tp<float>(x), tp<float>(y));
-  MGX_HIP(hipGetLastError());
+  ew_launch<EW_AXPY>(c, dt, n, a, x, y, y);
 }
 
 void launch_xpay(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x,
                  void *y) {
-  if (n <= 0) return;
-  if (dt == MGX_F64)
-    k_xpay<double><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, a, tp<double>(x), tp<double>(y));
-  else
-    k_xpay<float><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, (float)a, tp<float>(x), tp<float>(y));
-  MGX_HIP(hipGetLastError());
+  ew_launch<EW_XPAY>(c, dt, n, a, x, y, y);
 }
 
 void launch_axpyz(Ctx *c, mgx_dtype dt, int64_t n, double a, const void *x,
                   const void *y, void *z) {
-  if (n <= 0) return;
-  if (dt == MGX_F64)
-    k_axpyz<double><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, a, tp<double>(x), tp<double>(y), tp<double>(z));
-  else
-    k_axpyz<float><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, (float)a, tp<float>(x), tp<float>(y), tp<float>(z));
-  MGX_HIP(hipGetLastError());
+  ew_launch<EW_AXPY>(c, dt, n, a, x, y, z);
 }
 
 void launch_scal(Ctx *c, mgx_dtype dt, int64_t n, double a, void *x) {
-  if (n <= 0) return;
-  if (dt == MGX_F64)
-    k_scal<double><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, a, tp<double>(x));
-  else
-    k_scal<float><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, (float)a, tp<float>(x));
-  MGX_HIP(hipGetLastError());
+  ew_launch<EW_SCAL>(c, dt, n, a, x, nullptr, x);
 }
 
 void launch_copy(Ctx *c, mgx_dtype dt, int64_t n, const void *x, void *y) {
@@ -178,34 +191,17 @@ void launch_copy(Ctx *c, mgx_dtype dt, int64_t n, const void *x, void *y) {
 
 void launch_vv(Ctx *c, mgx_dtype dt, int op, int64_t n, const void *a,
                const void *b, void *y) {
-  if (n <= 0) return;
-  const int g = grid_for(c, n);
-  if (dt == MGX_F64) {
-    auto A = tp<double>(a);
-    auto B = tp<double>(b);
-    auto Y = tp<double>(y);
-    if (op == 0) k_vv<double, 0><<<g, kBlock, 0, c->stream>>>(n, A, B, Y);
-    else if (op == 1) k_vv<double, 1><<<g, kBlock, 0, c->stream>>>(n, A, B, Y);
-    else k_vv<double, 2><<<g, kBlock, 0, c->stream>>>(n, A, B, Y);
-  } else {
-    auto A = tp<float>(a);
-    auto B = tp<float>(b);
-    auto Y = tp<float>(y);
-    if (op == 0) k_vv<float, 0><<<g, kBlock, 0, c->stream>>>(n, A, B, Y);
-    else if (op == 1) k_vv<float, 1><<<g, kBlock, 0, c->stream>>>(n, A, B, Y);
-    else k_vv<float, 2><<<g, kBlock, 0, c->stream>>>(n, A, B, Y);
-  }
-  MGX_HIP(hipGetLastError());
+  if (op == 0)
+    ew_launch<EW_ADD>(c, dt, n, 0.0, a, b, y);
+  else if (op == 1)
+    ew_launch<EW_SUB>(c, dt, n, 0.0, a, b, y);
+  else
+    ew_launch<EW_MUL>(c, dt, n, 0.0, a, b, y);
 }
 
 void launch_reciprocal(Ctx *c, mgx_dtype dt, int64_t n, const void *a,
                        void *y) {
-  if (n <= 0) return;
-  if (dt == MGX_F64)
-    k_reciprocal<double><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, tp<double>(a), tp<double>(y));
-  else
-    k_reciprocal<float><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, tp<float>(a), tp<float>(y));
-  MGX_HIP(hipGetLastError());
+  ew_launch<EW_RECIP>(c, dt, n, 0.0, a, nullptr, y);
 }
 
 void launch_convert(Ctx *c, mgx_dtype from, mgx_dtype to, int64_t n,
@@ -215,10 +211,11 @@ void launch_convert(Ctx *c, mgx_dtype from, mgx_dtype to, int64_t n,
     launch_copy(c, from, n, x, y);
     return;
   }
+  int64_t g = std::min<int64_t>((n + kBlock * 4 - 1) / (kBlock * 4), (int64_t)c->num_cus * 16);
   if (from == MGX_F64)
-    k_convert<double, float><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, tp<double>(x), tp<float>(y));
+    k_convert<double, float><<<(unsigned)g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<float>(y));
   else
-    k_convert<float, double><<<grid_for(c, n), kBlock, 0, c->stream>>>(n, tp<float>(x), tp<double>(y));
+    k_convert<float, double><<<(unsigned)g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<double>(y));
   MGX_HIP(hipGetLastError());
 }
 
@@ -228,17 +225,23 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
     launch_fill(c, MGX_F64, d_out, 1, 0.0);
     return;
   }
-  const int64_t nt = ntiles_of(n);
-  ensure_partials(c, (size_t)nt * 8);
-  int64_t g = nt;
-  const int64_t cap = (int64_t)c->num_cus * 8;
-  if (g > cap) g = cap;
-  if (dt == MGX_F64)
-    k_dot<double><<<(int)g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y),
-        tp<double>(c->d_partials), nt, d_ticket, d_out);
-  else
-    k_dot<float><<<(int)g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y),
-        tp<float>(c->d_partials), nt, d_ticket, d_out);
+  ensure_partials(c, (size_t)kDotBlocks * 8);
+  const bool vec = aligned16(x) && aligned16(y);
+  if (dt == MGX_F64) {
+    if (vec)
+      k_dot<double, true><<<kDotBlocks, kBlock, 0, c->stream>>>(
+          n, tp<double>(x), tp<double>(y), tp<double>(c->d_partials), d_ticket, d_out);
+    else
+      k_dot<double, false><<<kDotBlocks, kBlock, 0, c->stream>>>(
+          n, tp<double>(x), tp<double>(y), tp<double>(c->d_partials), d_ticket, d_out);
+  } else {
+    if (vec)
+      k_dot<float, true><<<kDotBlocks, kBlock, 0, c->stream>>>(
+          n, tp<float>(x), tp<float>(y), tp<float>(c->d_partials), d_ticket, d_out);
+    else
+      k_dot<float, false><<<kDotBlocks, kBlock, 0, c->stream>>>(
+          n, tp<float>(x), tp<float>(y), tp<float>(c->d_partials), d_ticket, d_out);
+  }
   MGX_HIP(hipGetLastError());
 }
 

@@ -62,28 +62,50 @@ __device__ __forceinline__ void block_tree2(T &a, T &b, T *sh) {
   }
 }
 
+// Cross-block partial sums.  Partials are published with device-coherent
+// (agent-scope) stores and read back with device-coherent loads, so no L2
+// write-back/invalidate fence is needed: the writer waits until its store
+// has completed (s_waitcnt) before it takes a ticket, and the block that
+// takes the last ticket reads every partial coherently.
+template <class T>
+__device__ __forceinline__ void publish_partial(T *P, int64_t k, T v) {
+  __hip_atomic_store(&P[k], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T> __device__ __forceinline__ T load_partial(const T *P, int64_t k) {
+  return __hip_atomic_load(const_cast<T *>(&P[k]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // stage 2 of the canonical dot: lane t sums P[t], P[t+256], ... in order
 template <class T>
 __device__ __forceinline__ T reduce_partials(const T *P, int64_t ntiles,
                                              T *sh) {
   T s = 0;
-  for (int64_t k = threadIdx.x; k < ntiles; k += kBlock) {
-    s = s + P[k];
+  int64_t k = threadIdx.x;
+  // issue 8 coherent loads before adding them in order (latency bound)
+  for (; k + 7 * kBlock < ntiles; k += 8 * kBlock) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = load_partial(P, k + u * kBlock);
+#pragma unroll
+    for (int u = 0; u < 8; u++) s = s + v[u];
   }
+  for (; k < ntiles; k += kBlock) s = s + load_partial(P, k);
   return block_tree(s, sh);
 }
 
-// "last block done": every block calls this after writing its partials;
-// returns true in all threads of the block that arrives last, after which
-// that block may read every other block's partials.  The counter is reset
-// to zero by the last block for the next use.
+// "last block done": every block calls this after publishing its partials
+// (publish_partial); returns true in all threads of the block that arrives
+// last, which may then read every block's partials with load_partial.  The
+// counter is reset to zero by the last block for the next use.
 __device__ __forceinline__ bool last_block_arrives(uint32_t *ticket,
                                                    uint32_t nblocks,
                                                    int *sh_flag) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);  // this lane's published stores completed
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL,
+    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_AGENT);
     int last = (t == nblocks - 1);
     if (last)
@@ -92,9 +114,7 @@ __device__ __forceinline__ bool last_block_arrives(uint32_t *ticket,
     *sh_flag = last;
   }
   __syncthreads();
-  bool last = *sh_flag != 0;
-  if (last) __threadfence();
-  return last;
+  return *sh_flag != 0;
 }
 
 // workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on

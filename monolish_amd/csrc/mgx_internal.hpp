@@ -107,6 +107,10 @@ struct Csr {
   int32_t fused_nnz = 0;     // entry budget of a chunk
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
   Csr *trans = nullptr;      // A^T, built on first matvec_T
+  // replayable graph of fused solver iterations, reused while the kernel
+  // arguments (buffers, tolerance-independent) stay the same
+  hipGraphExec_t graph = nullptr;
+  std::string graph_key;
   size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
 };
 
@@ -145,7 +149,7 @@ inline void activate(Ctx *c) {
 }
 void ensure_partials(Ctx *c, size_t bytes);
 
-inline int64_t ntiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
+__host__ __device__ inline int64_t ntiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
 
 // kernel launchers (defined in the .hip files)
 void launch_fill(Ctx *c, mgx_dtype dt, void *y, int64_t n, double v);

@@ -179,6 +179,7 @@ static void csr_free(Csr *A) {
   dev_free(A->plan.d_blocks);
   dev_free(A->d_chunks);
   dev_free(A->d_tile_chunk);
+  if (A->graph) (void)hipGraphExecDestroy(A->graph);
   if (A->trans) csr_free(A->trans);
   delete static_cast<mgx_csr *>(A);
 }

@@ -22,7 +22,10 @@ namespace mgx {
 
 namespace {
 
-template <class T, bool AB>
+// U = entry budget / 256: each lane issues all of its val/col loads, then all
+// of its x gathers, before it multiplies, so up to U loads per lane are in
+// flight at once.
+template <class T, bool AB, int U>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const int32_t *__restrict__ blk, int64_t nblocks,
     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -58,10 +61,28 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     return;
   }
   const int32_t e0 = rp[r0], e1 = rp[r1];
-  for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += kBlock) {
-    T v = __builtin_nontemporal_load(&val[e]);
-    int32_t c = __builtin_nontemporal_load(&ci[e]);
-    prod[e - e0] = v * x[c];
+  {
+    T v[U];
+    int32_t c[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int32_t e = e0 + (int32_t)threadIdx.x + k * kBlock;
+      if (e < e1) {
+        v[k] = __builtin_nontemporal_load(&val[e]);
+        c[k] = __builtin_nontemporal_load(&ci[e]);
+      }
+    }
+    T xv[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int32_t e = e0 + (int32_t)threadIdx.x + k * kBlock;
+      if (e < e1) xv[k] = x[c[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int32_t e = e0 + (int32_t)threadIdx.x + k * kBlock;
+      if (e < e1) prod[e - e0] = v[k] * xv[k];
+    }
   }
   __syncthreads();
   const int32_t r = r0 + (int32_t)threadIdx.x;
@@ -145,39 +166,49 @@ void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp) {
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
 
-void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
-                 void *y, double alpha, double beta, bool ab) {
+template <class T, bool AB>
+static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
+                          T beta) {
   Ctx *c = A->ctx;
   const int64_t nb = A->plan.nblocks;
-  if (nb <= 0) return;
   const int remap = c->xcd_remap && nb >= 64;
-  int64_t grid = remap ? ((nb + 7) / 8) * 8 : nb;
-  const int nnz_max = A->plan.nnz_max;
+  const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
+  const size_t lds = (size_t)A->plan.nnz_max * sizeof(T);
+  const int u = (A->plan.nnz_max + kBlock - 1) / kBlock;  // <= 32
+  const int32_t *B = A->plan.d_blocks;
+  if (u <= 2)
+    k_spmv<T, AB, 2><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+  else if (u <= 4)
+    k_spmv<T, AB, 4><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+  else if (u <= 8)
+    k_spmv<T, AB, 8><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+  else if (u <= 16)
+    k_spmv<T, AB, 16><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+  else
+    k_spmv<T, AB, 32><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+  MGX_HIP(hipGetLastError());
+}
+
+void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
+                 void *y, double alpha, double beta, bool ab) {
+  if (A->plan.nblocks <= 0) return;
   if (dt == MGX_F64) {
-    const size_t lds = (size_t)nnz_max * sizeof(double);
     auto V = static_cast<const double *>(val);
     auto X = static_cast<const double *>(x);
     auto Y = static_cast<double *>(y);
     if (ab)
-      k_spmv<double, true><<<(unsigned)grid, kBlock, lds, c->stream>>>(
-          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+      spmv_dispatch<double, true>(A, V, X, Y, alpha, beta);
     else
-      k_spmv<double, false><<<(unsigned)grid, kBlock, lds, c->stream>>>(
-          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, 1.0, 0.0, remap);
+      spmv_dispatch<double, false>(A, V, X, Y, 1.0, 0.0);
   } else {
-    const size_t lds = (size_t)nnz_max * sizeof(float);
     auto V = static_cast<const float *>(val);
     auto X = static_cast<const float *>(x);
     auto Y = static_cast<float *>(y);
     if (ab)
-      k_spmv<float, true><<<(unsigned)grid, kBlock, lds, c->stream>>>(
-          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, (float)alpha,
-          (float)beta, remap);
+      spmv_dispatch<float, true>(A, V, X, Y, (float)alpha, (float)beta);
     else
-      k_spmv<float, false><<<(unsigned)grid, kBlock, lds, c->stream>>>(
-          A->plan.d_blocks, nb, A->rp, A->ci, V, X, Y, 1.0f, 0.0f, remap);
+      spmv_dispatch<float, false>(A, V, X, Y, 1.0f, 0.0f);
   }
-  MGX_HIP(hipGetLastError());
 }
 
 void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d) {

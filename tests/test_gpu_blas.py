@@ -238,6 +238,20 @@ def test_dot_nrm2_bit_exact(m, ctx, oracle, dtype, n):
     assert m.nrm2(X) == float(oracle.nrm2(x))
 
 
+def test_dot_repeatable_across_blocks(m, ctx, oracle):
+    """2048+ blocks publish partials that one block sums: 40 repeats must all
+    give the oracle's bits (a stale or lost partial would show here)."""
+    say("dot repeat")
+    n = 5_000_003
+    x = oracle.random_vector(n, -1.0, 1.0, 41)
+    y = oracle.random_vector(n, -1.0, 1.0, 42)
+    X = m.Vector(ctx, x)
+    Y = m.Vector(ctx, y)
+    ref = oracle.dot(x, y)
+    got = {m.dot(X, Y) for _ in range(40)}
+    assert got == {ref}
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_vector_ops_bit_exact(m, ctx, oracle, dtype):
     say("vector ops", dtype)
