@@ -60,6 +60,10 @@ int mgx_device_info(mgx_ctx *ctx, char *name, size_t len, int *compute_units,
  * threshold, XCD-aware block order.  Applies to CRS created afterwards. */
 int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
                         int long_row, int xcd_remap);
+/* 1: fused solvers and batches replay each group of iterations from a
+ * captured hipGraph (re-captured when the set of running systems changes);
+ * 0 (default): direct kernel launches */
+int mgx_set_graphs(mgx_ctx *ctx, int enable);
 
 /* ---- vectors: vector<T>::send/recv/nonfree_recv/device_free ------------- */
 /* vector<T>::send (src/utils/system/gpu_comm.cpp:8-24): device copy of n
@@ -148,7 +152,7 @@ int mgx_vreciprocal(mgx_vec *a, mgx_vec *y);
 /* how the iteration is executed; every mode follows the reference's
  * arithmetic, so FP64 results of REFERENCE and FUSED are bit-identical */
 #define MGX_MODE_REFERENCE 0 /* one kernel per BLAS call, scalars to host */
-#define MGX_MODE_FUSED 1     /* fused kernels, device scalars, graph replay */
+#define MGX_MODE_FUSED 1     /* fused kernels, device scalars (CG, BiCGSTAB) */
 
 typedef struct mgx_solve_params {
   double tol;          /* solver::set_tol (monolish_solver.hpp:86-215) */
@@ -196,6 +200,58 @@ int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                     const mgx_solve_params *p, double inner_tol,
                     int64_t max_outer, mgx_solve_result *res);
 
+/* ---- batched solves ------------------------------------------------------ */
+#define MGX_METHOD_CG 0
+#define MGX_METHOD_BICGSTAB 1
+
+/* equation::CG / BiCGSTAB ::solve over nsys independent systems at once
+ * (cg.cpp:12-86, bicgstab.cpp:12-135): A[i] x[i] = b[i], one fused launch
+ * per kernel stage covers every running system.  Each system follows exactly
+ * the iteration it would follow alone (per-system reductions in tile order),
+ * params apply to each system (tol_is_relative: tol * ||b[i]||) and
+ * results[i] receives its outcome.  Returns MGX_OK once every system has
+ * stopped (successfully or not); ABI errors as usual. */
+int mgx_batch_solve(mgx_ctx *ctx, int method, int64_t nsys, mgx_csr *const *A,
+                    mgx_vec *const *x, mgx_vec *const *b,
+                    const mgx_solve_params *p, mgx_solve_result *results);
+
+/* ---- solve server: continuous batching of arriving requests -------------- */
+typedef struct mgx_server mgx_server;
+typedef struct mgx_server_config {
+  size_t hbm_budget;        /* bytes for resident requests (0: 3/4 of free) */
+  int32_t max_active;       /* systems iterated together (0: 1024) */
+  int32_t iters_per_poll;   /* fused iterations between status checks (0: 8) */
+} mgx_server_config;
+
+typedef struct mgx_server_stats {
+  int64_t submitted, completed, active, queued;
+  double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s;
+  double mean_queue_s;      /* submit -> admission */
+  double solves_per_s;      /* completed / (last completion - first submit) */
+  size_t peak_bytes;        /* peak resident bytes (matrices + work) */
+} mgx_server_stats;
+
+/* A worker thread on the context's stream admits requests in arrival order
+ * while slots and the HBM budget allow, iterates all resident systems
+ * together, and retires each as soon as it stops.  The context must not be
+ * used by other calls while the server exists. */
+int mgx_server_create(mgx_ctx *ctx, const mgx_server_config *cfg,
+                      mgx_server **out);
+/* copies the FP64 system (CRS arrays, b, optional x0) and queues it;
+ * method MGX_METHOD_CG or MGX_METHOD_BICGSTAB; *ticket identifies it */
+int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz,
+                      const int32_t *row_ptr, const int32_t *col_ind,
+                      const double *val, const double *b, const double *x0,
+                      const mgx_solve_params *p, int64_t *ticket);
+/* waits up to timeout_s (< 0: forever) for a request; MGX_ERR_NOT_FOUND on
+ * timeout; copies the solution into x (n doubles) when x != NULL and
+ * releases the ticket */
+int mgx_server_wait(mgx_server *s, int64_t ticket, double timeout_s,
+                    mgx_solve_result *res, double *x);
+int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out);
+/* finishes queued and resident requests, stops the worker */
+int mgx_server_destroy(mgx_server *s);
+
 /* ---- workload generators (host) ----------------------------------------- */
 /* 3D stencil on an nx*ny*nz grid, row = (k*ny + j)*nx + i, ascending columns,
  * Dirichlet truncation.  points = 7: diag 6, off-diag -1; points = 27: diag
@@ -211,6 +267,20 @@ int mgx_gen_stencil3d(int64_t nx, int64_t ny, int64_t nz, int points,
 int mgx_gen_random_lognormal(int64_t rows, int64_t cols, double mean,
                              double sigma, int32_t maxlen, uint64_t seed,
                              int32_t *row_ptr, int32_t *col_ind, double *val);
+/* util::laplacian_matrix_2D_5p(m, n) (src/utils/create_matrix/
+ * laplacian_2D5p.cpp:6-50) directly as CRS; NULL arrays give nnz only */
+int mgx_gen_laplacian2d(int64_t m, int64_t n, int64_t *nnz, int32_t *row_ptr,
+                        int32_t *col_ind, double *val);
+/* 2D convection-diffusion -lap(u) + peclet du/dx on an m x m grid, upwind:
+ * diag 4 + c, west -1 - c, east/south/north -1, c = peclet / (m + 1) */
+int mgx_gen_convdiff2d(int64_t m, double peclet, int64_t *nnz, int32_t *row_ptr,
+                       int32_t *col_ind, double *val);
+/* SPD M-matrix: kmin..kmax random off-diagonals U[-1,0) per row,
+ * symmetrised (A + A^T)/2, diag = sum |off-diagonal| + shift.  Two calls as
+ * for mgx_gen_random_lognormal. */
+int mgx_gen_mmatrix(int64_t n, int32_t kmin, int32_t kmax, double shift,
+                    uint64_t seed, int32_t *row_ptr, int32_t *col_ind,
+                    double *val);
 
 #ifdef __cplusplus
 }

@@ -107,6 +107,32 @@ _sig("mgx_residual_l2", [_P, _P, _P, C.POINTER(_D)])
 _sig("mgx_ir_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64, C.POINTER(SolveResultC)])
 _sig("mgx_gen_stencil3d", [_I64, _I64, _I64, C.c_int, _D, C.POINTER(_I64), _P, _P, _P])
 _sig("mgx_gen_random_lognormal", [_I64, _I64, _D, _D, _I32, C.c_uint64, _P, _P, _P])
+_sig("mgx_gen_laplacian2d", [_I64, _I64, C.POINTER(_I64), _P, _P, _P])
+_sig("mgx_gen_convdiff2d", [_I64, _D, C.POINTER(_I64), _P, _P, _P])
+_sig("mgx_gen_mmatrix", [_I64, _I32, _I32, _D, C.c_uint64, _P, _P, _P])
+_sig("mgx_set_graphs", [_P, C.c_int])
+_sig("mgx_batch_solve", [_P, C.c_int, _I64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                         C.POINTER(SolveParams), C.POINTER(SolveResultC)])
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", _I32), ("iters_per_poll", _I32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", _I64), ("completed", _I64), ("active", _I64), ("queued", _I64),
+                ("p50_latency_s", _D), ("p99_latency_s", _D), ("mean_latency_s", _D),
+                ("max_latency_s", _D), ("mean_queue_s", _D), ("solves_per_s", _D),
+                ("peak_bytes", C.c_size_t)]
+
+
+_sig("mgx_server_create", [_P, C.POINTER(ServerConfig), C.POINTER(_P)])
+_sig("mgx_server_submit", [_P, C.c_int, _I64, _I64, _P, _P, _P, _P, _P, C.POINTER(SolveParams),
+                           C.POINTER(_I64)])
+_sig("mgx_server_wait", [_P, _I64, _D, C.POINTER(SolveResultC), _P])
+_sig("mgx_server_stats_get", [_P, C.POINTER(ServerStats)])
+_sig("mgx_server_destroy", [_P])
+METHOD_CG, METHOD_BICGSTAB = 0, 1
 
 
 def lib() -> C.CDLL:
@@ -466,3 +492,102 @@ def random_lognormal(rows: int, cols: int, mean: float = 32.0, sigma: float = 1.
     v = np.empty(nnz, np.float64)
     check(_lib.mgx_gen_random_lognormal(rows, cols, mean, sigma, maxlen, seed, _ptr(rp), _ptr(ci), _ptr(v)))
     return rp, ci, v
+
+
+def laplacian2d(m: int, n: int):
+    """util::laplacian_matrix_2D_5p(m, n) as CRS arrays (row_ptr, col_ind, val)."""
+    return _gen2(_lib.mgx_gen_laplacian2d, m, n)
+
+
+def convdiff2d(m: int, peclet: float):
+    """2D upwind convection-diffusion on an m x m grid (nonsymmetric)."""
+    return _gen2(_lib.mgx_gen_convdiff2d, m, float(peclet), rows=m * m)
+
+
+def _gen2(fn, *args, rows=None):
+    nnz = _I64(0)
+    check(fn(*args, C.byref(nnz), None, None, None))
+    n = rows if rows is not None else args[0] * args[1]
+    rp = np.empty(n + 1, np.int32)
+    ci = np.empty(nnz.value, np.int32)
+    v = np.empty(nnz.value, np.float64)
+    check(fn(*args, C.byref(nnz), _ptr(rp), _ptr(ci), _ptr(v)))
+    return rp, ci, v
+
+
+def mmatrix(n: int, kmin: int = 5, kmax: int = 15, shift: float = 0.5, seed: int = 4000):
+    """SPD M-matrix with kmin..kmax random off-diagonals per row, symmetrised."""
+    rp = np.empty(n + 1, np.int32)
+    check(_lib.mgx_gen_mmatrix(n, kmin, kmax, shift, seed, _ptr(rp), None, None))
+    nnz = int(rp[-1])
+    ci = np.empty(nnz, np.int32)
+    v = np.empty(nnz, np.float64)
+    check(_lib.mgx_gen_mmatrix(n, kmin, kmax, shift, seed, _ptr(rp), _ptr(ci), _ptr(v)))
+    return rp, ci, v
+
+
+def set_graphs(ctx: Context, enable: bool):
+    check(_lib.mgx_set_graphs(ctx.handle, int(enable)))
+
+
+def batch_solve(ctx: Context, method: int, As, xs, bs, tol=1e-8, maxiter=0, miniter=0,
+                precond=PRECOND_JACOBI, tol_is_relative=False, check_every=0):
+    """Solve many systems in shared fused launches; returns a SolveResult each."""
+    ns = len(As)
+    p, _ = _params(tol, maxiter, miniter, precond, MODE_FUSED, tol_is_relative, check_every, 0)
+    res = (SolveResultC * max(ns, 1))()
+    arrA = (_P * max(ns, 1))(*[a.handle for a in As])
+    arrX = (_P * max(ns, 1))(*[x.handle for x in xs])
+    arrB = (_P * max(ns, 1))(*[b.handle for b in bs])
+    check(_lib.mgx_batch_solve(ctx.handle, method, ns, arrA, arrX, arrB, C.byref(p), res))
+    return [SolveResult(r.status, r.final_iter, r.final_resid, r.seconds) for r in res[:ns]]
+
+
+class Server:
+    """Continuous-batching solve server on one context (see mgx_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, iters_per_poll: int = 0):
+        cfg = ServerConfig(hbm_budget, max_active, iters_per_poll)
+        self._h = _P()
+        check(_lib.mgx_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, method, rp, ci, val, b, x0=None, tol=1e-10, tol_is_relative=True, maxiter=0,
+               precond=PRECOND_JACOBI) -> int:
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x0p = None
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, np.float64)
+            x0p = _ptr(x0)
+        p, _ = _params(tol, maxiter, 0, precond, MODE_FUSED, tol_is_relative, 0, 0)
+        t = _I64(0)
+        check(_lib.mgx_server_submit(self._h, method, len(b), len(val), _ptr(rp), _ptr(ci), _ptr(val),
+                                     _ptr(b), x0p, C.byref(p), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, n: int | None = None, timeout: float = -1.0):
+        r = SolveResultC()
+        x = np.empty(n, np.float64) if n else None
+        rc = _lib.mgx_server_wait(self._h, ticket, timeout, C.byref(r), _ptr(x) if x is not None else None)
+        if rc == ERR_NOT_FOUND:
+            return None
+        check(rc)
+        return SolveResult(r.status, r.final_iter, r.final_resid, r.seconds), x
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(_lib.mgx_server_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(_lib.mgx_server_destroy(self._h))
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

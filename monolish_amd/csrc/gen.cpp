@@ -157,4 +157,136 @@ int mgx_gen_random_lognormal(int64_t rows, int64_t cols, double mean, double sig
   return MGX_OK;
 }
 
+// util::laplacian_matrix_2D_5p(m, n) (src/utils/create_matrix/
+// laplacian_2D5p.cpp:6-50) converted to CRS: row ii = i*m + j with entries
+// (ii-m), (ii-1), ii, (ii+1), (ii+m) inside the grid, already in column order
+int mgx_gen_laplacian2d(int64_t m, int64_t n, int64_t *nnz_out, int32_t *row_ptr,
+                        int32_t *col_ind, double *val) {
+  if (m <= 0 || n <= 0 || !nnz_out) return MGX_ERR_INVALID;
+  const int64_t size = m * n, nnz = 5 * m * n - 2 * m - 2 * n;
+  *nnz_out = nnz;
+  if (size >= INT32_MAX || nnz >= INT32_MAX) return MGX_ERR_INVALID;
+  if (!row_ptr) return MGX_OK;
+  int64_t e = 0;
+  for (int64_t ii = 0; ii < size; ii++) {
+    const int64_t i = ii / m, j = ii - i * m;
+    row_ptr[ii] = (int32_t)e;
+    auto put = [&](int64_t c, double v) {
+      if (col_ind) {
+        col_ind[e] = (int32_t)c;
+        val[e] = v;
+      }
+      e++;
+    };
+    if (i > 0) put(ii - m, -1.0);
+    if (j > 0) put(ii - 1, -1.0);
+    put(ii, 4.0);
+    if (j < m - 1) put(ii + 1, -1.0);
+    if (i < n - 1) put(ii + m, -1.0);
+  }
+  row_ptr[size] = (int32_t)e;
+  return MGX_OK;
+}
+
+// 2D convection-diffusion, -lap(u) + pe * du/dx on an m x m grid, first-order
+// upwind (flow in +x): diag 4 + c, west -1 - c, east -1, south/north -1 with
+// c = pe / (m + 1); nonsymmetric, diagonally dominant
+int mgx_gen_convdiff2d(int64_t m, double peclet, int64_t *nnz_out, int32_t *row_ptr,
+                       int32_t *col_ind, double *val) {
+  if (m <= 0 || peclet < 0 || !nnz_out) return MGX_ERR_INVALID;
+  const int64_t size = m * m, nnz = 5 * m * m - 4 * m;
+  *nnz_out = nnz;
+  if (size >= INT32_MAX || nnz >= INT32_MAX) return MGX_ERR_INVALID;
+  if (!row_ptr) return MGX_OK;
+  const double c = peclet / (double)(m + 1);
+  int64_t e = 0;
+  for (int64_t ii = 0; ii < size; ii++) {
+    const int64_t i = ii / m, j = ii - i * m;
+    row_ptr[ii] = (int32_t)e;
+    auto put = [&](int64_t col, double v) {
+      if (col_ind) {
+        col_ind[e] = (int32_t)col;
+        val[e] = v;
+      }
+      e++;
+    };
+    if (i > 0) put(ii - m, -1.0);
+    if (j > 0) put(ii - 1, -1.0 - c);
+    put(ii, 4.0 + c);
+    if (j < m - 1) put(ii + 1, -1.0);
+    if (i < m - 1) put(ii + m, -1.0);
+  }
+  row_ptr[size] = (int32_t)e;
+  return MGX_OK;
+}
+
+// SPD M-matrix: each row draws k in [kmin, kmax] distinct off-diagonal
+// columns with values U[-1, 0); the pattern is symmetrised as (A + A^T)/2 on
+// the union of both patterns, and diag = sum |off-diagonal| + shift.  Two
+// calls: row_ptr only (col_ind/val NULL) gives nnz, then the fill.
+int mgx_gen_mmatrix(int64_t n, int32_t kmin, int32_t kmax, double shift, uint64_t seed,
+                    int32_t *row_ptr, int32_t *col_ind, double *val) {
+  if (n <= 1 || kmin < 0 || kmax < kmin || !row_ptr) return MGX_ERR_INVALID;
+  struct Ent {
+    int32_t r, c;
+    double v;
+  };
+  std::vector<Ent> ents;
+  ents.reserve((size_t)n * (kmax + 1) * 2);
+  std::vector<int32_t> tmp;
+  for (int64_t r = 0; r < n; r++) {
+    uint64_t s = (seed + 0x51ED27ull) * 0x9E3779B97F4A7C15ull ^ (uint64_t)r * 0xD6E8FEB86659FD93ull;
+    const int32_t k = kmin + (int32_t)(splitmix64(s) % (uint64_t)(kmax - kmin + 1));
+    const int32_t kk = (int32_t)std::min<int64_t>(k, n - 1);
+    tmp.clear();
+    while ((int32_t)tmp.size() < kk) {
+      int32_t c = (int32_t)(splitmix64(s) % (uint64_t)n);
+      if (c == r || std::find(tmp.begin(), tmp.end(), c) != tmp.end()) continue;
+      tmp.push_back(c);
+    }
+    for (int32_t c : tmp) {
+      const double v = -u01(s);  // U(-1, 0]
+      ents.push_back({(int32_t)r, c, 0.5 * v});
+      ents.push_back({c, (int32_t)r, 0.5 * v});
+    }
+  }
+  std::sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+    return a.r != b.r ? a.r < b.r : a.c < b.c;
+  });
+  // merge duplicates (summed), add the diagonal
+  std::vector<Ent> out;
+  out.reserve(ents.size() + n);
+  std::vector<double> absum(n, 0.0);
+  for (size_t i = 0; i < ents.size();) {
+    size_t j = i;
+    double v = 0;
+    while (j < ents.size() && ents[j].r == ents[i].r && ents[j].c == ents[i].c) v += ents[j++].v;
+    out.push_back({ents[i].r, ents[i].c, v});
+    absum[ents[i].r] += std::fabs(v);
+    i = j;
+  }
+  std::vector<Ent> full;
+  full.reserve(out.size() + n);
+  size_t p = 0;
+  for (int64_t r = 0; r < n; r++) {
+    while (p < out.size() && out[p].r == r && out[p].c < r) full.push_back(out[p++]);
+    full.push_back({(int32_t)r, (int32_t)r, absum[r] + shift});
+    while (p < out.size() && out[p].r == r) full.push_back(out[p++]);
+  }
+  if ((int64_t)full.size() >= INT32_MAX) return MGX_ERR_INVALID;
+  int64_t e = 0;
+  for (int64_t r = 0; r < n; r++) {
+    row_ptr[r] = (int32_t)e;
+    while (e < (int64_t)full.size() && full[e].r == r) {
+      if (col_ind) {
+        col_ind[e] = full[e].c;
+        val[e] = full[e].v;
+      }
+      e++;
+    }
+  }
+  row_ptr[n] = (int32_t)e;
+  return MGX_OK;
+}
+
 }  // extern "C"

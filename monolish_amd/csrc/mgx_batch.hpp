@@ -1,0 +1,128 @@
+// mgx_batch.hpp -- multi-system fused Krylov engine (host side + descriptors).
+#pragma once
+
+#include <deque>
+#include <memory>
+#include <vector>
+
+#include "mgx_internal.hpp"
+
+namespace mgx {
+
+// one system of a fused launch; vectors are indexed by global row
+// (row0 + local row), matrix entries by the system's own row_ptr
+template <class T> struct SysDesc {
+  const int32_t *rp, *ci;
+  const T *val;
+  const int32_t *chunks, *tile_chunk;
+  const T *M;  // Jacobi 1/diag or nullptr
+  T *x;
+  const T *b;
+  T *r, *z, *q, *pa, *pb, *P;
+  T *r0, *s, *shat, *phat, *t, *va, *vb;  // BiCGSTAB
+  SolveState *st;
+  double *rhist;
+  int64_t rhist_len;
+  int64_t row0, n, ntiles;
+};
+
+struct TileRef {
+  int32_t slot, tile;
+};
+
+enum { kMethodCG = 0, kMethodBiCGSTAB = 1 };
+
+template <class T>
+void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
+                          int32_t *stopped, int lds_entries);
+template <class T>
+void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
+                                int64_t nlaunch, int32_t *stopped, int lds_entries);
+template <class T> void batch_launch_bcg_init(Ctx *c, SolveState *st, int32_t *stopped);
+// which: 0 <z,r>, 1 <b,b>, 2 <r,r>; result into the SolveState field at
+// byte offset field_off (scale != 0: scale * sqrt(result))
+template <class T>
+void batch_launch_sys_dot(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
+                          int which, int field_off, double scale);
+
+// device memory pool with reuse by size (no hipMalloc on the steady path)
+class DevicePool {
+ public:
+  explicit DevicePool(size_t budget) : budget_(budget) {}
+  ~DevicePool();
+  void *get(size_t bytes);  // nullptr when the budget would be exceeded
+  void put(void *p);
+  size_t in_use() const { return in_use_; }
+  size_t budget() const { return budget_; }
+  static size_t round(size_t bytes);
+
+ private:
+  size_t budget_, in_use_ = 0, reserved_ = 0;
+  std::vector<std::pair<size_t, void *>> free_;
+  std::vector<std::pair<void *, size_t>> live_;
+};
+
+struct SolveCfg {
+  int method = kMethodCG;
+  int precond = MGX_PRECOND_JACOBI;
+  double tol = 1e-8;  // absolute, or relative to ||b|| when tol_relative
+  int tol_relative = 0;
+  int64_t miniter = 0, maxiter = INT64_MAX / 4;
+  double *rhist = nullptr;  // device array or nullptr
+  int64_t rhist_len = 0;
+};
+
+// A set of systems iterated together.  add() uploads nothing: the matrix and
+// x/b are device resident; it allocates the work vectors from the pool and
+// queues the set-up kernels.  step() queues `iters` fused iterations of all
+// running systems; poll() reads which systems have stopped.
+template <class T> class Engine {
+ public:
+  Engine(Ctx *c, DevicePool *pool, int capacity);
+  ~Engine();
+  // returns the slot, or -1 when the pool budget or capacity is exhausted
+  int add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cfg);
+  size_t bytes_needed(const Csr *A, int method) const;
+  void step(int iters);
+  int running() const { return host_running_; }
+  // synchronises; returns slots that stopped since the last poll
+  std::vector<int> poll();
+  SolveState state(int slot);  // valid after poll() reported the slot
+  void release(int slot);
+  int active() const { return (int)active_.size(); }
+
+ private:
+  void rebuild_lists();
+  void launch_iters(int iters);
+  hipGraphExec_t gexec_ = nullptr;
+  int giters_ = 0;
+  bool gstale_ = true;
+  Ctx *c_;
+  DevicePool *pool_;
+  int cap_;
+  SysDesc<T> *d_sys_ = nullptr;
+  SolveState *d_states_ = nullptr;
+  int32_t *d_stopped_ = nullptr;
+  int32_t host_stopped_ = 0;
+  TileRef *d_tiles_[2] = {nullptr, nullptr};
+  int64_t tiles_cap_[2] = {0, 0};
+  int64_t ntiles_[2] = {0, 0};
+  bool dirty_ = true;
+  std::vector<SysDesc<T>> h_sys_;
+  std::vector<int> method_;
+  std::vector<int> free_slots_;
+  std::vector<int> active_;
+  std::vector<std::vector<void *>> bufs_;
+  std::vector<SolveState> h_states_;
+  std::vector<char> reported_;
+  int host_running_ = 0;
+  int lds_entries_ = 0;
+};
+
+// the context's pool and single-solve engines (created on first use, freed
+// by ctx_free_engines at mgx_finalize)
+DevicePool &ctx_pool(Ctx *c);
+template <class T> Engine<T> &ctx_engine(Ctx *c);
+void ctx_free_engines(Ctx *c);
+
+}  // namespace mgx

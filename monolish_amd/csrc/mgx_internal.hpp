@@ -107,10 +107,6 @@ struct Csr {
   int32_t fused_nnz = 0;     // entry budget of a chunk
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
   Csr *trans = nullptr;      // A^T, built on first matvec_T
-  // replayable graph of fused solver iterations, reused while the kernel
-  // arguments (buffers, tolerance-independent) stay the same
-  hipGraphExec_t graph = nullptr;
-  std::string graph_key;
   size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
 };
 
@@ -128,6 +124,11 @@ struct Ctx {
   hipEvent_t ev = nullptr;
   // SpMV tuning
   int spmv_rows = 256, spmv_nnz = 2048, spmv_long = 256, xcd_remap = 1;
+  int use_graphs = 0;  // replay fused iterations from a captured hipGraph
+  // fused-engine state owned by the context (engine.hip)
+  void *pool = nullptr;   // DevicePool for solver work vectors
+  void *eng64 = nullptr;  // Engine<double> for single fused solves
+  void *eng32 = nullptr;  // Engine<float>
   // workspace for solver vectors
   std::vector<std::pair<void *, size_t>> ws_free;
   std::vector<std::pair<void *, size_t>> ws_used;

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <thread>
 
+#include "mgx_batch.hpp"
 #include "mgx_internal.hpp"
 
 namespace mgx {
@@ -126,6 +127,18 @@ static bool overlap(const Vec *a, const Vec *b) {
   return a->n > 0 && b->n > 0 && a0 < b1 && b0 < a1;
 }
 
+// col_ind / val with kEntryPad zeroed entries after the last one: the SpMV
+// reads whole 16-byte vectors, which may extend past the final entry
+constexpr int64_t kEntryPad = 4;
+static void alloc_entries(Csr *A, int64_t nnz) {
+  const size_t cnt = (size_t)(std::max<int64_t>(nnz, 0) + kEntryPad);
+  A->ci = (int32_t *)dev_alloc(cnt * 4);
+  A->val = dev_alloc(cnt * A->elem());
+  MGX_HIP(hipMemsetAsync(A->ci + nnz, 0, kEntryPad * 4, A->ctx->stream));
+  MGX_HIP(hipMemsetAsync((char *)A->val + nnz * A->elem(), 0, kEntryPad * A->elem(),
+                         A->ctx->stream));
+}
+
 // build A^T with a stable counting sort by column: row c of A^T lists the
 // entries of column c in ascending original row, which is the order the
 // reference's serial scatter adds them (crs_matvec.hpp:94-97)
@@ -158,8 +171,7 @@ static Csr *build_transpose(Csr *A) {
   T->nnz = used;
   T->dtype = A->dtype;
   T->rp = (int32_t *)dev_alloc(trp.size() * 4);
-  T->ci = (int32_t *)dev_alloc(std::max<size_t>(tci.size(), 1) * 4);
-  T->val = dev_alloc(std::max<size_t>(tval.size(), 1));
+  alloc_entries(T, used);
   MGX_HIP(hipMemcpy(T->rp, trp.data(), trp.size() * 4, hipMemcpyHostToDevice));
   if (used) {
     MGX_HIP(hipMemcpy(T->ci, tci.data(), tci.size() * 4, hipMemcpyHostToDevice));
@@ -179,7 +191,6 @@ static void csr_free(Csr *A) {
   dev_free(A->plan.d_blocks);
   dev_free(A->d_chunks);
   dev_free(A->d_tile_chunk);
-  if (A->graph) (void)hipGraphExecDestroy(A->graph);
   if (A->trans) csr_free(A->trans);
   delete static_cast<mgx_csr *>(A);
 }
@@ -239,6 +250,7 @@ int mgx_finalize(mgx_ctx *ctx) {
     if (!ctx) return MGX_OK;
     activate(ctx);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx_free_engines(ctx);
     for (auto &e : ctx->ws_used) dev_free(e.first);
     for (auto &e : ctx->ws_free) dev_free(e.first);
     dev_free(ctx->d_partials);
@@ -289,6 +301,14 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
     ctx->spmv_nnz = nnz_per_block;
     ctx->spmv_long = long_row;
     ctx->xcd_remap = xcd_remap ? 1 : 0;
+    return MGX_OK;
+  });
+}
+
+int mgx_set_graphs(mgx_ctx *ctx, int enable) {
+  return guard([&] {
+    MGX_REQUIRE(ctx, "mgx_set_graphs: NULL ctx");
+    ctx->use_graphs = enable ? 1 : 0;
     return MGX_OK;
   });
 }
@@ -437,8 +457,7 @@ int mgx_csr_create(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
     A->dtype = dtype;
     try {
       A->rp = (int32_t *)dev_alloc((size_t)(rows + 1) * 4);
-      A->ci = (int32_t *)dev_alloc((size_t)std::max<int64_t>(nnz, 1) * 4);
-      A->val = dev_alloc((size_t)std::max<int64_t>(nnz, 1) * A->elem());
+      alloc_entries(A, nnz);
       MGX_HIP(hipMemcpy(A->rp, rp.data(), (size_t)(rows + 1) * 4, hipMemcpyHostToDevice));
       if (nnz > 0) {
         MGX_HIP(hipMemcpy(A->ci, col_ind, (size_t)nnz * 4, hipMemcpyHostToDevice));

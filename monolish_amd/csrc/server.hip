@@ -1,0 +1,407 @@
+// server.hip -- batched solves over many systems (mgx_batch_solve) and the
+// solve server: a request loop that keeps matrices and Krylov state resident
+// in HBM, admits arriving systems in order while slots and the HBM budget
+// allow, iterates every resident system together in fused launches
+// (continuous batching: a system joins or leaves between launches) and
+// reports latency percentiles beside throughput.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "mgx_batch.hpp"
+
+namespace mgx {
+namespace {
+
+SolveCfg cfg_of(const mgx_solve_params *p, int method) {
+  SolveCfg cfg;
+  cfg.method = method;
+  cfg.precond = p->precond;
+  cfg.tol = p->tol;
+  cfg.tol_relative = p->tol_is_relative;
+  cfg.miniter = p->miniter;
+  cfg.maxiter = p->maxiter <= 0 ? INT64_MAX / 4 : p->maxiter;
+  return cfg;
+}
+
+template <class T>
+int batch_solve_t(Ctx *c, int method, int64_t nsys, mgx_csr *const *A, mgx_vec *const *x,
+                  mgx_vec *const *b, const mgx_solve_params *p, mgx_solve_result *res) {
+  const double t0 = seconds_now();
+  Engine<T> eng(c, &ctx_pool(c), (int)nsys);
+  const SolveCfg cfg = cfg_of(p, method);
+  std::vector<int> slot_of(nsys, -1);
+  std::vector<int64_t> sys_of(nsys, -1);
+  for (int64_t i = 0; i < nsys; i++) {
+    const int s = eng.add(A[i], A[i]->val, (T *)x[i]->d, (const T *)b[i]->d, cfg);
+    if (s < 0) {
+      set_error("mgx_batch_solve: no device memory for system %lld", (long long)i);
+      throw Error{MGX_ERR_NOMEM};
+    }
+    slot_of[i] = s;
+    sys_of[s] = i;
+  }
+  const int every = p->check_every > 0 ? p->check_every : 8;
+  while (eng.running() > 0) {
+    eng.step(every);
+    for (int s : eng.poll()) {
+      const SolveState st = eng.state(s);
+      mgx_solve_result &r = res[sys_of[s]];
+      r.status = st.status;
+      r.final_iter = st.iter;
+      r.final_resid = st.resid;
+      r.seconds = seconds_now() - t0;
+      r.inner_iter = r.outer_iter = 0;
+    }
+  }
+  for (int64_t i = 0; i < nsys; i++) eng.release(slot_of[i]);
+  return MGX_OK;
+}
+
+// ------------------------------------------------------------- server ----
+struct Request {
+  int64_t id = 0;
+  int method = kMethodCG;
+  int64_t n = 0, nnz = 0;
+  std::vector<int32_t> rp, ci;
+  std::vector<double> val, b, x0;
+  SolveCfg cfg;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  // resident state
+  Csr *A = nullptr;
+  double *x = nullptr, *bd = nullptr;
+  size_t bytes = 0;
+  int slot = -1;
+  // outcome
+  bool done = false;
+  mgx_solve_result res{};
+  std::vector<double> x_out;
+};
+
+}  // namespace
+
+struct Server {
+  Ctx *ctx;
+  mgx_server_config cfg;
+  std::unique_ptr<DevicePool> pool;
+  std::unique_ptr<Engine<double>> eng;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::unordered_map<int64_t, std::shared_ptr<Request>> all;
+  std::vector<std::shared_ptr<Request>> by_slot;
+  std::thread worker;
+  bool stop = false;
+  int64_t next_id = 1, submitted = 0, completed = 0;
+  size_t resident = 0, peak = 0, budget = 0;
+  std::vector<double> lat, qwait;
+  double first_submit = -1, last_done = 0;
+  std::string fatal;
+
+  size_t request_bytes(const Request &r) const {
+    const size_t csr = (size_t)(r.n + 1) * 4 + (size_t)(r.nnz + 4) * 12 +
+                       (size_t)(r.n / 256 + 2) * 4 * 2;
+    Csr probe;
+    probe.rows = probe.cols = r.n;
+    return csr + (size_t)r.n * 16 + eng->bytes_needed(&probe, r.method);
+  }
+
+  void upload(Request &r) {
+    mgx_csr *A = nullptr;
+    int rc = mgx_csr_create(static_cast<mgx_ctx *>(ctx), r.n, r.n, r.nnz, r.rp.data(),
+                            r.ci.data(), r.val.data(), MGX_F64, &A);
+    if (rc != MGX_OK) throw Error{rc};
+    r.A = A;
+    r.x = (double *)dev_alloc((size_t)r.n * 8);
+    r.bd = (double *)dev_alloc((size_t)r.n * 8);
+    if (!r.x0.empty())
+      MGX_HIP(hipMemcpy(r.x, r.x0.data(), (size_t)r.n * 8, hipMemcpyHostToDevice));
+    else
+      MGX_HIP(hipMemset(r.x, 0, (size_t)r.n * 8));
+    MGX_HIP(hipMemcpy(r.bd, r.b.data(), (size_t)r.n * 8, hipMemcpyHostToDevice));
+    // host copies are no longer needed
+    std::vector<int32_t>().swap(r.rp);
+    std::vector<int32_t>().swap(r.ci);
+    std::vector<double>().swap(r.val);
+    std::vector<double>().swap(r.b);
+    std::vector<double>().swap(r.x0);
+  }
+
+  void free_resident(Request &r) {
+    if (r.A) (void)mgx_csr_destroy(static_cast<mgx_csr *>(r.A));
+    dev_free(r.x);
+    dev_free(r.bd);
+    r.A = nullptr;
+    r.x = r.bd = nullptr;
+  }
+
+  void finish(const std::shared_ptr<Request> &r, int status) {
+    std::lock_guard<std::mutex> lk(mu);
+    r->t_done = seconds_now();
+    r->res.status = status;
+    r->res.seconds = r->t_done - r->t_submit;
+    r->done = true;
+    completed++;
+    lat.push_back(r->t_done - r->t_submit);
+    qwait.push_back(r->t_admit > 0 ? r->t_admit - r->t_submit : 0);
+    last_done = r->t_done;
+    cv_done.notify_all();
+  }
+
+  void loop() {
+    activate(ctx);
+    for (;;) {
+      // admission, in arrival order, while slots and the budget allow
+      std::vector<std::shared_ptr<Request>> admit;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        while (!queue.empty() && eng->active() + (int)admit.size() < cfg.max_active) {
+          auto r = queue.front();
+          if (r->bytes > budget) {  // can never fit
+            queue.pop_front();
+            lk.unlock();
+            r->res.status = MGX_ERR_BUDGET;
+            finish(r, MGX_ERR_BUDGET);
+            lk.lock();
+            continue;
+          }
+          if (resident + r->bytes > budget) break;
+          queue.pop_front();
+          resident += r->bytes;
+          peak = std::max(peak, resident);
+          admit.push_back(r);
+        }
+        if (admit.empty() && eng->active() == 0) {
+          if (stop && queue.empty()) return;
+          cv_work.wait_for(lk, std::chrono::milliseconds(2));
+          continue;
+        }
+      }
+      for (auto &r : admit) {
+        try {
+          upload(*r);
+          r->slot = eng->add(r->A, r->A->val, r->x, r->bd, r->cfg);
+          if (r->slot < 0) throw Error{MGX_ERR_BUDGET};
+          r->t_admit = seconds_now();
+          by_slot[r->slot] = r;
+        } catch (const Error &e) {
+          free_resident(*r);
+          {
+            std::lock_guard<std::mutex> lk(mu);
+            resident -= r->bytes;
+          }
+          finish(r, e.code);
+        }
+      }
+      if (eng->active() == 0) continue;
+      eng->step(cfg.iters_per_poll);
+      for (int s : eng->poll()) {
+        auto r = by_slot[s];
+        const SolveState st = eng->state(s);
+        r->res.final_iter = st.iter;
+        r->res.final_resid = st.resid;
+        r->x_out.resize(r->n);
+        MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
+        eng->release(s);
+        by_slot[s].reset();
+        free_resident(*r);
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          resident -= r->bytes;
+        }
+        finish(r, st.status);
+      }
+    }
+  }
+
+  void run() {
+    try {
+      loop();
+    } catch (const Error &) {
+      std::lock_guard<std::mutex> lk(mu);
+      fatal = mgx_last_error();
+      stop = true;
+      cv_done.notify_all();
+    }
+  }
+};
+
+double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  size_t k = (size_t)std::ceil(q * (double)v.size());
+  if (k == 0) k = 1;
+  return v[std::min(k, v.size()) - 1];
+}
+
+}  // namespace mgx
+
+struct mgx_server : mgx::Server {};
+
+using namespace mgx;
+
+extern "C" {
+
+int mgx_batch_solve(mgx_ctx *ctx, int method, int64_t nsys, mgx_csr *const *A,
+                    mgx_vec *const *x, mgx_vec *const *b, const mgx_solve_params *p,
+                    mgx_solve_result *results) {
+  return guard([&] {
+    MGX_REQUIRE(ctx && p && results && (nsys == 0 || (A && x && b)),
+                "mgx_batch_solve: NULL argument");
+    MGX_REQUIRE(method == MGX_METHOD_CG || method == MGX_METHOD_BICGSTAB,
+                "mgx_batch_solve: unknown method %d", method);
+    MGX_REQUIRE(nsys >= 0 && nsys < (1 << 30), "mgx_batch_solve: bad system count");
+    if (nsys == 0) return MGX_OK;
+    const mgx_dtype dt = A[0]->dtype;
+    for (int64_t i = 0; i < nsys; i++) {
+      MGX_REQUIRE(A[i] && x[i] && b[i], "mgx_batch_solve: NULL system %lld", (long long)i);
+      MGX_REQUIRE(A[i]->ctx == ctx && x[i]->ctx == ctx && b[i]->ctx == ctx,
+                  "mgx_batch_solve: context mismatch in system %lld", (long long)i);
+      MGX_REQUIRE(A[i]->rows == A[i]->cols && A[i]->rows > 0,
+                  "mgx_batch_solve: system %lld is not square and non-empty", (long long)i);
+      MGX_REQUIRE(A[i]->dtype == dt && x[i]->dtype == dt && b[i]->dtype == dt,
+                  "mgx_batch_solve: dtype mismatch in system %lld", (long long)i);
+      MGX_REQUIRE(x[i]->n == A[i]->rows && b[i]->n == A[i]->rows,
+                  "mgx_batch_solve: size mismatch in system %lld", (long long)i);
+    }
+    activate(ctx);
+    if (dt == MGX_F64) return batch_solve_t<double>(ctx, method, nsys, A, x, b, p, results);
+    return batch_solve_t<float>(ctx, method, nsys, A, x, b, p, results);
+  });
+}
+
+int mgx_server_create(mgx_ctx *ctx, const mgx_server_config *cfg, mgx_server **out) {
+  return guard([&] {
+    MGX_REQUIRE(ctx && out, "mgx_server_create: NULL argument");
+    activate(ctx);
+    auto *s = new mgx_server();
+    s->ctx = ctx;
+    std::memset(&s->cfg, 0, sizeof s->cfg);
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.max_active <= 0) s->cfg.max_active = 1024;
+    if (s->cfg.iters_per_poll <= 0) s->cfg.iters_per_poll = 8;
+    size_t freeb = 0, total = 0;
+    MGX_HIP(hipMemGetInfo(&freeb, &total));
+    s->budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : freeb / 4 * 3;
+    s->pool.reset(new DevicePool(s->budget));
+    s->eng.reset(new Engine<double>(ctx, s->pool.get(), s->cfg.max_active));
+    s->by_slot.resize(s->cfg.max_active);
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return MGX_OK;
+  });
+}
+
+int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz, const int32_t *row_ptr,
+                      const int32_t *col_ind, const double *val, const double *b,
+                      const double *x0, const mgx_solve_params *p, int64_t *ticket) {
+  return guard([&] {
+    MGX_REQUIRE(s && row_ptr && b && p && ticket && (nnz == 0 || (col_ind && val)),
+                "mgx_server_submit: NULL argument");
+    MGX_REQUIRE(method == MGX_METHOD_CG || method == MGX_METHOD_BICGSTAB,
+                "mgx_server_submit: unknown method %d", method);
+    MGX_REQUIRE(n > 0 && nnz >= 0, "mgx_server_submit: empty system");
+    MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
+                "mgx_server_submit: unknown preconditioner");
+    auto r = std::make_shared<Request>();
+    r->method = method;
+    r->n = n;
+    r->nnz = nnz;
+    r->rp.assign(row_ptr, row_ptr + n + 1);
+    r->ci.assign(col_ind, col_ind + nnz);
+    r->val.assign(val, val + nnz);
+    r->b.assign(b, b + n);
+    if (x0) r->x0.assign(x0, x0 + n);
+    r->cfg = cfg_of(p, method);
+    r->bytes = s->request_bytes(*r);
+    std::lock_guard<std::mutex> lk(s->mu);
+    MGX_REQUIRE(!s->stop, "mgx_server_submit: server stopped (%s)", s->fatal.c_str());
+    r->id = s->next_id++;
+    r->t_submit = seconds_now();
+    if (s->first_submit < 0) s->first_submit = r->t_submit;
+    s->submitted++;
+    s->all[r->id] = r;
+    s->queue.push_back(r);
+    s->cv_work.notify_one();
+    *ticket = r->id;
+    return MGX_OK;
+  });
+}
+
+int mgx_server_wait(mgx_server *s, int64_t ticket, double timeout_s, mgx_solve_result *res,
+                    double *x) {
+  return guard([&] {
+    MGX_REQUIRE(s && res, "mgx_server_wait: NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    MGX_REQUIRE(it != s->all.end(), "mgx_server_wait: unknown ticket %lld", (long long)ticket);
+    auto r = it->second;
+    auto ready = [&] { return r->done || (s->stop && !s->fatal.empty()); };
+    if (timeout_s < 0)
+      s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+      return MGX_ERR_NOT_FOUND;
+    if (!r->done) {
+      set_error("mgx_server_wait: server failed: %s", s->fatal.c_str());
+      return MGX_ERR_HIP;
+    }
+    *res = r->res;
+    if (x && !r->x_out.empty()) std::memcpy(x, r->x_out.data(), (size_t)r->n * 8);
+    s->all.erase(it);
+    return MGX_OK;
+  });
+}
+
+int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out) {
+  return guard([&] {
+    MGX_REQUIRE(s && out, "mgx_server_stats_get: NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->queued = (int64_t)s->queue.size();
+    out->active = s->submitted - s->completed - out->queued;
+    out->p50_latency_s = percentile(s->lat, 0.50);
+    out->p99_latency_s = percentile(s->lat, 0.99);
+    double sum = 0, mx = 0, qs = 0;
+    for (double v : s->lat) {
+      sum += v;
+      mx = std::max(mx, v);
+    }
+    for (double v : s->qwait) qs += v;
+    out->mean_latency_s = s->lat.empty() ? 0 : sum / (double)s->lat.size();
+    out->max_latency_s = mx;
+    out->mean_queue_s = s->qwait.empty() ? 0 : qs / (double)s->qwait.size();
+    const double span = s->last_done - s->first_submit;
+    out->solves_per_s = span > 0 ? (double)s->completed / span : 0;
+    out->peak_bytes = s->peak;
+    return MGX_OK;
+  });
+}
+
+int mgx_server_destroy(mgx_server *s) {
+  return guard([&] {
+    if (!s) return MGX_OK;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+      s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    for (auto &r : s->by_slot)
+      if (r) s->free_resident(*r);
+    s->eng.reset();
+    s->pool.reset();
+    delete s;
+    return MGX_OK;
+  });
+}
+
+}  // extern "C"

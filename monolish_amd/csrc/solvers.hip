@@ -1,194 +1,25 @@
-// krylov.hip -- CG, BiCGSTAB and Jacobi solve loops (src/equation/cg.cpp,
-// bicgstab.cpp, jacobi.cpp) on gfx950.
+// solvers.hip -- CG, BiCGSTAB and Jacobi solve loops (src/equation/cg.cpp,
+// bicgstab.cpp, jacobi.cpp) and mixed-precision refinement on gfx950.
 //
-// MGX_MODE_REFERENCE runs the reference's call sequence one kernel per BLAS
-// call with every dot/nrm2 returned to the host (the "plain" GPU solver).
-// MGX_MODE_FUSED runs Jacobi-PCG as two kernels per iteration with all
-// scalars on the device:
-//   cg_spmv:   p = z + beta p is formed at each gather (p never makes an extra
-//              pass), q = A p with the CRS-order row sums, the new p is stored
-//              for the block's own rows, and <p,q> is reduced in tile order;
-//              the last block computes alpha = <z,r> / <p,q>.
-//   cg_update: x += alpha p, r -= alpha q, z = M r, with <z,r> and <r,r> in
-//              tile order; the last block forms beta, the residual and the
-//              stopping decision.
-// Every value is rounded exactly as in the reference loop (same operations,
-// same order, tile-order dots), so REFERENCE and FUSED give identical bits.
-// Kernels exit at once when the solve is no longer running, so the host can
-// queue several iterations between status checks.
+// MGX_MODE_REFERENCE runs the reference's call sequence, one kernel per BLAS
+// call, every dot/nrm2 returned to the host (the plain GPU solver).
+// MGX_MODE_FUSED runs CG and BiCGSTAB on the fused multi-system engine
+// (batch.hip) as a batch of one: CG takes two kernels per iteration and
+// BiCGSTAB three, all scalars stay on the device, and the host checks the
+// solve status every check_every iterations.  Every value is rounded exactly
+// as in the reference loop (same operations, same order, tile-order dots), so
+// both modes give identical bits.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 
+#include "mgx_batch.hpp"
 #include "mgx_device.hpp"
 
 namespace mgx {
 
 namespace {
-
-template <class T> __device__ __forceinline__ T dsqrt(T v);
-template <> __device__ __forceinline__ double dsqrt(double v) { return __builtin_sqrt(v); }
-template <> __device__ __forceinline__ float dsqrt(float v) { return __builtin_sqrtf(v); }
-
-template <class T> struct CgArgs {
-  int64_t n, ntiles;
-  const int32_t *rp, *ci;
-  const T *val;
-  const int32_t *chunks, *tile_chunk;
-  const T *M;  // Jacobi 1/diag, or nullptr for z = r
-  T *x, *r, *z, *q, *p_old, *p_new, *P;
-  SolveState *st;
-  double *rhist;
-  int64_t rhist_len;
-  int remap;
-};
-
-template <class T>
-__device__ __forceinline__ T p_value(const CgArgs<T> &a, int64_t c, bool first,
-                                     T beta) {
-  if (first) return a.z[c];
-  T bp = beta * a.p_old[c];
-  return a.z[c] + bp;
-}
-
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *prod = reinterpret_cast<T *>(smem_raw);
-  __shared__ T qloc[kTile];
-  __shared__ T sh[8];
-  __shared__ int flag;
-  SolveState *st = a.st;
-  if (st->status != kStatRunning) return;
-  const int64_t k = xcd_block(a.remap);
-  if (k >= a.ntiles) return;
-  const bool first = st->first != 0;
-  const T beta = (T)st->beta;
-  const int64_t R0 = k * kTile;
-  const int64_t R1 = R0 + kTile < a.n ? R0 + kTile : a.n;
-  const int t = threadIdx.x;
-  for (int32_t ck = a.tile_chunk[k]; ck < a.tile_chunk[k + 1]; ck++) {
-    const int32_t d0 = a.chunks[ck], d1 = a.chunks[ck + 1];
-    const int32_t ra = d0 & 0x7fffffff, rb = d1 & 0x7fffffff;
-    if (d0 < 0) {
-      const int32_t e0 = a.rp[ra], e1 = a.rp[ra + 1];
-      T s = 0;
-      for (int32_t e = e0 + t; e < e1; e += kBlock) {
-        T pr = a.val[e] * p_value(a, a.ci[e], first, beta);
-        s = s + pr;
-      }
-      T res = block_tree(s, sh);
-      if (t == 0) {
-        qloc[ra - R0] = res;
-        a.q[ra] = res;
-      }
-      __syncthreads();
-      continue;
-    }
-    const int32_t e0 = a.rp[ra], e1 = a.rp[rb];
-    for (int32_t e = e0 + t; e < e1; e += kBlock) {
-      T v = __builtin_nontemporal_load(&a.val[e]);
-      int32_t c = __builtin_nontemporal_load(&a.ci[e]);
-      prod[e - e0] = v * p_value(a, c, first, beta);
-    }
-    __syncthreads();
-    const int32_t row = ra + t;
-    if (row < rb) {
-      const int32_t lo = a.rp[row] - e0, hi = a.rp[row + 1] - e0;
-      T s = 0;
-      for (int32_t j = lo; j < hi; j++) s = s + prod[j];
-      qloc[row - R0] = s;
-      a.q[row] = s;
-    }
-    __syncthreads();
-  }
-  // new p for the tile's own rows and the tile partial of <p, q>
-  const int64_t i0 = R0 + 2 * t;
-  T acc0 = 0, acc1 = 0;
-  if (i0 < R1) {
-    T pn = p_value(a, i0, first, beta);
-    a.p_new[i0] = pn;
-    acc0 = pn * qloc[2 * t];
-  }
-  if (i0 + 1 < R1) {
-    T pn = p_value(a, i0 + 1, first, beta);
-    a.p_new[i0 + 1] = pn;
-    acc1 = pn * qloc[2 * t + 1];
-  }
-  T s = block_tree(acc0 + acc1, sh);
-  if (t == 0) publish_partial(a.P, k, s);
-  if (last_block_arrives(&st->ticket[0], (uint32_t)a.ntiles, &flag)) {
-    T pq = reduce_partials(a.P, a.ntiles, sh);
-    if (t == 0) {
-      T zr = (T)st->zr;
-      T alpha = zr / pq;
-      st->pq = (double)pq;
-      st->alpha = (double)alpha;
-    }
-  }
-}
-
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a) {
-  __shared__ T sh[8];
-  __shared__ int flag;
-  SolveState *st = a.st;
-  if (st->status != kStatRunning) return;
-  const int64_t k = xcd_block(a.remap);
-  if (k >= a.ntiles) return;
-  const T alpha = (T)st->alpha;
-  const T nalpha = -alpha;
-  const int t = threadIdx.x;
-  const int64_t i0 = k * kTile + 2 * t;
-  T zr2[2] = {0, 0}, rr2[2] = {0, 0};
-#pragma unroll
-  for (int e = 0; e < 2; e++) {
-    const int64_t i = i0 + e;
-    if (i < a.n) {
-      T pv = a.p_new[i], qv = a.q[i];
-      T t1 = alpha * pv;
-      a.x[i] = t1 + a.x[i];
-      T t2 = nalpha * qv;
-      T rv = t2 + a.r[i];
-      a.r[i] = rv;
-      T zv = a.M ? a.M[i] * rv : rv;
-      a.z[i] = zv;
-      zr2[e] = zv * rv;
-      rr2[e] = rv * rv;
-    }
-  }
-  T zr_a = zr2[0] + zr2[1];
-  T rr_a = rr2[0] + rr2[1];
-  block_tree2(zr_a, rr_a, sh);
-  if (t == 0) {
-    publish_partial(a.P, k, zr_a);
-    publish_partial(a.P, a.ntiles + k, rr_a);
-  }
-  if (last_block_arrives(&st->ticket[1], (uint32_t)a.ntiles, &flag)) {
-    T zr_new = reduce_partials(a.P, a.ntiles, sh);
-    __syncthreads();
-    T rr = reduce_partials(a.P + a.ntiles, a.ntiles, sh);
-    if (t == 0) {
-      const T zr_old = (T)st->zr;
-      const T beta = zr_new / zr_old;
-      const T resid = dsqrt(rr);
-      const int64_t it = st->iter;
-      if (a.rhist && it < a.rhist_len) a.rhist[it] = (double)resid;
-      st->beta = (double)beta;
-      st->zr = (double)zr_new;
-      st->rr = (double)rr;
-      st->resid = (double)resid;
-      st->first = 0;
-      st->iter = it + 1;
-      if ((double)resid < st->tol && st->miniter <= it + 1)
-        st->status = MGX_OK;
-      else if (resid != resid)
-        st->status = MGX_SOLVER_RESIDUAL_NAN;
-      else if (it + 1 >= st->maxiter)
-        st->status = MGX_SOLVER_MAXITER;
-    }
-  }
-}
 
 template <class T> T *tp(void *p) { return static_cast<T *>(p); }
 template <class T> const T *tp(const void *p) { return static_cast<const T *>(p); }
@@ -325,118 +156,53 @@ SolveOut cg_reference(Csr *A, const void *val, T *x, const T *b, int precond,
   return o;
 }
 
-// fused Jacobi-PCG; identical arithmetic to cg_reference
+// one-system run of the fused engine (CG or BiCGSTAB); identical arithmetic
+// to the reference-shaped loops
 template <class T>
-SolveOut cg_fused(Csr *A, const void *val, T *x, const T *b, int precond,
-                  double tol, int64_t miniter, int64_t maxiter, int check_every,
-                  double *rhist_host, int64_t rhist_len) {
+SolveOut fused_solve(int method, Csr *A, const void *val, T *x, const T *b, int precond,
+                     double tol, int64_t miniter, int64_t maxiter, int check_every,
+                     double *rhist_host, int64_t rhist_len) {
   Ctx *c = A->ctx;
-  const int64_t n = A->rows;
-  const mgx_dtype dt = DT<T>::v;
-  if (n == 0 || maxiter <= 0)
-    return cg_reference<T>(A, val, x, b, precond, tol, miniter, maxiter,
-                           rhist_host, rhist_len);
-  fused_plan_build(A);
-  const int64_t nt = ntiles_of(n);
-  const size_t bytes = (size_t)n * sizeof(T);
-  T *r = (T *)ws_get(c, bytes), *z = (T *)ws_get(c, bytes),
-    *q = (T *)ws_get(c, bytes), *pa = (T *)ws_get(c, bytes),
-    *pb = (T *)ws_get(c, bytes);
-  T *P = (T *)ws_get(c, (size_t)2 * nt * sizeof(T));
-  SolveState *st = (SolveState *)ws_get(c, sizeof(SolveState));
+  Engine<T> &eng = ctx_engine<T>(c);
   double *rh = nullptr;
   const int64_t rlen = rhist_host ? std::min(rhist_len, maxiter) : 0;
   if (rlen > 0) rh = (double *)ws_get(c, (size_t)rlen * sizeof(double));
-  T *M = make_precond<T>(A, val, precond);
-  launch_fill(c, dt, pa, n, 0.0);
-  // r = b - A x; z = M r; <z, r> (cg.cpp:39-44)
-  launch_spmv(A, val, dt, x, q, 1, 0, false);
-  launch_vv(c, dt, 1, n, b, q, r);
-  apply_precond(c, M, n, r, z);
-  SolveState hs;
-  std::memset(&hs, 0, sizeof hs);
-  hs.tol = tol;
-  hs.miniter = miniter;
-  hs.maxiter = maxiter;
-  hs.status = kStatRunning;
-  hs.first = 1;
-  MGX_HIP(hipMemcpyAsync(st, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream));
-  launch_dot(c, dt, n, z, r, &st->zr, &st->ticket[2]);
-
-  CgArgs<T> ar;
-  std::memset(&ar, 0, sizeof ar);  // the raw bytes key the graph cache
-  ar.n = n;
-  ar.ntiles = nt;
-  ar.rp = A->rp;
-  ar.ci = A->ci;
-  ar.val = static_cast<const T *>(val);
-  ar.chunks = A->d_chunks;
-  ar.tile_chunk = A->d_tile_chunk;
-  ar.M = M;
-  ar.x = x;
-  ar.r = r;
-  ar.z = z;
-  ar.q = q;
-  ar.P = P;
-  ar.st = st;
-  ar.rhist = rh;
-  ar.rhist_len = rlen;
-  ar.remap = c->xcd_remap && nt >= 64;
-  const unsigned grid = ar.remap ? (unsigned)(((nt + 7) / 8) * 8) : (unsigned)nt;
-  const size_t lds = (size_t)A->fused_nnz * sizeof(T);
-  // iterations per replay: even, so every replay starts on the same p buffer
-  int every = check_every > 0 ? check_every : 32;
-  every += every & 1;
-  // the graph holds `every` iterations; kernels become no-ops once the solve
-  // stops, so replaying past convergence or maxiter changes nothing
-  ar.p_old = nullptr;
-  ar.p_new = nullptr;
-  std::string key(reinterpret_cast<const char *>(&ar), sizeof ar);
-  key.append(reinterpret_cast<const char *>(&pa), sizeof pa);
-  key.append(reinterpret_cast<const char *>(&pb), sizeof pb);
-  key.append(reinterpret_cast<const char *>(&every), sizeof every);
-  key.append(reinterpret_cast<const char *>(&grid), sizeof grid);
-  key.push_back((char)sizeof(T));
-  if (!A->graph || A->graph_key != key) {
-    if (A->graph) {
-      MGX_HIP(hipGraphExecDestroy(A->graph));
-      A->graph = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    MGX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < every; i++) {
-      ar.p_old = (i & 1) ? pb : pa;
-      ar.p_new = (i & 1) ? pa : pb;
-      k_cg_spmv<T><<<grid, kBlock, lds, c->stream>>>(ar);
-      k_cg_update<T><<<grid, kBlock, 0, c->stream>>>(ar);
-    }
-    hipError_t launch_err = hipGetLastError();
-    hipError_t cap_err = hipStreamEndCapture(c->stream, &g);
-    MGX_HIP(launch_err);
-    MGX_HIP(cap_err);
-    hipError_t inst = hipGraphInstantiate(&A->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MGX_HIP(inst);
-    A->graph_key = key;
+  SolveCfg cfg;
+  cfg.method = method;
+  cfg.precond = precond;
+  cfg.tol = tol;
+  cfg.miniter = miniter;
+  cfg.maxiter = maxiter;
+  cfg.rhist = rh;
+  cfg.rhist_len = rlen;
+  const int slot = eng.add(A, val, x, b, cfg);
+  MGX_REQUIRE(slot >= 0, "fused solve: out of device memory for the work vectors");
+  const int every = check_every > 0 ? check_every : 32;
+  for (;;) {
+    eng.step(every);
+    if (!eng.poll().empty()) break;
   }
-  int32_t *h_status = reinterpret_cast<int32_t *>(c->h_pinned);
-  for (int64_t launched = 0; launched < maxiter; launched += every) {
-    MGX_HIP(hipGraphLaunch(A->graph, c->stream));
-    MGX_HIP(hipMemcpyAsync(h_status, &st->status, sizeof(int32_t),
-                           hipMemcpyDeviceToHost, c->stream));
-    MGX_HIP(hipStreamSynchronize(c->stream));
-    if (*h_status != kStatRunning) break;
-  }
-  MGX_HIP(hipMemcpy(&hs, st, sizeof hs, hipMemcpyDeviceToHost));
+  SolveState hs = eng.state(slot);
+  eng.release(slot);
   if (rlen > 0) {
     const int64_t got = std::min<int64_t>(hs.iter, rlen);
     if (got > 0) MGX_HIP(hipMemcpy(rhist_host, rh, (size_t)got * 8, hipMemcpyDeviceToHost));
   }
   SolveOut o;
-  o.status = hs.status == kStatRunning ? MGX_SOLVER_MAXITER : hs.status;
+  o.status = hs.status;
   o.iters = hs.iter;
   o.resid = hs.resid;
   return o;
+}
+
+template <class T>
+SolveOut cg_fused(Csr *A, const void *val, T *x, const T *b, int precond, double tol,
+                  int64_t miniter, int64_t maxiter, int check_every, double *rhist_host,
+                  int64_t rhist_len) {
+  if (A->rows == 0 || maxiter <= 0)
+    return cg_reference<T>(A, val, x, b, precond, tol, miniter, maxiter, rhist_host, rhist_len);
+  return fused_solve<T>(kMethodCG, A, val, x, b, precond, tol, miniter, maxiter, check_every,
+                        rhist_host, rhist_len);
 }
 
 // equation::BiCGSTAB, reference call sequence (bicgstab.cpp:27-134)
@@ -642,7 +408,9 @@ SolveOut ir_cg(Csr *A, double *x, const double *b, int precond, double thr,
   Ctx *c = A->ctx;
   const int64_t n = A->rows;
   if (!A->val32) {
-    A->val32 = (float *)dev_alloc((size_t)std::max<int64_t>(A->nnz, 1) * 4);
+    // padded like col_ind/val: vector loads may read past the last entry
+    A->val32 = (float *)dev_alloc((size_t)(A->nnz + 4) * 4);
+    MGX_HIP(hipMemsetAsync(A->val32 + A->nnz, 0, 16, c->stream));
     launch_convert(c, MGX_F64, MGX_F32, A->nnz, A->val, A->val32);
   }
   double *r = (double *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 8);
@@ -746,7 +514,16 @@ int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_param
     activate(A->ctx);
     const double thr = threshold(A->ctx, p, b);
     const int64_t mi = maxiter_of(p);
+    const bool fused = p->mode == MGX_MODE_FUSED && A->rows > 0 && mi > 0;
     return run_solver(A, res, [&] {
+      if (fused && A->dtype == MGX_F64)
+        return fused_solve<double>(kMethodBiCGSTAB, A, A->val, (double *)x->d,
+                                   (const double *)b->d, p->precond, thr, p->miniter, mi,
+                                   p->check_every, p->rhistory, p->rhistory_len);
+      if (fused)
+        return fused_solve<float>(kMethodBiCGSTAB, A, A->val, (float *)x->d,
+                                  (const float *)b->d, p->precond, thr, p->miniter, mi,
+                                  p->check_every, p->rhistory, p->rhistory_len);
       if (A->dtype == MGX_F64)
         return bicgstab_reference<double>(A, (double *)x->d, (const double *)b->d, p->precond,
                                           thr, p->miniter, mi, p->rhistory, p->rhistory_len);

@@ -62,26 +62,37 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
   }
   const int32_t e0 = rp[r0], e1 = rp[r1];
   {
-    T v[U];
-    int32_t c[U];
+    // 16-byte loads: W entries per lane per load, starting from e0 rounded
+    // down to a multiple of W (val/col_ind are padded by W entries, and
+    // entries outside [e0, e1) are never used)
+    constexpr int W = 16 / sizeof(T);
+    typedef T VT __attribute__((ext_vector_type(W)));
+    typedef int32_t VI __attribute__((ext_vector_type(W)));
+    const int32_t ea = e0 & ~(W - 1);
+    VT v[U];
+    VI c[U];
 #pragma unroll
     for (int k = 0; k < U; k++) {
-      const int32_t e = e0 + (int32_t)threadIdx.x + k * kBlock;
-      if (e < e1) {
-        v[k] = __builtin_nontemporal_load(&val[e]);
-        c[k] = __builtin_nontemporal_load(&ci[e]);
+      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+      if (base < e1) {
+        v[k] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(val + base));
+        c[k] = __builtin_nontemporal_load(reinterpret_cast<const VI *>(ci + base));
       }
     }
-    T xv[U];
+    T xv[U][W];
 #pragma unroll
     for (int k = 0; k < U; k++) {
-      const int32_t e = e0 + (int32_t)threadIdx.x + k * kBlock;
-      if (e < e1) xv[k] = x[c[k]];
+      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+#pragma unroll
+      for (int j = 0; j < W; j++)
+        if (base + j >= e0 && base + j < e1) xv[k][j] = x[c[k][j]];
     }
 #pragma unroll
     for (int k = 0; k < U; k++) {
-      const int32_t e = e0 + (int32_t)threadIdx.x + k * kBlock;
-      if (e < e1) prod[e - e0] = v[k] * xv[k];
+      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+#pragma unroll
+      for (int j = 0; j < W; j++)
+        if (base + j >= e0 && base + j < e1) prod[base + j - e0] = (T)v[k][j] * xv[k][j];
     }
   }
   __syncthreads();
@@ -174,18 +185,23 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
   const int remap = c->xcd_remap && nb >= 64;
   const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
   const size_t lds = (size_t)A->plan.nnz_max * sizeof(T);
-  const int u = (A->plan.nnz_max + kBlock - 1) / kBlock;  // <= 32
+  // vector loads per lane: cover nnz_max entries plus the alignment slack
+  constexpr int W = 16 / sizeof(T);
+  const int u = (A->plan.nnz_max + W - 1 + kBlock * W - 1) / (kBlock * W);  // <= 17
   const int32_t *B = A->plan.d_blocks;
-  if (u <= 2)
-    k_spmv<T, AB, 2><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
-  else if (u <= 4)
-    k_spmv<T, AB, 4><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
-  else if (u <= 8)
-    k_spmv<T, AB, 8><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
-  else if (u <= 16)
-    k_spmv<T, AB, 16><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
-  else
-    k_spmv<T, AB, 32><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, alpha, beta, remap);
+#define MGX_SPMV_U(UU)                                                          \
+  k_spmv<T, AB, UU><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
+                                                      Y, alpha, beta, remap)
+  if (u <= 1) MGX_SPMV_U(1);
+  else if (u <= 2) MGX_SPMV_U(2);
+  else if (u <= 3) MGX_SPMV_U(3);
+  else if (u <= 4) MGX_SPMV_U(4);
+  else if (u <= 5) MGX_SPMV_U(5);
+  else if (u <= 6) MGX_SPMV_U(6);
+  else if (u <= 8) MGX_SPMV_U(8);
+  else if (u <= 12) MGX_SPMV_U(12);
+  else MGX_SPMV_U(17);
+#undef MGX_SPMV_U
   MGX_HIP(hipGetLastError());
 }
 
