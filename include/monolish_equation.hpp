@@ -1,0 +1,245 @@
+// monolish_equation.hpp -- drop-in monolish::equation solvers for
+// matrix::CRS<T> on the device: none, Jacobi (solver and preconditioner), CG,
+// BiCGSTAB (include/monolish_equation.hpp:40-257 of monolish 0.17.3).
+//
+// With none / Jacobi / no preconditioner bound, CG and BiCGSTAB run the fused
+// device loop (mgx_cg_solve / mgx_bicgstab_solve); any other preconditioner
+// object runs the reference loop through its own callbacks with one device
+// BLAS call per step.  Both follow src/equation/{cg,bicgstab}.cpp step by
+// step, with the same return codes, stopping rule (absolute nrm2(r) < tol
+// after miniter) and final_iter / final_resid.
+#pragma once
+
+#include "monolish_blas.hpp"
+#include "monolish_solver.hpp"
+#include "monolish_vml.hpp"
+
+namespace monolish {
+namespace equation {
+
+namespace detail_eq {
+template <class MATRIX, class Float>
+void check_system(const MATRIX &A, const vector<Float> &x, const vector<Float> &b,
+                  const char *who) {
+  if (A.get_row() != A.get_col()) throw std::runtime_error("error, A.row != A.col");
+  if (A.get_device_mem_stat() != x.get_device_mem_stat() &&
+      A.get_device_mem_stat() != b.get_device_mem_stat())
+    throw std::runtime_error(
+        "error, A.get_device_mem_stat != x.get_device_mem_stat != b.get_device_mem_stat");
+  detail::need_device(A.get_device_mem_stat() && x.get_device_mem_stat() &&
+                          b.get_device_mem_stat(),
+                      who);
+}
+}  // namespace detail_eq
+
+// ------------------------------------------------------------------- none
+template <typename MATRIX, typename Float>
+class none : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  void create_precond(MATRIX &A) { (void)A; }
+  // none.cpp:40-46: z = r
+  void apply_precond(const vector<Float> &r, vector<Float> &z) { blas::copy(r, z); }
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    (void)A, (void)x, (void)b;
+    return MONOLISH_SOLVER_SUCCESS;
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::none"; }
+  [[nodiscard]] std::string solver_name() const { return "none"; }
+};
+
+// ----------------------------------------------------------------- Jacobi
+template <typename MATRIX, typename Float>
+class Jacobi : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  // jacobi.cpp:12-29: M = 1 / diag(A), resident when A is
+  void create_precond(MATRIX &A) {
+    if (A.get_row() != A.get_col()) throw std::runtime_error("error A.row != A.col");
+    auto &M = this->precond.M;
+    M.device_free();
+    M.resize(A.get_row());
+    if (A.get_device_mem_stat()) M.send();
+    A.diag(M);
+    vml::reciprocal(M, M);
+  }
+  // jacobi.cpp:53-58: z = M o r
+  void apply_precond(const vector<Float> &r, vector<Float> &z) { vml::mul(this->precond.M, r, z); }
+  // jacobi.cpp:83-150 (relative stopping test)
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    Logger::get_instance().solver_in(monolish_func);
+    int ret = 0;
+    if (this->lib == 0) {
+      detail_eq::check_system(A, x, b, "equation::Jacobi");
+      int kind = this->device_precond();
+      if (kind >= 0) this->precond.create_precond(A);
+      ret = this->run_device(mgx_jacobi_solve, A, x, b, kind >= 0 ? kind : MGX_PRECOND_DEFAULT,
+                             MGX_MODE_REFERENCE);
+    }
+    Logger::get_instance().solver_out();
+    return ret;
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::Jacobi"; }
+  [[nodiscard]] std::string solver_name() const { return "Jacobi"; }
+};
+
+// --------------------------------------------------------------------- CG
+template <typename MATRIX, typename Float>
+class CG : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    Logger::get_instance().solver_in(monolish_func);
+    int ret = 0;
+    if (this->lib == 0) ret = monolish_CG(A, x, b);
+    Logger::get_instance().solver_out();
+    return ret;
+  }
+  void create_precond(MATRIX &A) {
+    (void)A;
+    throw std::runtime_error("this precond. is not impl.");
+  }
+  void apply_precond(const vector<Float> &r, vector<Float> &z) {
+    (void)r, (void)z;
+    throw std::runtime_error("this precond. is not impl.");
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::CG"; }
+  [[nodiscard]] std::string solver_name() const { return "CG"; }
+
+ private:
+  int monolish_CG(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    detail_eq::check_system(A, x, b, "equation::CG");
+    if (this->resid_method != 0) throw std::runtime_error("error vector size is not same");
+    const int kind = this->device_precond();
+    if (kind >= 0) {
+      this->precond.create_precond(A);
+      return this->run_device(mgx_cg_solve, A, x, b, kind, MGX_MODE_FUSED);
+    }
+    // cg.cpp:27-85 with the bound callbacks
+    const size_t n = A.get_row();
+    vector<Float> r(n, 0.0), p(n, 0.0), q(n, 0.0), z(n, 0.0);
+    util::send(r, p, q, z);
+    this->precond.create_precond(A);
+    blas::matvec(A, x, q);
+    vml::sub(b, q, r);
+    this->precond.apply_precond(r, z);
+    blas::copy(z, p);
+    for (size_t iter = 0; iter < this->get_maxiter(); iter++) {
+      blas::matvec(A, p, q);
+      auto tmp = blas::dot(z, r);
+      auto alpha = tmp / blas::dot(p, q);
+      blas::axpy(alpha, p, x);
+      blas::axpy(-alpha, q, r);
+      this->precond.apply_precond(r, z);
+      auto beta = blas::dot(z, r) / tmp;
+      blas::xpay(beta, z, p);
+      Float resid = blas::nrm2(r);
+      if (this->get_print_rhistory())
+        *this->rhistory_stream << iter + 1 << "\t" << std::scientific << resid << std::endl;
+      if (resid < this->get_tol() && this->get_miniter() <= iter + 1) {
+        this->final_iter = iter + 1;
+        this->final_resid = blas::nrm2(r);
+        return MONOLISH_SOLVER_SUCCESS;
+      }
+      if (std::isnan(resid)) {
+        this->final_iter = iter + 1;
+        this->final_resid = blas::nrm2(r);
+        return MONOLISH_SOLVER_RESIDUAL_NAN;
+      }
+    }
+    this->final_iter = this->maxiter;
+    this->final_resid = blas::nrm2(r);
+    return MONOLISH_SOLVER_MAXITER;
+  }
+};
+
+// --------------------------------------------------------------- BiCGSTAB
+template <typename MATRIX, typename Float>
+class BiCGSTAB : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    Logger::get_instance().solver_in(monolish_func);
+    int ret = 0;
+    if (this->lib == 0) ret = monolish_BiCGSTAB(A, x, b);
+    Logger::get_instance().solver_out();
+    return ret;
+  }
+  void create_precond(MATRIX &A) {
+    (void)A;
+    throw std::runtime_error("this precond. is not impl.");
+  }
+  void apply_precond(const vector<Float> &r, vector<Float> &z) {
+    (void)r, (void)z;
+    throw std::runtime_error("this precond. is not impl.");
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::BiCGSTAB"; }
+  [[nodiscard]] std::string solver_name() const { return "BiCGSTAB"; }
+
+ private:
+  int monolish_BiCGSTAB(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    detail_eq::check_system(A, x, b, "equation::BiCGSTAB");
+    if (this->resid_method != 0) throw std::runtime_error("error vector size is not same");
+    const int kind = this->device_precond();
+    if (kind >= 0) {
+      this->precond.create_precond(A);
+      return this->run_device(mgx_bicgstab_solve, A, x, b, kind, MGX_MODE_FUSED);
+    }
+    // bicgstab.cpp:27-134 with the bound callbacks
+    const size_t n = A.get_row();
+    vector<Float> r(n, 0.0), r0(n, 0.0), p(n, 0.0), phat(n, 0.0), s(n, 0.0), shat(n, 0.0),
+        v(n, 0.0), t(n, 0.0);
+    util::send(r, r0, p, phat, s, shat, v, t);
+    Float rho_old = 1, rho = 1, alpha = 1, beta, omega = 1;
+    this->precond.create_precond(A);
+    blas::matvec(A, x, r);
+    vml::sub(b, r, r);
+    blas::copy(r, r0);
+    for (size_t iter = 0; iter < this->get_maxiter(); iter++) {
+      rho = blas::dot(r, r0);
+      if (rho == 0.0) {
+        this->final_iter = iter + 1;
+        this->final_resid = blas::nrm2(r);
+        return MONOLISH_SOLVER_BREAKDOWN;
+      }
+      if (iter == 0) {
+        blas::copy(r, p);
+      } else {
+        beta = (rho / rho_old) * (alpha / omega);
+        blas::axpy(-omega, v, p);
+        blas::xpay(beta, r, p);
+      }
+      this->precond.apply_precond(p, phat);
+      blas::matvec(A, phat, v);
+      alpha = rho / blas::dot(v, r0);
+      blas::axpyz(-alpha, v, r, s);
+      this->precond.apply_precond(s, shat);
+      blas::matvec(A, shat, t);
+      omega = blas::dot(t, s) / blas::dot(t, t);
+      if (omega == 0.0) {
+        this->final_iter = iter + 1;
+        this->final_resid = blas::nrm2(r);
+        return MONOLISH_SOLVER_BREAKDOWN;
+      }
+      blas::axpy(alpha, phat, x);
+      blas::axpy(omega, shat, x);
+      blas::axpyz(-omega, t, s, r);
+      auto resid = blas::nrm2(r);
+      if (this->get_print_rhistory())
+        *this->rhistory_stream << iter + 1 << "\t" << std::scientific << resid << std::endl;
+      if (resid < this->get_tol() && this->get_miniter() <= iter + 1) {
+        this->final_iter = iter + 1;
+        this->final_resid = blas::nrm2(r);
+        return MONOLISH_SOLVER_SUCCESS;
+      }
+      if (std::isnan(resid)) {
+        this->final_iter = iter + 1;
+        this->final_resid = blas::nrm2(r);
+        return MONOLISH_SOLVER_RESIDUAL_NAN;
+      }
+      rho_old = rho;
+    }
+    this->final_iter = this->maxiter;
+    this->final_resid = blas::nrm2(r);
+    return MONOLISH_SOLVER_MAXITER;
+  }
+};
+
+}  // namespace equation
+}  // namespace monolish
