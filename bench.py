@@ -205,6 +205,145 @@ def bench_mixed(m, ctx, side, reps):
     return res
 
 
+def bench_batched(m, ctx, nsys, seed0=4000):
+    """4096 SPD M-matrices, sizes log-uniform in [500, 20000], 5-15 random
+    off-diagonals per row U[-1,0) symmetrised, diag = row sum |offdiag| + 0.5;
+    b = U[-1,1) per system; Jacobi-PCG to 1e-10 relative, all systems in the
+    same fused launches."""
+    import concurrent.futures as cf
+
+    import oracle
+    rng = np.random.default_rng(seed0)
+    sizes = np.floor(np.exp(rng.uniform(np.log(500), np.log(20000), nsys))).astype(int)
+    t0 = time.perf_counter()
+
+    def gen(s):
+        rp, ci, v = m.mmatrix(int(sizes[s]), 5, 15, 0.5, seed=seed0 + s)
+        b = oracle.random_vector(int(sizes[s]), -1.0, 1.0, seed0 + s)
+        return rp, ci, v, b
+
+    with cf.ThreadPoolExecutor(16) as ex:
+        mats = list(ex.map(gen, range(nsys)))
+    t_gen = time.perf_counter() - t0
+    As = [m.CRS(ctx, len(b), len(b), rp, ci, v) for rp, ci, v, b in mats]
+    Bs = [m.Vector(ctx, b) for _, _, _, b in mats]
+    Xs = [m.Vector(ctx, n=len(b)) for _, _, _, b in mats]
+    nnz = int(sum(len(v) for _, _, v, _ in mats))
+    del mats
+    ctx.sync()
+    t_setup = time.perf_counter() - t0 - t_gen
+    # fused batch: every system in shared launches
+    t1 = time.perf_counter()
+    res = m.batch_solve(ctx, m.METHOD_CG, As, Xs, Bs, tol=1e-10, tol_is_relative=True,
+                        precond=m.PRECOND_JACOBI)
+    t_batch = time.perf_counter() - t1
+    ok = sum(r.status == 0 for r in res)
+    iters = [r.final_iter for r in res]
+    # the same systems one at a time: fused single-system and plain FP64 CG
+    # (a sample, extrapolated to all systems)
+    sample = list(range(0, nsys, max(1, nsys // 32)))
+    for X in Xs:
+        X.fill(0.0)
+
+    def one(mode):
+        t = time.perf_counter()
+        for s in sample:
+            Xs[s].fill(0.0)
+            m.cg(As[s], Xs[s], Bs[s], tol=1e-10, tol_is_relative=True, precond=m.PRECOND_JACOBI,
+                 mode=mode)
+        dt = time.perf_counter() - t
+        return dt / len(sample) * sum(1 for _ in range(nsys))
+
+    t_single = one(m.MODE_FUSED)
+    t_plain = one(m.MODE_REFERENCE)
+    out = {"workload": f"{nsys} SPD M-matrices, n log-uniform [500,20000] (sum n = {int(sizes.sum())}, "
+                       f"nnz = {nnz}), Jacobi-PCG FP64 to 1e-10 relative",
+           "batched_solves_per_s": round(nsys / t_batch, 1), "batched_seconds": round(t_batch, 3),
+           "converged": ok, "iterations_min_max": [int(min(iters)), int(max(iters))],
+           "one_at_a_time_fused_solves_per_s_est": round(nsys / t_single, 1),
+           "plain_fp64_gpu_cg_solves_per_s_est": round(nsys / t_plain, 1),
+           "batched_vs_plain": round(t_plain / t_batch, 2),
+           "estimate_sample": f"{len(sample)} systems solved one at a time, scaled to {nsys}",
+           "generation_s": round(t_gen, 2), "upload_s": round(t_setup, 2)}
+    log(f"batched C4: {out}")
+    for A in As:
+        A.free()
+    return out
+
+
+def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48):
+    """Poisson arrivals; 50% 2D/3D Poisson (CG), 50% 2D upwind convection-
+    diffusion with Peclet U[1,50] (BiCGSTAB); Jacobi, 1e-10 relative; sizes
+    log-uniform in [nmin, nmax].  Requests draw from a pool of pre-generated
+    systems so the client keeps up with the arrival rate."""
+    import oracle
+    rng = np.random.default_rng(seed)
+    systems = []
+    for k in range(pool):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        if k % 2 == 0:
+            if k % 4 == 0:
+                side = max(int(round(np.sqrt(n))), 4)
+                rp, ci, v = m.laplacian2d(side, side)
+            else:
+                side = max(int(round(n ** (1 / 3))), 3)
+                rp, ci, v = m.stencil3d(side, side, side, 7)
+            meth = m.METHOD_CG
+        else:
+            side = max(int(round(np.sqrt(n))), 4)
+            rp, ci, v = m.convdiff2d(side, float(rng.uniform(1, 50)))
+            meth = m.METHOD_BICGSTAB
+        b = oracle.random_vector(len(rp) - 1, -1.0, 1.0, seed + k)
+        systems.append((meth, rp, ci, v, b))
+    srv = m.Server(ctx, max_active=1024, iters_per_poll=8)
+    t0 = time.perf_counter()
+    t = 0.0
+    tickets = []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        k = int(rng.integers(pool))
+        meth, rp, ci, v, b = systems[k]
+        tickets.append((srv.submit(meth, rp, ci, v, b, tol=1e-10, tol_is_relative=True), k))
+    t_submitted = time.perf_counter() - t0
+    xs = {}
+    statuses = []
+    for tk, k in tickets:
+        r, x = srv.wait(tk, len(systems[k][4]))
+        statuses.append(r.status)
+        xs.setdefault(k, x)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    # a result in the mix equals the same system solved alone
+    same = True
+    for k in list(xs)[:4]:
+        meth, rp, ci, v, b = systems[k]
+        n = len(b)
+        A = m.CRS(ctx, n, n, rp, ci, v)
+        X = m.Vector(ctx, n=n)
+        fn = m.cg if meth == m.METHOD_CG else m.bicgstab
+        fn(A, X, m.Vector(ctx, b), tol=1e-10, tol_is_relative=True, precond=m.PRECOND_JACOBI,
+           mode=m.MODE_FUSED)
+        same = same and bool(np.array_equal(X.download().view(np.uint64), xs[k].view(np.uint64)))
+    out = {"workload": f"Poisson arrivals {rate}/s for {seconds}s, n log-uniform [{nmin},{nmax}], "
+                       f"50% 2D/3D Poisson (CG) / 50% upwind convection-diffusion Pe U[1,50] "
+                       f"(BiCGSTAB), Jacobi, 1e-10 relative; {pool} distinct systems",
+           "requests": len(tickets), "converged": int(sum(s == 0 for s in statuses)),
+           "solves_per_s": round(st["solves_per_s"], 1), "offered_per_s": round(len(tickets) / t_submitted, 1),
+           "p50_latency_ms": round(1e3 * st["p50_latency_s"], 2),
+           "p99_latency_ms": round(1e3 * st["p99_latency_s"], 2),
+           "mean_queue_ms": round(1e3 * st["mean_queue_s"], 2), "wall_s": round(wall, 2),
+           "peak_resident_MB": round(st["peak_bytes"] / 2 ** 20, 1),
+           "result_in_mix_equals_alone": same}
+    log(f"served C5: {out}")
+    return out
+
+
 def cpu_baseline_spmv(rp, ci, val, x, budget_s):
     import oracle
     oracle.spmv(rp, ci, val, x)  # warm-up
@@ -229,6 +368,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="skip the sweep, Krylov and CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--only", default="", help="comma list of legs: cpu,sweep,krylov,mixed,"
+                                                 "batched,served (default: all)")
+    ap.add_argument("--batch-systems", type=int, default=4096)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-nmax", type=int, default=100000,
+                    help="largest served system (rows); the full stream uses 1e6")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run 5 vadd launches of known size (counter calibration) and 5 "
                          "SpMVs on the 192^3 matrix, for rocprofv3 --pmc runs")
@@ -290,8 +435,13 @@ def main():
                    "timed_region_s": round(wall, 3)},
         "roofline": roofline,
     }
-    if not args.quick:
+    legs = set(args.only.split(",")) if args.only else {"cpu", "sweep", "krylov", "mixed", "batched",
+                                                         "served"}
+    if args.quick:
+        legs = set()
+    if "cpu" in legs:
         line["cpu_baseline"] = cpu_baseline_spmv(rp, ci, val, x, args.cpu_seconds)
+    if "sweep" in legs:
         r32, _, _ = bench_spmv_matrix(m, ev, ctx, "C2a fp32", rp, ci, val.astype(np.float32),
                                       x.astype(np.float32), args.steps, args.warmup)
         sweep["c2a_7pt_192_fp32"] = r32
@@ -307,8 +457,14 @@ def main():
         sweep["c2b_lognormal_4M_fp32"] = s32
         del rp2, ci2, v2, x2
         line["spmv_sweep"] = sweep
+    if "krylov" in legs:
         line["krylov"] = bench_krylov(m, ctx, reps_fused=30, reps_plain=5)
+    if "mixed" in legs:
         line["mixed_precision"] = bench_mixed(m, ctx, 128, reps=2)
+    if "batched" in legs:
+        line["batched"] = bench_batched(m, ctx, args.batch_systems)
+    if "served" in legs:
+        line["served"] = bench_served(m, ctx, 2000.0, args.serve_seconds, 1000, args.serve_nmax)
     line["device"] = info["name"]
     print(json.dumps(line))
     ctx.close()

@@ -73,11 +73,8 @@ __device__ __forceinline__ void tile_spmv(const SysDesc<T> &d, int64_t tile,
       continue;
     }
     const int32_t e0 = d.rp[ra], e1 = d.rp[rb];
-    for (int32_t e = e0 + t; e < e1; e += kBlock) {
-      T v = __builtin_nontemporal_load(&d.val[e]);
-      int32_t c = __builtin_nontemporal_load(&d.ci[e]);
-      prod[e - e0] = v * gv(c);
-    }
+    // a chunk holds at most kFusedNnz entries (fused_plan_build)
+    stream_products<T, stream_u<T>(kFusedNnz)>(d.val, d.ci, e0, e1, prod, gv);
     __syncthreads();
     const int32_t row = ra + t;
     if (row < rb) {
@@ -96,18 +93,20 @@ __device__ __forceinline__ void tile_spmv(const SysDesc<T> &d, int64_t tile,
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
                                                     const TileRef *tiles,
-                                                    int64_t nlaunch) {
+                                                    int64_t nlaunch, int remap) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *prod = reinterpret_cast<T *>(smem_raw);
-  __shared__ T qloc[kTile];
+  __shared__ T qloc[kQuarter];
   __shared__ T sh[8];
+  __shared__ T res[2];
   __shared__ int flag;
-  const int64_t b = blockIdx.x;
+  const int64_t b = xcd_block(remap);  // consecutive quarters share an XCD's L2
   if (b >= nlaunch) return;
   int64_t tile;
-  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  const SysDesc<T> &d = locate(sys, tiles, b >> 2, tile);
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
+  if (d.split) return;  // split iteration: plan-based SpMV and k_cg_pq instead
   const bool odd = (st->iter & 1) != 0;
   const T *p_old = odd ? d.pb : d.pa;
   T *p_new = odd ? d.pa : d.pb;
@@ -119,27 +118,32 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
     T bp = beta * p_old[c];
     return z[c] + bp;
   };
-  const int64_t R0 = d.row0 + tile * kTile;
-  const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
-  tile_spmv(d, tile, R0, prod, qloc, d.q, sh, pv);
+  // one block per 128-row quarter of the tile; wave 0 forms the quarter's
+  // share of the tile's <p, q> tree
+  const int64_t qi = tile * 4 + (b & 3);
+  const int64_t R0 = d.row0 + qi * kQuarter;
+  const int64_t R1 = min(d.row0 + d.n, R0 + kQuarter);
+  tile_spmv(d, qi, R0, prod, qloc, d.q, sh, pv);
   const int t = threadIdx.x;
-  const int64_t i0 = R0 + 2 * t;
   T acc0 = 0, acc1 = 0;
-  if (i0 < R1) {
-    T p = pv(i0);
-    p_new[i0] = p;
-    acc0 = p * qloc[2 * t];
+  if (t < 64) {
+    const int64_t i0 = R0 + 2 * t;
+    if (i0 < R1) {
+      T p = pv(i0);
+      p_new[i0] = p;
+      acc0 = p * qloc[2 * t];
+    }
+    if (i0 + 1 < R1) {
+      T p = pv(i0 + 1);
+      p_new[i0 + 1] = p;
+      acc1 = p * qloc[2 * t + 1];
+    }
   }
-  if (i0 + 1 < R1) {
-    T p = pv(i0 + 1);
-    p_new[i0 + 1] = p;
-    acc1 = p * qloc[2 * t + 1];
-  }
-  T s = block_tree(acc0 + acc1, sh);
-  if (t == 0) publish_partial(d.P, tile, s);
-  if (last_block_arrives(&st->ticket[0], (uint32_t)d.ntiles, &flag)) {
-    T pq = reduce_partials(d.P, d.ntiles, sh);
+  T s = wave_tree(acc0 + acc1);
+  if (t == 0) publish_partial(d.P, qi, s);
+  if (arrive_reduce<T, 1, 4>(st->tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
     if (t == 0) {
+      const T pq = res[0];
       T zr = (T)st->zr;
       T alpha = zr / pq;
       st->pq = (double)pq;
@@ -155,6 +159,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
                                                       int64_t nlaunch,
                                                       int32_t *stopped) {
   __shared__ T sh[8];
+  __shared__ T tp[2 * kBlock];
+  __shared__ T res[2];
   __shared__ int flag;
   const int64_t b = blockIdx.x;
   if (b >= nlaunch) return;
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const bool odd = (st->iter & 1) != 0;
-  const T *p_new = odd ? d.pa : d.pb;
+  const T *p_new = odd || d.split ? d.pa : d.pb;
   const T alpha = (T)st->alpha;
   const T nalpha = -alpha;
   const int t = threadIdx.x;
@@ -171,20 +177,47 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
   const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
   const int64_t i0 = R0 + 2 * t;
   T zr2[2] = {0, 0}, rr2[2] = {0, 0};
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  if (d.vec && i0 + 1 < R1) {
+    // i0 is even: one 2-element vector access per array and lane
+    const V2 pv = *reinterpret_cast<const V2 *>(p_new + i0);
+    const V2 qv = *reinterpret_cast<const V2 *>(d.q + i0);
+    V2 xv = *reinterpret_cast<const V2 *>(d.x + i0);
+    V2 rv = *reinterpret_cast<const V2 *>(d.r + i0);
+    V2 mv = {};
+    if (d.M) mv = *reinterpret_cast<const V2 *>(d.M + i0);
+    V2 zv;
 #pragma unroll
-  for (int e = 0; e < 2; e++) {
-    const int64_t i = i0 + e;
-    if (i < R1) {
-      T pv = p_new[i], qv = d.q[i];
-      T t1 = alpha * pv;
-      d.x[i] = t1 + d.x[i];
-      T t2 = nalpha * qv;
-      T rv = t2 + d.r[i];
-      d.r[i] = rv;
-      T zv = d.M ? d.M[i] * rv : rv;
-      d.z[i] = zv;
-      zr2[e] = zv * rv;
-      rr2[e] = rv * rv;
+    for (int e = 0; e < 2; e++) {
+      T t1 = alpha * pv[e];
+      xv[e] = t1 + xv[e];
+      T t2 = nalpha * qv[e];
+      T r_ = t2 + rv[e];
+      rv[e] = r_;
+      T z_ = d.M ? mv[e] * r_ : r_;
+      zv[e] = z_;
+      zr2[e] = z_ * r_;
+      rr2[e] = r_ * r_;
+    }
+    *reinterpret_cast<V2 *>(d.x + i0) = xv;
+    *reinterpret_cast<V2 *>(d.r + i0) = rv;
+    *reinterpret_cast<V2 *>(d.z + i0) = zv;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int64_t i = i0 + e;
+      if (i < R1) {
+        T pv = p_new[i], qv = d.q[i];
+        T t1 = alpha * pv;
+        d.x[i] = t1 + d.x[i];
+        T t2 = nalpha * qv;
+        T rv = t2 + d.r[i];
+        d.r[i] = rv;
+        T zv = d.M ? d.M[i] * rv : rv;
+        d.z[i] = zv;
+        zr2[e] = zv * rv;
+        rr2[e] = rv * rv;
+      }
     }
   }
   T zr_a = zr2[0] + zr2[1];
@@ -194,11 +227,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
     publish_partial(d.P, tile, zr_a);
     publish_partial(d.P, d.ntiles + tile, rr_a);
   }
-  if (last_block_arrives(&st->ticket[1], (uint32_t)d.ntiles, &flag)) {
-    T zr_new = reduce_partials(d.P, d.ntiles, sh);
-    __syncthreads();
-    T rr = reduce_partials(d.P + d.ntiles, d.ntiles, sh);
+  if (arrive_reduce<T, 2, 1>(st->tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
+                             &flag)) {
     if (t == 0) {
+      const T zr_new = res[0], rr = res[1];
       const T zr_old = (T)st->zr;
       const T beta = zr_new / zr_old;
       const T resid = dsqrt(rr);
@@ -223,22 +255,101 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
   }
 }
 
+// split iteration, stage 1: p = z in the first iteration, else
+// p = z + beta p (xpay, cg.cpp:76); stage 2 is q = A p by the plan-based SpMV
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const TileRef *tiles,
+                                                 int64_t nlaunch) {
+  const int64_t b = blockIdx.x;
+  if (b >= nlaunch) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (!d.split) return;
+  const SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  const bool first = st->first != 0;
+  const T beta = (T)st->beta;
+  const int64_t R1 = min(d.n, (tile + 1) * kTile);
+  const int64_t i0 = tile * kTile + 2 * threadIdx.x;
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  if (i0 + 1 < R1) {
+    // i0 is even and the work buffers are 256-byte aligned
+    const V2 zv = *reinterpret_cast<const V2 *>(d.z + i0);
+    V2 pv = zv;
+    if (!first) {
+      pv = *reinterpret_cast<const V2 *>(d.pa + i0);
+      const T b0 = beta * pv[0], b1 = beta * pv[1];
+      pv[0] = zv[0] + b0;
+      pv[1] = zv[1] + b1;
+    }
+    *reinterpret_cast<V2 *>(d.pa + i0) = pv;
+  } else if (i0 < R1) {
+    T pv = d.z[i0];
+    if (!first) {
+      const T bp = beta * d.pa[i0];
+      pv = d.z[i0] + bp;
+    }
+    d.pa[i0] = pv;
+  }
+}
+
+// split iteration, stage 3: <p, q> in tile order; alpha = <z,r> / <p,q>
+// (cg.cpp:55-58)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_pq(const SysDesc<T> *sys, const TileRef *tiles,
+                                                  int64_t nlaunch) {
+  __shared__ T sh[8];
+  __shared__ T tp[kBlock];
+  __shared__ T res[2];
+  __shared__ int flag;
+  const int64_t b = blockIdx.x;
+  if (b >= nlaunch) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (!d.split) return;
+  SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  const int64_t R1 = min(d.n, (tile + 1) * kTile);
+  const int64_t i0 = tile * kTile + 2 * threadIdx.x;
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  T p0 = 0, p1 = 0;
+  if (i0 + 1 < R1) {
+    const V2 pv = *reinterpret_cast<const V2 *>(d.pa + i0);
+    const V2 qv = *reinterpret_cast<const V2 *>(d.q + i0);
+    p0 = pv[0] * qv[0];
+    p1 = pv[1] * qv[1];
+  } else if (i0 < R1) {
+    p0 = d.pa[i0] * d.q[i0];
+  }
+  const T s = block_tree(p0 + p1, sh);
+  if (threadIdx.x == 0) publish_partial(d.P, tile, s);
+  if (arrive_reduce<T, 1, 1>(st->tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
+    if (threadIdx.x == 0) {
+      const T pq = res[0];
+      const T zr = (T)st->zr;
+      st->pq = (double)pq;
+      st->alpha = (double)(zr / pq);
+    }
+  }
+}
+
 // ---------------------------------------------------------- BiCGSTAB ----
 // K1: p = r + beta (p - omega v) and phat = M p at each gather;
 //     v = A phat; <v, r0>; alpha = rho / <v, r0>          (bicgstab.cpp:64-80)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
                                                       const TileRef *tiles,
-                                                      int64_t nlaunch) {
+                                                      int64_t nlaunch, int remap) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *prod = reinterpret_cast<T *>(smem_raw);
-  __shared__ T qloc[kTile];
+  __shared__ T qloc[kQuarter];
   __shared__ T sh[8];
+  __shared__ T res[2];
   __shared__ int flag;
-  const int64_t b = blockIdx.x;
+  const int64_t b = xcd_block(remap);  // consecutive quarters share an XCD's L2
   if (b >= nlaunch) return;
   int64_t tile;
-  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  const SysDesc<T> &d = locate(sys, tiles, b >> 2, tile);
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const bool odd = (st->iter & 1) != 0;
@@ -262,27 +373,30 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
     T p = pval(c);
     return M ? M[c] * p : p;
   };
-  const int64_t R0 = d.row0 + tile * kTile;
-  const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
-  tile_spmv(d, tile, R0, prod, qloc, v_new, sh, phv);
+  const int64_t qi = tile * 4 + (b & 3);
+  const int64_t R0 = d.row0 + qi * kQuarter;
+  const int64_t R1 = min(d.row0 + d.n, R0 + kQuarter);
+  tile_spmv(d, qi, R0, prod, qloc, v_new, sh, phv);
   const int t = threadIdx.x;
-  const int64_t i0 = R0 + 2 * t;
   T acc[2] = {0, 0};
+  if (t < 64) {
+    const int64_t i0 = R0 + 2 * t;
 #pragma unroll
-  for (int e = 0; e < 2; e++) {
-    const int64_t i = i0 + e;
-    if (i < R1) {
-      T p = pval(i);
-      p_new[i] = p;
-      d.phat[i] = M ? M[i] * p : p;
-      acc[e] = qloc[2 * t + e] * d.r0[i];
+    for (int e = 0; e < 2; e++) {
+      const int64_t i = i0 + e;
+      if (i < R1) {
+        T p = pval(i);
+        p_new[i] = p;
+        d.phat[i] = M ? M[i] * p : p;
+        acc[e] = qloc[2 * t + e] * d.r0[i];
+      }
     }
   }
-  T s = block_tree(acc[0] + acc[1], sh);
-  if (t == 0) publish_partial(d.P, tile, s);
-  if (last_block_arrives(&st->ticket[0], (uint32_t)d.ntiles, &flag)) {
-    T vr0 = reduce_partials(d.P, d.ntiles, sh);
+  T s = wave_tree(acc[0] + acc[1]);
+  if (t == 0) publish_partial(d.P, qi, s);
+  if (arrive_reduce<T, 1, 4>(st->tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
     if (t == 0) {
+      const T vr0 = res[0];
       T rho = (T)st->rho;
       st->vr0 = (double)vr0;
       st->alpha = (double)(rho / vr0);
@@ -295,17 +409,18 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
                                                       const TileRef *tiles,
-                                                      int64_t nlaunch,
+                                                      int64_t nlaunch, int remap,
                                                       int32_t *stopped) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *prod = reinterpret_cast<T *>(smem_raw);
-  __shared__ T qloc[kTile];
+  __shared__ T qloc[kQuarter];
   __shared__ T sh[8];
+  __shared__ T res[2];
   __shared__ int flag;
-  const int64_t b = blockIdx.x;
+  const int64_t b = xcd_block(remap);  // consecutive quarters share an XCD's L2
   if (b >= nlaunch) return;
   int64_t tile;
-  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  const SysDesc<T> &d = locate(sys, tiles, b >> 2, tile);
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const bool odd = (st->iter & 1) != 0;
@@ -321,36 +436,37 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
     T s = sval(c);
     return M ? M[c] * s : s;
   };
-  const int64_t R0 = d.row0 + tile * kTile;
-  const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
-  tile_spmv(d, tile, R0, prod, qloc, d.t, sh, shv);
+  const int64_t qi = tile * 4 + (b & 3);
+  const int64_t R0 = d.row0 + qi * kQuarter;
+  const int64_t R1 = min(d.row0 + d.n, R0 + kQuarter);
+  tile_spmv(d, qi, R0, prod, qloc, d.t, sh, shv);
   const int t = threadIdx.x;
-  const int64_t i0 = R0 + 2 * t;
   T ts2[2] = {0, 0}, tt2[2] = {0, 0};
+  if (t < 64) {
+    const int64_t i0 = R0 + 2 * t;
 #pragma unroll
-  for (int e = 0; e < 2; e++) {
-    const int64_t i = i0 + e;
-    if (i < R1) {
-      T sv = sval(i);
-      d.s[i] = sv;
-      d.shat[i] = M ? M[i] * sv : sv;
-      T tv = qloc[2 * t + e];
-      ts2[e] = tv * sv;
-      tt2[e] = tv * tv;
+    for (int e = 0; e < 2; e++) {
+      const int64_t i = i0 + e;
+      if (i < R1) {
+        T sv = sval(i);
+        d.s[i] = sv;
+        d.shat[i] = M ? M[i] * sv : sv;
+        T tv = qloc[2 * t + e];
+        ts2[e] = tv * sv;
+        tt2[e] = tv * tv;
+      }
     }
   }
-  T ts_a = ts2[0] + ts2[1];
-  T tt_a = tt2[0] + tt2[1];
-  block_tree2(ts_a, tt_a, sh);
+  T ts_a = wave_tree(ts2[0] + ts2[1]);
+  T tt_a = wave_tree(tt2[0] + tt2[1]);
   if (t == 0) {
-    publish_partial(d.P, tile, ts_a);
-    publish_partial(d.P, d.ntiles + tile, tt_a);
+    publish_partial(d.P, qi, ts_a);
+    publish_partial(d.P, 4 * d.ntiles + qi, tt_a);
   }
-  if (last_block_arrives(&st->ticket[1], (uint32_t)d.ntiles, &flag)) {
-    T ts = reduce_partials(d.P, d.ntiles, sh);
-    __syncthreads();
-    T tt = reduce_partials(d.P + d.ntiles, d.ntiles, sh);
+  if (arrive_reduce<T, 2, 4>(st->tick, d.P, 4 * d.ntiles, d.S, d.ntiles, qi, prod, sh, res,
+                             &flag)) {
     if (t == 0) {
+      const T ts = res[0], tt = res[1];
       T omega = ts / tt;
       st->ts = (double)ts;
       st->tt = (double)tt;
@@ -374,6 +490,8 @@ __global__ __launch_bounds__(kBlock) void k_bcg_update(const SysDesc<T> *sys,
                                                        int64_t nlaunch,
                                                        int32_t *stopped) {
   __shared__ T sh[8];
+  __shared__ T tp[2 * kBlock];
+  __shared__ T res[2];
   __shared__ int flag;
   const int64_t b = blockIdx.x;
   if (b >= nlaunch) return;
@@ -387,19 +505,44 @@ __global__ __launch_bounds__(kBlock) void k_bcg_update(const SysDesc<T> *sys,
   const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
   const int64_t i0 = R0 + 2 * t;
   T rr2[2] = {0, 0}, rq2[2] = {0, 0};
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  if (d.vec && i0 + 1 < R1) {
+    const V2 ph = *reinterpret_cast<const V2 *>(d.phat + i0);
+    const V2 sh_ = *reinterpret_cast<const V2 *>(d.shat + i0);
+    const V2 tv = *reinterpret_cast<const V2 *>(d.t + i0);
+    const V2 sv = *reinterpret_cast<const V2 *>(d.s + i0);
+    const V2 r0v = *reinterpret_cast<const V2 *>(d.r0 + i0);
+    V2 xv = *reinterpret_cast<const V2 *>(d.x + i0);
+    V2 rv;
 #pragma unroll
-  for (int e = 0; e < 2; e++) {
-    const int64_t i = i0 + e;
-    if (i < R1) {
-      T a1 = alpha * d.phat[i];
-      T xv = a1 + d.x[i];
-      T a2 = omega * d.shat[i];
-      d.x[i] = a2 + xv;
-      T a3 = nomega * d.t[i];
-      T rv = a3 + d.s[i];
-      d.r[i] = rv;
-      rr2[e] = rv * rv;
-      rq2[e] = rv * d.r0[i];
+    for (int e = 0; e < 2; e++) {
+      T a1 = alpha * ph[e];
+      T x1 = a1 + xv[e];
+      T a2 = omega * sh_[e];
+      xv[e] = a2 + x1;
+      T a3 = nomega * tv[e];
+      T r_ = a3 + sv[e];
+      rv[e] = r_;
+      rr2[e] = r_ * r_;
+      rq2[e] = r_ * r0v[e];
+    }
+    *reinterpret_cast<V2 *>(d.x + i0) = xv;
+    *reinterpret_cast<V2 *>(d.r + i0) = rv;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int64_t i = i0 + e;
+      if (i < R1) {
+        T a1 = alpha * d.phat[i];
+        T xv = a1 + d.x[i];
+        T a2 = omega * d.shat[i];
+        d.x[i] = a2 + xv;
+        T a3 = nomega * d.t[i];
+        T rv = a3 + d.s[i];
+        d.r[i] = rv;
+        rr2[e] = rv * rv;
+        rq2[e] = rv * d.r0[i];
+      }
     }
   }
   T rr_a = rr2[0] + rr2[1];
@@ -409,11 +552,10 @@ __global__ __launch_bounds__(kBlock) void k_bcg_update(const SysDesc<T> *sys,
     publish_partial(d.P, tile, rr_a);
     publish_partial(d.P, d.ntiles + tile, rq_a);
   }
-  if (last_block_arrives(&st->ticket[2], (uint32_t)d.ntiles, &flag)) {
-    T rr = reduce_partials(d.P, d.ntiles, sh);
-    __syncthreads();
-    T rho_next = reduce_partials(d.P + d.ntiles, d.ntiles, sh);
+  if (arrive_reduce<T, 2, 1>(st->tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
+                             &flag)) {
     if (t == 0) {
+      const T rr = res[0], rho_next = res[1];
       const T resid = dsqrt(rr);
       const int64_t it = st->iter;
       if (d.rhist && it < d.rhist_len) d.rhist[it] = (double)resid;
@@ -454,6 +596,8 @@ __global__ __launch_bounds__(kBlock) void k_sys_dot(const SysDesc<T> *sys,
                                                     int64_t nlaunch, int which,
                                                     int field_off, double scale) {
   __shared__ T sh[8];
+  __shared__ T tp[2 * kBlock];
+  __shared__ T res[2];
   __shared__ int flag;
   const int64_t b = blockIdx.x;
   if (b >= nlaunch) return;
@@ -466,13 +610,21 @@ __global__ __launch_bounds__(kBlock) void k_sys_dot(const SysDesc<T> *sys,
   const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
   const int64_t i0 = R0 + 2 * threadIdx.x;
   T p0 = 0, p1 = 0;
-  if (i0 < R1) p0 = xv[i0] * yv[i0];
-  if (i0 + 1 < R1) p1 = xv[i0 + 1] * yv[i0 + 1];
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  if (d.vec && i0 + 1 < R1) {
+    const V2 a = *reinterpret_cast<const V2 *>(xv + i0);
+    const V2 c = *reinterpret_cast<const V2 *>(yv + i0);
+    p0 = a[0] * c[0];
+    p1 = a[1] * c[1];
+  } else {
+    if (i0 < R1) p0 = xv[i0] * yv[i0];
+    if (i0 + 1 < R1) p1 = xv[i0 + 1] * yv[i0 + 1];
+  }
   T s = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(d.P, tile, s);
-  if (last_block_arrives(&st->ticket[3], (uint32_t)d.ntiles, &flag)) {
-    T v = reduce_partials(d.P, d.ntiles, sh);
+  if (arrive_reduce<T, 1, 1>(st->tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
     if (threadIdx.x == 0) {
+      const T v = res[0];
       double *f = reinterpret_cast<double *>(reinterpret_cast<char *>(st) + field_off);
       // scale != 0: store scale * sqrt(v) (a tolerance relative to ||b||)
       *f = scale != 0.0 ? scale * (double)dsqrt(v) : (double)v;
@@ -504,20 +656,39 @@ template void batch_launch_bcg_init<double>(Ctx *, SolveState *, int32_t *);
 template void batch_launch_bcg_init<float>(Ctx *, SolveState *, int32_t *);
 template <class T>
 void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
-                          int64_t nlaunch, int32_t *stopped, int lds_entries) {
+                          int64_t nlaunch, int32_t *stopped, int lds_entries, bool any_fused) {
   if (nlaunch <= 0) return;
-  const size_t lds = (size_t)lds_entries * sizeof(T);
-  k_cg_spmv<T><<<(unsigned)nlaunch, kBlock, lds, c->stream>>>(sys, tiles, nlaunch);
+  const size_t lds = (size_t)std::max(lds_entries, 2 * kBlock) * sizeof(T);
+  // SpMV stages run one block per 128-row quarter, updates one per tile
+  if (any_fused) {
+    const int64_t nq = 4 * nlaunch;
+    const int remap = nq >= 64;
+    const unsigned gq = (unsigned)(remap ? (nq + 7) / 8 * 8 : nq);
+    k_cg_spmv<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap);
+  }
   k_cg_update<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
+}
+
+template <class T>
+void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch) {
+  if (nlaunch > 0) k_cg_p<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch);
+}
+
+template <class T>
+void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch) {
+  if (nlaunch > 0) k_cg_pq<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch);
 }
 
 template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
                                 int64_t nlaunch, int32_t *stopped, int lds_entries) {
   if (nlaunch <= 0) return;
-  const size_t lds = (size_t)lds_entries * sizeof(T);
-  k_bcg_spmv1<T><<<(unsigned)nlaunch, kBlock, lds, c->stream>>>(sys, tiles, nlaunch);
-  k_bcg_spmv2<T><<<(unsigned)nlaunch, kBlock, lds, c->stream>>>(sys, tiles, nlaunch, stopped);
+  const size_t lds = (size_t)std::max(lds_entries, 2 * kBlock) * sizeof(T);
+  const int64_t nq = 4 * nlaunch;
+  const int remap = nq >= 64;
+  const unsigned gq = (unsigned)(remap ? (nq + 7) / 8 * 8 : nq);
+  k_bcg_spmv1<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap);
+  k_bcg_spmv2<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap, stopped);
   k_bcg_update<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
 }
 
@@ -530,9 +701,14 @@ void batch_launch_sys_dot(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
 }
 
 template void batch_launch_cg_iter<double>(Ctx *, const SysDesc<double> *, const TileRef *,
-                                           int64_t, int32_t *, int);
+                                           int64_t, int32_t *, int, bool);
 template void batch_launch_cg_iter<float>(Ctx *, const SysDesc<float> *, const TileRef *,
-                                          int64_t, int32_t *, int);
+                                          int64_t, int32_t *, int, bool);
+template void batch_launch_cg_p<double>(Ctx *, const SysDesc<double> *, const TileRef *, int64_t);
+template void batch_launch_cg_p<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);
+template void batch_launch_cg_pq<double>(Ctx *, const SysDesc<double> *, const TileRef *,
+                                         int64_t);
+template void batch_launch_cg_pq<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);
 template void batch_launch_bicgstab_iter<double>(Ctx *, const SysDesc<double> *,
                                                  const TileRef *, int64_t, int32_t *, int);
 template void batch_launch_bicgstab_iter<float>(Ctx *, const SysDesc<float> *, const TileRef *,

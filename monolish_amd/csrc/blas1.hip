@@ -6,11 +6,10 @@
 // pass, no grid-stride loop.  Every multiply and add rounds separately (built
 // with -ffp-contract=off), so results equal monolish's CPU loops bit for bit.
 //
-// dot/nrm2 follow the canonical tile order (mgx_device.hpp) with a fixed grid
-// of 256 blocks: block b owns the tiles k = b, b+256, b+512, ..., which are
-// exactly the partials stage-2 lane b adds in order, so each block adds its
-// tile partials itself and publishes a single value; the last block runs the
-// final 256-lane tree.
+// dot/nrm2 follow the canonical tile order (mgx_device.hpp): one block per
+// 512-element tile, whose partial is the block tree of the tile; the blocks
+// then meet in arrive_reduce, which runs stage 2 (lane sums in order, final
+// block tree) spread over the last-arriving blocks.
 #include "mgx_device.hpp"
 
 namespace mgx {
@@ -71,58 +70,33 @@ __global__ void k_convert(int64_t n, const S *x, D *y) {
     y[i] = (D)x[i];
 }
 
-constexpr int kDotBlocks = 256;  // = stage-2 lanes
-constexpr int kDotChunk = 32;    // tiles per block between LDS flushes
-
-// canonical dot; S holds one value per block
+// canonical dot: one block per 512-element tile (lane t holds elements 2t
+// and 2t+1), the tile partial from the block tree; the blocks then meet in
+// arrive_reduce, which adds the tile partials in the canonical lane order
 template <class T, bool VEC>
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *x,
-                                                const T *y, T *S,
-                                                uint32_t *ticket,
-                                                double *out) {
-  __shared__ T wsum[kDotChunk][4];
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *x, const T *y, T *P,
+                                                T *S, uint32_t *tick, double *out) {
   __shared__ T sh[8];
+  __shared__ T tp[kBlock];
+  __shared__ T res[2];
   __shared__ int flag;
-  const int64_t ntiles = ntiles_of(n);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  T acc = 0;
-  for (int64_t k0 = blockIdx.x; k0 < ntiles; k0 += (int64_t)kDotBlocks * kDotChunk) {
-    const int64_t left = (ntiles - k0 + kDotBlocks - 1) / kDotBlocks;
-    const int cnt = left < kDotChunk ? (int)left : kDotChunk;
-#pragma unroll 4
-    for (int j = 0; j < cnt; j++) {
-      const int64_t i0 = (k0 + (int64_t)kDotBlocks * j) * kTile + 2 * threadIdx.x;
-      T p0 = 0, p1 = 0;
-      if (VEC && i0 + 1 < n) {
-        typedef T V2 __attribute__((ext_vector_type(2)));
-        V2 xv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(x + i0));
-        V2 yv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(y + i0));
-        p0 = xv[0] * yv[0];
-        p1 = xv[1] * yv[1];
-      } else {
-        if (i0 < n) p0 = x[i0] * y[i0];
-        if (i0 + 1 < n) p1 = x[i0 + 1] * y[i0 + 1];
-      }
-      T w = wave_tree(p0 + p1);
-      if (lane == 0) wsum[j][wv] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int j = 0; j < cnt; j++) {
-        T lo = wsum[j][0] + wsum[j][1];
-        T hi = wsum[j][2] + wsum[j][3];
-        T p = lo + hi;
-        acc = acc + p;
-      }
-    }
-    __syncthreads();
+  const int64_t tile = blockIdx.x;
+  const int64_t i0 = tile * kTile + 2 * threadIdx.x;
+  T p0 = 0, p1 = 0;
+  if (VEC && i0 + 1 < n) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    const V2 xv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(x + i0));
+    const V2 yv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(y + i0));
+    p0 = xv[0] * yv[0];
+    p1 = xv[1] * yv[1];
+  } else {
+    if (i0 < n) p0 = x[i0] * y[i0];
+    if (i0 + 1 < n) p1 = x[i0 + 1] * y[i0 + 1];
   }
-  if (threadIdx.x == 0) publish_partial(S, blockIdx.x, acc);
-  if (last_block_arrives(ticket, kDotBlocks, &flag)) {
-    T v = load_partial(S, threadIdx.x);
-    T r = block_tree(v, sh);
-    if (threadIdx.x == 0) *out = (double)r;
-  }
+  const T s = block_tree(p0 + p1, sh);
+  if (threadIdx.x == 0) publish_partial(P, tile, s);
+  if (arrive_reduce<T, 1, 1>(tick, P, 0, S, ntiles_of(n), tile, tp, sh, res, &flag))
+    if (threadIdx.x == 0) *out = (double)res[0];
 }
 
 template <class T> T *tp(void *p) { return static_cast<T *>(p); }
@@ -225,22 +199,28 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
     launch_fill(c, MGX_F64, d_out, 1, 0.0);
     return;
   }
-  ensure_partials(c, (size_t)kDotBlocks * 8);
+  const int64_t nt = ntiles_of(n);
+  const size_t es = dt == MGX_F64 ? 8 : 4;
+  // tile partials, then the 256 lane sums
+  ensure_partials(c, (size_t)(nt + 2 * kRedGroups) * es);
   const bool vec = aligned16(x) && aligned16(y);
+  const unsigned g = (unsigned)nt;
   if (dt == MGX_F64) {
+    double *P = tp<double>(c->d_partials), *S = P + nt;
     if (vec)
-      k_dot<double, true><<<kDotBlocks, kBlock, 0, c->stream>>>(
-          n, tp<double>(x), tp<double>(y), tp<double>(c->d_partials), d_ticket, d_out);
+      k_dot<double, true><<<g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y), P, S,
+                                                       d_ticket, d_out);
     else
-      k_dot<double, false><<<kDotBlocks, kBlock, 0, c->stream>>>(
-          n, tp<double>(x), tp<double>(y), tp<double>(c->d_partials), d_ticket, d_out);
+      k_dot<double, false><<<g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y), P, S,
+                                                        d_ticket, d_out);
   } else {
+    float *P = tp<float>(c->d_partials), *S = P + nt;
     if (vec)
-      k_dot<float, true><<<kDotBlocks, kBlock, 0, c->stream>>>(
-          n, tp<float>(x), tp<float>(y), tp<float>(c->d_partials), d_ticket, d_out);
+      k_dot<float, true><<<g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y), P, S,
+                                                      d_ticket, d_out);
     else
-      k_dot<float, false><<<kDotBlocks, kBlock, 0, c->stream>>>(
-          n, tp<float>(x), tp<float>(y), tp<float>(c->d_partials), d_ticket, d_out);
+      k_dot<float, false><<<g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y), P, S,
+                                                       d_ticket, d_out);
   }
   MGX_HIP(hipGetLastError());
 }
@@ -248,7 +228,7 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
 double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y) {
   double *slot = &c->d_state->rr;
-  launch_dot(c, dt, n, x, y, slot, &c->d_state->ticket[0]);
+  launch_dot(c, dt, n, x, y, slot, c->d_state->tick);
   MGX_HIP(hipMemcpyAsync(c->h_pinned, slot, sizeof(double),
                          hipMemcpyDeviceToHost, c->stream));
   MGX_HIP(hipStreamSynchronize(c->stream));

@@ -76,6 +76,8 @@ Engine<T>::Engine(Ctx *c, DevicePool *pool, int capacity) : c_(c), pool_(pool), 
   bufs_.resize(cap_);
   h_states_.resize(cap_);
   reported_.assign(cap_, 0);
+  csr_.assign(cap_, nullptr);
+  val_.assign(cap_, nullptr);
   for (int s = cap_ - 1; s >= 0; s--) free_slots_.push_back(s);
 }
 
@@ -95,7 +97,9 @@ template <class T> size_t Engine<T>::bytes_needed(const Csr *A, int method) cons
   const int64_t n = std::max<int64_t>(A->rows, 1);
   const size_t v = DevicePool::round((size_t)n * sizeof(T));
   const int nvec = method == kMethodCG ? 6 : 11;
-  return nvec * v + DevicePool::round((size_t)2 * ntiles_of(n) * sizeof(T));
+  return nvec * v + DevicePool::round(((size_t)(method == kMethodCG ? 4 : 8) * ntiles_of(n) +
+                                       2 * kRedGroups) *
+                                      sizeof(T));
 }
 
 template <class T>
@@ -140,8 +144,12 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
     need(d.va);
     need(d.vb);
   }
-  d.P = take((size_t)2 * nt * sizeof(T));
+  // partials: per quarter for the SpMV stages (CG 1 sum, BiCGSTAB 2), per
+  // tile for the updates (2 sums); then the stage-2 lane sums
+  const int64_t np = (cfg.method == kMethodCG ? 4 : 8) * nt;
+  d.P = take((size_t)(np + 2 * kRedGroups) * sizeof(T));
   ok = ok && d.P != nullptr;
+  if (ok) d.S = d.P + np;
   if (!ok) {
     for (void *p : got) pool_->put(p);
     return -1;
@@ -163,6 +171,10 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   d.row0 = 0;
   d.n = n;
   d.ntiles = nt;
+  d.split = cfg.method == kMethodCG && n >= kSplitRows ? 1 : 0;
+  d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+  csr_[slot] = A;
+  val_[slot] = val;
   lds_entries_ = std::max(lds_entries_, A->fused_nnz);
   const mgx_dtype dt = DTof<T>::v;
   Ctx *c = c_;
@@ -210,9 +222,17 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
 
 template <class T> void Engine<T>::rebuild_lists() {
   std::vector<TileRef> lst[2];
+  any_fused_ = false;
+  split_slots_.clear();
   for (int s : active_) {
     if (reported_[s]) continue;
     const int m = method_[s];
+    if (m == kMethodCG) {
+      if (h_sys_[s].split)
+        split_slots_.push_back(s);
+      else
+        any_fused_ = true;
+    }
     for (int64_t k = 0; k < h_sys_[s].ntiles; k++) lst[m].push_back(TileRef{s, (int32_t)k});
   }
   MGX_HIP(hipStreamSynchronize(c_->stream));
@@ -233,8 +253,18 @@ template <class T> void Engine<T>::rebuild_lists() {
 
 template <class T> void Engine<T>::launch_iters(int iters) {
   for (int i = 0; i < iters; i++) {
-    if (ntiles_[0])
-      batch_launch_cg_iter<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_, lds_entries_);
+    if (ntiles_[0]) {
+      if (!split_slots_.empty()) {
+        // split form: p, then q = A p per system, then <p, q>
+        batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
+        for (int s : split_slots_)
+          launch_spmv(csr_[s], val_[s], DTof<T>::v, h_sys_[s].pa, h_sys_[s].q, 1.0, 0.0, false,
+                      &d_states_[s].status);
+        batch_launch_cg_pq<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
+      }
+      batch_launch_cg_iter<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_, lds_entries_,
+                              any_fused_);
+    }
     if (ntiles_[1])
       batch_launch_bicgstab_iter<T>(c_, d_sys_, d_tiles_[1], ntiles_[1], d_stopped_,
                                     lds_entries_);

@@ -18,12 +18,18 @@ template <class T> struct SysDesc {
   const T *M;  // Jacobi 1/diag or nullptr
   T *x;
   const T *b;
-  T *r, *z, *q, *pa, *pb, *P;
+  T *r, *z, *q, *pa, *pb;
+  T *P;  // per-unit partials of the reductions
+  T *S;  // stage-2 lane sums (2 * kRedGroups)
   T *r0, *s, *shat, *phat, *t, *va, *vb;  // BiCGSTAB
   SolveState *st;
   double *rhist;
   int64_t rhist_len;
   int64_t row0, n, ntiles;
+  // CG only: 1 = split iteration for large systems (p = z + beta p; q = A p
+  // by the plan-based SpMV; <p, q>; update), 0 = two fused kernels
+  int32_t split;
+  int32_t vec;  // row0 even and x, b 16-byte aligned: vector loads allowed
 };
 
 struct TileRef {
@@ -32,9 +38,22 @@ struct TileRef {
 
 enum { kMethodCG = 0, kMethodBiCGSTAB = 1 };
 
+// CG systems of at least this many rows run the split iteration: launch
+// overhead is small against their SpMV, and the plan-based SpMV streams
+// better than the fused quarter blocks, which gather two vectors
+constexpr int64_t kSplitRows = 1 << 19;
+
+// one CG iteration of the listed systems: the fused SpMV stage (when a
+// listed system uses the two-kernel form) and the update stage; systems of
+// the split form need batch_launch_cg_p, their SpMV and batch_launch_cg_pq
+// first
 template <class T>
 void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
-                          int32_t *stopped, int lds_entries);
+                          int32_t *stopped, int lds_entries, bool any_fused);
+template <class T>
+void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
+template <class T>
+void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
 template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
                                 int64_t nlaunch, int32_t *stopped, int lds_entries);
@@ -117,6 +136,10 @@ template <class T> class Engine {
   std::vector<char> reported_;
   int host_running_ = 0;
   int lds_entries_ = 0;
+  bool any_fused_ = false;        // a listed CG system uses the two-kernel form
+  std::vector<int> split_slots_;  // listed CG systems of the split form
+  std::vector<Csr *> csr_;        // per slot: matrix and values (split SpMV)
+  std::vector<const void *> val_;
 };
 
 // the context's pool and single-solve engines (created on first use, freed

@@ -95,26 +95,212 @@ __device__ __forceinline__ T reduce_partials(const T *P, int64_t ntiles,
   return block_tree(s, sh);
 }
 
+// stage 2 over quarter (wave) partials W: tile k's partial is
+// (W[4k] + W[4k+1]) + (W[4k+2] + W[4k+3]), the same value block_tree gives
+// for the tile; then the canonical lane sums and block tree
+template <class T>
+__device__ __forceinline__ T reduce_quarters(const T *W, int64_t ntiles, T *sh) {
+  T s = 0;
+  for (int64_t k = threadIdx.x; k < ntiles; k += kBlock) {
+    T w0 = load_partial(W, 4 * k), w1 = load_partial(W, 4 * k + 1);
+    T w2 = load_partial(W, 4 * k + 2), w3 = load_partial(W, 4 * k + 3);
+    T lo = w0 + w1;
+    T hi = w2 + w3;
+    T p = lo + hi;
+    s = s + p;
+  }
+  return block_tree(s, sh);
+}
+
 // "last block done": every block calls this after publishing its partials
 // (publish_partial); returns true in all threads of the block that arrives
-// last, which may then read every block's partials with load_partial.  The
-// counter is reset to zero by the last block for the next use.
-__device__ __forceinline__ bool last_block_arrives(uint32_t *ticket,
-                                                   uint32_t nblocks,
-                                                   int *sh_flag) {
+// last, which may then read every block's partials with load_partial.
+// tick[0] counts finished groups, tick[1 + g] the arrivals of group g; block
+// blk (0 <= blk < nblocks) belongs to group blk % G with G = min(nblocks,
+// kTicketGroups).  The last arrival of a group takes a group ticket; the
+// block that takes the last group ticket is the last block overall.  Every
+// counter it finishes is reset to zero for the next use.
+__device__ __forceinline__ bool last_block_arrives(uint32_t *tick, uint32_t nblocks,
+                                                   uint32_t blk, int *sh_flag) {
   __builtin_amdgcn_s_waitcnt(0);  // this lane's published stores completed
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED,
+    const uint32_t G = nblocks < kTicketGroups ? nblocks : kTicketGroups;
+    const uint32_t g = blk % G;
+    const uint32_t members = (nblocks - g + G - 1) / G;
+    int last = 0;
+    uint32_t t = __hip_atomic_fetch_add(&tick[1 + g], 1u, __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_AGENT);
-    int last = (t == nblocks - 1);
-    if (last)
-      __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+    if (t == members - 1) {
+      __hip_atomic_store(&tick[1 + g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t t2 = __hip_atomic_fetch_add(&tick[0], 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      if (t2 == G - 1) {
+        __hip_atomic_store(&tick[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = 1;
+      }
+    }
     *sh_flag = last;
   }
   __syncthreads();
   return *sh_flag != 0;
+}
+
+// Stage 2 of the canonical reduction, spread over the arriving blocks.
+// NV independent sums; sum v has one partial per unit at P + v * pstride.
+// Units are tiles (Q = 1) or 128-row quarters (Q = 4, tile k's partial is
+// (W[4k] + W[4k+1]) + (W[4k+2] + W[4k+3])).  Every block of the launch calls
+// this once, after publishing its unit's partials with publish_partial.
+// Stage-2 lane g of the canonical order adds the tile partials g, g + 256,
+// ... in order; here the blocks of those tiles form group g, and the last of
+// them to arrive computes lane g's sum and publishes it in S[v * 256 + g].
+// Groups arrive at supergroups (g mod 16), those at the top counter; the
+// block that completes the top counter runs the final block tree over the
+// 256 lane sums.  It gets true and res[0..NV) (LDS, visible to all its
+// threads); every other block gets false.  tp is LDS scratch of NV * 256
+// values; tick holds zeroed counters and is zero again after the call.
+template <class T, int NV, int Q>
+__device__ __forceinline__ bool arrive_reduce(uint32_t *tick, const T *P, int64_t pstride,
+                                              T *S, int64_t ntiles, int64_t unit, T *tp,
+                                              T *sh, T *res, int *flag) {
+  __builtin_amdgcn_s_waitcnt(0);  // this lane's published stores completed
+  __syncthreads();
+  const int64_t G = ntiles < kRedGroups ? ntiles : kRedGroups;
+  const int64_t g = (Q == 4 ? (unit >> 2) : unit) % G;
+  const int64_t m = (ntiles - g + G - 1) / G;  // tiles of group g
+  uint32_t *gtick = tick + 1 + kRedSuper;
+  if (threadIdx.x == 0) {
+    const uint32_t members = (uint32_t)(Q * m);
+    int last = 0;
+    const uint32_t t = __hip_atomic_fetch_add(&gtick[g], 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if (t == members - 1) {
+      __hip_atomic_store(&gtick[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = 1;
+    }
+    *flag = last;
+  }
+  __syncthreads();
+  if (!*flag) return false;
+  // lane g's chain: tile partials loaded in parallel, added in order
+  T acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0;
+  for (int64_t base = 0; base < m; base += kBlock) {
+    const int64_t l = base + threadIdx.x;
+    if (l < m) {
+      const int64_t k = g + l * G;
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const T *Pv = P + v * pstride;
+        T p;
+        if (Q == 4) {
+          const T w0 = load_partial(Pv, 4 * k), w1 = load_partial(Pv, 4 * k + 1);
+          const T w2 = load_partial(Pv, 4 * k + 2), w3 = load_partial(Pv, 4 * k + 3);
+          const T lo = w0 + w1;
+          const T hi = w2 + w3;
+          p = lo + hi;
+        } else {
+          p = load_partial(Pv, k);
+        }
+        tp[v * kBlock + threadIdx.x] = p;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int64_t cnt = m - base < kBlock ? m - base : kBlock;
+      for (int64_t j = 0; j < cnt; j++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) acc[v] = acc[v] + tp[v * kBlock + j];
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t SG = G < kRedSuper ? G : kRedSuper;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) publish_partial(S, v * kRedGroups + g, acc[v]);
+    __builtin_amdgcn_s_waitcnt(0);
+    const int64_t sg = g % SG;
+    const uint32_t smembers = (uint32_t)((G - sg + SG - 1) / SG);
+    int last = 0;
+    const uint32_t t = __hip_atomic_fetch_add(&tick[1 + sg], 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if (t == smembers - 1) {
+      __hip_atomic_store(&tick[1 + sg], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t t2 = __hip_atomic_fetch_add(&tick[0], 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if (t2 == (uint32_t)SG - 1) {
+        __hip_atomic_store(&tick[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = 1;
+      }
+    }
+    *flag = last;
+  }
+  __syncthreads();
+  if (!*flag) return false;
+  // final block tree over the lane sums (lanes without tiles hold 0)
+  T s0 = threadIdx.x < G ? load_partial(S, threadIdx.x) : T(0);
+  if (NV == 1) {
+    const T r = block_tree(s0, sh);
+    if (threadIdx.x == 0) res[0] = r;
+  } else {
+    T s1 = threadIdx.x < G ? load_partial(S, kRedGroups + threadIdx.x) : T(0);
+    block_tree2(s0, s1, sh);
+    if (threadIdx.x == 0) {
+      res[0] = s0;
+      res[1] = s1;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
+// Products val[e] * gv(ci[e]) for e in [e0, e1) into prod[e - e0] by the 256
+// lanes of a block.  val / col_ind are read with 16-byte non-temporal loads
+// from e0 rounded down to W = 16 / sizeof(T) entries (the arrays are padded
+// by W entries; entries outside [e0, e1) are loaded but never used); each
+// lane issues its U vector loads before any gather, then all gathers before
+// any product.  Requires e1 - e0 + W - 1 <= U * 256 * W.
+template <class T, int U, class G>
+__device__ __forceinline__ void stream_products(const T *__restrict__ val,
+                                                const int32_t *__restrict__ ci, int32_t e0,
+                                                int32_t e1, T *prod, G gv) {
+  constexpr int W = 16 / sizeof(T);
+  typedef T VT __attribute__((ext_vector_type(W)));
+  typedef int32_t VI __attribute__((ext_vector_type(W)));
+  const int32_t ea = e0 & ~(W - 1);
+  VT v[U];
+  VI c[U];
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+    if (base < e1) {
+      v[k] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(val + base));
+      c[k] = __builtin_nontemporal_load(reinterpret_cast<const VI *>(ci + base));
+    }
+  }
+  T xv[U][W];
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+#pragma unroll
+    for (int j = 0; j < W; j++)
+      if (base + j >= e0 && base + j < e1) xv[k][j] = gv(c[k][j]);
+  }
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+#pragma unroll
+    for (int j = 0; j < W; j++)
+      if (base + j >= e0 && base + j < e1) prod[base + j - e0] = (T)v[k][j] * xv[k][j];
+  }
+}
+
+// vector loads per lane that stream_products needs for `entries` entries
+template <class T> constexpr int stream_u(int entries) {
+  return (entries + (int)(16 / sizeof(T)) - 1 + kBlock * (int)(16 / sizeof(T)) - 1) /
+         (kBlock * (int)(16 / sizeof(T)));
 }
 
 // workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on

@@ -57,7 +57,16 @@ template <class F> int guard(F &&f) {
 // ------------------------------------------------------- device constants --
 constexpr int kBlock = 256;      // threads per block for every kernel
 constexpr int kTile = 512;       // canonical reduction tile (elements)
+constexpr int kQuarter = 128;    // rows of one wave's share of a tile
+constexpr int kFusedNnz = 2048;  // entry budget of a fused-SpMV chunk
 constexpr int kStatRunning = 1;  // solver status while iterating
+
+// arrival counters of the cross-block reductions (mgx_device.hpp): groups
+// of blocks, supergroups of groups, one top counter
+constexpr uint32_t kTicketGroups = 64;  // last_block_arrives
+constexpr int kRedGroups = 256;         // arrive_reduce: one group per stage-2 lane
+constexpr int kRedSuper = 16;
+constexpr int kTickWords = 1 + kRedSuper + kRedGroups;
 
 // device-resident solver scalars (one per solve / per batched system)
 struct SolveState {
@@ -66,8 +75,10 @@ struct SolveState {
   double tol, bnorm;
   int64_t iter, miniter, maxiter;
   int32_t status, first;
-  uint32_t ticket[4];  // last-block-done counters
-  double pad[2];
+  // arrival counters, all zero between launches: [0] top, [1, 1 + kRedSuper)
+  // supergroups, then the groups (last_block_arrives uses [0, 1 + 64))
+  uint32_t tick[kTickWords];
+  uint32_t pad32;
 };
 
 // ------------------------------------------------------------ handles --
@@ -176,8 +187,11 @@ double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y);
 
 void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp_host);
+// live: optional device status word; the launch does nothing unless it
+// reads kStatRunning (device-driven solver loops)
 void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
-                 void *y, double alpha, double beta, bool ab);
+                 void *y, double alpha, double beta, bool ab,
+                 const int32_t *live = nullptr);
 void launch_diag(Csr *A, void *d);
 void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
 // fused-CG row chunks (built on first use of a fused solver)

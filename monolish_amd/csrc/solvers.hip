@@ -32,21 +32,24 @@ template <class T> T host_sqrt(T v) { return std::sqrt(v); }
 
 }  // namespace
 
-// row chunks for the fused kernels: <= 256 rows, <= budget entries, never
-// crossing a 512-row tile; rows longer than the budget become long chunks
+// row chunks for the fused kernels: <= budget entries, never crossing a
+// 128-row quarter of a 512-row tile (one fused-SpMV block per quarter);
+// rows longer than the budget become long chunks.  d_tile_chunk[q] is the
+// first chunk of quarter q (4 per tile, empty past the last row).
 void fused_plan_build(Csr *A) {
   if (A->d_chunks) return;
   Ctx *c = A->ctx;
   const int64_t n = A->rows;
   std::vector<int32_t> rp(n + 1);
   MGX_HIP(hipMemcpy(rp.data(), A->rp, (n + 1) * 4, hipMemcpyDeviceToHost));
-  const int budget = std::max(c->spmv_nnz, 256) * 2;
-  const int64_t nt = ntiles_of(n);
-  std::vector<int32_t> chunks, tile_chunk(nt + 1);
-  for (int64_t k = 0; k < nt; k++) {
+  (void)c;
+  const int budget = kFusedNnz;
+  const int64_t nq = 4 * ntiles_of(n);
+  std::vector<int32_t> chunks, tile_chunk(nq + 1);
+  for (int64_t k = 0; k < nq; k++) {
     tile_chunk[k] = (int32_t)chunks.size();
-    const int64_t R1 = std::min(n, (k + 1) * (int64_t)kTile);
-    int64_t r = k * kTile;
+    const int64_t R1 = std::min(n, (k + 1) * (int64_t)kQuarter);
+    int64_t r = std::min(n, k * kQuarter);
     while (r < R1) {
       int32_t len = rp[r + 1] - rp[r];
       if (len > budget) {
@@ -65,7 +68,7 @@ void fused_plan_build(Csr *A) {
       chunks.push_back((int32_t)s);
     }
   }
-  tile_chunk[nt] = (int32_t)chunks.size();
+  tile_chunk[nq] = (int32_t)chunks.size();
   chunks.push_back((int32_t)n);
   A->d_chunks = (int32_t *)dev_alloc(chunks.size() * 4);
   A->d_tile_chunk = (int32_t *)dev_alloc(tile_chunk.size() * 4);

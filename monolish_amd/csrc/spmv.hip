@@ -30,12 +30,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const int32_t *__restrict__ blk, int64_t nblocks,
     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y,
-    T alpha, T beta, int remap) {
+    T alpha, T beta, int remap, const int32_t *live) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *prod = reinterpret_cast<T *>(smem_raw);
   __shared__ T sh[8];
   const int64_t b = xcd_block(remap);
   if (b >= nblocks) return;
+  // inside a device-driven solve: nothing to do once the solve has stopped
+  if (live && *live != kStatRunning) return;
   const int32_t d0 = blk[b], d1 = blk[b + 1];
   const int32_t r0 = d0 & 0x7fffffff, r1 = d1 & 0x7fffffff;
   if (d0 < 0) {
@@ -61,40 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     return;
   }
   const int32_t e0 = rp[r0], e1 = rp[r1];
-  {
-    // 16-byte loads: W entries per lane per load, starting from e0 rounded
-    // down to a multiple of W (val/col_ind are padded by W entries, and
-    // entries outside [e0, e1) are never used)
-    constexpr int W = 16 / sizeof(T);
-    typedef T VT __attribute__((ext_vector_type(W)));
-    typedef int32_t VI __attribute__((ext_vector_type(W)));
-    const int32_t ea = e0 & ~(W - 1);
-    VT v[U];
-    VI c[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
-      if (base < e1) {
-        v[k] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(val + base));
-        c[k] = __builtin_nontemporal_load(reinterpret_cast<const VI *>(ci + base));
-      }
-    }
-    T xv[U][W];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
-#pragma unroll
-      for (int j = 0; j < W; j++)
-        if (base + j >= e0 && base + j < e1) xv[k][j] = x[c[k][j]];
-    }
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
-#pragma unroll
-      for (int j = 0; j < W; j++)
-        if (base + j >= e0 && base + j < e1) prod[base + j - e0] = (T)v[k][j] * xv[k][j];
-    }
-  }
+  stream_products<T, U>(val, ci, e0, e1, prod, [&](int32_t c) { return x[c]; });
   __syncthreads();
   const int32_t r = r0 + (int32_t)threadIdx.x;
   if (r < r1) {
@@ -114,17 +83,43 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
   }
 }
 
-// CRS::diag: d[i] = A(i,i), last stored match wins, 0 if absent
+// CRS::diag: d[i] = A(i,i), last stored match wins, 0 if absent.  A block
+// owns 256 consecutive rows and scans their entries with consecutive lanes on
+// consecutive entries (coalesced); the owning row of an entry comes from a
+// binary search of the block's row_ptr slice in LDS, and the largest matching
+// entry index per row (= the last stored match) is kept with an LDS atomicMax.
 template <class T>
-__global__ void k_diag(int64_t n, const int32_t *__restrict__ rp,
-                       const int32_t *__restrict__ ci,
-                       const T *__restrict__ val, T *__restrict__ d) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
-    T v = 0;
-    for (int32_t j = rp[i]; j < rp[i + 1]; j++)
-      if (ci[j] == (int32_t)i) v = val[j];
-    d[i] = v;
+__global__ __launch_bounds__(kBlock) void k_diag(int64_t n, const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ ci,
+                                                 const T *__restrict__ val,
+                                                 T *__restrict__ d) {
+  __shared__ int32_t srp[kBlock + 1];
+  __shared__ int32_t best[kBlock];
+  const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+  const int nr = (int)(n - r0 < kBlock ? n - r0 : kBlock);
+  const int t = threadIdx.x;
+  // nr + 1 row pointers (up to 257) from 256 lanes
+  if (t < nr) srp[t] = rp[r0 + t];
+  if (t == 0) srp[nr] = rp[r0 + nr];
+  best[t] = -1;
+  __syncthreads();
+  const int32_t e0 = srp[0], e1 = srp[nr];
+  for (int32_t j = e0 + t; j < e1; j += kBlock) {
+    // largest local row lr with srp[lr] <= j
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (srp[mid] <= j)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    if (ci[j] == (int32_t)(r0 + lo)) atomicMax(&best[lo], j);
+  }
+  __syncthreads();
+  if (t < nr) {
+    const int32_t b = best[t];
+    d[r0 + t] = b >= 0 ? val[b] : T(0);
   }
 }
 
@@ -179,7 +174,7 @@ void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp) {
 
 template <class T, bool AB>
 static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
-                          T beta) {
+                          T beta, const int32_t *live) {
   Ctx *c = A->ctx;
   const int64_t nb = A->plan.nblocks;
   const int remap = c->xcd_remap && nb >= 64;
@@ -191,7 +186,7 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
   const int32_t *B = A->plan.d_blocks;
 #define MGX_SPMV_U(UU)                                                          \
   k_spmv<T, AB, UU><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
-                                                      Y, alpha, beta, remap)
+                                                      Y, alpha, beta, remap, live)
   if (u <= 1) MGX_SPMV_U(1);
   else if (u <= 2) MGX_SPMV_U(2);
   else if (u <= 3) MGX_SPMV_U(3);
@@ -206,24 +201,24 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
 }
 
 void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
-                 void *y, double alpha, double beta, bool ab) {
+                 void *y, double alpha, double beta, bool ab, const int32_t *live) {
   if (A->plan.nblocks <= 0) return;
   if (dt == MGX_F64) {
     auto V = static_cast<const double *>(val);
     auto X = static_cast<const double *>(x);
     auto Y = static_cast<double *>(y);
     if (ab)
-      spmv_dispatch<double, true>(A, V, X, Y, alpha, beta);
+      spmv_dispatch<double, true>(A, V, X, Y, alpha, beta, live);
     else
-      spmv_dispatch<double, false>(A, V, X, Y, 1.0, 0.0);
+      spmv_dispatch<double, false>(A, V, X, Y, 1.0, 0.0, live);
   } else {
     auto V = static_cast<const float *>(val);
     auto X = static_cast<const float *>(x);
     auto Y = static_cast<float *>(y);
     if (ab)
-      spmv_dispatch<float, true>(A, V, X, Y, (float)alpha, (float)beta);
+      spmv_dispatch<float, true>(A, V, X, Y, (float)alpha, (float)beta, live);
     else
-      spmv_dispatch<float, false>(A, V, X, Y, 1.0f, 0.0f);
+      spmv_dispatch<float, false>(A, V, X, Y, 1.0f, 0.0f, live);
   }
 }
 
@@ -231,8 +226,7 @@ void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d) {
   Ctx *c = A->ctx;
   const int64_t n = std::min(A->rows, A->cols);
   if (n <= 0) return;
-  int64_t g = (n + kBlock - 1) / kBlock;
-  g = std::min<int64_t>(g, (int64_t)c->num_cus * 16);
+  const int64_t g = (n + kBlock - 1) / kBlock;  // one block per 256 rows
   if (dt == MGX_F64)
     k_diag<double><<<(unsigned)g, kBlock, 0, c->stream>>>(
         n, A->rp, A->ci, static_cast<const double *>(val),

@@ -303,3 +303,29 @@ def test_crs_diag(m, ctx, oracle):
     d = m.Vector(ctx, n=3)
     A.diag(d)
     assert d.download().tolist() == oracle.crs_diag(3, 3, rp, ci, val).tolist() == [9.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("rows,cols", [(256, 256), (1024, 1024), (1300, 1297), (700, 1000)])
+def test_crs_diag_blocks(m, ctx, oracle, rows, cols):
+    """diag over several full 256-row blocks: empty rows, duplicated diagonal
+    entries (the last stored one wins), a long row, rectangular shapes."""
+    say("crs diag blocks", rows, cols)
+    rng = np.random.default_rng(rows + cols)
+    lens = rng.integers(0, 6, rows)
+    lens[rng.random(rows) < 0.2] = 0
+    lens[rows // 3] = 600
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    nnz = int(rp[-1])
+    ci = rng.integers(0, cols, nnz).astype(np.int32)
+    for i in range(min(rows, cols)):
+        if lens[i] > 0 and rng.random() < 0.7:
+            ci[rp[i] + rng.integers(0, lens[i])] = i
+        if lens[i] > 1 and rng.random() < 0.2:
+            ci[rp[i]] = i
+            ci[rp[i + 1] - 1] = i
+    for dt in (np.float64, np.float32):
+        val = rng.standard_normal(nnz).astype(dt)
+        A = m.CRS(ctx, rows, cols, rp, ci, val)
+        d = m.Vector(ctx, n=min(rows, cols), dtype=dt)
+        A.diag(d)
+        assert bits_equal(d.download(), oracle.crs_diag(rows, cols, rp, ci, val))

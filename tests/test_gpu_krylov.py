@@ -130,6 +130,28 @@ def test_cg_fused_equals_reference_mode(m, ctx, oracle, dtype, precond):
         assert bits_equal(res.rhistory, ref[4]), mode
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cg_fused_large_three_kernel_form(m, ctx, oracle, dtype):
+    """n >= 2^19 rows: the fused solver runs the three-kernel iteration (the
+    SpMV gathers p alone); 40 iterations stay bit-identical to the oracle and
+    to the reference-shaped GPU loop."""
+    say("cg fused three-kernel", dtype.__name__)
+    rp, ci, val = m.stencil3d(96, 96, 64, 7, 0.0)
+    val = val.astype(dtype)
+    n = len(rp) - 1
+    assert n >= 1 << 19
+    b = oracle.random_vector(n, -1.0, 1.0, 3001, dtype=dtype)
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n, dtype), b, precond=2, tol=1e-30,
+                       maxiter=40, history=True)
+    assert ref[0] == m.SOLVER_MAXITER
+    for mode in (0, 1):
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n, dtype), b, 2, 1e-30,
+                           maxiter=40, mode=mode, history=40)
+        assert res.status == ref[0] and res.final_iter == 40, mode
+        assert bits_equal(x, ref[1]), mode
+        assert bits_equal(res.rhistory, ref[4]), mode
+
+
 def test_cg_error_paths(m, ctx, oracle):
     say("cg error paths")
     rp, ci, val = _poisson(oracle, 16)
