@@ -174,11 +174,16 @@ def bench_mixed(m, ctx, side, reps):
     B = m.Vector(ctx, b)
     X = m.Vector(ctx, n=n)
     res = {}
-    for name, fn in (("fp64_fused_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI)),
-                     ("fp64_plain_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI,
-                                                    mode=m.MODE_REFERENCE)),
-                     ("mixed_ir_cg", lambda: m.ir_cg(A, X, B, tol=1e-10, tol_is_relative=True,
-                                                     inner_tol=1e-4))):
+    # the inner FP32 solves restart from zero at every refinement step, so
+    # the inner tolerance trades restarts against FP32 stagnation
+    inner_tols = (1e-4, 1e-5, 1e-6)
+    legs = [("fp64_fused_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI)),
+            ("fp64_plain_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI,
+                                           mode=m.MODE_REFERENCE))]
+    for it in inner_tols:
+        legs.append((f"mixed_ir_cg_inner{it:g}",
+                     lambda it=it: m.ir_cg(A, X, B, tol=1e-10, tol_is_relative=True, inner_tol=it)))
+    for name, fn in legs:
         ts = []
         r = None
         for i in range(reps + 1):
@@ -193,12 +198,17 @@ def bench_mixed(m, ctx, side, reps):
                      "outer": r.outer_iter, "true_rel_residual": true_rel, "status": r.status}
         res[name + "_x"] = x
     ref = res["fp64_fused_cg_x"]
-    for name in ("fp64_plain_cg", "mixed_ir_cg"):
+    for name, _ in legs[1:]:
         xx = res.pop(name + "_x")
         res[name]["rel_err_vs_fp64"] = float(np.linalg.norm(xx - ref) / np.linalg.norm(ref))
     res.pop("fp64_fused_cg_x")
+    # the fastest converged inner tolerance is the reported mixed solve
+    mixed = [n for n, _ in legs[2:] if res[n]["status"] == 0 and res[n]["true_rel_residual"] <= 1e-10]
+    best = min(mixed, key=lambda n: res[n]["ms"]) if mixed else legs[2][0]
+    res["mixed_ir_cg"] = dict(res[best], inner_tol=float(best.split("inner")[1]))
     res["mixed_vs_plain_fp64"] = round(res["fp64_plain_cg"]["ms"] / res["mixed_ir_cg"]["ms"], 2)
     res["mixed_vs_fused_fp64"] = round(res["fp64_fused_cg"]["ms"] / res["mixed_ir_cg"]["ms"], 2)
+    res["fused_vs_plain_fp64"] = round(res["fp64_plain_cg"]["ms"] / res["fp64_fused_cg"]["ms"], 2)
     res["workload"] = (f"27-point stencil {side}^3 (n={n}, nnz={len(val)}), diag 26+1e-2, "
                        f"b=U[-1,1) seed 3001, ||b-Ax|| < 1e-10||b||")
     log(f"mixed C3 {side}^3: {res}")
@@ -239,6 +249,19 @@ def bench_batched(m, ctx, nsys, seed0=4000):
     t_batch = time.perf_counter() - t1
     ok = sum(r.status == 0 for r in res)
     iters = [r.final_iter for r in res]
+    # FP32-refined form: FP64 residuals and stopping test, batched FP32 PCG
+    # corrections; error against the FP64 batch on a sample of systems
+    x64 = {s: Xs[s].download() for s in range(0, nsys, max(1, nsys // 64))}
+    for X in Xs:
+        X.fill(0.0)
+    ctx.sync()
+    t1 = time.perf_counter()
+    res_ir = m.batch_ir_cg(ctx, As, Xs, Bs, tol=1e-10, tol_is_relative=True, inner_tol=1e-5,
+                           precond=m.PRECOND_JACOBI)
+    t_ir = time.perf_counter() - t1
+    ok_ir = sum(r.status == 0 for r in res_ir)
+    err_ir = max(float(np.linalg.norm(Xs[s].download() - x) / np.linalg.norm(x))
+                 for s, x in x64.items())
     # the same systems one at a time: fused single-system and plain FP64 CG
     # (a sample, extrapolated to all systems)
     sample = list(range(0, nsys, max(1, nsys // 32)))
@@ -263,6 +286,14 @@ def bench_batched(m, ctx, nsys, seed0=4000):
            "one_at_a_time_fused_solves_per_s_est": round(nsys / t_single, 1),
            "plain_fp64_gpu_cg_solves_per_s_est": round(nsys / t_plain, 1),
            "batched_vs_plain": round(t_plain / t_batch, 2),
+           "fp32_refined": {"solves_per_s": round(nsys / t_ir, 1), "seconds": round(t_ir, 3),
+                            "converged": ok_ir,
+                            "outer_min_max": [int(min(r.outer_iter for r in res_ir)),
+                                              int(max(r.outer_iter for r in res_ir))],
+                            "inner_iterations_min_max": [int(min(r.inner_iter for r in res_ir)),
+                                                         int(max(r.inner_iter for r in res_ir))],
+                            "max_rel_err_vs_fp64_batch": err_ir,
+                            "vs_plain": round(t_plain / t_ir, 2)},
            "estimate_sample": f"{len(sample)} systems solved one at a time, scaled to {nsys}",
            "generation_s": round(t_gen, 2), "upload_s": round(t_setup, 2)}
     log(f"batched C4: {out}")

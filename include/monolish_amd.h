@@ -60,10 +60,29 @@ int mgx_device_info(mgx_ctx *ctx, char *name, size_t len, int *compute_units,
  * threshold, XCD-aware block order.  Applies to CRS created afterwards. */
 int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
                         int long_row, int xcd_remap);
+/* Autotuner of the SpMV block plan (no monolish counterpart: monolish hands
+ * CRS SpMV to cuSPARSE): times A's SpMV under candidate plans -- rows per
+ * block {64, 128, 256} x entries per block {1024, 2048, 4096} x XCD-aware
+ * order on/off -- `reps` launches each, keeps the fastest for A and reports
+ * it.  The long-row threshold is unchanged, so results do not depend on the
+ * choice. */
+typedef struct mgx_spmv_tuning {
+  int32_t rows_per_block, nnz_per_block, long_row, xcd_remap;
+  double us;           /* time of one SpMV under the chosen plan */
+  int32_t candidates;  /* plans timed */
+  int32_t pad;
+} mgx_spmv_tuning;
+int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out);
 /* 1: fused solvers and batches replay each group of iterations from a
  * captured hipGraph (re-captured when the set of running systems changes);
  * 0 (default): direct kernel launches */
 int mgx_set_graphs(mgx_ctx *ctx, int enable);
+/* 1: a fused CG solve of one system small enough for the two-kernel
+ * iteration runs its iterations inside one cooperative launch (grid barriers
+ * between the stages); 0 (default, measured faster on MI355X): one launch per
+ * stage; a value above 1 enables it with at most that many blocks (each
+ * block then owns several 128-row quarters) */
+int mgx_set_persistent(mgx_ctx *ctx, int enable);
 
 /* ---- vectors: vector<T>::send/recv/nonfree_recv/device_free ------------- */
 /* vector<T>::send (src/utils/system/gpu_comm.cpp:8-24): device copy of n
@@ -214,6 +233,19 @@ int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
 int mgx_batch_solve(mgx_ctx *ctx, int method, int64_t nsys, mgx_csr *const *A,
                     mgx_vec *const *x, mgx_vec *const *b,
                     const mgx_solve_params *p, mgx_solve_result *results);
+
+/* FP32-refined form of mgx_batch_solve for CG on FP64 systems (no monolish
+ * counterpart: the precision form of equation::CG, cg.cpp:12-86, over many
+ * systems): iterative refinement with FP64 residuals r = b - A x, updates
+ * and stopping tests (||r|| < tol, or tol * ||b|| when tol_is_relative),
+ * and the corrections of all running systems solved together by the
+ * batched FP32 PCG (p->precond) to the accuracy inner_tol.  Each system
+ * follows the arithmetic of mgx_ir_cg_solve.  results[i]: final_iter and
+ * inner_iter = FP32 iterations, outer_iter = refinement steps, final_resid
+ * = FP64 ||r||; p->maxiter bounds the inner iterations per system. */
+int mgx_batch_ir_cg_solve(mgx_ctx *ctx, int64_t nsys, mgx_csr *const *A, mgx_vec *const *x,
+                          mgx_vec *const *b, const mgx_solve_params *p, double inner_tol,
+                          int64_t max_outer, mgx_solve_result *results);
 
 /* ---- solve server: continuous batching of arriving requests -------------- */
 typedef struct mgx_server mgx_server;

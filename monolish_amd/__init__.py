@@ -52,6 +52,11 @@ class SolveResultC(C.Structure):
                 ("seconds", _D), ("inner_iter", _I64), ("outer_iter", _I64)]
 
 
+class SpmvTuning(C.Structure):
+    _fields_ = [("rows_per_block", _I32), ("nnz_per_block", _I32), ("long_row", _I32),
+                ("xcd_remap", _I32), ("us", _D), ("candidates", _I32), ("pad", _I32)]
+
+
 class CsrStats(C.Structure):
     _fields_ = [("rows", _I64), ("cols", _I64), ("nnz", _I64), ("max_row", _I32),
                 ("min_row", _I32), ("mean_row", _D), ("stream_blocks", _I64), ("long_rows", _I64)]
@@ -87,6 +92,7 @@ _sig("mgx_csr_download", [_P, _P, _P, _P])
 _sig("mgx_csr_destroy", [_P])
 _sig("mgx_csr_diag", [_P, _P])
 _sig("mgx_csr_stats_get", [_P, C.POINTER(CsrStats)])
+_sig("mgx_csr_autotune", [_P, C.c_int, C.POINTER(SpmvTuning)])
 _sig("mgx_matvec", [_P, _P, _P])
 _sig("mgx_matvec_ab", [_D, _P, _P, _D, _P])
 _sig("mgx_matvec_t", [_P, _P, _P])
@@ -111,8 +117,11 @@ _sig("mgx_gen_laplacian2d", [_I64, _I64, C.POINTER(_I64), _P, _P, _P])
 _sig("mgx_gen_convdiff2d", [_I64, _D, C.POINTER(_I64), _P, _P, _P])
 _sig("mgx_gen_mmatrix", [_I64, _I32, _I32, _D, C.c_uint64, _P, _P, _P])
 _sig("mgx_set_graphs", [_P, C.c_int])
+_sig("mgx_set_persistent", [_P, C.c_int])
 _sig("mgx_batch_solve", [_P, C.c_int, _I64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                          C.POINTER(SolveParams), C.POINTER(SolveResultC)])
+_sig("mgx_batch_ir_cg_solve", [_P, _I64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                               C.POINTER(SolveParams), C.c_double, _I64, C.POINTER(SolveResultC)])
 
 
 class ServerConfig(C.Structure):
@@ -318,6 +327,12 @@ class CRS:
         s = CsrStats()
         check(_lib.mgx_csr_stats_get(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in CsrStats._fields_}
+
+    def autotune(self, reps: int = 20) -> dict:
+        """Time the SpMV under candidate block plans and keep the fastest."""
+        t = SpmvTuning()
+        check(_lib.mgx_csr_autotune(self._h, int(reps), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in SpmvTuning._fields_ if f != "pad"}
 
     def free(self):
         if getattr(self, "_h", None):
@@ -530,6 +545,13 @@ def set_graphs(ctx: Context, enable: bool):
     check(_lib.mgx_set_graphs(ctx.handle, int(enable)))
 
 
+def set_persistent(ctx: Context, enable, max_blocks: int = 0):
+    """Run single small fused CG solves in one cooperative launch (default off);
+    max_blocks > 1 caps its grid."""
+    check(_lib.mgx_set_persistent(ctx.handle, int(max_blocks) if enable and max_blocks > 1
+                                  else int(bool(enable))))
+
+
 def batch_solve(ctx: Context, method: int, As, xs, bs, tol=1e-8, maxiter=0, miniter=0,
                 precond=PRECOND_JACOBI, tol_is_relative=False, check_every=0):
     """Solve many systems in shared fused launches; returns a SolveResult each."""
@@ -541,6 +563,21 @@ def batch_solve(ctx: Context, method: int, As, xs, bs, tol=1e-8, maxiter=0, mini
     arrB = (_P * max(ns, 1))(*[b.handle for b in bs])
     check(_lib.mgx_batch_solve(ctx.handle, method, ns, arrA, arrX, arrB, C.byref(p), res))
     return [SolveResult(r.status, r.final_iter, r.final_resid, r.seconds) for r in res[:ns]]
+
+
+def batch_ir_cg(ctx: Context, As, xs, bs, tol=1e-10, tol_is_relative=True, inner_tol=1e-5,
+                max_outer=100, maxiter=0, precond=PRECOND_JACOBI, check_every=0):
+    """FP32-refined batched CG over FP64 systems; returns a SolveResult each."""
+    ns = len(As)
+    p, _ = _params(tol, maxiter, 0, precond, MODE_FUSED, tol_is_relative, check_every, 0)
+    res = (SolveResultC * max(ns, 1))()
+    arrA = (_P * max(ns, 1))(*[a.handle for a in As])
+    arrX = (_P * max(ns, 1))(*[x.handle for x in xs])
+    arrB = (_P * max(ns, 1))(*[b.handle for b in bs])
+    check(_lib.mgx_batch_ir_cg_solve(ctx.handle, ns, arrA, arrX, arrB, C.byref(p), inner_tol,
+                                     int(max_outer), res))
+    return [SolveResult(r.status, r.final_iter, r.final_resid, r.seconds, r.inner_iter,
+                        r.outer_iter) for r in res[:ns]]
 
 
 class Server:

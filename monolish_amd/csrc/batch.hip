@@ -88,6 +88,44 @@ __device__ __forceinline__ void tile_spmv(const SysDesc<T> &d, int64_t tile,
   }
 }
 
+// Solve-state fields are read and written with device-coherent (agent-scope)
+// accesses: the persistent kernels read them in every block after a grid
+// barrier, and the other kernels share the same helpers.
+template <class V> __device__ __forceinline__ V sld(const V *p) {
+  return __hip_atomic_load(const_cast<V *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class V> __device__ __forceinline__ void sst(V *p, V v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// end of a CG iteration (cg.cpp:64-84): beta = <z,r>_new / <z,r>_old,
+// residual nrm2(r), history, then the stop tests in the reference's order;
+// called by one thread of the block that completed the reductions
+template <class T>
+__device__ __forceinline__ void cg_finish(const SysDesc<T> &d, SolveState *st, T zr_new, T rr,
+                                          int32_t *stopped) {
+  const T zr_old = (T)sld(&st->zr);
+  const T beta = zr_new / zr_old;
+  const T resid = dsqrt(rr);
+  const int64_t it = sld(&st->iter);
+  if (d.rhist && it < d.rhist_len) d.rhist[it] = (double)resid;
+  sst(&st->beta, (double)beta);
+  sst(&st->zr, (double)zr_new);
+  sst(&st->rr, (double)rr);
+  sst(&st->resid, (double)resid);
+  sst(&st->first, 0);
+  sst(&st->iter, it + 1);
+  int32_t s = kStatRunning;
+  if ((double)resid < sld(&st->tol) && sld(&st->miniter) <= it + 1)
+    s = MGX_OK;
+  else if (resid != resid)
+    s = MGX_SOLVER_RESIDUAL_NAN;
+  else if (it + 1 >= sld(&st->maxiter))
+    s = MGX_SOLVER_MAXITER;
+  sst(&st->status, s);
+  if (s != kStatRunning && stopped) atomicAdd(stopped, 1);
+}
+
 // ---------------------------------------------------------------- CG ----
 // p = z + beta p formed at each gather; q = A p; <p, q>; alpha = <z,r>/<p,q>
 template <class T>
@@ -141,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
   }
   T s = wave_tree(acc0 + acc1);
   if (t == 0) publish_partial(d.P, qi, s);
-  if (arrive_reduce<T, 1, 4>(st->tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
+  if (arrive_reduce<T, 1, 4>(d.tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
     if (t == 0) {
       const T pq = res[0];
       T zr = (T)st->zr;
@@ -227,30 +265,178 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
     publish_partial(d.P, tile, zr_a);
     publish_partial(d.P, d.ntiles + tile, rr_a);
   }
-  if (arrive_reduce<T, 2, 1>(st->tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
+  if (arrive_reduce<T, 2, 1>(d.tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
                              &flag)) {
-    if (t == 0) {
-      const T zr_new = res[0], rr = res[1];
-      const T zr_old = (T)st->zr;
-      const T beta = zr_new / zr_old;
-      const T resid = dsqrt(rr);
-      const int64_t it = st->iter;
-      if (d.rhist && it < d.rhist_len) d.rhist[it] = (double)resid;
-      st->beta = (double)beta;
-      st->zr = (double)zr_new;
-      st->rr = (double)rr;
-      st->resid = (double)resid;
-      st->first = 0;
-      st->iter = it + 1;
-      int32_t s = kStatRunning;
-      if ((double)resid < st->tol && st->miniter <= it + 1)
-        s = MGX_OK;
-      else if (resid != resid)
-        s = MGX_SOLVER_RESIDUAL_NAN;
-      else if (it + 1 >= st->maxiter)
-        s = MGX_SOLVER_MAXITER;
-      st->status = s;
-      if (s != kStatRunning && stopped) atomicAdd(stopped, 1);
+    if (t == 0) cg_finish<T>(d, st, res[0], res[1], stopped);
+  }
+}
+
+// ------------------------------------------------- persistent CG (small) --
+// One cooperative launch runs up to max_iters CG iterations of one system.
+// Every block owns the 128-row quarters qi = b, b + G, ... in both stages.
+// Cross-block data (p, z, partials, solve state) moves through
+// device-coherent accesses; the blocks meet at a grid barrier after each
+// stage, where the last block to arrive runs the canonical stage 2 and
+// writes the scalars before it releases the others.  Waits are bounded:
+// a block that waits too long sets gs->err and leaves, and so do the others.
+constexpr uint32_t kSpinLimit = 1u << 22;  // polls of about 0.5-1 us each
+
+__device__ __forceinline__ bool grid_arrive(GridSync *gs, uint32_t G, uint32_t ngroups,
+                                            uint32_t blk, uint64_t gen, int *flag) {
+  __builtin_amdgcn_s_waitcnt(0);  // this lane's coherent stores completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t g = blk % ngroups;
+    const uint64_t members = (G - g + ngroups - 1) / ngroups;
+    int last = 0;
+    const uint64_t t = __hip_atomic_fetch_add(&gs->group[g], (uint64_t)1, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if (t == members * gen - 1) {
+      const uint64_t t2 = __hip_atomic_fetch_add(&gs->top, (uint64_t)1, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if (t2 == (uint64_t)ngroups * gen - 1) last = 1;
+    }
+    *flag = last;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+__device__ __forceinline__ void grid_release(GridSync *gs, uint64_t gen) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) sst(&gs->release, gen);
+}
+
+// false when the wait timed out or another block reported an error
+__device__ __forceinline__ bool grid_wait(GridSync *gs, uint64_t gen, int *flag) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    uint32_t spins = 0;
+    while (sld(&gs->release) < gen) {
+      __builtin_amdgcn_s_sleep(2);
+      if ((++spins & 1023u) == 0 && sld(&gs->err) != 0) {
+        ok = 0;
+        break;
+      }
+      if (spins >= kSpinLimit) {
+        sst(&gs->err, 1u);
+        ok = 0;
+        break;
+      }
+    }
+    *flag = ok;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_persist(const SysDesc<T> *sys, GridSync *gs,
+                                                       uint32_t ngroups, int32_t *stopped,
+                                                       int max_iters) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *prod = reinterpret_cast<T *>(smem_raw);
+  __shared__ T qloc[kQuarter];
+  __shared__ T sh[8];
+  __shared__ int flag;
+  const SysDesc<T> &d = sys[0];
+  SolveState *st = d.st;
+  const uint32_t G = gridDim.x, blk = blockIdx.x;
+  const int t = threadIdx.x;
+  const int64_t nq = 4 * d.ntiles;
+  uint64_t gen = 0;
+  for (int k = 0; k < max_iters; k++) {
+    if (sld(&st->status) != kStatRunning) break;  // the same value in every block
+    const int64_t iter = sld(&st->iter);
+    const bool first = sld(&st->first) != 0;
+    const T beta = (T)sld(&st->beta);
+    const bool odd = (iter & 1) != 0;
+    const T *p_old = odd ? d.pb : d.pa;
+    T *p_new = odd ? d.pa : d.pb;
+    const T *z = d.z;
+    auto pv = [&](int64_t c) -> T {
+      const T zc = sld(&z[c]);
+      if (first) return zc;
+      T bp = beta * sld(&p_old[c]);
+      return zc + bp;
+    };
+    // stage 1: q = A p over the owned quarters, p formed at each gather,
+    // quarter partials of <p, q>
+    for (int64_t qi = blk; qi < nq; qi += G) {
+      const int64_t R0 = qi * kQuarter;
+      const int64_t R1 = min(d.n, R0 + kQuarter);
+      tile_spmv(d, qi, R0, prod, qloc, d.q, sh, pv);
+      T acc0 = 0, acc1 = 0;
+      if (t < 64) {
+        const int64_t i0 = R0 + 2 * t;
+        if (i0 < R1) {
+          const T p = pv(i0);
+          sst(&p_new[i0], p);
+          acc0 = p * qloc[2 * t];
+        }
+        if (i0 + 1 < R1) {
+          const T p = pv(i0 + 1);
+          sst(&p_new[i0 + 1], p);
+          acc1 = p * qloc[2 * t + 1];
+        }
+      }
+      const T s = wave_tree(acc0 + acc1);
+      if (t == 0) publish_partial(d.P, qi, s);
+      __syncthreads();  // qloc and prod are reused by the next quarter
+    }
+    if (grid_arrive(gs, G, ngroups, blk, ++gen, &flag)) {
+      const T pq = reduce_quarters(d.P, d.ntiles, sh);
+      if (t == 0) {
+        const T zr = (T)sld(&st->zr);
+        sst(&st->pq, (double)pq);
+        sst(&st->alpha, (double)(zr / pq));
+      }
+      grid_release(gs, gen);
+    } else if (!grid_wait(gs, gen, &flag)) {
+      return;
+    }
+    // stage 2: x += alpha p, r -= alpha q, z = M r over the owned quarters,
+    // quarter partials of <z, r> and <r, r>
+    const T alpha = (T)sld(&st->alpha);
+    const T nalpha = -alpha;
+    for (int64_t qi = blk; qi < nq; qi += G) {
+      const int64_t R0 = qi * kQuarter;
+      const int64_t R1 = min(d.n, R0 + kQuarter);
+      T zr2[2] = {0, 0}, rr2[2] = {0, 0};
+      if (t < 64) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const int64_t i = R0 + 2 * t + e;
+          if (i < R1) {
+            const T pvv = sld(&p_new[i]), qv = d.q[i];
+            T t1 = alpha * pvv;
+            d.x[i] = t1 + d.x[i];
+            T t2 = nalpha * qv;
+            T rv = t2 + d.r[i];
+            d.r[i] = rv;
+            T zv = d.M ? d.M[i] * rv : rv;
+            sst(&d.z[i], zv);
+            zr2[e] = zv * rv;
+            rr2[e] = rv * rv;
+          }
+        }
+      }
+      const T zr_w = wave_tree(zr2[0] + zr2[1]);
+      const T rr_w = wave_tree(rr2[0] + rr2[1]);
+      if (t == 0) {
+        publish_partial(d.P, qi, zr_w);
+        publish_partial(d.P, nq + qi, rr_w);
+      }
+    }
+    if (grid_arrive(gs, G, ngroups, blk, ++gen, &flag)) {
+      const T zr_new = reduce_quarters(d.P, d.ntiles, sh);
+      __syncthreads();
+      const T rr = reduce_quarters(d.P + nq, d.ntiles, sh);
+      if (t == 0) cg_finish<T>(d, st, zr_new, rr, stopped);
+      grid_release(gs, gen);
+    } else if (!grid_wait(gs, gen, &flag)) {
+      return;
     }
   }
 }
@@ -323,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq(const SysDesc<T> *sys, const T
   }
   const T s = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(d.P, tile, s);
-  if (arrive_reduce<T, 1, 1>(st->tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
+  if (arrive_reduce<T, 1, 1>(d.tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
     if (threadIdx.x == 0) {
       const T pq = res[0];
       const T zr = (T)st->zr;
@@ -394,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
   }
   T s = wave_tree(acc[0] + acc[1]);
   if (t == 0) publish_partial(d.P, qi, s);
-  if (arrive_reduce<T, 1, 4>(st->tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
+  if (arrive_reduce<T, 1, 4>(d.tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
     if (t == 0) {
       const T vr0 = res[0];
       T rho = (T)st->rho;
@@ -463,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
     publish_partial(d.P, qi, ts_a);
     publish_partial(d.P, 4 * d.ntiles + qi, tt_a);
   }
-  if (arrive_reduce<T, 2, 4>(st->tick, d.P, 4 * d.ntiles, d.S, d.ntiles, qi, prod, sh, res,
+  if (arrive_reduce<T, 2, 4>(d.tick, d.P, 4 * d.ntiles, d.S, d.ntiles, qi, prod, sh, res,
                              &flag)) {
     if (t == 0) {
       const T ts = res[0], tt = res[1];
@@ -552,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_bcg_update(const SysDesc<T> *sys,
     publish_partial(d.P, tile, rr_a);
     publish_partial(d.P, d.ntiles + tile, rq_a);
   }
-  if (arrive_reduce<T, 2, 1>(st->tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
+  if (arrive_reduce<T, 2, 1>(d.tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
                              &flag)) {
     if (t == 0) {
       const T rr = res[0], rho_next = res[1];
@@ -588,13 +774,103 @@ __global__ __launch_bounds__(kBlock) void k_bcg_update(const SysDesc<T> *sys,
   }
 }
 
-// per-system tile-order dot for set-up: result (as double) into a SolveState
-// field; the field is selected by byte offset
+// ------------------------------------------------------------- set-up ----
+// The set-up of every system admitted since the last launch runs in a few
+// launches over the list of their tiles (cg.cpp:35-45, bicgstab.cpp:41-52).
+
+// copies staged descriptors and states into their slots
+template <class T>
+__global__ void k_scatter(const SysDesc<T> *ds, const SolveState *ss, const int32_t *slots,
+                          SysDesc<T> *sys, SolveState *states) {
+  const int i = blockIdx.x, slot = slots[i];
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&ds[i]);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&sys[slot]);
+  for (int w = threadIdx.x; w < (int)(sizeof(SysDesc<T>) / 4); w += blockDim.x) dst[w] = src[w];
+  const uint32_t *s2 = reinterpret_cast<const uint32_t *>(&ss[i]);
+  uint32_t *d2 = reinterpret_cast<uint32_t *>(&states[slot]);
+  for (int w = threadIdx.x; w < (int)(sizeof(SolveState) / 4); w += blockDim.x) d2[w] = s2[w];
+}
+
+// Jacobi: M = 1 / diag(A) for the tile's 512 rows (crs_getvec.cpp:8-55 then
+// vml::reciprocal); the diagonal search is that of k_diag, per 256 rows
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_setup_diag(const SysDesc<T> *sys,
+                                                       const TileRef *tiles, int64_t nlaunch) {
+  __shared__ int32_t srp[kBlock + 1];
+  __shared__ int32_t best[kBlock];
+  const int64_t b = blockIdx.x;
+  if (b >= nlaunch) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (!d.M) return;
+  T *M = const_cast<T *>(d.M);
+  const int t = threadIdx.x;
+  for (int half = 0; half < 2; half++) {
+    const int64_t r0 = tile * kTile + half * kBlock;
+    if (r0 >= d.n) break;
+    const int nr = (int)(d.n - r0 < kBlock ? d.n - r0 : kBlock);
+    if (t < nr) srp[t] = d.rp[r0 + t];
+    if (t == 0) srp[nr] = d.rp[r0 + nr];
+    best[t] = -1;
+    __syncthreads();
+    const int32_t e0 = srp[0], e1 = srp[nr];
+    for (int32_t j = e0 + t; j < e1; j += kBlock) {
+      int lo = 0, hi = nr - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (srp[mid] <= j)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      if (d.ci[j] == (int32_t)(r0 + lo)) atomicMax(&best[lo], j);
+    }
+    __syncthreads();
+    if (t < nr) {
+      const int32_t bj = best[t];
+      const T dv = bj >= 0 ? d.val[bj] : T(0);
+      M[r0 + t] = T(1) / dv;
+    }
+    __syncthreads();
+  }
+}
+
+// r = b - A x per 128-row quarter; CG: z = M r (or r); BiCGSTAB: r0 = r
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_setup_r(const SysDesc<T> *sys, const TileRef *tiles,
+                                                    int64_t nlaunch, int remap) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *prod = reinterpret_cast<T *>(smem_raw);
+  __shared__ T qloc[kQuarter];
+  __shared__ T sh[8];
+  const int64_t b = xcd_block(remap);
+  if (b >= nlaunch) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b >> 2, tile);
+  const int64_t qi = tile * 4 + (b & 3);
+  const int64_t R0 = qi * kQuarter;
+  const int64_t R1 = min(d.n, R0 + kQuarter);
+  const T *x = d.x;
+  T *scratch = d.method == kMethodCG ? d.q : d.t;
+  tile_spmv(d, qi, R0, prod, qloc, scratch, sh, [&](int64_t c) { return x[c]; });
+  const int64_t i = R0 + threadIdx.x;
+  if (threadIdx.x < kQuarter && i < R1) {
+    const T rv = d.b[i] - qloc[threadIdx.x];
+    d.r[i] = rv;
+    if (d.method == kMethodCG)
+      d.z[i] = d.M ? d.M[i] * rv : rv;
+    else
+      d.r0[i] = rv;
+  }
+}
+
+// per-system tile-order dots of the set-up.  pass 0: tol = tol_scale *
+// nrm2(b) for systems given a relative tolerance (others skip); pass 1:
+// CG <z, r> into zr, BiCGSTAB <r, r0> (r0 = r) into rho
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_sys_dot(const SysDesc<T> *sys,
                                                     const TileRef *tiles,
-                                                    int64_t nlaunch, int which,
-                                                    int field_off, double scale) {
+                                                    int64_t nlaunch, int pass) {
   __shared__ T sh[8];
   __shared__ T tp[2 * kBlock];
   __shared__ T res[2];
@@ -603,11 +879,13 @@ __global__ __launch_bounds__(kBlock) void k_sys_dot(const SysDesc<T> *sys,
   if (b >= nlaunch) return;
   int64_t tile;
   const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (pass == 0 && d.tol_scale == 0.0) return;
   SolveState *st = d.st;
-  const T *xv = which == 0 ? d.z : (which == 1 ? d.b : d.r);  // <z,r>, <b,b>, <r,r>
-  const T *yv = which == 0 ? d.r : (which == 1 ? d.b : d.r);
-  const int64_t R0 = d.row0 + tile * kTile;
-  const int64_t R1 = min(d.row0 + d.n, R0 + kTile);
+  const bool cg = d.method == kMethodCG;
+  const T *xv = pass == 0 ? d.b : (cg ? d.z : d.r);
+  const T *yv = pass == 0 ? d.b : d.r;
+  const int64_t R0 = tile * kTile;
+  const int64_t R1 = min(d.n, R0 + kTile);
   const int64_t i0 = R0 + 2 * threadIdx.x;
   T p0 = 0, p1 = 0;
   typedef T V2 __attribute__((ext_vector_type(2)));
@@ -622,21 +900,31 @@ __global__ __launch_bounds__(kBlock) void k_sys_dot(const SysDesc<T> *sys,
   }
   T s = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(d.P, tile, s);
-  if (arrive_reduce<T, 1, 1>(st->tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
+  if (arrive_reduce<T, 1, 1>(d.tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
     if (threadIdx.x == 0) {
       const T v = res[0];
-      double *f = reinterpret_cast<double *>(reinterpret_cast<char *>(st) + field_off);
-      // scale != 0: store scale * sqrt(v) (a tolerance relative to ||b||)
-      *f = scale != 0.0 ? scale * (double)dsqrt(v) : (double)v;
+      if (pass == 0)
+        sst(&st->tol, d.tol_scale * (double)dsqrt(v));
+      else if (cg)
+        sst(&st->zr, (double)v);
+      else
+        sst(&st->rho, (double)v);
     }
   }
 }
 
 // BiCGSTAB set-up: rho = <r, r0> is in st->rho (r0 = r); the residual of
 // r is sqrt(rho); rho == 0 is a breakdown at the first iteration
+// (bicgstab.cpp:56-62).  One thread per BiCGSTAB system (its tile 0).
 template <class T>
-__global__ void k_bcg_init(SolveState *st, int32_t *stopped) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_bcg_init(const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
+                           int32_t *stopped) {
+  const int64_t b = blockIdx.x;
+  if (b >= nlaunch || threadIdx.x != 0) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (tile != 0 || d.method != kMethodBiCGSTAB) return;
+  SolveState *st = d.st;
   const T rho = (T)st->rho;
   st->resid = (double)dsqrt(rho);
   if (rho == T(0)) {
@@ -649,11 +937,36 @@ __global__ void k_bcg_init(SolveState *st, int32_t *stopped) {
 }  // namespace
 
 // ------------------------------------------------------------ launchers --
-template <class T> void batch_launch_bcg_init(Ctx *c, SolveState *st, int32_t *stopped) {
-  k_bcg_init<T><<<1, 64, 0, c->stream>>>(st, stopped);
+template <class T>
+void batch_launch_setup(Ctx *c, const SysDesc<T> *stage_d, const SolveState *stage_s,
+                        const int32_t *stage_slots, int nstage, SysDesc<T> *sys,
+                        SolveState *states, const TileRef *tiles, int64_t ntiles, bool any_m,
+                        bool any_rel, bool any_bcg, int lds_entries, int32_t *stopped) {
+  if (nstage > 0)
+    k_scatter<T><<<(unsigned)nstage, 64, 0, c->stream>>>(stage_d, stage_s, stage_slots, sys,
+                                                         states);
+  if (ntiles <= 0) return;
+  const unsigned gt = (unsigned)ntiles;
+  if (any_m) k_setup_diag<T><<<gt, kBlock, 0, c->stream>>>(sys, tiles, ntiles);
+  const size_t lds = (size_t)std::max(lds_entries, 2 * kBlock) * sizeof(T);
+  const int64_t nq = 4 * ntiles;
+  const int remap = nq >= 64;
+  const unsigned gq = (unsigned)(remap ? (nq + 7) / 8 * 8 : nq);
+  k_setup_r<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap);
+  if (any_rel) k_sys_dot<T><<<gt, kBlock, 0, c->stream>>>(sys, tiles, ntiles, 0);
+  k_sys_dot<T><<<gt, kBlock, 0, c->stream>>>(sys, tiles, ntiles, 1);
+  if (any_bcg) k_bcg_init<T><<<gt, 64, 0, c->stream>>>(sys, tiles, ntiles, stopped);
+  MGX_HIP(hipGetLastError());
 }
-template void batch_launch_bcg_init<double>(Ctx *, SolveState *, int32_t *);
-template void batch_launch_bcg_init<float>(Ctx *, SolveState *, int32_t *);
+template void batch_launch_setup<double>(Ctx *, const SysDesc<double> *, const SolveState *,
+                                         const int32_t *, int, SysDesc<double> *, SolveState *,
+                                         const TileRef *, int64_t, bool, bool, bool, int,
+                                         int32_t *);
+template void batch_launch_setup<float>(Ctx *, const SysDesc<float> *, const SolveState *,
+                                        const int32_t *, int, SysDesc<float> *, SolveState *,
+                                        const TileRef *, int64_t, bool, bool, bool, int,
+                                        int32_t *);
+
 template <class T>
 void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
                           int64_t nlaunch, int32_t *stopped, int lds_entries, bool any_fused) {
@@ -667,6 +980,28 @@ void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
     k_cg_spmv<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap);
   }
   k_cg_update<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
+}
+
+template <class T>
+int batch_cg_persist_blocks(Ctx *c, int lds_entries) {
+  const size_t lds = (size_t)std::max(lds_entries, 2 * kBlock) * sizeof(T);
+  int per_cu = 0;
+  MGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(&k_cg_persist<T>), kBlock, lds));
+  return per_cu * c->num_cus;
+}
+
+template <class T>
+void batch_launch_cg_persist(Ctx *c, const SysDesc<T> *sys, GridSync *gs, int grid,
+                             int32_t *stopped, int lds_entries, int max_iters) {
+  size_t lds = (size_t)std::max(lds_entries, 2 * kBlock) * sizeof(T);
+  uint32_t ngroups = 1;
+  while (ngroups * ngroups < (uint32_t)grid && ngroups < 64) ngroups++;
+  void *args[] = {(void *)&sys, (void *)&gs, (void *)&ngroups, (void *)&stopped,
+                  (void *)&max_iters};
+  MGX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_cg_persist<T>),
+                                     dim3((unsigned)grid), dim3(kBlock), args, (unsigned)lds,
+                                     c->stream));
 }
 
 template <class T>
@@ -692,18 +1027,16 @@ void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *ti
   k_bcg_update<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
 }
 
-template <class T>
-void batch_launch_sys_dot(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
-                          int64_t nlaunch, int which, int field_off, double scale) {
-  if (nlaunch <= 0) return;
-  k_sys_dot<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, which,
-                                                           field_off, scale);
-}
-
 template void batch_launch_cg_iter<double>(Ctx *, const SysDesc<double> *, const TileRef *,
                                            int64_t, int32_t *, int, bool);
 template void batch_launch_cg_iter<float>(Ctx *, const SysDesc<float> *, const TileRef *,
                                           int64_t, int32_t *, int, bool);
+template int batch_cg_persist_blocks<double>(Ctx *, int);
+template int batch_cg_persist_blocks<float>(Ctx *, int);
+template void batch_launch_cg_persist<double>(Ctx *, const SysDesc<double> *, GridSync *, int,
+                                              int32_t *, int, int);
+template void batch_launch_cg_persist<float>(Ctx *, const SysDesc<float> *, GridSync *, int,
+                                             int32_t *, int, int);
 template void batch_launch_cg_p<double>(Ctx *, const SysDesc<double> *, const TileRef *, int64_t);
 template void batch_launch_cg_p<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);
 template void batch_launch_cg_pq<double>(Ctx *, const SysDesc<double> *, const TileRef *,
@@ -713,9 +1046,5 @@ template void batch_launch_bicgstab_iter<double>(Ctx *, const SysDesc<double> *,
                                                  const TileRef *, int64_t, int32_t *, int);
 template void batch_launch_bicgstab_iter<float>(Ctx *, const SysDesc<float> *, const TileRef *,
                                                 int64_t, int32_t *, int);
-template void batch_launch_sys_dot<double>(Ctx *, const SysDesc<double> *, const TileRef *,
-                                           int64_t, int, int, double);
-template void batch_launch_sys_dot<float>(Ctx *, const SysDesc<float> *, const TileRef *,
-                                          int64_t, int, int, double);
 
 }  // namespace mgx

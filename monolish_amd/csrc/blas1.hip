@@ -228,7 +228,7 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
 double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y) {
   double *slot = &c->d_state->rr;
-  launch_dot(c, dt, n, x, y, slot, c->d_state->tick);
+  launch_dot(c, dt, n, x, y, slot, c->d_tick);
   MGX_HIP(hipMemcpyAsync(c->h_pinned, slot, sizeof(double),
                          hipMemcpyDeviceToHost, c->stream));
   MGX_HIP(hipStreamSynchronize(c->stream));

@@ -20,44 +20,46 @@ size_t DevicePool::round(size_t bytes) {
 }
 
 DevicePool::~DevicePool() {
-  for (auto &f : free_) dev_free(f.second);
+  for (auto &f : free_)
+    for (void *p : f.second) dev_free(p);
   for (auto &l : live_) dev_free(l.first);
 }
 
+// free blocks are kept per rounded size; lookups and returns are O(1)
 void *DevicePool::get(size_t bytes) {
   const size_t sz = round(bytes);
-  for (size_t i = 0; i < free_.size(); i++)
-    if (free_[i].first == sz) {
-      void *p = free_[i].second;
-      free_.erase(free_.begin() + i);
-      live_.push_back({p, sz});
-      in_use_ += sz;
-      return p;
-    }
+  auto it = free_.find(sz);
+  if (it != free_.end() && !it->second.empty()) {
+    void *p = it->second.back();
+    it->second.pop_back();
+    live_.emplace(p, sz);
+    in_use_ += sz;
+    return p;
+  }
   if (in_use_ + sz > budget_) return nullptr;
-  // drop cached blocks of other sizes if they keep us over budget
-  while (reserved_ + sz > budget_ && !free_.empty()) {
-    dev_free(free_.back().second);
-    reserved_ -= free_.back().first;
-    free_.pop_back();
+  // drop cached blocks of other sizes while they keep us over budget
+  for (auto f = free_.begin(); reserved_ + sz > budget_ && f != free_.end(); ++f) {
+    while (!f->second.empty() && reserved_ + sz > budget_) {
+      dev_free(f->second.back());
+      f->second.pop_back();
+      reserved_ -= f->first;
+    }
   }
   void *p = nullptr;
   if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
   reserved_ += sz;
-  live_.push_back({p, sz});
+  live_.emplace(p, sz);
   in_use_ += sz;
   return p;
 }
 
 void DevicePool::put(void *p) {
   if (!p) return;
-  for (size_t i = 0; i < live_.size(); i++)
-    if (live_[i].first == p) {
-      free_.push_back({live_[i].second, p});
-      in_use_ -= live_[i].second;
-      live_.erase(live_.begin() + i);
-      return;
-    }
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_[it->second].push_back(p);
+  in_use_ -= it->second;
+  live_.erase(it);
 }
 
 // ---------------------------------------------------------------- Engine --
@@ -71,6 +73,9 @@ Engine<T>::Engine(Ctx *c, DevicePool *pool, int capacity) : c_(c), pool_(pool), 
   d_states_ = (SolveState *)dev_alloc(sizeof(SolveState) * cap_);
   d_stopped_ = (int32_t *)dev_alloc(sizeof(int32_t));
   MGX_HIP(hipMemset(d_stopped_, 0, sizeof(int32_t)));
+  d_ticks_ = (uint32_t *)dev_alloc(sizeof(uint32_t) * kTickWords * cap_);
+  MGX_HIP(hipMemset(d_ticks_, 0, sizeof(uint32_t) * kTickWords * cap_));
+  rel_.assign(cap_, 0);
   h_sys_.resize(cap_);
   method_.assign(cap_, kMethodCG);
   bufs_.resize(cap_);
@@ -91,23 +96,24 @@ template <class T> Engine<T>::~Engine() {
   dev_free(d_stopped_);
   dev_free(d_tiles_[0]);
   dev_free(d_tiles_[1]);
+  dev_free(d_gs_);
+  dev_free(d_ticks_);
+  dev_free(d_stage_);
+  dev_free(d_ptiles_);
 }
 
 template <class T> size_t Engine<T>::bytes_needed(const Csr *A, int method) const {
   const int64_t n = std::max<int64_t>(A->rows, 1);
   const size_t v = DevicePool::round((size_t)n * sizeof(T));
   const int nvec = method == kMethodCG ? 6 : 11;
-  return nvec * v + DevicePool::round(((size_t)(method == kMethodCG ? 4 : 8) * ntiles_of(n) +
-                                       2 * kRedGroups) *
-                                      sizeof(T));
+  return nvec * v + DevicePool::round(((size_t)8 * ntiles_of(n) + 2 * kRedGroups) * sizeof(T));
 }
 
 template <class T>
 int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cfg) {
   MGX_REQUIRE(A->rows == A->cols && A->rows > 0, "engine: square, non-empty system required");
   if (free_slots_.empty()) return -1;
-  MGX_HIP(hipStreamSynchronize(c_->stream));
-  fused_plan_build(A);
+  fused_plan_build(A);  // normally built at upload
   const int64_t n = A->rows, nt = ntiles_of(n);
   const int slot = free_slots_.back();
   std::vector<void *> got;
@@ -145,8 +151,9 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
     need(d.vb);
   }
   // partials: per quarter for the SpMV stages (CG 1 sum, BiCGSTAB 2), per
-  // tile for the updates (2 sums); then the stage-2 lane sums
-  const int64_t np = (cfg.method == kMethodCG ? 4 : 8) * nt;
+  // tile for the updates (2 sums), per quarter for the persistent CG (2
+  // sums); then the stage-2 lane sums
+  const int64_t np = 8 * nt;
   d.P = take((size_t)(np + 2 * kRedGroups) * sizeof(T));
   ok = ok && d.P != nullptr;
   if (ok) d.S = d.P + np;
@@ -166,6 +173,7 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   d.x = x;
   d.b = b;
   d.st = d_states_ + slot;
+  d.tick = d_ticks_ + (size_t)slot * kTickWords;
   d.rhist = cfg.rhist;
   d.rhist_len = cfg.rhist ? cfg.rhist_len : 0;
   d.row0 = 0;
@@ -173,28 +181,12 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   d.ntiles = nt;
   d.split = cfg.method == kMethodCG && n >= kSplitRows ? 1 : 0;
   d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+  d.method = cfg.method;
+  d.tol_scale = cfg.tol_relative ? cfg.tol : 0.0;
   csr_[slot] = A;
   val_[slot] = val;
   lds_entries_ = std::max(lds_entries_, A->fused_nnz);
-  const mgx_dtype dt = DTof<T>::v;
-  Ctx *c = c_;
-  if (d.M) {
-    launch_diag_v(A, val, dt, (void *)d.M);
-    launch_reciprocal(c, dt, n, d.M, (void *)d.M);
-  }
-  // r = b - A x (cg.cpp:39-40, bicgstab.cpp:47-48)
-  T *tmp = cfg.method == kMethodCG ? d.q : d.t;
-  launch_spmv(A, val, dt, x, tmp, 1, 0, false);
-  launch_vv(c, dt, 1, n, b, tmp, d.r);
-  if (cfg.method == kMethodCG) {
-    if (d.M)
-      launch_vv(c, dt, 2, n, d.M, d.r, d.z);
-    else
-      launch_copy(c, dt, n, d.r, d.z);
-  } else {
-    launch_copy(c, dt, n, d.r, d.r0);
-  }
-  SolveState hs;
+  SolveState &hs = h_states_[slot];
   std::memset(&hs, 0, sizeof hs);
   hs.tol = cfg.tol;
   hs.miniter = cfg.miniter;
@@ -202,22 +194,62 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   hs.status = kStatRunning;
   hs.first = 1;
   h_sys_[slot] = d;
-  MGX_HIP(hipMemcpy(d_states_ + slot, &hs, sizeof hs, hipMemcpyHostToDevice));
-  MGX_HIP(hipMemcpy(d_sys_ + slot, &d, sizeof d, hipMemcpyHostToDevice));
-  if (cfg.tol_relative)
-    batch_launch_sys_dot<T>(c, d_sys_ + slot, nullptr, nt, 1, (int)offsetof(SolveState, tol),
-                            cfg.tol);
-  if (cfg.method == kMethodCG) {
-    batch_launch_sys_dot<T>(c, d_sys_ + slot, nullptr, nt, 0, (int)offsetof(SolveState, zr), 0.0);
-  } else {
-    batch_launch_sys_dot<T>(c, d_sys_ + slot, nullptr, nt, 2, (int)offsetof(SolveState, rho), 0.0);
-    batch_launch_bcg_init<T>(c, d_states_ + slot, d_stopped_);
-  }
-  MGX_HIP(hipGetLastError());
+  rel_[slot] = cfg.tol_relative ? 1 : 0;
+  pending_.push_back(slot);
   active_.push_back(slot);
   host_running_++;
   dirty_ = true;
   return slot;
+}
+
+template <class T> void Engine<T>::flush_pending() {
+  if (pending_.empty()) return;
+  // the staging buffers of the previous flush may still be in use
+  MGX_HIP(hipStreamSynchronize(c_->stream));
+  const int np = (int)pending_.size();
+  const size_t sd = sizeof(SysDesc<T>) * np, ss = sizeof(SolveState) * np,
+               sl = sizeof(int32_t) * np;
+  const size_t bytes = sd + ss + sl;
+  h_stage_.resize(bytes);
+  std::vector<TileRef> tl;
+  bool any_m = false, any_rel = false, any_bcg = false;
+  for (int i = 0; i < np; i++) {
+    const int slot = pending_[i];
+    const SysDesc<T> &d = h_sys_[slot];
+    std::memcpy(h_stage_.data() + sizeof(SysDesc<T>) * i, &d, sizeof d);
+    std::memcpy(h_stage_.data() + sd + sizeof(SolveState) * i, &h_states_[slot],
+                sizeof(SolveState));
+    const int32_t s32 = slot;
+    std::memcpy(h_stage_.data() + sd + ss + sizeof(int32_t) * i, &s32, sizeof s32);
+    for (int64_t k = 0; k < d.ntiles; k++) tl.push_back(TileRef{slot, (int32_t)k});
+    any_m = any_m || d.M != nullptr;
+    any_rel = any_rel || rel_[slot];
+    any_bcg = any_bcg || method_[slot] == kMethodBiCGSTAB;
+  }
+  if (bytes > stage_cap_) {
+    dev_free(d_stage_);
+    d_stage_ = nullptr;
+    stage_cap_ = std::max(bytes, 2 * stage_cap_);
+    d_stage_ = dev_alloc(stage_cap_);
+  }
+  const int64_t nt = (int64_t)tl.size();
+  if (nt > ptiles_cap_) {
+    dev_free(d_ptiles_);
+    d_ptiles_ = nullptr;
+    ptiles_cap_ = std::max<int64_t>(nt, 2 * ptiles_cap_);
+    d_ptiles_ = (TileRef *)dev_alloc(sizeof(TileRef) * ptiles_cap_);
+  }
+  MGX_HIP(hipMemcpyAsync(d_stage_, h_stage_.data(), bytes, hipMemcpyHostToDevice, c_->stream));
+  MGX_HIP(hipMemcpyAsync(d_ptiles_, tl.data(), sizeof(TileRef) * nt, hipMemcpyHostToDevice,
+                         c_->stream));
+  char *base = static_cast<char *>(d_stage_);
+  batch_launch_setup<T>(c_, reinterpret_cast<const SysDesc<T> *>(base),
+                        reinterpret_cast<const SolveState *>(base + sd),
+                        reinterpret_cast<const int32_t *>(base + sd + ss), np, d_sys_, d_states_,
+                        d_ptiles_, nt, any_m, any_rel, any_bcg, lds_entries_, d_stopped_);
+  // tl is a local: the copies above must finish before it goes away
+  MGX_HIP(hipStreamSynchronize(c_->stream));
+  pending_.clear();
 }
 
 template <class T> void Engine<T>::rebuild_lists() {
@@ -272,6 +304,7 @@ template <class T> void Engine<T>::launch_iters(int iters) {
 }
 
 template <class T> void Engine<T>::step(int iters) {
+  flush_pending();
   if (dirty_) {
     rebuild_lists();
     gstale_ = true;
@@ -304,6 +337,38 @@ template <class T> void Engine<T>::step(int iters) {
   MGX_HIP(hipGraphLaunch(gexec_, c_->stream));
 }
 
+template <class T> bool Engine<T>::persistent_ok(int slot) const {
+  if (!c_->use_persistent || active_.size() != 1 || active_[0] != slot) return false;
+  const SysDesc<T> &d = h_sys_[slot];
+  return method_[slot] == kMethodCG && !d.split;
+}
+
+template <class T> void Engine<T>::run_persistent(int slot) {
+  MGX_REQUIRE(persistent_ok(slot), "engine: persistent solve needs a single CG system");
+  flush_pending();
+  if (!d_gs_) d_gs_ = (GridSync *)dev_alloc(sizeof(GridSync));
+  const int64_t nq = 4 * h_sys_[slot].ntiles;
+  const int cap = batch_cg_persist_blocks<T>(c_, lds_entries_);
+  MGX_REQUIRE(cap > 0, "engine: persistent kernel does not fit on the device");
+  int64_t grid = std::min<int64_t>(nq, cap);
+  if (c_->use_persistent > 1) grid = std::min<int64_t>(grid, c_->use_persistent);
+  // each launch runs at most this many iterations, then the host looks at
+  // the state and launches again if the solve is still running
+  const int per_launch = 4096;
+  for (;;) {
+    MGX_HIP(hipMemsetAsync(d_gs_, 0, sizeof(GridSync), c_->stream));
+    batch_launch_cg_persist<T>(c_, d_sys_ + slot, d_gs_, (int)grid, d_stopped_, lds_entries_,
+                               per_launch);
+    uint32_t err = 0;
+    MGX_HIP(hipMemcpyAsync(c_->h_pinned, &d_gs_->err, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c_->stream));
+    MGX_HIP(hipStreamSynchronize(c_->stream));
+    std::memcpy(&err, c_->h_pinned, sizeof err);
+    MGX_REQUIRE(err == 0, "persistent CG: a grid barrier wait timed out");
+    if (!poll().empty()) return;
+  }
+}
+
 template <class T> std::vector<int> Engine<T>::poll() {
   std::vector<int> out;
   int32_t stopped = 0;
@@ -327,12 +392,19 @@ template <class T> std::vector<int> Engine<T>::poll() {
 
 template <class T> SolveState Engine<T>::state(int slot) { return h_states_[slot]; }
 
+template <class T> void Engine<T>::fetch_states() {
+  MGX_HIP(hipStreamSynchronize(c_->stream));
+  MGX_HIP(hipMemcpy(h_states_.data(), d_states_, sizeof(SolveState) * cap_,
+                    hipMemcpyDeviceToHost));
+}
+
 template <class T> void Engine<T>::release(int slot) {
   MGX_HIP(hipStreamSynchronize(c_->stream));
   auto it = std::find(active_.begin(), active_.end(), slot);
   if (it == active_.end()) return;
   if (!reported_[slot]) host_running_--;
   active_.erase(it);
+  pending_.erase(std::remove(pending_.begin(), pending_.end(), slot), pending_.end());
   for (void *p : bufs_[slot]) pool_->put(p);
   bufs_[slot].clear();
   free_slots_.push_back(slot);

@@ -3,6 +3,7 @@
 
 #include <deque>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 
 #include "mgx_internal.hpp"
@@ -23,6 +24,7 @@ template <class T> struct SysDesc {
   T *S;  // stage-2 lane sums (2 * kRedGroups)
   T *r0, *s, *shat, *phat, *t, *va, *vb;  // BiCGSTAB
   SolveState *st;
+  uint32_t *tick;  // kTickWords arrival counters of this system's reductions
   double *rhist;
   int64_t rhist_len;
   int64_t row0, n, ntiles;
@@ -30,6 +32,9 @@ template <class T> struct SysDesc {
   // by the plan-based SpMV; <p, q>; update), 0 = two fused kernels
   int32_t split;
   int32_t vec;  // row0 even and x, b 16-byte aligned: vector loads allowed
+  int32_t method;  // kMethodCG or kMethodBiCGSTAB
+  int32_t pad_;
+  double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
 };
 
 struct TileRef {
@@ -57,12 +62,32 @@ void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int
 template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
                                 int64_t nlaunch, int32_t *stopped, int lds_entries);
-template <class T> void batch_launch_bcg_init(Ctx *c, SolveState *st, int32_t *stopped);
-// which: 0 <z,r>, 1 <b,b>, 2 <r,r>; result into the SolveState field at
-// byte offset field_off (scale != 0: scale * sqrt(result))
+
+// grid barrier state of the persistent kernels; zero before each launch
+struct GridSync {
+  uint64_t group[64];
+  uint64_t top;
+  uint64_t release;
+  uint32_t err;  // a block timed out waiting
+  uint32_t pad;
+};
+// blocks of the persistent CG kernel that fit on the device at once
+template <class T> int batch_cg_persist_blocks(Ctx *c, int lds_entries);
+// up to max_iters iterations of the single system sys[0] in one cooperative
+// launch of `grid` blocks (grid <= batch_cg_persist_blocks)
 template <class T>
-void batch_launch_sys_dot(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
-                          int which, int field_off, double scale);
+void batch_launch_cg_persist(Ctx *c, const SysDesc<T> *sys, GridSync *gs, int grid,
+                             int32_t *stopped, int lds_entries, int max_iters);
+
+// set-up of newly admitted systems: scatter their staged descriptors and
+// states into their slots, then (over the list of their tiles) M = 1/diag,
+// r = b - A x with z = M r or r0 = r, the relative tolerances, <z,r> or
+// <r,r0>, and the BiCGSTAB first-iteration breakdown test
+template <class T>
+void batch_launch_setup(Ctx *c, const SysDesc<T> *stage_d, const SolveState *stage_s,
+                        const int32_t *stage_slots, int nstage, SysDesc<T> *sys,
+                        SolveState *states, const TileRef *tiles, int64_t ntiles, bool any_m,
+                        bool any_rel, bool any_bcg, int lds_entries, int32_t *stopped);
 
 // device memory pool with reuse by size (no hipMalloc on the steady path)
 class DevicePool {
@@ -77,8 +102,8 @@ class DevicePool {
 
  private:
   size_t budget_, in_use_ = 0, reserved_ = 0;
-  std::vector<std::pair<size_t, void *>> free_;
-  std::vector<std::pair<void *, size_t>> live_;
+  std::unordered_map<size_t, std::vector<void *>> free_;  // by rounded size
+  std::unordered_map<void *, size_t> live_;
 };
 
 struct SolveCfg {
@@ -93,8 +118,10 @@ struct SolveCfg {
 
 // A set of systems iterated together.  add() uploads nothing: the matrix and
 // x/b are device resident; it allocates the work vectors from the pool and
-// queues the set-up kernels.  step() queues `iters` fused iterations of all
-// running systems; poll() reads which systems have stopped.
+// records the system.  The set-up of all systems added since the last launch
+// runs in a few shared launches ahead of the next step() (or
+// run_persistent()).  step() queues `iters` fused iterations of all running
+// systems; poll() reads which systems have stopped.
 template <class T> class Engine {
  public:
   Engine(Ctx *c, DevicePool *pool, int capacity);
@@ -109,10 +136,21 @@ template <class T> class Engine {
   SolveState state(int slot);  // valid after poll() reported the slot
   void release(int slot);
   int active() const { return (int)active_.size(); }
+  // run the set-up of the systems added since the last launch now
+  void prepare() { flush_pending(); }
+  // copy every slot's device state to the host (state() then reads it)
+  void fetch_states();
+  const SysDesc<T> &desc(int slot) const { return h_sys_[slot]; }
+  // the one running system is a CG system of the two-kernel form, and the
+  // context allows the persistent kernel
+  bool persistent_ok(int slot) const;
+  // iterate that system to its end with the persistent kernel
+  void run_persistent(int slot);
 
  private:
   void rebuild_lists();
   void launch_iters(int iters);
+  void flush_pending();  // set-up of the systems added since the last launch
   hipGraphExec_t gexec_ = nullptr;
   int giters_ = 0;
   bool gstale_ = true;
@@ -136,6 +174,15 @@ template <class T> class Engine {
   std::vector<char> reported_;
   int host_running_ = 0;
   int lds_entries_ = 0;
+  GridSync *d_gs_ = nullptr;      // grid barrier of the persistent kernel
+  uint32_t *d_ticks_ = nullptr;   // kTickWords arrival counters per slot
+  std::vector<int> pending_;      // slots added since the last launch
+  std::vector<char> rel_;         // per slot: relative tolerance
+  std::vector<char> h_stage_;     // staged descriptors / states / slots
+  void *d_stage_ = nullptr;
+  size_t stage_cap_ = 0;
+  TileRef *d_ptiles_ = nullptr;   // tiles of the pending systems
+  int64_t ptiles_cap_ = 0;
   bool any_fused_ = false;        // a listed CG system uses the two-kernel form
   std::vector<int> split_slots_;  // listed CG systems of the split form
   std::vector<Csr *> csr_;        // per slot: matrix and values (split SpMV)

@@ -62,7 +62,9 @@ constexpr int kFusedNnz = 2048;  // entry budget of a fused-SpMV chunk
 constexpr int kStatRunning = 1;  // solver status while iterating
 
 // arrival counters of the cross-block reductions (mgx_device.hpp): groups
-// of blocks, supergroups of groups, one top counter
+// of blocks, supergroups of groups, one top counter; an array of kTickWords
+// zeros per reduction user: [0] top, [1, 1 + kRedSuper) supergroups, then
+// the groups (last_block_arrives uses [0, 1 + 64))
 constexpr uint32_t kTicketGroups = 64;  // last_block_arrives
 constexpr int kRedGroups = 256;         // arrive_reduce: one group per stage-2 lane
 constexpr int kRedSuper = 16;
@@ -75,10 +77,6 @@ struct SolveState {
   double tol, bnorm;
   int64_t iter, miniter, maxiter;
   int32_t status, first;
-  // arrival counters, all zero between launches: [0] top, [1, 1 + kRedSuper)
-  // supergroups, then the groups (last_block_arrives uses [0, 1 + 64))
-  uint32_t tick[kTickWords];
-  uint32_t pad32;
 };
 
 // ------------------------------------------------------------ handles --
@@ -99,7 +97,11 @@ struct SpmvPlan {
   int64_t nlong = 0;
   int32_t max_row = 0, min_row = 0;
   int32_t nnz_max = 0;  // entry budget of a stream block (LDS products)
+  int32_t rows_max = 0;  // row budget of a stream block
+  int32_t long_row = 0;  // rows longer than this get a block of their own
+  int32_t remap = 1;     // XCD-aware block order
   double mean_row = 0;
+  double tuned_us = 0;   // SpMV time measured by the autotuner (0: not tuned)
 };
 
 struct Csr {
@@ -131,11 +133,14 @@ struct Ctx {
   void *d_partials = nullptr;
   size_t partials_bytes = 0;
   SolveState *d_state = nullptr;     // scratch solve state for BLAS calls
+  uint32_t *d_tick = nullptr;        // arrival counters of the BLAS reductions
   double *h_pinned = nullptr;        // pinned host scalars (64 doubles)
   hipEvent_t ev = nullptr;
   // SpMV tuning
   int spmv_rows = 256, spmv_nnz = 2048, spmv_long = 256, xcd_remap = 1;
   int use_graphs = 0;  // replay fused iterations from a captured hipGraph
+  int use_persistent = 0;  // single small CG solves in one cooperative launch
+                           // (> 1: also the largest grid it may use)
   // fused-engine state owned by the context (engine.hip)
   void *pool = nullptr;   // DevicePool for solver work vectors
   void *eng64 = nullptr;  // Engine<double> for single fused solves
@@ -187,15 +192,22 @@ double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y);
 
 void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp_host);
+// plan with explicit budgets (the context's defaults otherwise)
+void spmv_plan_build_with(Csr *A, const int32_t *rp_host, int rows_max, int nnz_max,
+                          int long_row, int remap);
 // live: optional device status word; the launch does nothing unless it
 // reads kStatRunning (device-driven solver loops)
 void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
                  void *y, double alpha, double beta, bool ab,
                  const int32_t *live = nullptr);
 void launch_diag(Csr *A, void *d);
+void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out);
 void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
-// fused-CG row chunks (built on first use of a fused solver)
-void fused_plan_build(Csr *A);
+// fused-solver row chunks (built at upload for square matrices, from the
+// host row_ptr when given, else from a copy of the device one)
+void fused_plan_build(Csr *A, const int32_t *rp_host = nullptr);
+// FP32 shadow of an FP64 matrix's values (mixed precision), made once
+void ensure_val32(Csr *A);
 
 double seconds_now();
 

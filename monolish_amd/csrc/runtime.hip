@@ -237,6 +237,8 @@ int mgx_init(int device, mgx_ctx **out) {
     MGX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->d_state = (SolveState *)dev_alloc(sizeof(SolveState));
     MGX_HIP(hipMemset(c->d_state, 0, sizeof(SolveState)));
+    c->d_tick = (uint32_t *)dev_alloc(kTickWords * sizeof(uint32_t));
+    MGX_HIP(hipMemset(c->d_tick, 0, kTickWords * sizeof(uint32_t)));
     MGX_HIP(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
     MGX_HIP(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming));
     ensure_partials(c, 1 << 20);
@@ -255,6 +257,7 @@ int mgx_finalize(mgx_ctx *ctx) {
     for (auto &e : ctx->ws_free) dev_free(e.first);
     dev_free(ctx->d_partials);
     dev_free(ctx->d_state);
+    dev_free(ctx->d_tick);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev) (void)hipEventDestroy(ctx->ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -309,6 +312,26 @@ int mgx_set_graphs(mgx_ctx *ctx, int enable) {
   return guard([&] {
     MGX_REQUIRE(ctx, "mgx_set_graphs: NULL ctx");
     ctx->use_graphs = enable ? 1 : 0;
+    return MGX_OK;
+  });
+}
+
+int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out) {
+  return guard([&] {
+    MGX_REQUIRE(A, "mgx_csr_autotune: NULL matrix");
+    MGX_REQUIRE(reps >= 1 && reps <= 10000, "mgx_csr_autotune: reps must be in [1, 10000]");
+    activate(A->ctx);
+    MGX_HIP(hipStreamSynchronize(A->ctx->stream));
+    spmv_autotune(A, reps, out);
+    return MGX_OK;
+  });
+}
+
+int mgx_set_persistent(mgx_ctx *ctx, int enable) {
+  return guard([&] {
+    MGX_REQUIRE(ctx, "mgx_set_persistent: NULL ctx");
+    MGX_REQUIRE(enable >= 0, "mgx_set_persistent: negative value");
+    ctx->use_persistent = enable;
     return MGX_OK;
   });
 }
@@ -465,6 +488,7 @@ int mgx_csr_create(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
       }
       if (raw_differs) A->rp_raw.assign(row_ptr, row_ptr + rows + 1);
       spmv_plan_build(ctx, A, rp.data());
+      if (rows == cols) fused_plan_build(A, rp.data());
     } catch (...) {
       csr_free(A);
       throw;

@@ -36,13 +36,25 @@ template <class T> T host_sqrt(T v) { return std::sqrt(v); }
 // 128-row quarter of a 512-row tile (one fused-SpMV block per quarter);
 // rows longer than the budget become long chunks.  d_tile_chunk[q] is the
 // first chunk of quarter q (4 per tile, empty past the last row).
-void fused_plan_build(Csr *A) {
-  if (A->d_chunks) return;
+void ensure_val32(Csr *A) {
+  if (A->val32) return;
   Ctx *c = A->ctx;
+  // padded like col_ind/val: vector loads may read past the last entry
+  A->val32 = (float *)dev_alloc((size_t)(A->nnz + 4) * 4);
+  MGX_HIP(hipMemsetAsync(A->val32 + A->nnz, 0, 16, c->stream));
+  launch_convert(c, MGX_F64, MGX_F32, A->nnz, A->val, A->val32);
+}
+
+void fused_plan_build(Csr *A, const int32_t *rp_host) {
+  if (A->d_chunks) return;
   const int64_t n = A->rows;
-  std::vector<int32_t> rp(n + 1);
-  MGX_HIP(hipMemcpy(rp.data(), A->rp, (n + 1) * 4, hipMemcpyDeviceToHost));
-  (void)c;
+  std::vector<int32_t> rp_copy;
+  const int32_t *rp = rp_host;
+  if (!rp) {
+    rp_copy.resize(n + 1);
+    MGX_HIP(hipMemcpy(rp_copy.data(), A->rp, (n + 1) * 4, hipMemcpyDeviceToHost));
+    rp = rp_copy.data();
+  }
   const int budget = kFusedNnz;
   const int64_t nq = 4 * ntiles_of(n);
   std::vector<int32_t> chunks, tile_chunk(nq + 1);
@@ -180,10 +192,14 @@ SolveOut fused_solve(int method, Csr *A, const void *val, T *x, const T *b, int 
   cfg.rhist_len = rlen;
   const int slot = eng.add(A, val, x, b, cfg);
   MGX_REQUIRE(slot >= 0, "fused solve: out of device memory for the work vectors");
-  const int every = check_every > 0 ? check_every : 32;
-  for (;;) {
-    eng.step(every);
-    if (!eng.poll().empty()) break;
+  if (eng.persistent_ok(slot)) {
+    eng.run_persistent(slot);
+  } else {
+    const int every = check_every > 0 ? check_every : 32;
+    for (;;) {
+      eng.step(every);
+      if (!eng.poll().empty()) break;
+    }
   }
   SolveState hs = eng.state(slot);
   eng.release(slot);
@@ -410,12 +426,7 @@ SolveOut ir_cg(Csr *A, double *x, const double *b, int precond, double thr,
                int check_every, int64_t *inner_total, int64_t *outer_steps) {
   Ctx *c = A->ctx;
   const int64_t n = A->rows;
-  if (!A->val32) {
-    // padded like col_ind/val: vector loads may read past the last entry
-    A->val32 = (float *)dev_alloc((size_t)(A->nnz + 4) * 4);
-    MGX_HIP(hipMemsetAsync(A->val32 + A->nnz, 0, 16, c->stream));
-    launch_convert(c, MGX_F64, MGX_F32, A->nnz, A->val, A->val32);
-  }
+  ensure_val32(A);
   double *r = (double *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 8);
   float *r32 = (float *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 4);
   float *d32 = (float *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 4);

@@ -126,10 +126,16 @@ __global__ __launch_bounds__(kBlock) void k_diag(int64_t n, const int32_t *__res
 }  // namespace
 
 void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp) {
+  spmv_plan_build_with(A, rp, c->spmv_rows, c->spmv_nnz, c->spmv_long, c->xcd_remap);
+}
+
+void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, int long_req,
+                          int remap_req) {
+  Ctx *c = A->ctx;
   const int64_t rows = A->rows;
-  const int rows_max = std::min(std::max(c->spmv_rows, 1), kBlock);
-  const int nnz_max = std::max(c->spmv_nnz, 64);
-  const int long_t = std::min(std::max(c->spmv_long, 1), nnz_max);
+  const int rows_max = std::min(std::max(rows_req, 1), kBlock);
+  const int nnz_max = std::max(nnz_req, 64);
+  const int long_t = std::min(std::max(long_req, 1), nnz_max);
   std::vector<int32_t> blk;
   blk.reserve((size_t)(rows / std::max(rows_max, 1)) + 16);
   int32_t mx = 0, mn = rows > 0 ? INT32_MAX : 0;
@@ -164,6 +170,10 @@ void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp) {
   A->plan.min_row = mn;
   A->plan.mean_row = rows > 0 ? (double)A->nnz / (double)rows : 0.0;
   A->plan.nnz_max = nnz_max;
+  A->plan.rows_max = rows_max;
+  A->plan.long_row = long_t;
+  A->plan.remap = remap_req ? 1 : 0;
+  A->plan.tuned_us = 0;
   if (A->plan.d_blocks) dev_free(A->plan.d_blocks);
   A->plan.d_blocks = (int32_t *)dev_alloc(blk.size() * sizeof(int32_t));
   MGX_HIP(hipMemcpyAsync(A->plan.d_blocks, blk.data(),
@@ -177,7 +187,7 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
                           T beta, const int32_t *live) {
   Ctx *c = A->ctx;
   const int64_t nb = A->plan.nblocks;
-  const int remap = c->xcd_remap && nb >= 64;
+  const int remap = A->plan.remap && nb >= 64;
   const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
   const size_t lds = (size_t)A->plan.nnz_max * sizeof(T);
   // vector loads per lane: cover nnz_max entries plus the alignment slack
@@ -239,5 +249,67 @@ void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d) {
 }
 
 void launch_diag(Csr *A, void *d) { launch_diag_v(A, A->val, A->dtype, d); }
+
+// Autotuner: times A's SpMV under candidate block plans (rows and entries
+// per stream block, XCD order) and keeps the fastest.  The long-row
+// threshold is kept, so every candidate computes the same bits.
+void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
+  Ctx *c = A->ctx;
+  const int64_t rows = A->rows, cols = A->cols;
+  std::vector<int32_t> rp(rows + 1);
+  MGX_HIP(hipMemcpy(rp.data(), A->rp, (size_t)(rows + 1) * 4, hipMemcpyDeviceToHost));
+  const size_t es = A->elem();
+  void *x = dev_alloc((size_t)std::max<int64_t>(cols, 1) * es);
+  void *y = dev_alloc((size_t)std::max<int64_t>(rows, 1) * es);
+  MGX_HIP(hipMemsetAsync(x, 0, (size_t)std::max<int64_t>(cols, 1) * es, c->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const int long_t = A->plan.long_row > 0 ? A->plan.long_row : c->spmv_long;
+  struct Cand {
+    int rows_max, nnz_max, remap;
+    double us;
+  };
+  std::vector<Cand> cands;
+  for (int rm : {64, 128, 256})
+    for (int nm : {1024, 2048, 4096})
+      for (int rr : {1, 0})
+        if (nm >= long_t) cands.push_back(Cand{rm, nm, rr, 0.0});
+  try {
+    MGX_HIP(hipEventCreate(&e0));
+    MGX_HIP(hipEventCreate(&e1));
+    for (Cand &k : cands) {
+      spmv_plan_build_with(A, rp.data(), k.rows_max, k.nnz_max, long_t, k.remap);
+      for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+      MGX_HIP(hipEventRecord(e0, c->stream));
+      for (int i = 0; i < reps; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+      MGX_HIP(hipEventRecord(e1, c->stream));
+      MGX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      k.us = 1e3 * ms / reps;
+    }
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(x);
+    dev_free(y);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(x);
+  dev_free(y);
+  const Cand best = *std::min_element(cands.begin(), cands.end(),
+                                      [](const Cand &a, const Cand &b) { return a.us < b.us; });
+  spmv_plan_build_with(A, rp.data(), best.rows_max, best.nnz_max, long_t, best.remap);
+  A->plan.tuned_us = best.us;
+  if (out) {
+    out->rows_per_block = best.rows_max;
+    out->nnz_per_block = best.nnz_max;
+    out->long_row = long_t;
+    out->xcd_remap = best.remap;
+    out->us = best.us;
+    out->candidates = (int32_t)cands.size();
+  }
+}
 
 }  // namespace mgx

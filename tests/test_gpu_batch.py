@@ -152,3 +152,26 @@ def test_server_mix_equals_alone(m, ctx, oracle):
     assert r.status == m.ERR_BUDGET
     srv.close()
     c2.close()
+
+
+def test_batch_ir_cg_equals_single_ir(m, ctx, oracle):
+    """FP32-refined batched CG: every system follows exactly the arithmetic
+    of the single-system mixed-precision solve (same outer and inner
+    iteration counts, same x bit for bit) and reaches the FP64 tolerance."""
+    say("batch ir")
+    syss = [m.mmatrix(n, 5, 15, 0.5, seed=70 + n) for n in (700, 1500, 3100, 5000)]
+    bs = [oracle.random_vector(len(rp) - 1, -1.0, 1.0, 71 + i) for i, (rp, _, _) in enumerate(syss)]
+    As = [m.CRS(ctx, len(b), len(b), rp, ci, v) for (rp, ci, v), b in zip(syss, bs)]
+    Bs = [m.Vector(ctx, b) for b in bs]
+    Xs = [m.Vector(ctx, n=len(b)) for b in bs]
+    res = m.batch_ir_cg(ctx, As, Xs, Bs, tol=1e-10, tol_is_relative=True, inner_tol=1e-5)
+    for A, X, B, (rp, ci, v), b, r in zip(As, Xs, Bs, syss, bs, res):
+        assert r.status == 0 and r.outer_iter >= 1
+        x = X.download()
+        assert oracle.residual_l2(rp, ci, v, x, b) < 1e-10 * oracle.nrm2(b)
+        X1 = m.Vector(ctx, n=len(b))
+        r1 = m.ir_cg(A, X1, B, tol=1e-10, tol_is_relative=True, inner_tol=1e-5)
+        say(len(b), r.outer_iter, r.inner_iter, r1.outer_iter, r1.inner_iter)
+        assert (r.outer_iter, r.inner_iter) == (r1.outer_iter, r1.inner_iter)
+        assert r.final_resid == r1.final_resid
+        assert np.array_equal(bits(x), bits(X1.download()))

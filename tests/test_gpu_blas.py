@@ -135,6 +135,30 @@ def test_matvec_tuning_invariance(m, oracle):
     assert bits_equal(outs[2], ref)  # every row sequential when long_row >= max length
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_autotune(m, ctx, oracle, dtype):
+    """The autotuner picks one of its candidate plans; the product is the same
+    bit for bit before and after, and equals the CPU row loop."""
+    say("matvec autotune", dtype.__name__)
+    rp, ci, val = m.stencil3d(40, 37, 29, 27, 1e-2)
+    val = val.astype(dtype)
+    n = len(rp) - 1
+    x = oracle.random_vector(n, -1.0, 1.0, 2001, dtype=dtype)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    X = m.Vector(ctx, x, dtype=dtype)
+    Y = m.Vector(ctx, n=n, dtype=dtype)
+    m.matvec(A, X, Y)
+    before = Y.download()
+    t = A.autotune(reps=3)
+    assert t["rows_per_block"] in (64, 128, 256) and t["nnz_per_block"] in (1024, 2048, 4096)
+    assert t["candidates"] == 18 and t["us"] > 0 and t["long_row"] == 256
+    Y.fill(0.0)
+    m.matvec(A, X, Y)
+    after = Y.download()
+    assert bits_equal(before, after)
+    assert bits_equal(after, oracle.spmv(rp, ci, val, x))
+
+
 def test_matvec_edge_shapes(m, ctx, oracle):
     say("matvec edge shapes")
     # empty rows, trailing empty rows left at 0 by CRS(COO&), one long row

@@ -130,12 +130,41 @@ def test_cg_fused_equals_reference_mode(m, ctx, oracle, dtype, precond):
         assert bits_equal(res.rhistory, ref[4]), mode
 
 
+@pytest.mark.parametrize("blocks", [1, 0, 7, 64])
+def test_cg_persistent_kernel(m, ctx, oracle, blocks):
+    """The fused CG of one small system in one cooperative launch (blocks=1:
+    a grid of one block per 128-row quarter; 7, 64: capped grids, blocks own
+    several quarters; 0: one launch per stage) is bit-identical to the
+    oracle, including the maxiter and NaN exits."""
+    say("cg persistent", blocks)
+    rp, ci, val = _poisson(oracle, 60)  # n = 3600: 8 tiles, 32 quarters
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 1001)
+    tol = 1e-10 * float(oracle.nrm2(b))
+    m.set_persistent(ctx, blocks != 0, max_blocks=blocks)
+    try:
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=tol, history=True)
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, tol, mode=1, history=4000)
+        assert res.status == ref[0] == 0 and res.final_iter == ref[2]
+        assert bits_equal(x, ref[1]) and bits_equal(res.rhistory, ref[4])
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=1, tol=1e-30, maxiter=9)
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 1, 1e-30, maxiter=9, mode=1)
+        assert res.status == m.SOLVER_MAXITER and res.final_iter == 9
+        assert bits_equal(x, ref[1]) and res.final_resid == ref[3]
+        bn = b.copy()
+        bn[77] = np.nan
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), bn, 2, 1e-8, mode=1)
+        assert res.status == m.SOLVER_RESIDUAL_NAN and res.final_iter == 1
+    finally:
+        m.set_persistent(ctx, False)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_cg_fused_large_three_kernel_form(m, ctx, oracle, dtype):
-    """n >= 2^19 rows: the fused solver runs the three-kernel iteration (the
-    SpMV gathers p alone); 40 iterations stay bit-identical to the oracle and
-    to the reference-shaped GPU loop."""
-    say("cg fused three-kernel", dtype.__name__)
+def test_cg_fused_large_split_iteration(m, ctx, oracle, dtype):
+    """n >= 2^19 rows: the fused solver runs the split iteration (p update,
+    plan-based SpMV, <p, q>, update); 40 iterations stay bit-identical to the
+    oracle and to the reference-shaped GPU loop."""
+    say("cg fused split", dtype.__name__)
     rp, ci, val = m.stencil3d(96, 96, 64, 7, 0.0)
     val = val.astype(dtype)
     n = len(rp) - 1
