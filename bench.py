@@ -96,11 +96,13 @@ def cpu_info():
     return model
 
 
-def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup):
+def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup, autotune=True):
     rows = len(rp) - 1
     A = m.CRS(ctx, rows, len(x), rp, ci, val)
     X = m.Vector(ctx, x, dtype=val.dtype)
     Y = m.Vector(ctx, n=rows, dtype=val.dtype)
+    # the library's autotuner picks the block plan (outside the timed region)
+    plan = A.autotune(reps=10) if autotune else None
     ms, wall = time_spmv(m, ev, ctx, A, X, Y, steps, warmup)
     nbytes = spmv_bytes(len(val), rows, len(x), val.dtype.itemsize)
     gbs = nbytes / (ms * 1e-3) / 1e9
@@ -108,9 +110,13 @@ def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup):
     A.free()
     X.free()
     Y.free()
-    return {"us_per_spmv": round(ms * 1e3, 2), "GB_s": round(gbs, 1),
-            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes, "nnz": int(len(val)),
-            "rows": rows}, ms, wall
+    out = {"us_per_spmv": round(ms * 1e3, 2), "GB_s": round(gbs, 1),
+           "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes, "nnz": int(len(val)),
+           "rows": rows}
+    if plan:
+        out["plan"] = {k: plan[k] for k in ("rows_per_block", "nnz_per_block", "long_row",
+                                            "xcd_remap", "candidates")}
+    return out, ms, wall
 
 
 def bench_krylov(m, ctx, reps_fused, reps_plain):
@@ -121,6 +127,7 @@ def bench_krylov(m, ctx, reps_fused, reps_plain):
     b = oracle.random_vector(n, -1.0, 1.0, 1001)
     tol = 1e-10 * float(oracle.nrm2(b))
     A = m.CRS(ctx, n, n, rp, ci, val)
+    A.autotune(reps=10)
     B = m.Vector(ctx, b)
     X = m.Vector(ctx, n=n)
     out = {"workload": "Jacobi-PCG, 2D 5-point Poisson 256x256 (n=65536, nnz=326656), "
@@ -171,6 +178,7 @@ def bench_mixed(m, ctx, side, reps):
     b = oracle.random_vector(n, -1.0, 1.0, 3001)
     bn = float(oracle.nrm2(b))
     A = m.CRS(ctx, n, n, rp, ci, val)
+    A.autotune(reps=5)
     B = m.Vector(ctx, b)
     X = m.Vector(ctx, n=n)
     res = {}
@@ -406,8 +414,9 @@ def main():
     ap.add_argument("--serve-nmax", type=int, default=100000,
                     help="largest served system (rows); the full stream uses 1e6")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="only run 5 vadd launches of known size (counter calibration) and 5 "
-                         "SpMVs on the 192^3 matrix, for rocprofv3 --pmc runs")
+                    help="only run 5 vadd launches of known size (counter calibration) and, "
+                         "after the autotuner, 5 SpMVs on the 192^3 matrix (the last 5 SpMV "
+                         "dispatches), for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
@@ -433,6 +442,8 @@ def main():
         A = m.CRS(ctx, len(rp) - 1, len(rp) - 1, rp, ci, val)
         X = m.Vector(ctx, np.ones(len(rp) - 1))
         Y = m.Vector(ctx, n=len(rp) - 1)
+        A.autotune(reps=10)  # the plan the headline measures
+        ctx.sync()
         for _ in range(5):
             m.matvec(A, X, Y)
         ctx.sync()

@@ -265,9 +265,37 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
     publish_partial(d.P, tile, zr_a);
     publish_partial(d.P, d.ntiles + tile, rr_a);
   }
+  if (d.split) return;  // k_cg_finish reduces the partials of the split form
   if (arrive_reduce<T, 2, 1>(d.tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
                              &flag)) {
     if (t == 0) cg_finish<T>(d, st, res[0], res[1], stopped);
+  }
+}
+
+// split form: stage 2 of the canonical reductions by one block per system,
+// after the kernel that published the tile partials.  stage 0: <p, q> ->
+// alpha (cg.cpp:55-58); stage 1: <z, r> and <r, r> -> cg_finish
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_finish(const SysDesc<T> *sys,
+                                                      const int32_t *slots, int nslots,
+                                                      int stage, int32_t *stopped) {
+  __shared__ T sh[8];
+  if ((int)blockIdx.x >= nslots) return;
+  const SysDesc<T> &d = sys[slots[blockIdx.x]];
+  SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  if (stage == 0) {
+    const T pq = reduce_partials(d.P, d.ntiles, sh);
+    if (threadIdx.x == 0) {
+      const T zr = (T)st->zr;
+      st->pq = (double)pq;
+      st->alpha = (double)(zr / pq);
+    }
+  } else {
+    const T zr_new = reduce_partials(d.P, d.ntiles, sh);
+    __syncthreads();
+    const T rr = reduce_partials(d.P + d.ntiles, d.ntiles, sh);
+    if (threadIdx.x == 0) cg_finish<T>(d, st, zr_new, rr, stopped);
   }
 }
 
@@ -479,15 +507,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const Ti
   }
 }
 
-// split iteration, stage 3: <p, q> in tile order; alpha = <z,r> / <p,q>
-// (cg.cpp:55-58)
+// split iteration, stage 3: tile partials of <p, q>
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_pq(const SysDesc<T> *sys, const TileRef *tiles,
                                                   int64_t nlaunch) {
   __shared__ T sh[8];
-  __shared__ T tp[kBlock];
-  __shared__ T res[2];
-  __shared__ int flag;
   const int64_t b = blockIdx.x;
   if (b >= nlaunch) return;
   int64_t tile;
@@ -509,14 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq(const SysDesc<T> *sys, const T
   }
   const T s = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(d.P, tile, s);
-  if (arrive_reduce<T, 1, 1>(d.tick, d.P, 0, d.S, d.ntiles, tile, tp, sh, res, &flag)) {
-    if (threadIdx.x == 0) {
-      const T pq = res[0];
-      const T zr = (T)st->zr;
-      st->pq = (double)pq;
-      st->alpha = (double)(zr / pq);
-    }
-  }
+  // k_cg_finish (stage 0) adds the tile partials and computes alpha
 }
 
 // ---------------------------------------------------------- BiCGSTAB ----
@@ -1015,6 +1032,14 @@ void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int
 }
 
 template <class T>
+void batch_launch_cg_finish(Ctx *c, const SysDesc<T> *sys, const int32_t *slots, int nslots,
+                            int stage, int32_t *stopped) {
+  if (nslots > 0)
+    k_cg_finish<T><<<(unsigned)nslots, kBlock, 0, c->stream>>>(sys, slots, nslots, stage,
+                                                               stopped);
+}
+
+template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
                                 int64_t nlaunch, int32_t *stopped, int lds_entries) {
   if (nlaunch <= 0) return;
@@ -1042,6 +1067,10 @@ template void batch_launch_cg_p<float>(Ctx *, const SysDesc<float> *, const Tile
 template void batch_launch_cg_pq<double>(Ctx *, const SysDesc<double> *, const TileRef *,
                                          int64_t);
 template void batch_launch_cg_pq<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);
+template void batch_launch_cg_finish<double>(Ctx *, const SysDesc<double> *, const int32_t *, int,
+                                             int, int32_t *);
+template void batch_launch_cg_finish<float>(Ctx *, const SysDesc<float> *, const int32_t *, int,
+                                            int, int32_t *);
 template void batch_launch_bicgstab_iter<double>(Ctx *, const SysDesc<double> *,
                                                  const TileRef *, int64_t, int32_t *, int);
 template void batch_launch_bicgstab_iter<float>(Ctx *, const SysDesc<float> *, const TileRef *,

@@ -100,6 +100,7 @@ template <class T> Engine<T>::~Engine() {
   dev_free(d_ticks_);
   dev_free(d_stage_);
   dev_free(d_ptiles_);
+  dev_free(d_split_slots_);
 }
 
 template <class T> size_t Engine<T>::bytes_needed(const Csr *A, int method) const {
@@ -280,6 +281,17 @@ template <class T> void Engine<T>::rebuild_lists() {
       MGX_HIP(hipMemcpy(d_tiles_[m], lst[m].data(), sizeof(TileRef) * ntiles_[m],
                         hipMemcpyHostToDevice));
   }
+  const int64_t ns = (int64_t)split_slots_.size();
+  if (ns > split_cap_) {
+    dev_free(d_split_slots_);
+    d_split_slots_ = nullptr;
+    split_cap_ = std::max<int64_t>(ns, 2 * split_cap_);
+    d_split_slots_ = (int32_t *)dev_alloc(sizeof(int32_t) * split_cap_);
+  }
+  if (ns) {
+    std::vector<int32_t> sl(split_slots_.begin(), split_slots_.end());
+    MGX_HIP(hipMemcpy(d_split_slots_, sl.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+  }
   dirty_ = false;
 }
 
@@ -293,9 +305,14 @@ template <class T> void Engine<T>::launch_iters(int iters) {
           launch_spmv(csr_[s], val_[s], DTof<T>::v, h_sys_[s].pa, h_sys_[s].q, 1.0, 0.0, false,
                       &d_states_[s].status);
         batch_launch_cg_pq<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
+        batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 0,
+                                  d_stopped_);
       }
       batch_launch_cg_iter<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_, lds_entries_,
                               any_fused_);
+      if (!split_slots_.empty())
+        batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 1,
+                                  d_stopped_);
     }
     if (ntiles_[1])
       batch_launch_bicgstab_iter<T>(c_, d_sys_, d_tiles_[1], ntiles_[1], d_stopped_,

@@ -59,6 +59,12 @@ template <class T>
 void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
 template <class T>
 void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
+// split form: stage 2 of <p,q> (stage 0, after batch_launch_cg_pq) or of
+// <z,r>, <r,r> with the convergence test (stage 1, after the update), one
+// block per listed split system
+template <class T>
+void batch_launch_cg_finish(Ctx *c, const SysDesc<T> *sys, const int32_t *slots, int nslots,
+                            int stage, int32_t *stopped);
 template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
                                 int64_t nlaunch, int32_t *stopped, int lds_entries);
@@ -185,6 +191,8 @@ template <class T> class Engine {
   int64_t ptiles_cap_ = 0;
   bool any_fused_ = false;        // a listed CG system uses the two-kernel form
   std::vector<int> split_slots_;  // listed CG systems of the split form
+  int32_t *d_split_slots_ = nullptr;
+  int64_t split_cap_ = 0;
   std::vector<Csr *> csr_;        // per slot: matrix and values (split SpMV)
   std::vector<const void *> val_;
 };

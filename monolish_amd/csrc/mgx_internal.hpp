@@ -92,7 +92,7 @@ struct Vec {
 };
 
 struct SpmvPlan {
-  int32_t *d_blocks = nullptr;  // nblocks+1 descriptors (row start | kind)
+  int2 *d_blocks = nullptr;  // nblocks+1 descriptors (first row | long flag, first entry)
   int64_t nblocks = 0;
   int64_t nlong = 0;
   int32_t max_row = 0, min_row = 0;

@@ -10,9 +10,9 @@
 //  * long blocks: one row longer than the long-row threshold; the 256 lanes
 //    stride over it and the partial sums meet in the canonical block tree
 //    (deterministic, within the forward-error bound of the CPU loop).
-// Descriptors are int32 row starts; the sign bit marks a long block.  The
-// kernel is HBM bound (roofline: nnz*(sv+4) + (rows+1)*4 + (rows+cols)*sv
-// bytes); x gathers are served from L2/MALL.
+// Descriptors are (first row, first entry) pairs; the sign bit of the row
+// marks a long block.  The kernel is HBM bound (roofline: nnz*(sv+4) +
+// (rows+1)*4 + (rows+cols)*sv bytes); x gathers are served from L2/MALL.
 #include <algorithm>
 #include <cstring>
 
@@ -27,7 +27,7 @@ namespace {
 // flight at once.
 template <class T, bool AB, int U>
 __global__ __launch_bounds__(kBlock) void k_spmv(
-    const int32_t *__restrict__ blk, int64_t nblocks,
+    const int2 *__restrict__ blk, int64_t nblocks,
     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y,
     T alpha, T beta, int remap, const int32_t *live) {
@@ -38,10 +38,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
   if (b >= nblocks) return;
   // inside a device-driven solve: nothing to do once the solve has stopped
   if (live && *live != kStatRunning) return;
-  const int32_t d0 = blk[b], d1 = blk[b + 1];
-  const int32_t r0 = d0 & 0x7fffffff, r1 = d1 & 0x7fffffff;
-  if (d0 < 0) {
-    const int32_t e0 = rp[r0], e1 = rp[r0 + 1];
+  // descriptor: (first row | long-row flag, first entry)
+  const int2 d0 = blk[b], d1 = blk[b + 1];
+  const int32_t r0 = d0.x & 0x7fffffff, r1 = d1.x & 0x7fffffff;
+  const int32_t e0 = d0.y, e1 = d1.y;
+  if (d0.x < 0) {
     T s = 0;
     for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += kBlock) {
       T p = val[e] * x[ci[e]];
@@ -62,12 +63,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     }
     return;
   }
-  const int32_t e0 = rp[r0], e1 = rp[r1];
+  // this lane's row bounds are read before the products, so their latency
+  // overlaps the entry loads
+  const int32_t r = r0 + (int32_t)threadIdx.x;
+  int32_t ra = 0, rb = 0;
+  if (r < r1) {
+    ra = rp[r];
+    rb = rp[r + 1];
+  }
   stream_products<T, U>(val, ci, e0, e1, prod, [&](int32_t c) { return x[c]; });
   __syncthreads();
-  const int32_t r = r0 + (int32_t)threadIdx.x;
   if (r < r1) {
-    const int32_t a = rp[r] - e0, bnd = rp[r + 1] - e0;
+    const int32_t a = ra - e0, bnd = rb - e0;
     T s = 0;
     for (int32_t j = a; j < bnd; j++) s = s + prod[j];
     if (AB) {
@@ -136,7 +143,7 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   const int rows_max = std::min(std::max(rows_req, 1), kBlock);
   const int nnz_max = std::max(nnz_req, 64);
   const int long_t = std::min(std::max(long_req, 1), nnz_max);
-  std::vector<int32_t> blk;
+  std::vector<int2> blk;
   blk.reserve((size_t)(rows / std::max(rows_max, 1)) + 16);
   int32_t mx = 0, mn = rows > 0 ? INT32_MAX : 0;
   int64_t nlong = 0, r = 0;
@@ -145,7 +152,7 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
     mx = std::max(mx, len);
     mn = std::min(mn, len);
     if (len > long_t) {
-      blk.push_back((int32_t)r | (int32_t)0x80000000u);
+      blk.push_back(make_int2((int32_t)r | (int32_t)0x80000000u, rp[r]));
       nlong++;
       r++;
       continue;
@@ -160,10 +167,10 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
       acc += len;
       r++;
     }
-    blk.push_back((int32_t)start);
+    blk.push_back(make_int2((int32_t)start, rp[start]));
   }
   const int64_t nb = (int64_t)blk.size();
-  blk.push_back((int32_t)rows);
+  blk.push_back(make_int2((int32_t)rows, rp[rows]));
   A->plan.nblocks = nb;
   A->plan.nlong = nlong;
   A->plan.max_row = mx;
@@ -175,10 +182,9 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   A->plan.remap = remap_req ? 1 : 0;
   A->plan.tuned_us = 0;
   if (A->plan.d_blocks) dev_free(A->plan.d_blocks);
-  A->plan.d_blocks = (int32_t *)dev_alloc(blk.size() * sizeof(int32_t));
-  MGX_HIP(hipMemcpyAsync(A->plan.d_blocks, blk.data(),
-                         blk.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                         c->stream));
+  A->plan.d_blocks = (int2 *)dev_alloc(blk.size() * sizeof(int2));
+  MGX_HIP(hipMemcpyAsync(A->plan.d_blocks, blk.data(), blk.size() * sizeof(int2),
+                         hipMemcpyHostToDevice, c->stream));
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -193,7 +199,7 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
   // vector loads per lane: cover nnz_max entries plus the alignment slack
   constexpr int W = 16 / sizeof(T);
   const int u = (A->plan.nnz_max + W - 1 + kBlock * W - 1) / (kBlock * W);  // <= 17
-  const int32_t *B = A->plan.d_blocks;
+  const int2 *B = A->plan.d_blocks;
 #define MGX_SPMV_U(UU)                                                          \
   k_spmv<T, AB, UU><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
                                                       Y, alpha, beta, remap, live)
