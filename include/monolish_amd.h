@@ -49,6 +49,8 @@ int mgx_version(int *major, int *minor, int *patch);
 int mgx_device_count(int *count);
 /* util::set_default_device / device query, src/utils/system/gpu_deivce.cpp:6-34 */
 int mgx_init(int device, mgx_ctx **out);
+/* releases the context and every vector and matrix of it still alive; their
+ * handles must not be used afterwards */
 int mgx_finalize(mgx_ctx *ctx);
 int mgx_sync(mgx_ctx *ctx);
 /* the context's hipStream_t (as void*) for callers that record events */
@@ -63,16 +65,31 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
 /* Autotuner of the SpMV block plan (no monolish counterpart: monolish hands
  * CRS SpMV to cuSPARSE): times A's SpMV under candidate plans -- rows per
  * block {64, 128, 256} x entries per block {1024, 2048, 4096} x XCD-aware
- * order on/off -- `reps` launches each, keeps the fastest for A and reports
- * it.  The long-row threshold is unchanged, so results do not depend on the
- * choice. */
+ * order on/off with products staged in LDS, and rows per block {128, 256}
+ * x 1024 entries x XCD order with the row walk -- then, for matrices whose x does not fit an XCD's L2 and
+ * whose rows list columns in ascending order, the best of these with 4, 8
+ * or 16 column panels (the short rows' entries stored a second time,
+ * panel-major, so each pass gathers x from an L2-sized slice) -- `reps`
+ * launches each, keeps the fastest for A and reports it.  The long-row
+ * threshold is unchanged, so results do not depend on the choice.  The
+ * panel copy is dropped by mgx_csr_update_values. */
 typedef struct mgx_spmv_tuning {
   int32_t rows_per_block, nnz_per_block, long_row, xcd_remap;
   double us;           /* time of one SpMV under the chosen plan */
   int32_t candidates;  /* plans timed */
+  int32_t column_panels; /* 0, or the column panels of the chosen plan */
+  int32_t row_walk;    /* 1: stream blocks walk rows lane by lane (LDS-staged
+                          val / col_ind; coalesced gathers on banded rows) */
   int32_t pad;
 } mgx_spmv_tuning;
 int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out);
+/* Sets A's SpMV plan explicitly (the candidates the autotuner times):
+ * rows_per_block in [1, 256], nnz_per_block in [64, 8192] (<= 1024 with
+ * row_walk), xcd_remap and row_walk 0/1, column_panels 0 or 2..64 (needs
+ * ascending columns within rows: MGX_ERR_INVALID otherwise).  The long-row
+ * threshold stays as it is. */
+int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_remap,
+                     int row_walk, int column_panels);
 /* 1: fused solvers and batches replay each group of iterations from a
  * captured hipGraph (re-captured when the set of running systems changes);
  * 0 (default): direct kernel launches */
@@ -275,9 +292,19 @@ int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz,
                       const int32_t *row_ptr, const int32_t *col_ind,
                       const double *val, const double *b, const double *x0,
                       const mgx_solve_params *p, int64_t *ticket);
+/* queues a solve of a device-resident FP64 system without copying it: A
+ * and b belong to the caller, may be shared by many requests and must stay
+ * alive and unchanged until the request completes.  x (optional, not shared
+ * with another pending request) is the initial guess and receives the
+ * solution; without x the server starts from zero and keeps only the
+ * statistics.  Admission charges the work vectors (and x when the server
+ * allocates it) against the HBM budget. */
+int mgx_server_submit_resident(mgx_server *s, int method, mgx_csr *A, mgx_vec *b, mgx_vec *x,
+                               const mgx_solve_params *p, int64_t *ticket);
 /* waits up to timeout_s (< 0: forever) for a request; MGX_ERR_NOT_FOUND on
- * timeout; copies the solution into x (n doubles) when x != NULL and
- * releases the ticket */
+ * timeout; copies the solution into x (n doubles) when x != NULL and the
+ * request was submitted with its arrays (mgx_server_submit), and releases
+ * the ticket */
 int mgx_server_wait(mgx_server *s, int64_t ticket, double timeout_s,
                     mgx_solve_result *res, double *x);
 int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out);

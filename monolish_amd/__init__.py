@@ -54,7 +54,8 @@ class SolveResultC(C.Structure):
 
 class SpmvTuning(C.Structure):
     _fields_ = [("rows_per_block", _I32), ("nnz_per_block", _I32), ("long_row", _I32),
-                ("xcd_remap", _I32), ("us", _D), ("candidates", _I32), ("pad", _I32)]
+                ("xcd_remap", _I32), ("us", _D), ("candidates", _I32), ("column_panels", _I32),
+                ("row_walk", _I32), ("pad", _I32)]
 
 
 class CsrStats(C.Structure):
@@ -93,6 +94,7 @@ _sig("mgx_csr_destroy", [_P])
 _sig("mgx_csr_diag", [_P, _P])
 _sig("mgx_csr_stats_get", [_P, C.POINTER(CsrStats)])
 _sig("mgx_csr_autotune", [_P, C.c_int, C.POINTER(SpmvTuning)])
+_sig("mgx_csr_set_plan", [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int])
 _sig("mgx_matvec", [_P, _P, _P])
 _sig("mgx_matvec_ab", [_D, _P, _P, _D, _P])
 _sig("mgx_matvec_t", [_P, _P, _P])
@@ -138,6 +140,8 @@ class ServerStats(C.Structure):
 _sig("mgx_server_create", [_P, C.POINTER(ServerConfig), C.POINTER(_P)])
 _sig("mgx_server_submit", [_P, C.c_int, _I64, _I64, _P, _P, _P, _P, _P, C.POINTER(SolveParams),
                            C.POINTER(_I64)])
+_sig("mgx_server_submit_resident", [_P, C.c_int, _P, _P, _P, C.POINTER(SolveParams),
+                                    C.POINTER(_I64)])
 _sig("mgx_server_wait", [_P, _I64, _D, C.POINTER(SolveResultC), _P])
 _sig("mgx_server_stats_get", [_P, C.POINTER(ServerStats)])
 _sig("mgx_server_destroy", [_P])
@@ -276,7 +280,9 @@ class Vector:
 
     def free(self):
         if getattr(self, "_h", None):
-            check(_lib.mgx_vec_destroy(self._h))
+            # Context.close() already released it (mgx_finalize)
+            if self.ctx.handle:
+                check(_lib.mgx_vec_destroy(self._h))
             self._h = _P()
 
     def __del__(self):
@@ -328,6 +334,12 @@ class CRS:
         check(_lib.mgx_csr_stats_get(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in CsrStats._fields_}
 
+    def set_plan(self, rows_per_block=256, nnz_per_block=2048, xcd_remap=True, row_walk=False,
+                 column_panels=0):
+        """Fix the SpMV block plan (the autotuner's candidates)."""
+        check(_lib.mgx_csr_set_plan(self._h, rows_per_block, nnz_per_block, int(xcd_remap),
+                                    int(row_walk), column_panels))
+
     def autotune(self, reps: int = 20) -> dict:
         """Time the SpMV under candidate block plans and keep the fastest."""
         t = SpmvTuning()
@@ -336,7 +348,8 @@ class CRS:
 
     def free(self):
         if getattr(self, "_h", None):
-            check(_lib.mgx_csr_destroy(self._h))
+            if self.ctx.handle:
+                check(_lib.mgx_csr_destroy(self._h))
             self._h = _P()
 
     def __del__(self):
@@ -602,6 +615,17 @@ class Server:
         t = _I64(0)
         check(_lib.mgx_server_submit(self._h, method, len(b), len(val), _ptr(rp), _ptr(ci), _ptr(val),
                                      _ptr(b), x0p, C.byref(p), C.byref(t)))
+        return t.value
+
+    def submit_resident(self, method, A, b, x=None, tol=1e-10, tol_is_relative=True, maxiter=0,
+                        precond=PRECOND_JACOBI) -> int:
+        """Queue a solve of a device-resident system (A, b stay the caller's and may be
+        shared by many requests); x, if given, is the start and receives the solution."""
+        p, _ = _params(tol, maxiter, 0, precond, MODE_FUSED, tol_is_relative, 0, 0)
+        t = _I64(0)
+        check(_lib.mgx_server_submit_resident(self._h, method, A.handle, b.handle,
+                                              x.handle if x is not None else None, C.byref(p),
+                                              C.byref(t)))
         return t.value
 
     def wait(self, ticket: int, n: int | None = None, timeout: float = -1.0):

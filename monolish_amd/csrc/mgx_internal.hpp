@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/monolish_amd.h"
@@ -100,8 +102,21 @@ struct SpmvPlan {
   int32_t rows_max = 0;  // row budget of a stream block
   int32_t long_row = 0;  // rows longer than this get a block of their own
   int32_t remap = 1;     // XCD-aware block order
+  int32_t walk = 0;      // stream blocks: 0 product-staged, 1 row walk
   double mean_row = 0;
   double tuned_us = 0;   // SpMV time measured by the autotuner (0: not tuned)
+  // column-panel mode (0: off): the short rows' entries are also stored
+  // panel-major (panel k holds columns [k*pwidth, (k+1)*pwidth)), each pass
+  // gathers x from one panel and continues the rows' sums in y; long rows
+  // keep their own pass over the long blocks of d_blocks
+  int32_t npanels = 0;
+  int64_t pwidth = 0;
+  int32_t *d_prp = nullptr;    // npanels x (rows + 1) row pointers into pci/pval
+  int32_t *d_pci = nullptr;
+  void *d_pval = nullptr;
+  int2 *d_pblocks = nullptr;   // stream-block descriptors of all panels
+  std::vector<int64_t> pblock_off;  // panel k: descriptors [off[k], off[k+1]) (+1 end)
+  int32_t *d_lrows = nullptr;  // the long rows (nlong of them)
 };
 
 struct Csr {
@@ -148,6 +163,11 @@ struct Ctx {
   // workspace for solver vectors
   std::vector<std::pair<void *, size_t>> ws_free;
   std::vector<std::pair<void *, size_t>> ws_used;
+  // vectors and matrices created in this context and not destroyed yet;
+  // mgx_finalize releases them
+  std::mutex objs_mu;
+  std::unordered_set<Vec *> live_vecs;
+  std::unordered_set<Csr *> live_csrs;
 };
 
 // ------------------------------------------------------- host helpers --
@@ -194,7 +214,7 @@ double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
 void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp_host);
 // plan with explicit budgets (the context's defaults otherwise)
 void spmv_plan_build_with(Csr *A, const int32_t *rp_host, int rows_max, int nnz_max,
-                          int long_row, int remap);
+                          int long_row, int remap, int walk);
 // live: optional device status word; the launch does nothing unless it
 // reads kStatRunning (device-driven solver loops)
 void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
@@ -202,6 +222,10 @@ void launch_spmv(Csr *A, const void *val, mgx_dtype dt, const void *x,
                  const int32_t *live = nullptr);
 void launch_diag(Csr *A, void *d);
 void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out);
+// build / drop the column-panel mode of A's plan (false: not applicable, e.g.
+// columns not ascending within a row)
+bool spmv_panels_build(Csr *A, int npanels);
+void spmv_panels_free(Csr *A);
 void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
 // fused-solver row chunks (built at upload for square matrices, from the
 // host row_ptr when given, else from a copy of the device one)

@@ -189,6 +189,7 @@ static void csr_free(Csr *A) {
   dev_free(A->val);
   dev_free(A->val32);
   dev_free(A->plan.d_blocks);
+  spmv_panels_free(A);
   dev_free(A->d_chunks);
   dev_free(A->d_tile_chunk);
   if (A->trans) csr_free(A->trans);
@@ -253,6 +254,16 @@ int mgx_finalize(mgx_ctx *ctx) {
     activate(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     ctx_free_engines(ctx);
+    {
+      std::lock_guard<std::mutex> lk(ctx->objs_mu);
+      for (Csr *A : ctx->live_csrs) csr_free(A);
+      for (Vec *v : ctx->live_vecs) {
+        if (v->owner) dev_free(v->d);
+        delete static_cast<mgx_vec *>(v);
+      }
+      ctx->live_csrs.clear();
+      ctx->live_vecs.clear();
+    }
     for (auto &e : ctx->ws_used) dev_free(e.first);
     for (auto &e : ctx->ws_free) dev_free(e.first);
     dev_free(ctx->d_partials);
@@ -327,6 +338,32 @@ int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out) {
   });
 }
 
+int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_remap,
+                     int row_walk, int column_panels) {
+  return guard([&] {
+    MGX_REQUIRE(A, "mgx_csr_set_plan: NULL matrix");
+    MGX_REQUIRE(rows_per_block >= 1 && rows_per_block <= kBlock,
+                "rows_per_block must be in [1, %d]", kBlock);
+    MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk ? 1024 : 8192),
+                "nnz_per_block must be in [64, %d]", row_walk ? 1024 : 8192);
+    MGX_REQUIRE(column_panels == 0 || (column_panels >= 2 && column_panels <= 64),
+                "column_panels must be 0 or in [2, 64]");
+    activate(A->ctx);
+    MGX_HIP(hipStreamSynchronize(A->ctx->stream));
+    std::vector<int32_t> rp(A->rows + 1);
+    MGX_HIP(hipMemcpy(rp.data(), A->rp, (size_t)(A->rows + 1) * 4, hipMemcpyDeviceToHost));
+    const int long_t = A->plan.long_row > 0 ? A->plan.long_row : A->ctx->spmv_long;
+    MGX_REQUIRE(long_t <= nnz_per_block, "nnz_per_block below the long-row threshold %d",
+                long_t);
+    spmv_plan_build_with(A, rp.data(), rows_per_block, nnz_per_block, long_t, xcd_remap,
+                         row_walk);
+    if (column_panels)
+      MGX_REQUIRE(spmv_panels_build(A, column_panels),
+                  "mgx_csr_set_plan: column panels need ascending columns within rows");
+    return MGX_OK;
+  });
+}
+
 int mgx_set_persistent(mgx_ctx *ctx, int enable) {
   return guard([&] {
     MGX_REQUIRE(ctx, "mgx_set_persistent: NULL ctx");
@@ -362,6 +399,10 @@ int mgx_vec_create(mgx_ctx *ctx, int64_t n, mgx_dtype dtype, const void *host,
       delete v;
       throw;
     }
+    {
+      std::lock_guard<std::mutex> lk(ctx->objs_mu);
+      ctx->live_vecs.insert(v);
+    }
     *out = v;
     return MGX_OK;
   });
@@ -379,6 +420,10 @@ int mgx_vec_view(mgx_vec *base, int64_t offset, int64_t n, mgx_vec **out) {
     v->dtype = base->dtype;
     v->d = (char *)base->d + offset * (int64_t)base->elem();
     v->owner = false;
+    {
+      std::lock_guard<std::mutex> lk(v->ctx->objs_mu);
+      v->ctx->live_vecs.insert(v);
+    }
     *out = v;
     return MGX_OK;
   });
@@ -409,6 +454,10 @@ int mgx_vec_download(mgx_vec *v, void *host) {
 int mgx_vec_destroy(mgx_vec *v) {
   return guard([&] {
     if (!v) return MGX_OK;
+    {
+      std::lock_guard<std::mutex> lk(v->ctx->objs_mu);
+      v->ctx->live_vecs.erase(v);
+    }
     activate(v->ctx);
     if (v->owner) {
       (void)hipStreamSynchronize(v->ctx->stream);
@@ -493,6 +542,10 @@ int mgx_csr_create(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
       csr_free(A);
       throw;
     }
+    {
+      std::lock_guard<std::mutex> lk(ctx->objs_mu);
+      ctx->live_csrs.insert(A);
+    }
     *out = A;
     return MGX_OK;
   });
@@ -505,6 +558,7 @@ int mgx_csr_update_values(mgx_csr *A, const void *val) {
     MGX_HIP(hipStreamSynchronize(A->ctx->stream));
     if (A->nnz) MGX_HIP(hipMemcpy(A->val, val, (size_t)A->nnz * A->elem(), hipMemcpyHostToDevice));
     if (A->val32) launch_convert(A->ctx, A->dtype, MGX_F32, A->nnz, A->val, A->val32);
+    spmv_panels_free(A);  // its value copy would be stale
     if (A->trans) {
       csr_free(A->trans);
       A->trans = nullptr;
@@ -534,6 +588,10 @@ int mgx_csr_download(mgx_csr *A, int32_t *row_ptr, int32_t *col_ind, void *val) 
 int mgx_csr_destroy(mgx_csr *A) {
   return guard([&] {
     if (!A) return MGX_OK;
+    {
+      std::lock_guard<std::mutex> lk(A->ctx->objs_mu);
+      A->ctx->live_csrs.erase(A);
+    }
     activate(A->ctx);
     csr_free(A);
     return MGX_OK;

@@ -80,6 +80,9 @@ struct Request {
   double *x = nullptr, *bd = nullptr;
   size_t bytes = 0;
   int slot = -1;
+  // submitted by handle: A, b (and x when given) belong to the caller
+  bool by_handle = false;
+  bool own_x = true;
   // outcome
   bool done = false;
   mgx_solve_result res{};
@@ -107,14 +110,23 @@ struct Server {
   std::string fatal;
 
   size_t request_bytes(const Request &r) const {
-    const size_t csr = (size_t)(r.n + 1) * 4 + (size_t)(r.nnz + 4) * 12 +
-                       (size_t)(r.n / 256 + 2) * 4 * 2;
     Csr probe;
     probe.rows = probe.cols = r.n;
-    return csr + (size_t)r.n * 16 + eng->bytes_needed(&probe, r.method);
+    const size_t work = eng->bytes_needed(&probe, r.method);
+    if (r.by_handle) return work + (r.own_x ? (size_t)r.n * 8 : 0);
+    const size_t csr = (size_t)(r.n + 1) * 4 + (size_t)(r.nnz + 4) * 12 +
+                       (size_t)(r.n / 256 + 2) * 4 * 2;
+    return csr + (size_t)r.n * 16 + work;
   }
 
   void upload(Request &r) {
+    if (r.by_handle) {
+      if (r.own_x) {
+        r.x = (double *)dev_alloc((size_t)r.n * 8);
+        MGX_HIP(hipMemsetAsync(r.x, 0, (size_t)r.n * 8, ctx->stream));
+      }
+      return;
+    }
     mgx_csr *A = nullptr;
     int rc = mgx_csr_create(static_cast<mgx_ctx *>(ctx), r.n, r.n, r.nnz, r.rp.data(),
                             r.ci.data(), r.val.data(), MGX_F64, &A);
@@ -136,6 +148,14 @@ struct Server {
   }
 
   void free_resident(Request &r) {
+    if (r.by_handle) {
+      if (r.own_x) {
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(r.x);
+      }
+      r.x = nullptr;
+      return;
+    }
     if (r.A) (void)mgx_csr_destroy(static_cast<mgx_csr *>(r.A));
     dev_free(r.x);
     dev_free(r.bd);
@@ -208,8 +228,10 @@ struct Server {
         const SolveState st = eng->state(s);
         r->res.final_iter = st.iter;
         r->res.final_resid = st.resid;
-        r->x_out.resize(r->n);
-        MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
+        if (!r->by_handle) {
+          r->x_out.resize(r->n);
+          MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
+        }
         eng->release(s);
         by_slot[s].reset();
         free_resident(*r);
@@ -324,6 +346,45 @@ int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz, const i
     r->bytes = s->request_bytes(*r);
     std::lock_guard<std::mutex> lk(s->mu);
     MGX_REQUIRE(!s->stop, "mgx_server_submit: server stopped (%s)", s->fatal.c_str());
+    r->id = s->next_id++;
+    r->t_submit = seconds_now();
+    if (s->first_submit < 0) s->first_submit = r->t_submit;
+    s->submitted++;
+    s->all[r->id] = r;
+    s->queue.push_back(r);
+    s->cv_work.notify_one();
+    *ticket = r->id;
+    return MGX_OK;
+  });
+}
+
+int mgx_server_submit_resident(mgx_server *s, int method, mgx_csr *A, mgx_vec *b, mgx_vec *x,
+                               const mgx_solve_params *p, int64_t *ticket) {
+  return guard([&] {
+    MGX_REQUIRE(s && A && b && p && ticket, "mgx_server_submit_resident: NULL argument");
+    MGX_REQUIRE(method == MGX_METHOD_CG || method == MGX_METHOD_BICGSTAB,
+                "mgx_server_submit_resident: unknown method %d", method);
+    MGX_REQUIRE(A->ctx == s->ctx && b->ctx == s->ctx && (!x || x->ctx == s->ctx),
+                "mgx_server_submit_resident: context mismatch");
+    MGX_REQUIRE(A->dtype == MGX_F64 && b->dtype == MGX_F64 && (!x || x->dtype == MGX_F64),
+                "mgx_server_submit_resident: FP64 system required");
+    MGX_REQUIRE(A->rows == A->cols && A->rows > 0 && b->n == A->rows && (!x || x->n == A->rows),
+                "mgx_server_submit_resident: size mismatch");
+    MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
+                "mgx_server_submit_resident: unknown preconditioner");
+    auto r = std::make_shared<Request>();
+    r->method = method;
+    r->n = A->rows;
+    r->nnz = A->nnz;
+    r->by_handle = true;
+    r->A = A;
+    r->bd = static_cast<double *>(b->d);
+    r->own_x = x == nullptr;
+    r->x = x ? static_cast<double *>(x->d) : nullptr;
+    r->cfg = cfg_of(p, method);
+    r->bytes = s->request_bytes(*r);
+    std::lock_guard<std::mutex> lk(s->mu);
+    MGX_REQUIRE(!s->stop, "mgx_server_submit_resident: server stopped (%s)", s->fatal.c_str());
     r->id = s->next_id++;
     r->t_submit = seconds_now();
     if (s->first_submit < 0) s->first_submit = r->t_submit;

@@ -25,7 +25,7 @@ namespace {
 // U = entry budget / 256: each lane issues all of its val/col loads, then all
 // of its x gathers, before it multiplies, so up to U loads per lane are in
 // flight at once.
-template <class T, bool AB, int U>
+template <class T, bool AB, int U, bool WALK>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const int2 *__restrict__ blk, int64_t nblocks,
     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -71,12 +71,62 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     ra = rp[r];
     rb = rp[r + 1];
   }
-  stream_products<T, U>(val, ci, e0, e1, prod, [&](int32_t c) { return x[c]; });
-  __syncthreads();
-  if (r < r1) {
-    const int32_t a = ra - e0, bnd = rb - e0;
-    T s = 0;
-    for (int32_t j = a; j < bnd; j++) s = s + prod[j];
+  T s = 0;
+  if (WALK) {
+    // row walk: the block's val / col_ind go to LDS with coalesced 16-byte
+    // loads; then lane t walks row r0 + t, 8 entries at a time (LDS reads,
+    // then the gathers, then the products added in order).  At each step
+    // the lanes gather x for consecutive rows, which share cache lines on
+    // banded and stencil matrices.
+    constexpr int W = 16 / sizeof(T);
+    typedef T VT __attribute__((ext_vector_type(W)));
+    typedef int32_t VI __attribute__((ext_vector_type(W)));
+    const int32_t ea = e0 & ~(W - 1);
+    T *sv = prod;
+    int32_t *sc = reinterpret_cast<int32_t *>(sv + U * kBlock * W);
+    VT v[U];
+    VI c[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+      if (base < e1) {
+        v[k] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(val + base));
+        c[k] = __builtin_nontemporal_load(reinterpret_cast<const VI *>(ci + base));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
+      if (base < e1) {
+        *reinterpret_cast<VT *>(sv + (base - ea)) = v[k];
+        *reinterpret_cast<VI *>(sc + (base - ea)) = c[k];
+      }
+    }
+    __syncthreads();
+    if (r >= r1) return;
+    const int32_t je = rb - ea;
+    for (int32_t j = ra - ea; j < je; j += 8) {
+      T vv[8], xv[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (j + q < je) {
+          vv[q] = sv[j + q];
+          xv[q] = x[sc[j + q]];
+        }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (j + q < je) {
+          T p = vv[q] * xv[q];
+          s = s + p;
+        }
+    }
+  } else {
+    stream_products<T, U>(val, ci, e0, e1, prod, [&](int32_t c) { return x[c]; });
+    __syncthreads();
+    if (r >= r1) return;
+    for (int32_t j = ra - e0; j < rb - e0; j++) s = s + prod[j];
+  }
+  {
     if (AB) {
       T t = alpha * s;
       if (beta != T(0)) {
@@ -88,6 +138,57 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
       y[r] = s;
     }
   }
+}
+
+// Column-panel pass: the stream blocks of one panel over the panel-major
+// copy of the short rows' entries.  Each row continues its sum in y from the
+// previous panel (FIRST: from 0), so the products are added in the row's
+// storage order -- the CPU row loop's order -- across the passes.
+template <class T, int U, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_spmv_panel(
+    const int2 *__restrict__ blk, int64_t nblocks, const int32_t *__restrict__ prp,
+    const int32_t *__restrict__ pci, const T *__restrict__ pval, const T *__restrict__ x,
+    T *__restrict__ y, int remap) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *prod = reinterpret_cast<T *>(smem_raw);
+  const int64_t b = xcd_block(remap);
+  if (b >= nblocks) return;
+  const int2 d0 = blk[b], d1 = blk[b + 1];
+  const int32_t r0 = d0.x, r1 = d1.x, e0 = d0.y, e1 = d1.y;
+  const int32_t r = r0 + (int32_t)threadIdx.x;
+  int32_t ra = 0, rb = 0;
+  T s = 0;
+  if (r < r1) {
+    ra = prp[r];
+    rb = prp[r + 1];
+    if (!FIRST) s = y[r];
+  }
+  stream_products<T, U>(pval, pci, e0, e1, prod, [&](int32_t c) { return x[c]; });
+  __syncthreads();
+  if (r < r1) {
+    for (int32_t j = ra - e0; j < rb - e0; j++) s = s + prod[j];
+    y[r] = s;
+  }
+}
+
+// rows longer than the long-row threshold in column-panel mode: the long
+// path of k_spmv (lane-strided sums, canonical block tree), one block per row
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spmv_longrows(
+    const int32_t *__restrict__ rows, int64_t nlong, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const T *__restrict__ val, const T *__restrict__ x,
+    T *__restrict__ y) {
+  __shared__ T sh[8];
+  if ((int64_t)blockIdx.x >= nlong) return;
+  const int32_t r0 = rows[blockIdx.x];
+  const int32_t e0 = rp[r0], e1 = rp[r0 + 1];
+  T s = 0;
+  for (int32_t e = e0 + (int32_t)threadIdx.x; e < e1; e += kBlock) {
+    T p = val[e] * x[ci[e]];
+    s = s + p;
+  }
+  T res = block_tree(s, sh);
+  if (threadIdx.x == 0) y[r0] = res;
 }
 
 // CRS::diag: d[i] = A(i,i), last stored match wins, 0 if absent.  A block
@@ -133,12 +234,13 @@ __global__ __launch_bounds__(kBlock) void k_diag(int64_t n, const int32_t *__res
 }  // namespace
 
 void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp) {
-  spmv_plan_build_with(A, rp, c->spmv_rows, c->spmv_nnz, c->spmv_long, c->xcd_remap);
+  spmv_plan_build_with(A, rp, c->spmv_rows, c->spmv_nnz, c->spmv_long, c->xcd_remap, 0);
 }
 
 void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, int long_req,
-                          int remap_req) {
+                          int remap_req, int walk) {
   Ctx *c = A->ctx;
+  spmv_panels_free(A);  // built for the previous block budgets
   const int64_t rows = A->rows;
   const int rows_max = std::min(std::max(rows_req, 1), kBlock);
   const int nnz_max = std::max(nnz_req, 64);
@@ -180,6 +282,7 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   A->plan.rows_max = rows_max;
   A->plan.long_row = long_t;
   A->plan.remap = remap_req ? 1 : 0;
+  A->plan.walk = walk ? 1 : 0;
   A->plan.tuned_us = 0;
   if (A->plan.d_blocks) dev_free(A->plan.d_blocks);
   A->plan.d_blocks = (int2 *)dev_alloc(blk.size() * sizeof(int2));
@@ -188,21 +291,199 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
 
+void spmv_panels_free(Csr *A) {
+  SpmvPlan &p = A->plan;
+  if (!p.npanels) return;
+  if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
+  dev_free(p.d_prp);
+  dev_free(p.d_pci);
+  dev_free(p.d_pval);
+  dev_free(p.d_pblocks);
+  dev_free(p.d_lrows);
+  p.d_prp = p.d_pci = p.d_lrows = nullptr;
+  p.d_pval = nullptr;
+  p.d_pblocks = nullptr;
+  p.pblock_off.clear();
+  p.npanels = 0;
+  p.pwidth = 0;
+}
+
+bool spmv_panels_build(Csr *A, int npanels) {
+  Ctx *c = A->ctx;
+  SpmvPlan &p = A->plan;
+  spmv_panels_free(A);
+  const int64_t n = A->rows, cols = A->cols, nnz = A->nnz;
+  if (npanels < 2 || n <= 0 || cols <= 0 || nnz <= 0) return false;
+  const size_t es = A->elem();
+  std::vector<int32_t> rp(n + 1), ci(nnz);
+  std::vector<char> val((size_t)nnz * es);
+  MGX_HIP(hipStreamSynchronize(c->stream));
+  MGX_HIP(hipMemcpy(rp.data(), A->rp, (size_t)(n + 1) * 4, hipMemcpyDeviceToHost));
+  MGX_HIP(hipMemcpy(ci.data(), A->ci, (size_t)nnz * 4, hipMemcpyDeviceToHost));
+  MGX_HIP(hipMemcpy(val.data(), A->val, (size_t)nnz * es, hipMemcpyDeviceToHost));
+  const int long_t = p.long_row;
+  const int64_t pw = (cols + npanels - 1) / npanels;
+  // short rows must list their columns in ascending order: then a row's
+  // entries of panel k are a contiguous run, and panel order is storage order
+  std::vector<int32_t> lrows;
+  std::vector<int32_t> cnt((size_t)npanels * n, 0);
+  for (int64_t r = 0; r < n; r++) {
+    const int32_t a = rp[r], e = rp[r + 1];
+    if (e - a > long_t) {
+      lrows.push_back((int32_t)r);
+      continue;
+    }
+    for (int32_t j = a; j < e; j++) {
+      if (j > a && ci[j] <= ci[j - 1]) return false;
+      cnt[(size_t)(ci[j] / pw) * n + r]++;
+    }
+  }
+  if ((int64_t)lrows.size() != p.nlong) return false;
+  // panel-major row pointers (absolute offsets into the panel arrays)
+  std::vector<int32_t> prp((size_t)npanels * (n + 1));
+  int64_t off = 0;
+  for (int k = 0; k < npanels; k++) {
+    int32_t *pr = prp.data() + (size_t)k * (n + 1);
+    for (int64_t r = 0; r < n; r++) {
+      pr[r] = (int32_t)off;
+      off += cnt[(size_t)k * n + r];
+    }
+    pr[n] = (int32_t)off;
+  }
+  const int64_t nshort = off;
+  std::vector<int32_t> pci((size_t)nshort + 4, 0);
+  std::vector<char> pval(((size_t)nshort + 4) * es, 0);
+  for (int64_t r = 0; r < n; r++) {
+    const int32_t a = rp[r], e = rp[r + 1];
+    if (e - a > long_t) continue;
+    for (int32_t j = a; j < e; j++) {
+      const int k = (int)(ci[j] / pw);
+      const int64_t d = prp[(size_t)k * (n + 1) + r]++;
+      pci[d] = ci[j];
+      std::memcpy(&pval[(size_t)d * es], &val[(size_t)j * es], es);
+    }
+  }
+  // the scatter advanced prp[k][r] to the start of row r + 1's run: shift back
+  for (int k = 0; k < npanels; k++) {
+    int32_t *pr = prp.data() + (size_t)k * (n + 1);
+    for (int64_t r = n; r > 0; r--) pr[r] = pr[r - 1];
+    pr[0] = k == 0 ? 0 : prp[(size_t)(k - 1) * (n + 1) + n];
+  }
+  // stream blocks of every panel: <= rows_max rows, <= nnz_max entries
+  std::vector<int2> pblk;
+  std::vector<int64_t> boff;
+  for (int k = 0; k < npanels; k++) {
+    boff.push_back((int64_t)pblk.size());
+    const int32_t *pr = prp.data() + (size_t)k * (n + 1);
+    int64_t r = 0;
+    while (r < n) {
+      const int64_t start = r;
+      int64_t acc = 0;
+      while (r < n) {
+        const int64_t len = pr[r + 1] - pr[r];
+        if (r - start >= p.rows_max || acc + len > p.nnz_max) break;
+        acc += len;
+        r++;
+      }
+      pblk.push_back(make_int2((int32_t)start, pr[start]));
+    }
+    pblk.push_back(make_int2((int32_t)n, pr[n]));  // end of this panel's list
+  }
+  boff.push_back((int64_t)pblk.size());
+  try {
+    p.d_prp = (int32_t *)dev_alloc(prp.size() * 4);
+    p.d_pci = (int32_t *)dev_alloc(pci.size() * 4);
+    p.d_pval = dev_alloc(pval.size());
+    p.d_pblocks = (int2 *)dev_alloc(pblk.size() * sizeof(int2));
+    p.d_lrows = (int32_t *)dev_alloc(std::max<size_t>(lrows.size(), 1) * 4);
+    MGX_HIP(hipMemcpy(p.d_prp, prp.data(), prp.size() * 4, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_pci, pci.data(), pci.size() * 4, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_pval, pval.data(), pval.size(), hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_pblocks, pblk.data(), pblk.size() * sizeof(int2),
+                      hipMemcpyHostToDevice));
+    if (!lrows.empty())
+      MGX_HIP(hipMemcpy(p.d_lrows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice));
+  } catch (...) {
+    p.npanels = 1;  // let spmv_panels_free release what was allocated
+    spmv_panels_free(A);
+    throw;
+  }
+  p.pblock_off = boff;
+  p.pwidth = pw;
+  p.npanels = npanels;
+  return true;
+}
+
+template <class T>
+static void spmv_panels_dispatch(Csr *A, const T *X, T *Y) {
+  Ctx *c = A->ctx;
+  const SpmvPlan &p = A->plan;
+  const size_t lds = (size_t)p.nnz_max * sizeof(T);
+  constexpr int W = 16 / sizeof(T);
+  const int u = (p.nnz_max + W - 1 + kBlock * W - 1) / (kBlock * W);
+  const int64_t n1 = A->rows + 1;
+  const T *pval = static_cast<const T *>(p.d_pval);
+  for (int k = 0; k < p.npanels; k++) {
+    const int64_t nb = p.pblock_off[k + 1] - p.pblock_off[k] - 1;
+    if (nb <= 0) continue;
+    const int remap = p.remap && nb >= 64;
+    const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
+    const int2 *B = p.d_pblocks + p.pblock_off[k];
+    const int32_t *prp = p.d_prp + (size_t)k * n1;
+#define MGX_PANEL_U(UU)                                                                  \
+  do {                                                                                    \
+    if (k == 0)                                                                           \
+      k_spmv_panel<T, UU, true><<<grid, kBlock, lds, c->stream>>>(B, nb, prp, p.d_pci, pval, \
+                                                                  X, Y, remap);           \
+    else                                                                                  \
+      k_spmv_panel<T, UU, false><<<grid, kBlock, lds, c->stream>>>(B, nb, prp, p.d_pci,    \
+                                                                   pval, X, Y, remap);    \
+  } while (0)
+    if (u <= 1) MGX_PANEL_U(1);
+    else if (u <= 2) MGX_PANEL_U(2);
+    else if (u <= 3) MGX_PANEL_U(3);
+    else if (u <= 5) MGX_PANEL_U(5);
+    else if (u <= 8) MGX_PANEL_U(8);
+    else MGX_PANEL_U(17);
+#undef MGX_PANEL_U
+  }
+  if (p.nlong > 0)
+    k_spmv_longrows<T><<<(unsigned)p.nlong, kBlock, 0, c->stream>>>(
+        p.d_lrows, p.nlong, A->rp, A->ci, static_cast<const T *>(A->val), X, Y);
+  MGX_HIP(hipGetLastError());
+}
+
 template <class T, bool AB>
 static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
                           T beta, const int32_t *live) {
   Ctx *c = A->ctx;
+  // column-panel mode: plain y = A x with A's own values, outside solver loops
+  if (!AB && !live && A->plan.npanels > 0 && V == static_cast<const T *>(A->val)) {
+    spmv_panels_dispatch<T>(A, X, Y);
+    return;
+  }
   const int64_t nb = A->plan.nblocks;
   const int remap = A->plan.remap && nb >= 64;
   const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
-  const size_t lds = (size_t)A->plan.nnz_max * sizeof(T);
   // vector loads per lane: cover nnz_max entries plus the alignment slack
   constexpr int W = 16 / sizeof(T);
   const int u = (A->plan.nnz_max + W - 1 + kBlock * W - 1) / (kBlock * W);  // <= 17
   const int2 *B = A->plan.d_blocks;
-#define MGX_SPMV_U(UU)                                                          \
-  k_spmv<T, AB, UU><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
-                                                      Y, alpha, beta, remap, live)
+  const bool walk = A->plan.walk != 0;
+  // products (nnz_max values), or for the row walk the staged val / col_ind
+  // of every load slot
+  size_t lds = (size_t)A->plan.nnz_max * sizeof(T);
+#define MGX_SPMV_U(UU)                                                                    \
+  do {                                                                                    \
+    if (walk) {                                                                           \
+      lds = (size_t)(UU) * kBlock * W * (sizeof(T) + 4);                                  \
+      k_spmv<T, AB, UU, true><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
+                                                                Y, alpha, beta, remap, live); \
+    } else {                                                                              \
+      k_spmv<T, AB, UU, false><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
+                                                                 Y, alpha, beta, remap, live); \
+    }                                                                                     \
+  } while (0)
   if (u <= 1) MGX_SPMV_U(1);
   else if (u <= 2) MGX_SPMV_U(2);
   else if (u <= 3) MGX_SPMV_U(3);
@@ -271,19 +552,23 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int long_t = A->plan.long_row > 0 ? A->plan.long_row : c->spmv_long;
   struct Cand {
-    int rows_max, nnz_max, remap;
+    int rows_max, nnz_max, remap, walk;
     double us;
   };
   std::vector<Cand> cands;
   for (int rm : {64, 128, 256})
     for (int nm : {1024, 2048, 4096})
       for (int rr : {1, 0})
-        if (nm >= long_t) cands.push_back(Cand{rm, nm, rr, 0.0});
+        if (nm >= long_t) cands.push_back(Cand{rm, nm, rr, 0, 0.0});
+  // the row walk stages val and col_ind (12 bytes per FP64 entry) in LDS
+  for (int rm : {128, 256})
+    for (int rr : {1, 0})
+      if (1024 >= long_t) cands.push_back(Cand{rm, 1024, rr, 1, 0.0});
   try {
     MGX_HIP(hipEventCreate(&e0));
     MGX_HIP(hipEventCreate(&e1));
     for (Cand &k : cands) {
-      spmv_plan_build_with(A, rp.data(), k.rows_max, k.nnz_max, long_t, k.remap);
+      spmv_plan_build_with(A, rp.data(), k.rows_max, k.nnz_max, long_t, k.remap, k.walk);
       for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
       MGX_HIP(hipEventRecord(e0, c->stream));
       for (int i = 0; i < reps; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
@@ -306,15 +591,61 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
   dev_free(y);
   const Cand best = *std::min_element(cands.begin(), cands.end(),
                                       [](const Cand &a, const Cand &b) { return a.us < b.us; });
-  spmv_plan_build_with(A, rp.data(), best.rows_max, best.nnz_max, long_t, best.remap);
-  A->plan.tuned_us = best.us;
+  spmv_plan_build_with(A, rp.data(), best.rows_max, best.nnz_max, long_t, best.remap,
+                       best.walk);
+  // column panels, when x outgrows an XCD's L2 (4 MiB)
+  double best_us = best.us;
+  int best_panels = 0, ncand = (int)cands.size();
+  if ((size_t)cols * es > ((size_t)4 << 20)) {
+    x = dev_alloc((size_t)cols * es);
+    y = dev_alloc((size_t)std::max<int64_t>(rows, 1) * es);
+    try {
+      MGX_HIP(hipMemsetAsync(x, 0, (size_t)cols * es, c->stream));
+      MGX_HIP(hipEventCreate(&e0));
+      MGX_HIP(hipEventCreate(&e1));
+      for (int np : {4, 8, 16}) {
+        if (!spmv_panels_build(A, np)) break;
+        ncand++;
+        for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+        MGX_HIP(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+        MGX_HIP(hipEventRecord(e1, c->stream));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / reps;
+        if (us < best_us) {
+          best_us = us;
+          best_panels = np;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      dev_free(x);
+      dev_free(y);
+      spmv_panels_free(A);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(x);
+    dev_free(y);
+    if (best_panels)
+      spmv_panels_build(A, best_panels);
+    else
+      spmv_panels_free(A);
+  }
+  A->plan.tuned_us = best_us;
   if (out) {
     out->rows_per_block = best.rows_max;
     out->nnz_per_block = best.nnz_max;
     out->long_row = long_t;
     out->xcd_remap = best.remap;
-    out->us = best.us;
-    out->candidates = (int32_t)cands.size();
+    out->us = best_us;
+    out->candidates = (int32_t)ncand;
+    out->column_panels = best_panels;
+    out->row_walk = best.walk;
   }
 }
 

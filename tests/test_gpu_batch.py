@@ -175,3 +175,41 @@ def test_batch_ir_cg_equals_single_ir(m, ctx, oracle):
         assert (r.outer_iter, r.inner_iter) == (r1.outer_iter, r1.inner_iter)
         assert r.final_resid == r1.final_resid
         assert np.array_equal(bits(x), bits(X1.download()))
+
+
+def test_server_resident_requests(m, ctx, oracle):
+    """Requests by handle: several requests share one resident matrix and
+    right-hand side; those given an x receive the solution, bit-identical to
+    the oracle's solve of the system alone; requests without x only count."""
+    say("server resident")
+    syss = systems(m, oracle)
+    c2 = m.Context(0)
+    res_sys = []
+    for meth, rp, ci, v, b in syss:
+        n = len(b)
+        res_sys.append((meth, m.CRS(c2, n, n, rp, ci, v), m.Vector(c2, b), rp, ci, v, b))
+    srv = m.Server(c2, max_active=5, iters_per_poll=3)
+    tickets = []
+    for rep in range(3):
+        for i, (meth, A, B, rp, ci, v, b) in enumerate(res_sys):
+            code = m.METHOD_CG if meth == "cg" else m.METHOD_BICGSTAB
+            X = m.Vector(c2, n=len(b)) if rep == 1 else None
+            tickets.append((srv.submit_resident(code, A, B, X, tol=1e-10, tol_is_relative=True),
+                            i, X))
+    for t, i, X in tickets:
+        r, _ = srv.wait(t, timeout=120.0)
+        meth, A, B, rp, ci, v, b = res_sys[i]
+        ref = oracle_solve(oracle, meth, rp, ci, v, b)
+        assert r.status == 0 and r.final_iter == ref[2]
+        if X is not None:
+            assert np.array_equal(bits(X.download()), bits(ref[1]))
+    st = srv.stats()
+    assert st["completed"] == len(tickets) and st["queued"] == 0
+    srv.close()
+    for t, i, X in tickets:
+        if X is not None:
+            X.free()
+    for _, A, B, *_ in res_sys:
+        A.free()
+        B.free()
+    c2.close()

@@ -151,7 +151,7 @@ def test_matvec_autotune(m, ctx, oracle, dtype):
     before = Y.download()
     t = A.autotune(reps=3)
     assert t["rows_per_block"] in (64, 128, 256) and t["nnz_per_block"] in (1024, 2048, 4096)
-    assert t["candidates"] == 18 and t["us"] > 0 and t["long_row"] == 256
+    assert t["candidates"] >= 22 and t["us"] > 0 and t["long_row"] == 256
     Y.fill(0.0)
     m.matvec(A, X, Y)
     after = Y.download()
@@ -353,3 +353,42 @@ def test_crs_diag_blocks(m, ctx, oracle, rows, cols):
         d = m.Vector(ctx, n=min(rows, cols), dtype=dt)
         A.diag(d)
         assert bits_equal(d.download(), oracle.crs_diag(rows, cols, rp, ci, val))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_plans_bit_identical(m, ctx, oracle, dtype):
+    """Every SpMV plan the autotuner can pick -- products staged in LDS, the
+    row walk, column panels (entries stored a second time panel-major, rows
+    continuing their sums across the passes), XCD order on and off --
+    computes the same bits: the CPU row loop's for rows up to the long-row
+    threshold, one block tree for longer rows."""
+    say("matvec plans", dtype.__name__)
+    n, cols = 40000, 700000  # x of 700k columns: 5.6 MB (FP64)
+    rp, ci, val = m.random_lognormal(n, cols, 24.0, 1.0, 900, seed=11)
+    val = val.astype(dtype)
+    x = oracle.random_vector(cols, -1.0, 1.0, 12, dtype=dtype)
+    A = m.CRS(ctx, n, cols, rp, ci, val)
+    X = m.Vector(ctx, x, dtype=dtype)
+    Y = m.Vector(ctx, n=n, dtype=dtype)
+    m.matvec(A, X, Y)
+    base = Y.download()
+    ref = oracle.spmv(rp, ci, val, x)
+    short = np.diff(rp) <= 256
+    assert short.sum() < n  # some long rows
+    assert bits_equal(base[short], ref[short])
+    tol = 1e-13 if dtype == np.float64 else 1e-5
+    assert np.linalg.norm(base - ref) <= tol * np.linalg.norm(ref)
+    for plan in [dict(rows_per_block=128, nnz_per_block=1024, row_walk=True),
+                 dict(rows_per_block=256, nnz_per_block=1024, row_walk=True, xcd_remap=False),
+                 dict(rows_per_block=64, nnz_per_block=4096),
+                 dict(rows_per_block=256, nnz_per_block=2048, column_panels=4),
+                 dict(rows_per_block=128, nnz_per_block=1024, column_panels=16, xcd_remap=False)]:
+        A.set_plan(**plan)
+        Y.fill(0.0)
+        m.matvec(A, X, Y)
+        assert bits_equal(Y.download(), base), plan
+    t = A.autotune(reps=2)
+    assert t["candidates"] >= 22 + 3 * (dtype == np.float64)
+    Y.fill(0.0)
+    m.matvec(A, X, Y)
+    assert bits_equal(Y.download(), base), t
