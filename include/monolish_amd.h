@@ -65,29 +65,31 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
 /* Autotuner of the SpMV block plan (no monolish counterpart: monolish hands
  * CRS SpMV to cuSPARSE): times A's SpMV under candidate plans -- rows per
  * block {64, 128, 256} x entries per block {1024, 2048, 4096} x XCD-aware
- * order on/off with products staged in LDS, and rows per block {128, 256}
- * x 1024 entries x XCD order with the row walk -- then, for matrices whose x does not fit an XCD's L2 and
+ * order on/off with products staged in LDS in entry order; rows per block
+ * {128, 256} x entries {1024, 2048} x XCD order with the row walk -- then, for matrices whose x does not fit an XCD's L2 and
  * whose rows list columns in ascending order, the best of these with 4, 8
  * or 16 column panels (the short rows' entries stored a second time,
  * panel-major, so each pass gathers x from an L2-sized slice) -- `reps`
- * launches each, keeps the fastest for A and reports it.  The long-row
- * threshold is unchanged, so results do not depend on the choice.  The
- * panel copy is dropped by mgx_csr_update_values. */
+ * launches each, keeps the fastest for A and reports it (within 2 %, an
+ * XCD-ordered plan).  The long-row threshold is unchanged, so results do
+ * not depend on the choice.  The panel copy is dropped by
+ * mgx_csr_update_values. */
 typedef struct mgx_spmv_tuning {
   int32_t rows_per_block, nnz_per_block, long_row, xcd_remap;
   double us;           /* time of one SpMV under the chosen plan */
   int32_t candidates;  /* plans timed */
   int32_t column_panels; /* 0, or the column panels of the chosen plan */
-  int32_t row_walk;    /* 1: stream blocks walk rows lane by lane (LDS-staged
+  int32_t row_walk;    /* stream blocks: 0 products staged in LDS in entry
+                          order; 1 row walk (lane per row over LDS-staged
                           val / col_ind; coalesced gathers on banded rows) */
   int32_t pad;
 } mgx_spmv_tuning;
 int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out);
 /* Sets A's SpMV plan explicitly (the candidates the autotuner times):
- * rows_per_block in [1, 256], nnz_per_block in [64, 8192] (<= 1024 with
+ * rows_per_block in [1, 256], nnz_per_block in [64, 8192] (<= 4096 with
  * row_walk), xcd_remap and row_walk 0/1, column_panels 0 or 2..64 (needs
- * ascending columns within rows: MGX_ERR_INVALID otherwise).  The long-row
- * threshold stays as it is. */
+ * ascending columns within rows: MGX_ERR_INVALID otherwise).
+ * The long-row threshold stays as it is. */
 int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_remap,
                      int row_walk, int column_panels);
 /* 1: fused solvers and batches replay each group of iterations from a
@@ -235,6 +237,18 @@ int mgx_residual_l2(mgx_csr *A, mgx_vec *x, mgx_vec *b, double *out);
 int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                     const mgx_solve_params *p, double inner_tol,
                     int64_t max_outer, mgx_solve_result *res);
+
+/* Mixed-precision Jacobi-PCG with FP64 residual replacement: the CG
+ * iterations run on FP32 values and vectors with FP32 accumulation; each
+ * time the FP32 residual has dropped by the factor delta since the last
+ * replacement, the FP32 correction is added to the FP64 x and the FP64
+ * residual r = b - A x replaces the FP32 one, keeping the search direction.
+ * The stopping test ||r|| < tol (or tol*||b||) is made on that FP64
+ * residual.  No monolish counterpart: a precision form of equation::CG
+ * (cg.cpp).  res: inner_iter = FP32 iterations, outer_iter = replacements. */
+int mgx_mixed_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                       const mgx_solve_params *p, double delta,
+                       int64_t max_updates, mgx_solve_result *res);
 
 /* ---- batched solves ------------------------------------------------------ */
 #define MGX_METHOD_CG 0

@@ -113,6 +113,8 @@ for _n in ("mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve"):
     _sig(_n, [_P, _P, _P, C.POINTER(SolveParams), C.POINTER(SolveResultC)])
 _sig("mgx_residual_l2", [_P, _P, _P, C.POINTER(_D)])
 _sig("mgx_ir_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64, C.POINTER(SolveResultC)])
+_sig("mgx_mixed_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64,
+                            C.POINTER(SolveResultC)])
 _sig("mgx_gen_stencil3d", [_I64, _I64, _I64, C.c_int, _D, C.POINTER(_I64), _P, _P, _P])
 _sig("mgx_gen_random_lognormal", [_I64, _I64, _D, _D, _I32, C.c_uint64, _P, _P, _P])
 _sig("mgx_gen_laplacian2d", [_I64, _I64, C.POINTER(_I64), _P, _P, _P])
@@ -334,9 +336,10 @@ class CRS:
         check(_lib.mgx_csr_stats_get(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in CsrStats._fields_}
 
-    def set_plan(self, rows_per_block=256, nnz_per_block=2048, xcd_remap=True, row_walk=False,
+    def set_plan(self, rows_per_block=256, nnz_per_block=2048, xcd_remap=True, row_walk=0,
                  column_panels=0):
-        """Fix the SpMV block plan (the autotuner's candidates)."""
+        """Fix the SpMV block plan (the autotuner's candidates).  row_walk: 0
+        products staged in LDS in entry order, 1 row walk."""
         check(_lib.mgx_csr_set_plan(self._h, rows_per_block, nnz_per_block, int(xcd_remap),
                                     int(row_walk), column_panels))
 
@@ -493,6 +496,18 @@ def ir_cg(A, x, b, tol=1e-10, tol_is_relative=True, inner_tol=1e-4, max_outer=10
     r = SolveResultC()
     rc = check(_lib.mgx_ir_cg_solve(A.handle, x.handle, b.handle, C.byref(p), inner_tol,
                                     int(max_outer), C.byref(r)))
+    return SolveResult(rc, r.final_iter, r.final_resid, r.seconds, r.inner_iter, r.outer_iter)
+
+
+def mixed_cg(A, x, b, tol=1e-10, tol_is_relative=True, delta=0.1, max_updates=1000, maxiter=0,
+             precond=PRECOND_JACOBI, check_every=0) -> SolveResult:
+    """Mixed-precision Jacobi-PCG: FP32 iterations, FP64 residual replacement
+    each time the FP32 residual has dropped by ``delta`` (search direction
+    kept).  inner_iter = FP32 iterations, outer_iter = replacements."""
+    p, _ = _params(tol, maxiter, 0, precond, MODE_FUSED, tol_is_relative, check_every, 0)
+    r = SolveResultC()
+    rc = check(_lib.mgx_mixed_cg_solve(A.handle, x.handle, b.handle, C.byref(p), delta,
+                                       int(max_updates), C.byref(r)))
     return SolveResult(rc, r.final_iter, r.final_resid, r.seconds, r.inner_iter, r.outer_iter)
 
 

@@ -409,6 +409,20 @@ template <class T> std::vector<int> Engine<T>::poll() {
 
 template <class T> SolveState Engine<T>::state(int slot) { return h_states_[slot]; }
 
+template <class T> void Engine<T>::resume(int slot, const SolveState &s) {
+  MGX_REQUIRE(slot >= 0 && slot < cap_ && reported_[slot] &&
+                  std::find(active_.begin(), active_.end(), slot) != active_.end(),
+              "engine: resume needs a stopped, unreleased slot");
+  MGX_REQUIRE(s.status == kStatRunning, "engine: resume needs a running state");
+  MGX_HIP(hipStreamSynchronize(c_->stream));
+  h_states_[slot] = s;
+  MGX_HIP(hipMemcpy(d_states_ + slot, &h_states_[slot], sizeof(SolveState),
+                    hipMemcpyHostToDevice));
+  reported_[slot] = 0;
+  host_running_++;
+  dirty_ = true;
+}
+
 template <class T> void Engine<T>::fetch_states() {
   MGX_HIP(hipStreamSynchronize(c_->stream));
   MGX_HIP(hipMemcpy(h_states_.data(), d_states_, sizeof(SolveState) * cap_,

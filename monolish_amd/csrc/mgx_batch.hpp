@@ -140,6 +140,8 @@ template <class T> class Engine {
   // synchronises; returns slots that stopped since the last poll
   std::vector<int> poll();
   SolveState state(int slot);  // valid after poll() reported the slot
+  // continue a stopped slot from the state s (its vectors as they are now)
+  void resume(int slot, const SolveState &s);
   void release(int slot);
   int active() const { return (int)active_.size(); }
   // run the set-up of the systems added since the last launch now

@@ -344,8 +344,10 @@ int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_
     MGX_REQUIRE(A, "mgx_csr_set_plan: NULL matrix");
     MGX_REQUIRE(rows_per_block >= 1 && rows_per_block <= kBlock,
                 "rows_per_block must be in [1, %d]", kBlock);
-    MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk ? 1024 : 8192),
-                "nnz_per_block must be in [64, %d]", row_walk ? 1024 : 8192);
+    MGX_REQUIRE(row_walk == 0 || row_walk == 1, "row_walk must be 0 or 1");
+    // the row walk stages 12 bytes per FP64 entry in LDS: 4096 entries fit 64 KiB
+    MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk ? 4096 : 8192),
+                "nnz_per_block must be in [64, %d]", row_walk ? 4096 : 8192);
     MGX_REQUIRE(column_panels == 0 || (column_panels >= 2 && column_panels <= 64),
                 "column_panels must be 0 or in [2, 64]");
     activate(A->ctx);

@@ -477,6 +477,124 @@ SolveOut ir_cg(Csr *A, double *x, const double *b, int precond, double thr,
   return o;
 }
 
+// r32 = (float) r; z = M r32 (Jacobi) or r32: the FP32 CG's residual and
+// preconditioned residual after a residual replacement
+__global__ void k_mp_replace(int64_t n, const double *r, const float *M, float *r32,
+                             float *z) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const float v = (float)r[i];
+    r32[i] = v;
+    z[i] = M ? M[i] * v : v;
+  }
+}
+
+// Mixed-precision PCG with FP64 residual replacement ("reliable updates",
+// Sleijpen & van der Vorst, Computing 56 (1996); the FP32/FP64 form of
+// Clark et al., Comput. Phys. Commun. 181 (2010), alg. 3).  The FP32 PCG
+// iterates a correction d32 from the residual r32.  Once its residual has
+// dropped by the factor delta since the last replacement, the correction is
+// folded into the FP64 x, the FP64 residual r = b - A x replaces r32 and z =
+// M r32, and the iteration continues with the same search direction p:
+// beta = <z,r>_new / <z,r>_{k-1}.  Unlike a restarted refinement the Krylov
+// direction survives each replacement, so the FP32 iteration count stays
+// close to the FP64 solver's.
+SolveOut mp_cg(Csr *A, double *x, const double *b, int precond, double thr,
+               int64_t max_inner_total, double delta, int64_t max_updates, int check_every,
+               int64_t *inner_total, int64_t *updates) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  ensure_val32(A);
+  double *r = (double *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 8);
+  float *r32 = (float *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 4);
+  float *d32 = (float *)ws_get(c, (size_t)std::max<int64_t>(n, 1) * 4);
+  const int g = stream_grid(c, n);
+  auto residual = [&]() {
+    launch_spmv(A, A->val, MGX_F64, x, r, 1, 0, false);
+    launch_vv(c, MGX_F64, 1, n, b, r, r);
+    return std::sqrt(dot_host(c, MGX_F64, n, r, r));
+  };
+  SolveOut o;
+  *inner_total = 0;
+  *updates = 0;
+  double rn = residual();
+  o.resid = rn;
+  if (rn < thr) {
+    o.status = MGX_OK;
+    return o;
+  }
+  if (std::isnan(rn)) {
+    o.status = MGX_SOLVER_RESIDUAL_NAN;
+    return o;
+  }
+  if (max_inner_total <= 0) {
+    o.status = MGX_SOLVER_MAXITER;
+    return o;
+  }
+  k_to_f32_scaled<<<g, kBlock, 0, c->stream>>>(n, r, 1.0, r32);
+  MGX_HIP(hipGetLastError());
+  launch_fill(c, MGX_F32, d32, n, 0.0);
+  Engine<float> &eng = ctx_engine<float>(c);
+  SolveCfg cfg;
+  cfg.method = kMethodCG;
+  cfg.precond = precond;
+  cfg.tol = std::max(delta * rn, thr);
+  cfg.miniter = 0;
+  cfg.maxiter = max_inner_total;
+  const int slot = eng.add(A, A->val32, d32, r32, cfg);
+  MGX_REQUIRE(slot >= 0, "mixed CG: out of device memory for the work vectors");
+  const int every = check_every > 0 ? check_every : 8;
+  try {
+    for (;;) {
+      for (;;) {
+        eng.step(every);
+        if (!eng.poll().empty()) break;
+      }
+      const SolveState hs = eng.state(slot);
+      *inner_total = hs.iter;
+      k_add_scaled_f32<<<g, kBlock, 0, c->stream>>>(n, 1.0, d32, x);
+      MGX_HIP(hipGetLastError());
+      (*updates)++;
+      rn = residual();
+      o.resid = rn;
+      if (rn < thr) {
+        o.status = MGX_OK;
+        break;
+      }
+      if (std::isnan(rn) || hs.status == MGX_SOLVER_RESIDUAL_NAN) {
+        o.status = MGX_SOLVER_RESIDUAL_NAN;
+        break;
+      }
+      if (hs.status != MGX_OK || hs.iter >= max_inner_total || *updates >= max_updates) {
+        o.status = MGX_SOLVER_MAXITER;
+        break;
+      }
+      // replace the FP32 residual; d32 restarts from 0, p is kept
+      const SysDesc<float> &d = eng.desc(slot);
+      launch_fill(c, MGX_F32, d32, n, 0.0);
+      k_mp_replace<<<g, kBlock, 0, c->stream>>>(n, r, d.M, d.r, d.z);
+      MGX_HIP(hipGetLastError());
+      const double zr = dot_host(c, MGX_F32, n, d.z, d.r);
+      const double rr = dot_host(c, MGX_F32, n, d.r, d.r);
+      SolveState s = hs;
+      // hs.beta = zr_k / zr_{k-1} and hs.zr = zr_k, both of the replaced residual
+      s.beta = hs.beta * (zr / hs.zr);
+      s.zr = zr;
+      s.rr = rr;
+      s.resid = std::sqrt(rr);
+      s.tol = std::max(delta * rn, thr);
+      s.status = kStatRunning;
+      eng.resume(slot, s);
+    }
+  } catch (...) {
+    eng.release(slot);
+    throw;
+  }
+  eng.release(slot);
+  o.iters = *inner_total;
+  return o;
+}
+
 }  // namespace
 
 }  // namespace mgx
@@ -599,6 +717,26 @@ int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *
     });
     res->inner_iter = inner;
     res->outer_iter = outer;
+    return st;
+  });
+}
+
+int mgx_mixed_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                       double delta, int64_t max_updates, mgx_solve_result *res) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, "mgx_mixed_cg_solve");
+    MGX_REQUIRE(A->dtype == MGX_F64, "mgx_mixed_cg_solve: needs an FP64 matrix");
+    MGX_REQUIRE(delta > 0 && delta < 1, "mgx_mixed_cg_solve: delta in (0,1)");
+    activate(A->ctx);
+    const double thr = threshold(A->ctx, p, b);
+    int64_t inner = 0, updates = 0;
+    int st = run_solver(A, res, [&] {
+      return mp_cg(A, (double *)x->d, (const double *)b->d, p->precond, thr, maxiter_of(p),
+                   delta, max_updates > 0 ? max_updates : 1000, p->check_every, &inner,
+                   &updates);
+    });
+    res->inner_iter = inner;
+    res->outer_iter = updates;
     return st;
   });
 }

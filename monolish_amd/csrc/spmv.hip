@@ -25,7 +25,11 @@ namespace {
 // U = entry budget / 256: each lane issues all of its val/col loads, then all
 // of its x gathers, before it multiplies, so up to U loads per lane are in
 // flight at once.
-template <class T, bool AB, int U, bool WALK>
+//
+// MODE 0: products staged in LDS in entry order (consecutive lanes,
+// consecutive entries).  MODE 1: row walk (lane per row over LDS-staged
+// val / col_ind).
+template <class T, bool AB, int U, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv(
     const int2 *__restrict__ blk, int64_t nblocks,
     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -72,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     rb = rp[r + 1];
   }
   T s = 0;
-  if (WALK) {
+  if (MODE == 1) {
     // row walk: the block's val / col_ind go to LDS with coalesced 16-byte
     // loads; then lane t walks row r0 + t, 8 entries at a time (LDS reads,
     // then the gathers, then the products added in order).  At each step
@@ -84,22 +88,28 @@ __global__ __launch_bounds__(kBlock) void k_spmv(
     const int32_t ea = e0 & ~(W - 1);
     T *sv = prod;
     int32_t *sc = reinterpret_cast<int32_t *>(sv + U * kBlock * W);
-    VT v[U];
-    VI c[U];
+    // load slots in groups (at most 4: 8 loads in flight per lane): fully
+    // unrolled, larger budgets compiled to 74-176 VGPRs here
+    constexpr int G = U <= 3 ? U : (U == 4 ? 2 : 4);
+#pragma unroll 1
+    for (int k0 = 0; k0 < U; k0 += G) {
+      VT v[G];
+      VI c[G];
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
-      if (base < e1) {
-        v[k] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(val + base));
-        c[k] = __builtin_nontemporal_load(reinterpret_cast<const VI *>(ci + base));
+      for (int k = 0; k < G; k++) {
+        const int32_t base = ea + ((int32_t)threadIdx.x + (k0 + k) * kBlock) * W;
+        if (k0 + k < U && base < e1) {
+          v[k] = __builtin_nontemporal_load(reinterpret_cast<const VT *>(val + base));
+          c[k] = __builtin_nontemporal_load(reinterpret_cast<const VI *>(ci + base));
+        }
       }
-    }
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-      const int32_t base = ea + ((int32_t)threadIdx.x + k * kBlock) * W;
-      if (base < e1) {
-        *reinterpret_cast<VT *>(sv + (base - ea)) = v[k];
-        *reinterpret_cast<VI *>(sc + (base - ea)) = c[k];
+      for (int k = 0; k < G; k++) {
+        const int32_t base = ea + ((int32_t)threadIdx.x + (k0 + k) * kBlock) * W;
+        if (k0 + k < U && base < e1) {
+          *reinterpret_cast<VT *>(sv + (base - ea)) = v[k];
+          *reinterpret_cast<VI *>(sc + (base - ea)) = c[k];
+        }
       }
     }
     __syncthreads();
@@ -477,11 +487,11 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
   do {                                                                                    \
     if (walk) {                                                                           \
       lds = (size_t)(UU) * kBlock * W * (sizeof(T) + 4);                                  \
-      k_spmv<T, AB, UU, true><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
-                                                                Y, alpha, beta, remap, live); \
+      k_spmv<T, AB, UU, 1><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, \
+                                                             alpha, beta, remap, live);   \
     } else {                                                                              \
-      k_spmv<T, AB, UU, false><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, \
-                                                                 Y, alpha, beta, remap, live); \
+      k_spmv<T, AB, UU, 0><<<grid, kBlock, lds, c->stream>>>(B, nb, A->rp, A->ci, V, X, Y, \
+                                                             alpha, beta, remap, live);   \
     }                                                                                     \
   } while (0)
   if (u <= 1) MGX_SPMV_U(1);
@@ -491,6 +501,7 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
   else if (u <= 5) MGX_SPMV_U(5);
   else if (u <= 6) MGX_SPMV_U(6);
   else if (u <= 8) MGX_SPMV_U(8);
+  else if (u <= 9) MGX_SPMV_U(9);
   else if (u <= 12) MGX_SPMV_U(12);
   else MGX_SPMV_U(17);
 #undef MGX_SPMV_U
@@ -562,8 +573,9 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
         if (nm >= long_t) cands.push_back(Cand{rm, nm, rr, 0, 0.0});
   // the row walk stages val and col_ind (12 bytes per FP64 entry) in LDS
   for (int rm : {128, 256})
-    for (int rr : {1, 0})
-      if (1024 >= long_t) cands.push_back(Cand{rm, 1024, rr, 1, 0.0});
+    for (int nm : {1024, 2048})
+      for (int rr : {1, 0})
+        if (nm >= long_t) cands.push_back(Cand{rm, nm, rr, 1, 0.0});
   try {
     MGX_HIP(hipEventCreate(&e0));
     MGX_HIP(hipEventCreate(&e1));
@@ -589,8 +601,16 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
   (void)hipEventDestroy(e1);
   dev_free(x);
   dev_free(y);
-  const Cand best = *std::min_element(cands.begin(), cands.end(),
-                                      [](const Cand &a, const Cand &b) { return a.us < b.us; });
+  const auto faster = [](const Cand &a, const Cand &b) { return a.us < b.us; };
+  Cand best = *std::min_element(cands.begin(), cands.end(), faster);
+  // Within 2 % (the spread of short repeated timings) an XCD-ordered plan
+  // wins: it keeps each x line in one XCD's L2, where the round-robin order
+  // fetches the lines shared by neighbouring blocks into several L2s.
+  if (!best.remap) {
+    const double limit = best.us * 1.02;
+    for (const Cand &k : cands)
+      if (k.remap && k.us <= limit && (!best.remap || k.us < best.us)) best = k;
+  }
   spmv_plan_build_with(A, rp.data(), best.rows_max, best.nnz_max, long_t, best.remap,
                        best.walk);
   // column panels, when x outgrows an XCD's L2 (4 MiB)

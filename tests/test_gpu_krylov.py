@@ -261,3 +261,56 @@ def test_mixed_precision_refinement(m, ctx, oracle):
     ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=1e-10 * bn)
     # both meet ||r|| < 1e-10 ||b||; kappa(27-pt, shift 1e-2) < 3e3
     assert np.linalg.norm(x - ref[1]) <= 3e3 * 2e-10 * np.linalg.norm(ref[1])
+
+
+@pytest.mark.parametrize("grid,delta", [(32, 0.1), (32, 1e-3), (81, 0.05)])
+def test_mixed_cg_residual_replacement(m, ctx, oracle, grid, delta):
+    """FP32 PCG with FP64 residual replacement on the 27-point shifted
+    stencil (32^3: the two-kernel iteration; 81^3 = 531,441 rows: the split
+    iteration).  The FP64 true residual meets 1e-10 ||b||, and because the
+    search direction survives every replacement the FP32 iteration count
+    stays within 10 % of the FP64 CG's (a wrong beta after a replacement
+    restarts or breaks the recurrence and shows up here first)."""
+    say("mixed-cg 27pt", grid, delta)
+    rp, ci, val = m.stencil3d(grid, grid, grid, 27, 1e-2)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3001)
+    bn = oracle.nrm2(b)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    B = m.Vector(ctx, b)
+    X64 = m.Vector(ctx, np.zeros(n))
+    ref = m.cg(A, X64, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI)
+    assert ref.status == 0
+    X = m.Vector(ctx, np.zeros(n))
+    res = m.mixed_cg(A, X, B, tol=1e-10, tol_is_relative=True, delta=delta)
+    say("mixed", res, "fp64 iterations", ref.final_iter)
+    x = X.download()
+    assert res.status == 0 and res.outer_iter >= 2
+    assert oracle.residual_l2(rp, ci, val, x, b) < 1e-10 * bn
+    assert res.inner_iter <= 1.10 * ref.final_iter + 5
+    x64 = X64.download()
+    # both meet ||r|| < 1e-10 ||b||; kappa(27-pt, shift 1e-2) < 3e3
+    assert np.linalg.norm(x - x64) <= 3e3 * 2e-10 * np.linalg.norm(x64)
+    for v in (A, B, X64, X):
+        v.free()
+
+
+def test_mixed_cg_edge_cases(m, ctx, oracle):
+    """x0 already a solution: no FP32 iteration; maxiter reached: status
+    MAXITER with the FP64 residual reported; delta outside (0, 1) rejected."""
+    say("mixed-cg edges")
+    rp, ci, val = m.stencil3d(8, 8, 8, 7)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 5)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    B = m.Vector(ctx, np.zeros(n))
+    X = m.Vector(ctx, np.zeros(n))
+    res = m.mixed_cg(A, X, B, tol=1e-10, tol_is_relative=False)
+    assert res.status == 0 and res.inner_iter == 0 and res.outer_iter == 0
+    B = m.Vector(ctx, b)
+    X = m.Vector(ctx, np.zeros(n))
+    res = m.mixed_cg(A, X, B, tol=1e-12, tol_is_relative=True, maxiter=3)
+    assert res.status == m.SOLVER_MAXITER and res.inner_iter == 3
+    assert abs(res.final_resid - oracle.residual_l2(rp, ci, val, X.download(), b)) <= 1e-12 * oracle.nrm2(b)
+    with pytest.raises(m.MgxError):
+        m.mixed_cg(A, X, B, delta=1.0)
