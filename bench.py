@@ -120,17 +120,21 @@ def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup, autotune=
     return out, ms, wall
 
 
+def c1_problem(m):
+    """C1: laplacian_matrix_2D_5p(256, 256) as CRS, b = vector(n, -1, 1, 1001)."""
+    rp, ci, val = m.laplacian2d(256, 256)
+    return rp, ci, val, m.random_vector(len(rp) - 1, -1.0, 1.0, 1001)
+
+
 def bench_krylov(m, ctx, reps_fused, reps_plain):
-    import oracle
-    rows, cols, r, c, v = oracle.laplacian_2d5p(256, 256)
-    rp, ci, val = oracle.crs_from_coo(rows, cols, r, c, v)
-    n = rows
-    b = oracle.random_vector(n, -1.0, 1.0, 1001)
-    tol = 1e-10 * float(oracle.nrm2(b))
+    rp, ci, val, b = c1_problem(m)
+    n = len(rp) - 1
     A = m.CRS(ctx, n, n, rp, ci, val)
     A.autotune(reps=10)
     B = m.Vector(ctx, b)
     X = m.Vector(ctx, n=n)
+    bn = m.nrm2(B)
+    tol = 1e-10 * bn
     out = {"workload": "Jacobi-PCG, 2D 5-point Poisson 256x256 (n=65536, nnz=326656), "
                        "b=U[-1,1) seed 1001, x0=0, ||r||<1e-10||b||"}
 
@@ -159,29 +163,24 @@ def bench_krylov(m, ctx, reps_fused, reps_plain):
     out["fused_fp64_gpu_cg"] = fused
     out["fused_vs_plain"] = round(fused["solves_per_s"] / plain["solves_per_s"], 2)
     out["fused_bit_identical_to_plain"] = bool(np.array_equal(xp.view(np.uint64), xf.view(np.uint64)))
-    out["true_rel_residual"] = float(oracle.residual_l2(rp, ci, val, xf, b)) / float(oracle.nrm2(b))
-    # CPU: the oracle's restatement of the reference CG loop, same inputs
-    t0 = time.perf_counter()
-    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_JACOBI, tol=tol)
-    t_cpu = time.perf_counter() - t0
-    out["cpu_cg"] = {"solves_per_s": round(1.0 / t_cpu, 2), "iterations": ref[2],
-                     "threads": oracle.num_threads(),
-                     "gpu_bit_identical": bool(np.array_equal(ref[1].view(np.uint64), xf.view(np.uint64)))}
+    out["true_rel_residual"] = m.residual_l2(A, X, B) / bn
     log(f"krylov C1: plain {plain['solves_per_s']}/s fused {fused['solves_per_s']}/s "
         f"(x{out['fused_vs_plain']}), {fused['iterations']} it, p99 {fused['p99_ms']} ms")
-    return out
+    A.free()
+    B.free()
+    X.free()
+    return out, xf, tol
 
 
 def bench_mixed(m, ctx, side, reps):
-    import oracle
     rp, ci, val = m.stencil3d(side, side, side, 27, 1e-2)
     n = len(rp) - 1
-    b = oracle.random_vector(n, -1.0, 1.0, 3001)
-    bn = float(oracle.nrm2(b))
+    b = m.random_vector(n, -1.0, 1.0, 3001)
     A = m.CRS(ctx, n, n, rp, ci, val)
     A.autotune(reps=5)
     B = m.Vector(ctx, b)
     X = m.Vector(ctx, n=n)
+    bn = m.nrm2(B)
     res = {}
     # restarted refinement: the inner FP32 solves restart from zero at every
     # refinement step, so the inner tolerance trades restarts against FP32
@@ -208,7 +207,7 @@ def bench_mixed(m, ctx, side, reps):
             r = fn()
             ts.append(time.perf_counter() - t0)
         x = X.download()
-        true_rel = float(oracle.residual_l2(rp, ci, val, x, b)) / bn
+        true_rel = m.residual_l2(A, X, B) / bn
         res[name] = {"ms": round(1e3 * statistics.median(ts[1:]), 3), "iterations": r.final_iter,
                      "outer": r.outer_iter, "true_rel_residual": true_rel, "status": r.status}
         res[name + "_x"] = x
@@ -249,14 +248,13 @@ def bench_batched(m, ctx, nsys, seed0=4000):
     same fused launches."""
     import concurrent.futures as cf
 
-    import oracle
     rng = np.random.default_rng(seed0)
     sizes = np.floor(np.exp(rng.uniform(np.log(500), np.log(20000), nsys))).astype(int)
     t0 = time.perf_counter()
 
     def gen(s):
         rp, ci, v = m.mmatrix(int(sizes[s]), 5, 15, 0.5, seed=seed0 + s)
-        b = oracle.random_vector(int(sizes[s]), -1.0, 1.0, seed0 + s)
+        b = m.random_vector(int(sizes[s]), -1.0, 1.0, seed0 + s)
         return rp, ci, v, b
 
     with cf.ThreadPoolExecutor(16) as ex:
@@ -336,7 +334,6 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48):
     are uploaded once (the served matrices and right-hand sides stay resident
     in HBM); each request is submitted by handle, so the client keeps up with
     the arrival rate and queued requests hold no host copies."""
-    import oracle
     rng = np.random.default_rng(seed)
     systems = []
     for k in range(pool):
@@ -354,7 +351,7 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48):
             rp, ci, v = m.convdiff2d(side, float(rng.uniform(1, 50)))
             meth = m.METHOD_BICGSTAB
         nn = len(rp) - 1
-        b = oracle.random_vector(nn, -1.0, 1.0, seed + k)
+        b = m.random_vector(nn, -1.0, 1.0, seed + k)
         systems.append((meth, m.CRS(ctx, nn, nn, rp, ci, v), m.Vector(ctx, b), nn))
     ctx.sync()
     srv = m.Server(ctx, max_active=1024, iters_per_poll=8)
@@ -412,6 +409,99 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48):
     return out
 
 
+def bench_ingest(m, ctx, side, with_cpu):
+    """MatrixMarket -> device CRS (COO::input_mm + CRS(COO&), optionally
+    COO::sort(true)) on the 3D 7-point side^3 matrix: the file is written by
+    the library's output_mm, read back with sort = 0 and sort = 2, and the
+    device conversion is timed alone from shuffled host triplets."""
+    import shutil
+    import tempfile
+    rp, ci, val = m.stencil3d(side, side, side, 7)
+    n, nnz = len(rp) - 1, len(val)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    d = tempfile.mkdtemp(prefix="mgx_mm_")
+    out = {"workload": f"3D 7-point {side}^3 (n={n}, nnz={nnz}) as a MatrixMarket file written "
+                       f"by output_mm (%.17e values)"}
+    try:
+        path = os.path.join(d, "a.mtx")
+        t0 = time.perf_counter()
+        A.write_mm(path)
+        t_write = time.perf_counter() - t0
+        size = os.path.getsize(path)
+
+        def timed(fn):
+            ctx.sync()
+            t = time.perf_counter()
+            B = fn()
+            ctx.sync()
+            return B, time.perf_counter() - t
+
+        B0, t0s = timed(lambda: m.CRS.read_mm(ctx, path, sort=0))
+        B2, t2s = timed(lambda: m.CRS.read_mm(ctx, path, sort=2))
+        rows_coo = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
+        perm = np.random.default_rng(6001).permutation(nnz)
+        r_s, c_s, v_s = rows_coo[perm], ci[perm], val[perm]
+        Bc, tcs = timed(lambda: m.CRS.from_coo(ctx, n, n, r_s, c_s, v_s, sort=2))
+        same = True
+        for B in (B0, B2, Bc):
+            brp, bci, bv = B.download()
+            same = same and np.array_equal(brp, rp) and np.array_equal(bci, ci) and \
+                np.array_equal(bv.view(np.uint64), val.view(np.uint64))
+            B.free()
+        out.update({
+            "file_MB": round(size / 1e6, 1), "nnz": nnz,
+            "write_mm_s": round(t_write, 3),
+            "read_mm_sort0_s": round(t0s, 3),
+            "read_mm_sort0_Mentries_per_s": round(nnz / t0s / 1e6, 1),
+            "read_mm_sort0_MB_per_s": round(size / t0s / 1e6, 1),
+            "read_mm_sort2_s": round(t2s, 3),
+            "from_coo_shuffled_sort2_s": round(tcs, 3),
+            "from_coo_shuffled_sort2_Mentries_per_s": round(nnz / tcs / 1e6, 1),
+            "identical_to_source": same})
+        if with_cpu:
+            out["cpu"] = cpu_baseline_ingest(path, r_s, c_s, v_s, n)
+    finally:
+        A.free()
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"ingest: {out}")
+    return out
+
+
+def cpu_baseline_ingest(path, r_s, c_s, v_s, n):
+    """The oracle's COO::input_mm + CRS(COO&) on the same file, and its
+    COO::sort(true) on a 1M-entry sample of the shuffled triplets (the
+    reference's quicksort takes the last element as pivot)."""
+    import oracle
+    t = time.perf_counter()
+    rows, cols, r, c, v = oracle.input_mm(path)
+    t_read = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.crs_from_coo(rows, cols, r, c, v)
+    t_conv = time.perf_counter() - t
+    k = min(len(r_s), 1_000_000)
+    t = time.perf_counter()
+    oracle.coo_sort(r_s[:k], c_s[:k], v_s[:k], merge=True)
+    t_sort = time.perf_counter() - t
+    return {"input_mm_s": round(t_read, 3), "crs_from_coo_s": round(t_conv, 3),
+            "Mentries_per_s": round(len(v) / (t_read + t_conv) / 1e6, 2),
+            "coo_sort_merge_1M_s": round(t_sort, 3), "threads": 1, "kind": "port",
+            "sample": "full file for input_mm + CRS(COO&); 1M shuffled entries for the sort"}
+
+
+def cpu_baseline_cg(rp, ci, val, b, tol, x_gpu):
+    """The oracle's restatement of the reference CG loop on the C1 inputs,
+    timed on this host and compared with the GPU's fused solution."""
+    import oracle
+    n = len(rp) - 1
+    t0 = time.perf_counter()
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_JACOBI, tol=tol)
+    t_cpu = time.perf_counter() - t0
+    return {"solves_per_s": round(1.0 / t_cpu, 2), "iterations": ref[2],
+            "threads": oracle.num_threads(), "kind": "port",
+            "gpu_bit_identical": bool(np.array_equal(ref[1].view(np.uint64),
+                                                      x_gpu.view(np.uint64)))}
+
+
 def cpu_baseline_spmv(rp, ci, val, x, budget_s):
     import oracle
     oracle.spmv(rp, ci, val, x)  # warm-up
@@ -437,7 +527,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip the sweep, Krylov and CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--only", default="", help="comma list of legs: cpu,sweep,krylov,mixed,"
-                                                 "batched,served (default: all)")
+                                                 "batched,served,ingest (default: all)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-nmax", type=int, default=100000,
@@ -483,8 +573,7 @@ def main():
 
     # ---- headline: SpMV, 3D 7-point 192^3, FP64 ---------------------------
     rp, ci, val = m.stencil3d(192, 192, 192, 7)
-    import oracle
-    x = oracle.random_vector(len(rp) - 1, -1.0, 1.0, 2001)
+    x = m.random_vector(len(rp) - 1, -1.0, 1.0, 2001)
     head, ms, wall = bench_spmv_matrix(m, ev, ctx, "C2a fp64", rp, ci, val, x, args.steps, args.warmup)
     sweep = {"c2a_7pt_192_fp64": head}
     traffic = None
@@ -507,7 +596,7 @@ def main():
         "roofline": roofline,
     }
     legs = set(args.only.split(",")) if args.only else {"cpu", "sweep", "krylov", "mixed", "batched",
-                                                         "served"}
+                                                         "served", "ingest"}
     if args.quick:
         legs = set()
     if "cpu" in legs:
@@ -519,7 +608,7 @@ def main():
         del rp, ci, val, x
         n2 = 4_000_000
         rp2, ci2, v2 = m.random_lognormal(n2, n2, 32.0, 1.0, 2048, seed=2002)
-        x2 = oracle.random_vector(n2, -1.0, 1.0, 2002)
+        x2 = m.random_vector(n2, -1.0, 1.0, 2002)
         s64, _, _ = bench_spmv_matrix(m, ev, ctx, "C2b fp64", rp2, ci2, v2, x2, max(args.steps // 4, 10),
                                       args.warmup)
         s32, _, _ = bench_spmv_matrix(m, ev, ctx, "C2b fp32", rp2, ci2, v2.astype(np.float32),
@@ -529,13 +618,18 @@ def main():
         del rp2, ci2, v2, x2
         line["spmv_sweep"] = sweep
     if "krylov" in legs:
-        line["krylov"] = bench_krylov(m, ctx, reps_fused=30, reps_plain=5)
+        line["krylov"], xf, tol = bench_krylov(m, ctx, reps_fused=30, reps_plain=5)
+        if "cpu" in legs:
+            rp1, ci1, val1, b1 = c1_problem(m)
+            line["krylov"]["cpu_cg"] = cpu_baseline_cg(rp1, ci1, val1, b1, tol, xf)
     if "mixed" in legs:
         line["mixed_precision"] = bench_mixed(m, ctx, 128, reps=2)
     if "batched" in legs:
         line["batched"] = bench_batched(m, ctx, args.batch_systems)
     if "served" in legs:
         line["served"] = bench_served(m, ctx, 2000.0, args.serve_seconds, 1000, args.serve_nmax)
+    if "ingest" in legs:
+        line["ingest"] = bench_ingest(m, ctx, 128, "cpu" in legs)
     line["device"] = info["name"]
     print(json.dumps(line))
     ctx.close()

@@ -130,6 +130,32 @@ void *mgx_vec_device_ptr(mgx_vec *v);
 int mgx_csr_create(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
                    const int32_t *row_ptr, const int32_t *col_ind,
                    const void *val, mgx_dtype dtype, mgx_csr **out);
+/* CRS<T>(COO<T>&) (src/utils/convert/convert_crs.cpp:9-43) from host COO
+ * triplets (0-based int32 row, col; values of dtype), built on the device.
+ * sort = 0: the triplets are taken in the given order, which must list rows
+ * in ascending order, as CRS(COO&) reads them (MGX_ERR_INVALID otherwise);
+ * sort = 1: first ordered by (row, col) like COO::sort(false)
+ * (src/utils/at_insert_sort/at_insert_sort_coo.cpp:119-144), stably, so
+ * equal (row, col) keep their input order; sort = 2: ordered and merged
+ * like COO::sort(true): each run of equal (row, col) collapses to one entry
+ * holding the value of the last one in input order.  Indices outside
+ * rows x cols are rejected.  row_ptr of rows after the last non-empty one
+ * reads back as 0, as CRS(COO&) leaves it. */
+int mgx_csr_from_coo(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
+                     const int32_t *row, const int32_t *col, const void *val,
+                     mgx_dtype dtype, int sort, mgx_csr **out);
+/* COO<T>::input_mm (src/utils/IO/IO_coo.cpp:69-153) followed by
+ * mgx_csr_from_coo: a MatrixMarket "matrix coordinate real general" file
+ * (other banners rejected), '%' comment lines, the size line, then nnz
+ * lines "i j value" with 1-based indices; values parsed as dtype (FP32 text
+ * rounded once to float, as istream >> float).  The text is parsed by host
+ * threads; sorting, merging and row_ptr run on the device. */
+int mgx_csr_read_mm(mgx_ctx *ctx, const char *path, mgx_dtype dtype, int sort,
+                    mgx_csr **out);
+/* COO<T>::output_mm (src/utils/IO/IO_coo.cpp:50-66) of COO(A): the same
+ * banner, size line and "i j value" lines (row order, 1-based, values in
+ * scientific notation with max_digits10 digits), written by host threads. */
+int mgx_csr_write_mm(mgx_csr *A, const char *path);
 /* CRS<T>::send on a resident matrix: value update (gpu_comm.cpp:95-99) */
 int mgx_csr_update_values(mgx_csr *A, const void *val);
 /* CRS<T>::nonfree_recv (gpu_comm.cpp:134-150); any pointer may be NULL */
@@ -354,6 +380,13 @@ int mgx_gen_convdiff2d(int64_t m, double peclet, int64_t *nnz, int32_t *row_ptr,
 int mgx_gen_mmatrix(int64_t n, int32_t kmin, int32_t kmax, double shift,
                     uint64_t seed, int32_t *row_ptr, int32_t *col_ind,
                     double *val);
+
+/* vector<T>(N, min, max, seed) / util::random_vector
+ * (src/utils/vector_constructor.cpp:78-113): std::mt19937(seed) and
+ * std::uniform_real_distribution<double>(min, max) drawn in order into the
+ * host array out (FP32: float bounds, each draw rounded to float). */
+int mgx_gen_random_vector(int64_t n, double min, double max, uint32_t seed,
+                          mgx_dtype dtype, void *out);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,9 @@ _sig("mgx_vec_fill", [_P, _D])
 _sig("mgx_vec_size", [_P], _I64)
 _sig("mgx_vec_device_ptr", [_P], _P)
 _sig("mgx_csr_create", [_P, _I64, _I64, _I64, _P, _P, _P, C.c_int, C.POINTER(_P)])
+_sig("mgx_csr_from_coo", [_P, _I64, _I64, _I64, _P, _P, _P, C.c_int, C.c_int, C.POINTER(_P)])
+_sig("mgx_csr_read_mm", [_P, C.c_char_p, C.c_int, C.c_int, C.POINTER(_P)])
+_sig("mgx_csr_write_mm", [_P, C.c_char_p])
 _sig("mgx_csr_update_values", [_P, _P])
 _sig("mgx_csr_download", [_P, _P, _P, _P])
 _sig("mgx_csr_destroy", [_P])
@@ -115,6 +118,7 @@ _sig("mgx_residual_l2", [_P, _P, _P, C.POINTER(_D)])
 _sig("mgx_ir_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64, C.POINTER(SolveResultC)])
 _sig("mgx_mixed_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64,
                             C.POINTER(SolveResultC)])
+_sig("mgx_gen_random_vector", [_I64, _D, _D, C.c_uint32, C.c_int, _P])
 _sig("mgx_gen_stencil3d", [_I64, _I64, _I64, C.c_int, _D, C.POINTER(_I64), _P, _P, _P])
 _sig("mgx_gen_random_lognormal", [_I64, _I64, _D, _D, _I32, C.c_uint64, _P, _P, _P])
 _sig("mgx_gen_laplacian2d", [_I64, _I64, C.POINTER(_I64), _P, _P, _P])
@@ -310,6 +314,46 @@ class CRS:
         check(_lib.mgx_csr_create(ctx.handle, rows, cols, v.size, _ptr(rp), _ptr(ci), _ptr(v),
                                   _dt(dtype), C.byref(self._h)))
         self.rows, self.cols, self.nnz, self.dtype = rows, cols, int(v.size), dtype
+
+    @classmethod
+    def _adopt(cls, ctx: Context, h, dtype):
+        self = cls.__new__(cls)
+        self.ctx, self._h, self.dtype = ctx, h, np.dtype(dtype)
+        s = CsrStats()
+        check(_lib.mgx_csr_stats_get(h, C.byref(s)))
+        self.rows, self.cols, self.nnz = int(s.rows), int(s.cols), int(s.nnz)
+        return self
+
+    @classmethod
+    def from_coo(cls, ctx: Context, rows: int, cols: int, row, col, val, sort: int = 0,
+                 dtype=None):
+        """CRS(COO&) built on the device from COO triplets.  sort: 0 the
+        given (row-ordered) order, 1 stable (row, col) sort, 2 sort and merge
+        duplicates (the last in input order wins)."""
+        val = np.asarray(val)
+        dtype = np.dtype(dtype or val.dtype)
+        r = np.ascontiguousarray(row, dtype=np.int32)
+        c = np.ascontiguousarray(col, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=dtype)
+        if not (r.size == c.size == v.size):
+            raise ValueError("inconsistent COO arrays")
+        h = _P()
+        check(_lib.mgx_csr_from_coo(ctx.handle, rows, cols, v.size, _ptr(r), _ptr(c), _ptr(v),
+                                    _dt(dtype), int(sort), C.byref(h)))
+        return cls._adopt(ctx, h, dtype)
+
+    @classmethod
+    def read_mm(cls, ctx: Context, path: str, dtype=np.float64, sort: int = 0):
+        """COO::input_mm(path) then CRS(COO&) (optionally sorted / merged),
+        the conversion on the device."""
+        h = _P()
+        check(_lib.mgx_csr_read_mm(ctx.handle, os.fsencode(path), _dt(dtype), int(sort),
+                                   C.byref(h)))
+        return cls._adopt(ctx, h, dtype)
+
+    def write_mm(self, path: str):
+        """COO(A).output_mm(path): MatrixMarket text, max_digits10 scientific."""
+        check(_lib.mgx_csr_write_mm(self._h, os.fsencode(path)))
 
     @property
     def handle(self):
@@ -567,6 +611,14 @@ def mmatrix(n: int, kmin: int = 5, kmax: int = 15, shift: float = 0.5, seed: int
     v = np.empty(nnz, np.float64)
     check(_lib.mgx_gen_mmatrix(n, kmin, kmax, shift, seed, _ptr(rp), _ptr(ci), _ptr(v)))
     return rp, ci, v
+
+
+def random_vector(n: int, vmin: float, vmax: float, seed: int, dtype=np.float64) -> np.ndarray:
+    """vector<T>(n, vmin, vmax, seed): std::mt19937 + uniform_real_distribution
+    draws in order (FP32: rounded draws)."""
+    out = np.empty(n, dtype)
+    check(_lib.mgx_gen_random_vector(n, vmin, vmax, seed, _dt(dtype), _ptr(out)))
+    return out
 
 
 def set_graphs(ctx: Context, enable: bool):

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -286,6 +287,27 @@ int mgx_gen_mmatrix(int64_t n, int32_t kmin, int32_t kmax, double shift, uint64_
     }
   }
   row_ptr[n] = (int32_t)e;
+  return MGX_OK;
+}
+
+// vector<T>(N, min, max, seed) and util::random_vector
+// (src/utils/vector_constructor.cpp:78-113): std::mt19937 seeded with seed,
+// std::uniform_real_distribution<double>(min, max), drawn in index order.
+// The FP32 forms pass float bounds to the double distribution and store
+// each double draw rounded to float.
+int mgx_gen_random_vector(int64_t n, double min, double max, uint32_t seed, mgx_dtype dtype,
+                          void *out) {
+  if (n < 0 || (n > 0 && !out) || (dtype != MGX_F64 && dtype != MGX_F32)) return MGX_ERR_INVALID;
+  std::mt19937 mt(seed);
+  if (dtype == MGX_F64) {
+    std::uniform_real_distribution<> rand(min, max);
+    double *o = static_cast<double *>(out);
+    for (int64_t i = 0; i < n; i++) o[i] = rand(mt);
+  } else {
+    std::uniform_real_distribution<> rand((float)min, (float)max);
+    float *o = static_cast<float *>(out);
+    for (int64_t i = 0; i < n; i++) o[i] = (float)rand(mt);
+  }
   return MGX_OK;
 }
 

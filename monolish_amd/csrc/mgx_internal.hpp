@@ -230,6 +230,9 @@ void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
 // fused-solver row chunks (built at upload for square matrices, from the
 // host row_ptr when given, else from a copy of the device one)
 void fused_plan_build(Csr *A, const int32_t *rp_host = nullptr);
+// col_ind / val of nnz entries plus 4 zeroed padding entries
+void alloc_entries(Csr *A, int64_t nnz);
+void csr_free(Csr *A);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);
 

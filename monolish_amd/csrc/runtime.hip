@@ -130,7 +130,7 @@ static bool overlap(const Vec *a, const Vec *b) {
 // col_ind / val with kEntryPad zeroed entries after the last one: the SpMV
 // reads whole 16-byte vectors, which may extend past the final entry
 constexpr int64_t kEntryPad = 4;
-static void alloc_entries(Csr *A, int64_t nnz) {
+void alloc_entries(Csr *A, int64_t nnz) {
   const size_t cnt = (size_t)(std::max<int64_t>(nnz, 0) + kEntryPad);
   A->ci = (int32_t *)dev_alloc(cnt * 4);
   A->val = dev_alloc(cnt * A->elem());
@@ -181,7 +181,7 @@ static Csr *build_transpose(Csr *A) {
   return T;
 }
 
-static void csr_free(Csr *A) {
+void csr_free(Csr *A) {
   if (!A) return;
   if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
   dev_free(A->rp);

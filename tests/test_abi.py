@@ -87,3 +87,39 @@ def test_random_lognormal_properties():
     assert np.all(v >= -1) and np.all(v < 1)
     rp2, ci2, v2 = m.random_lognormal(20000, 50000, 32.0, 1.0, 2048, seed=5)
     assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and np.array_equal(v, v2)
+
+
+def test_random_vector_matches_libstdcxx_golden():
+    """The library's vector(N, min, max, seed) generator against libstdc++'s
+    draws recorded in the golden file (and the oracle's restatement)."""
+    import json
+    import struct
+    import numpy as np
+    import monolish_amd as m
+    from conftest import GOLDEN
+    g = json.load(open(os.path.join(GOLDEN, "std_random.json")))
+    for case in g["uniform_double"]:
+        ref = np.array([struct.unpack("<d", bytes.fromhex(v)[::-1])[0] for v in case["values"]])
+        got = m.random_vector(len(ref), case["min"], case["max"], case["seed"])
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), case["seed"]
+    for case in g["uniform_float"]:
+        ref = np.array([struct.unpack("<f", bytes.fromhex(v)[::-1])[0] for v in case["values"]],
+                       dtype=np.float32)
+        got = m.random_vector(len(ref), -1.0, 1.0, case["seed"], dtype=np.float32)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), case["seed"]
+    v = m.random_vector(65536, -1.0, 1.0, 1001)
+    assert v[-1] == struct.unpack("<d", bytes.fromhex(g["rhs_65536_seed1001"]["last"])[::-1])[0]
+
+
+def test_laplacian2d_generator_equals_reference_construction():
+    """mgx_gen_laplacian2d writes CRS(laplacian_matrix_2D_5p(m, n)) directly;
+    the oracle builds it the reference's way (COO inserts, then CRS(COO))."""
+    import numpy as np
+    import monolish_amd as m
+    import oracle
+    for (a, b) in ((256, 256), (5, 3), (1, 7)):
+        rp, ci, v = m.laplacian2d(a, b)
+        rows, cols, r, c, vv = oracle.laplacian_2d5p(a, b)
+        rp2, ci2, v2 = oracle.crs_from_coo(rows, cols, r, c, vv)
+        assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2)
+        assert np.array_equal(v.view(np.uint64), v2.view(np.uint64))
