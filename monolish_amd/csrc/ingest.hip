@@ -226,12 +226,37 @@ int bits_for(int64_t v) {  // bits to hold 0..v-1 (at least 1)
 
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>((n + kBlock - 1) / kBlock, 1); }
 
+// stable sort of n (key, payload) pairs on the low `bits` bits of the keys;
+// the result is left in (k0, v0), (k1, v1) is scratch of the same size
+void radix_sort(Ctx *c, uint64_t *&k0, uint32_t *&v0, uint64_t *&k1, uint32_t *&v1, int64_t n,
+                int bits) {
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  uint32_t *hist = (uint32_t *)dev_alloc((size_t)std::max<int64_t>(ntiles, 1) * 256 * 4);
+  try {
+    for (int shift = 0; shift < bits; shift += 8) {
+      k_sort_hist<<<(unsigned)ntiles, kSortThreads, 0, c->stream>>>(k0, n, shift, hist, ntiles);
+      MGX_HIP(hipGetLastError());
+      scan_exclusive(c, hist, ntiles * 256);
+      k_sort_scatter<<<(unsigned)ntiles, kSortThreads, 0, c->stream>>>(k0, v0, k1, v1, n, shift,
+                                                                      hist, ntiles);
+      MGX_HIP(hipGetLastError());
+      std::swap(k0, k1);
+      std::swap(v0, v1);
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(hist);
+    throw;
+  }
+  MGX_HIP(hipStreamSynchronize(c->stream));
+  dev_free(hist);
+}
+
 template <class T>
 void sort_entries(Ctx *c, int64_t rows, int64_t cols, int64_t nnz, int32_t *d_row, int32_t *d_col,
                   T *d_val) {
   const int cbits = bits_for(cols);
   const int bits = bits_for(rows) + cbits;
-  const int64_t ntiles = (nnz + kSortTile - 1) / kSortTile;
   std::vector<void *> tmp;
   auto take = [&](size_t b) {
     void *p = dev_alloc(b);
@@ -241,21 +266,11 @@ void sort_entries(Ctx *c, int64_t rows, int64_t cols, int64_t nnz, int32_t *d_ro
   try {
     uint64_t *k0 = (uint64_t *)take((size_t)nnz * 8), *k1 = (uint64_t *)take((size_t)nnz * 8);
     uint32_t *v0 = (uint32_t *)take((size_t)nnz * 4), *v1 = (uint32_t *)take((size_t)nnz * 4);
-    uint32_t *hist = (uint32_t *)take((size_t)ntiles * 256 * 4);
     T *vals = (T *)take((size_t)nnz * sizeof(T));
     k_make_keys<<<grid_for(nnz), kBlock, 0, c->stream>>>(d_row, d_col, nnz, cbits, k0);
     k_iota<<<grid_for(nnz), kBlock, 0, c->stream>>>(v0, nnz);
     MGX_HIP(hipGetLastError());
-    for (int shift = 0; shift < bits; shift += 8) {
-      k_sort_hist<<<(unsigned)ntiles, kSortThreads, 0, c->stream>>>(k0, nnz, shift, hist, ntiles);
-      MGX_HIP(hipGetLastError());
-      scan_exclusive(c, hist, ntiles * 256);
-      k_sort_scatter<<<(unsigned)ntiles, kSortThreads, 0, c->stream>>>(k0, v0, k1, v1, nnz, shift,
-                                                                      hist, ntiles);
-      MGX_HIP(hipGetLastError());
-      std::swap(k0, k1);
-      std::swap(v0, v1);
-    }
+    radix_sort(c, k0, v0, k1, v1, nnz, bits);
     MGX_HIP(hipMemcpyAsync(vals, d_val, (size_t)nnz * sizeof(T), hipMemcpyDeviceToDevice,
                            c->stream));
     k_unpack<T><<<grid_for(nnz), kBlock, 0, c->stream>>>(k0, v0, vals, nnz, cbits, d_row, d_col,
@@ -268,6 +283,61 @@ void sort_entries(Ctx *c, int64_t rows, int64_t cols, int64_t nnz, int32_t *d_ro
     throw;
   }
   for (void *p : tmp) dev_free(p);
+}
+
+// row index of every entry of a CRS (one thread per row)
+__global__ void k_entry_rows(const int32_t *rp, int64_t rows, int32_t *row) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  for (int32_t j = rp[r]; j < rp[r + 1]; j++) row[j] = (int32_t)r;
+}
+
+// A^T on the device: entries sorted stably by (col, row), so row c of A^T
+// lists column c's entries in ascending original row -- the order in which
+// the reference's serial transposed product (crs_matvec.hpp:94-97) adds them
+template <class T> Csr *transpose_device(Csr *A) {
+  Ctx *c = A->ctx;
+  std::vector<int32_t> rp(A->rows + 1);
+  MGX_HIP(hipStreamSynchronize(c->stream));
+  MGX_HIP(hipMemcpy(rp.data(), A->rp, rp.size() * 4, hipMemcpyDeviceToHost));
+  const int64_t used = rp[A->rows];
+  Csr *Tm = new mgx_csr();
+  Tm->ctx = c;
+  Tm->rows = A->cols;
+  Tm->cols = A->rows;
+  Tm->nnz = used;
+  Tm->dtype = A->dtype;
+  int32_t *d_row = nullptr, *d_col = nullptr;
+  try {
+    const size_t ne = (size_t)std::max<int64_t>(used, 1);
+    d_row = (int32_t *)dev_alloc(ne * 4);  // becomes A^T's row = A's column
+    d_col = (int32_t *)dev_alloc(ne * 4);
+    Tm->rp = (int32_t *)dev_alloc((size_t)(Tm->rows + 1) * 4);
+    alloc_entries(Tm, used);
+    if (used > 0) {
+      k_entry_rows<<<grid_for(A->rows), kBlock, 0, c->stream>>>(A->rp, A->rows, d_col);
+      MGX_HIP(hipGetLastError());
+      MGX_HIP(hipMemcpyAsync(d_row, A->ci, (size_t)used * 4, hipMemcpyDeviceToDevice, c->stream));
+      MGX_HIP(hipMemcpyAsync(Tm->val, A->val, (size_t)used * sizeof(T), hipMemcpyDeviceToDevice,
+                             c->stream));
+      sort_entries<T>(c, Tm->rows, Tm->cols, used, d_row, d_col, static_cast<T *>(Tm->val));
+      MGX_HIP(hipMemcpyAsync(Tm->ci, d_col, (size_t)used * 4, hipMemcpyDeviceToDevice, c->stream));
+    }
+    k_row_ptr<<<grid_for(used + 1), kBlock, 0, c->stream>>>(d_row, used, Tm->rows, Tm->rp);
+    MGX_HIP(hipGetLastError());
+    std::vector<int32_t> trp(Tm->rows + 1);
+    MGX_HIP(hipMemcpyAsync(trp.data(), Tm->rp, trp.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    MGX_HIP(hipStreamSynchronize(c->stream));
+    spmv_plan_build(c, Tm, trp.data());
+  } catch (...) {
+    dev_free(d_row);
+    dev_free(d_col);
+    csr_free(Tm);
+    throw;
+  }
+  dev_free(d_row);
+  dev_free(d_col);
+  return Tm;
 }
 
 // duplicates (equal row and col, adjacent after the sort) collapse to the
@@ -652,6 +722,14 @@ template <class T> void write_mm(Csr *A, const char *path) {
   ok = (fclose(fp) == 0) && ok;
   MGX_REQUIRE(ok, "output_mm: writing %s failed", path);
 }
+
+}  // namespace
+
+Csr *csr_transpose(Csr *A) {
+  return A->dtype == MGX_F64 ? transpose_device<double>(A) : transpose_device<float>(A);
+}
+
+namespace {
 
 void register_csr(Ctx *c, Csr *A) {
   std::lock_guard<std::mutex> lk(c->objs_mu);

@@ -117,6 +117,23 @@ struct SpmvPlan {
   int2 *d_pblocks = nullptr;   // stream-block descriptors of all panels
   std::vector<int64_t> pblock_off;  // panel k: descriptors [off[k], off[k+1]) (+1 end)
   int32_t *d_lrows = nullptr;  // the long rows (nlong of them)
+  // sliced mode (0: off): the short rows' entries are also stored in slices
+  // of 64 rows and w-entry tuples, entry j of a slice's row l at
+  // soff[slice] + ((j / w) * 64 + l) * w + j % w, w = 16 / sizeof(value):
+  // `sl` for A's values, `sl32` (w = 4) for the FP32 copy of an FP64
+  // matrix; one lane per row; long rows keep the block-tree path
+  struct Sliced {
+    int w = 0;
+    int64_t entries = 0;
+    int64_t *d_soff = nullptr;
+    int32_t *d_sci = nullptr;
+    void *d_val = nullptr;
+  };
+  int32_t sell = 0;
+  Sliced sl, sl32;
+  std::vector<int64_t> soff32;  // host offsets of sl32, kept until it is built
+  int32_t *d_slrows = nullptr;
+  int64_t nslong = 0;
 };
 
 struct Csr {
@@ -226,6 +243,9 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out);
 // columns not ascending within a row)
 bool spmv_panels_build(Csr *A, int npanels);
 void spmv_panels_free(Csr *A);
+bool spmv_sell_build(Csr *A, const int32_t *rp_host);
+void spmv_sell_free(Csr *A);
+void spmv_sell_values(Csr *A);  // refresh the sliced values after a change
 void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
 // fused-solver row chunks (built at upload for square matrices, from the
 // host row_ptr when given, else from a copy of the device one)
@@ -233,6 +253,8 @@ void fused_plan_build(Csr *A, const int32_t *rp_host = nullptr);
 // col_ind / val of nnz entries plus 4 zeroed padding entries
 void alloc_entries(Csr *A, int64_t nnz);
 void csr_free(Csr *A);
+// A^T built on the device (stable sort of the entries by column)
+Csr *csr_transpose(Csr *A);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);
 

@@ -139,48 +139,6 @@ void alloc_entries(Csr *A, int64_t nnz) {
                          A->ctx->stream));
 }
 
-// build A^T with a stable counting sort by column: row c of A^T lists the
-// entries of column c in ascending original row, which is the order the
-// reference's serial scatter adds them (crs_matvec.hpp:94-97)
-static Csr *build_transpose(Csr *A) {
-  Ctx *c = A->ctx;
-  std::vector<int32_t> rp(A->rows + 1), ci(A->nnz);
-  std::vector<unsigned char> val(A->nnz * A->elem());
-  MGX_HIP(hipMemcpy(rp.data(), A->rp, rp.size() * 4, hipMemcpyDeviceToHost));
-  if (A->nnz) {
-    MGX_HIP(hipMemcpy(ci.data(), A->ci, ci.size() * 4, hipMemcpyDeviceToHost));
-    MGX_HIP(hipMemcpy(val.data(), A->val, val.size(), hipMemcpyDeviceToHost));
-  }
-  const int64_t used = rp[A->rows];
-  std::vector<int32_t> trp(A->cols + 1, 0), tci(used);
-  std::vector<unsigned char> tval(used * A->elem());
-  for (int64_t j = 0; j < used; j++) trp[ci[j] + 1]++;
-  for (int64_t k = 0; k < A->cols; k++) trp[k + 1] += trp[k];
-  std::vector<int32_t> pos(trp.begin(), trp.end() - 1);
-  const size_t es = A->elem();
-  for (int64_t i = 0; i < A->rows; i++)
-    for (int32_t j = rp[i]; j < rp[i + 1]; j++) {
-      const int32_t d = pos[ci[j]]++;
-      tci[d] = (int32_t)i;
-      std::memcpy(&tval[(size_t)d * es], &val[(size_t)j * es], es);
-    }
-  Csr *T = new mgx_csr();
-  T->ctx = c;
-  T->rows = A->cols;
-  T->cols = A->rows;
-  T->nnz = used;
-  T->dtype = A->dtype;
-  T->rp = (int32_t *)dev_alloc(trp.size() * 4);
-  alloc_entries(T, used);
-  MGX_HIP(hipMemcpy(T->rp, trp.data(), trp.size() * 4, hipMemcpyHostToDevice));
-  if (used) {
-    MGX_HIP(hipMemcpy(T->ci, tci.data(), tci.size() * 4, hipMemcpyHostToDevice));
-    MGX_HIP(hipMemcpy(T->val, tval.data(), tval.size(), hipMemcpyHostToDevice));
-  }
-  spmv_plan_build(c, T, trp.data());
-  return T;
-}
-
 void csr_free(Csr *A) {
   if (!A) return;
   if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
@@ -190,6 +148,7 @@ void csr_free(Csr *A) {
   dev_free(A->val32);
   dev_free(A->plan.d_blocks);
   spmv_panels_free(A);
+  spmv_sell_free(A);
   dev_free(A->d_chunks);
   dev_free(A->d_tile_chunk);
   if (A->trans) csr_free(A->trans);
@@ -344,7 +303,7 @@ int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_
     MGX_REQUIRE(A, "mgx_csr_set_plan: NULL matrix");
     MGX_REQUIRE(rows_per_block >= 1 && rows_per_block <= kBlock,
                 "rows_per_block must be in [1, %d]", kBlock);
-    MGX_REQUIRE(row_walk == 0 || row_walk == 1, "row_walk must be 0 or 1");
+    MGX_REQUIRE(row_walk >= 0 && row_walk <= 2, "row_walk must be 0, 1 or 2");
     // the row walk stages 12 bytes per FP64 entry in LDS: 4096 entries fit 64 KiB
     MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk ? 4096 : 8192),
                 "nnz_per_block must be in [64, %d]", row_walk ? 4096 : 8192);
@@ -357,11 +316,16 @@ int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_
     const int long_t = A->plan.long_row > 0 ? A->plan.long_row : A->ctx->spmv_long;
     MGX_REQUIRE(long_t <= nnz_per_block, "nnz_per_block below the long-row threshold %d",
                 long_t);
+    MGX_REQUIRE(row_walk != 2 || column_panels == 0,
+                "mgx_csr_set_plan: the sliced layout and column panels exclude each other");
     spmv_plan_build_with(A, rp.data(), rows_per_block, nnz_per_block, long_t, xcd_remap,
-                         row_walk);
+                         row_walk == 1);
     if (column_panels)
       MGX_REQUIRE(spmv_panels_build(A, column_panels),
                   "mgx_csr_set_plan: column panels need ascending columns within rows");
+    if (row_walk == 2)
+      MGX_REQUIRE(spmv_sell_build(A, rp.data()),
+                  "mgx_csr_set_plan: the sliced layout would pad the entries by more than 30 %%");
     return MGX_OK;
   });
 }
@@ -561,6 +525,7 @@ int mgx_csr_update_values(mgx_csr *A, const void *val) {
     if (A->nnz) MGX_HIP(hipMemcpy(A->val, val, (size_t)A->nnz * A->elem(), hipMemcpyHostToDevice));
     if (A->val32) launch_convert(A->ctx, A->dtype, MGX_F32, A->nnz, A->val, A->val32);
     spmv_panels_free(A);  // its value copy would be stale
+    spmv_sell_values(A);
     if (A->trans) {
       csr_free(A->trans);
       A->trans = nullptr;
@@ -659,7 +624,7 @@ int mgx_matvec_t(mgx_csr *A, mgx_vec *x, mgx_vec *y) {
   return guard([&] {
     matvec_checks(A, x, y, true, "mgx_matvec_t");
     activate(A->ctx);
-    if (!A->trans) A->trans = build_transpose(A);
+    if (!A->trans) A->trans = csr_transpose(A);
     launch_spmv(A->trans, A->trans->val, A->dtype, x->d, y->d, 1.0, 0.0, false);
     return MGX_OK;
   });

@@ -43,6 +43,7 @@ void ensure_val32(Csr *A) {
   A->val32 = (float *)dev_alloc((size_t)(A->nnz + 4) * 4);
   MGX_HIP(hipMemsetAsync(A->val32 + A->nnz, 0, 16, c->stream));
   launch_convert(c, MGX_F64, MGX_F32, A->nnz, A->val, A->val32);
+  spmv_sell_values(A);  // the sliced copy of the FP32 values, if A has one
 }
 
 void fused_plan_build(Csr *A, const int32_t *rp_host) {

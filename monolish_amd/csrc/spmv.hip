@@ -181,15 +181,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_panel(
   }
 }
 
-// rows longer than the long-row threshold in column-panel mode: the long
-// path of k_spmv (lane-strided sums, canonical block tree), one block per row
-template <class T>
+// rows longer than the long-row threshold in the column-panel and sliced
+// modes: the long path of k_spmv (lane-strided sums, canonical block tree),
+// one block per row
+template <class T, bool AB>
 __global__ __launch_bounds__(kBlock) void k_spmv_longrows(
     const int32_t *__restrict__ rows, int64_t nlong, const int32_t *__restrict__ rp,
     const int32_t *__restrict__ ci, const T *__restrict__ val, const T *__restrict__ x,
-    T *__restrict__ y) {
+    T *__restrict__ y, T alpha, T beta, const int32_t *live) {
   __shared__ T sh[8];
   if ((int64_t)blockIdx.x >= nlong) return;
+  if (live && *live != kStatRunning) return;
   const int32_t r0 = rows[blockIdx.x];
   const int32_t e0 = rp[r0], e1 = rp[r0 + 1];
   T s = 0;
@@ -198,7 +200,103 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longrows(
     s = s + p;
   }
   T res = block_tree(s, sh);
-  if (threadIdx.x == 0) y[r0] = res;
+  if (threadIdx.x == 0) {
+    if (AB) {
+      T t = alpha * res;
+      if (beta != T(0)) {
+        T u = beta * y[r0];
+        t = t + u;
+      }
+      y[r0] = t;
+    } else {
+      y[r0] = res;
+    }
+  }
+}
+
+// Sliced layout: rows in slices of kSlice = 64 and entries in tuples of
+// W = 16 / sizeof(value) (one 16-byte value load per lane; the FP32 copy of
+// an FP64 matrix gets a layout of its own with W = 4); slice s stores
+// entry j of its row l at soff[s] + ((j / W) * 64 + l) * W + j % W, the
+// slice as wide as its longest short row (rounded up to W).  One lane per
+// row walks its row in storage order with no LDS: a wavefront loads 1 KiB
+// of consecutive values per instruction, and on banded and stencil matrices
+// gathers x from consecutive addresses.
+constexpr int kSlice = 64;
+
+__host__ __device__ constexpr int64_t sell_index(int64_t base, int32_t j, int l, int w) {
+  return base + ((int64_t)(j / w) * kSlice + l) * w + j % w;
+}
+
+template <class T, int W>
+__global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
+                            const int32_t *__restrict__ ci, const T *__restrict__ val,
+                            int long_t, const int64_t *__restrict__ soff, int32_t *sci, T *sval) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  const int32_t ra = rp[r], len = rp[r + 1] - ra;
+  if (len > long_t) return;
+  const int64_t base = soff[r / kSlice];
+  const int l = (int)(r % kSlice);
+  for (int32_t j = 0; j < len; j++) {
+    const int64_t o = sell_index(base, j, l, W);
+    if (sci) sci[o] = ci[ra + j];
+    sval[o] = val[ra + j];
+  }
+}
+
+template <class T, bool AB, int W>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    int64_t rows, int64_t nblocks, const int32_t *__restrict__ rp,
+    const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
+    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+    int long_t, int remap, const int32_t *live) {
+  const int64_t b = xcd_block(remap);
+  if (b >= nblocks) return;
+  if (live && *live != kStatRunning) return;
+  const int64_t r = b * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  const int32_t ra = rp[r], len = rp[r + 1] - ra;
+  if (len > long_t) return;  // a long row: k_spmv_longrows
+  constexpr int QT = 8 / W;  // tuples per step: 8 entries
+  typedef T VT __attribute__((ext_vector_type(W)));
+  typedef int32_t VI __attribute__((ext_vector_type(W)));
+  const int64_t base = soff[r / kSlice] + (r % kSlice) * W;
+  const VT *v = reinterpret_cast<const VT *>(sval + base);
+  const VI *c = reinterpret_cast<const VI *>(sci + base);
+  T s = 0;
+  for (int32_t j = 0; j < len; j += QT * W) {
+    VT vv[QT];
+    T xv[QT * W];
+#pragma unroll
+    for (int q = 0; q < QT; q++)
+      if (j + q * W < len) {
+        const int64_t o = (int64_t)(j / W + q) * kSlice;
+        vv[q] = __builtin_nontemporal_load(v + o);
+        const VI cc = __builtin_nontemporal_load(c + o);
+#pragma unroll
+        for (int e = 0; e < W; e++)
+          if (j + q * W + e < len) xv[q * W + e] = x[cc[e]];
+      }
+#pragma unroll
+    for (int q = 0; q < QT; q++)
+#pragma unroll
+      for (int e = 0; e < W; e++)
+        if (j + q * W + e < len) {
+          T p = vv[q][e] * xv[q * W + e];
+          s = s + p;
+        }
+  }
+  if (AB) {
+    T t = alpha * s;
+    if (beta != T(0)) {
+      T u = beta * y[r];
+      t = t + u;
+    }
+    y[r] = t;
+  } else {
+    y[r] = s;
+  }
 }
 
 // CRS::diag: d[i] = A(i,i), last stored match wins, 0 if absent.  A block
@@ -251,6 +349,7 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
                           int remap_req, int walk) {
   Ctx *c = A->ctx;
   spmv_panels_free(A);  // built for the previous block budgets
+  spmv_sell_free(A);
   const int64_t rows = A->rows;
   const int rows_max = std::min(std::max(rows_req, 1), kBlock);
   const int nnz_max = std::max(nnz_req, 64);
@@ -301,6 +400,130 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
 
+static void sliced_free(SpmvPlan::Sliced &l) {
+  dev_free(l.d_soff);
+  dev_free(l.d_sci);
+  dev_free(l.d_val);
+  l = SpmvPlan::Sliced();
+}
+
+void spmv_sell_free(Csr *A) {
+  SpmvPlan &p = A->plan;
+  if (!p.sell) return;
+  if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
+  sliced_free(p.sl);
+  sliced_free(p.sl32);
+  p.soff32.clear();
+  dev_free(p.d_slrows);
+  p.d_slrows = nullptr;
+  p.sell = 0;
+  p.nslong = 0;
+}
+
+// slice offsets for w-entry tuples; false when the padding exceeds 30 % of
+// the short rows' entries
+static bool sliced_offsets(const int32_t *rp, int64_t rows, int long_t, int w,
+                           std::vector<int64_t> &soff, std::vector<int32_t> *lrows) {
+  const int64_t ns = (rows + kSlice - 1) / kSlice;
+  soff.assign(ns + 1, 0);
+  int64_t short_nnz = 0;
+  for (int64_t s = 0; s < ns; s++) {
+    int32_t width = 0;
+    for (int64_t r = s * kSlice; r < std::min(rows, (s + 1) * kSlice); r++) {
+      const int32_t len = rp[r + 1] - rp[r];
+      if (len > long_t) {
+        if (lrows) lrows->push_back((int32_t)r);
+        continue;
+      }
+      width = std::max(width, len);
+      short_nnz += len;
+    }
+    width = (width + w - 1) / w * w;
+    soff[s + 1] = soff[s] + (int64_t)width * kSlice;
+  }
+  return (double)soff[ns] <= 1.3 * (double)short_nnz + kSlice * w;
+}
+
+// allocates a layout from host offsets and writes its indices and values
+template <class T, int W>
+static void sliced_fill(Csr *A, SpmvPlan::Sliced &l, const std::vector<int64_t> &soff,
+                        const T *val) {
+  Ctx *c = A->ctx;
+  l.w = W;
+  l.entries = soff.back();
+  l.d_soff = (int64_t *)dev_alloc(soff.size() * 8);
+  l.d_sci = (int32_t *)dev_alloc((size_t)(l.entries + W) * 4);
+  l.d_val = dev_alloc((size_t)(l.entries + W) * sizeof(T));
+  MGX_HIP(hipMemcpyAsync(l.d_soff, soff.data(), soff.size() * 8, hipMemcpyHostToDevice,
+                         c->stream));
+  const unsigned g = (unsigned)std::max<int64_t>((A->rows + kBlock - 1) / kBlock, 1);
+  k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, A->plan.long_row,
+                                                 l.d_soff, l.d_sci, static_cast<T *>(l.d_val));
+  MGX_HIP(hipGetLastError());
+  MGX_HIP(hipStreamSynchronize(c->stream));
+}
+
+template <class T, int W>
+static void sliced_refill(Csr *A, SpmvPlan::Sliced &l, const T *val) {
+  Ctx *c = A->ctx;
+  const unsigned g = (unsigned)std::max<int64_t>((A->rows + kBlock - 1) / kBlock, 1);
+  k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, A->plan.long_row,
+                                                 l.d_soff, nullptr, static_cast<T *>(l.d_val));
+  MGX_HIP(hipGetLastError());
+  MGX_HIP(hipStreamSynchronize(c->stream));
+}
+
+// (re)writes the sliced values of A and, for an FP64 matrix with an FP32
+// copy, builds or refreshes the FP32 layout
+void spmv_sell_values(Csr *A) {
+  SpmvPlan &p = A->plan;
+  if (!p.sell) return;
+  if (A->dtype == MGX_F64) {
+    sliced_refill<double, 2>(A, p.sl, static_cast<const double *>(A->val));
+    if (A->val32 && p.sl32.d_soff)
+      sliced_refill<float, 4>(A, p.sl32, A->val32);
+    else if (A->val32 && !p.soff32.empty())
+      sliced_fill<float, 4>(A, p.sl32, p.soff32, A->val32);
+  } else {
+    sliced_refill<float, 4>(A, p.sl, static_cast<const float *>(A->val));
+  }
+}
+
+// Builds the sliced copy for the plan's long-row threshold; false (nothing
+// built) when the slices' padding would exceed 30 % of the short rows'
+// entries.  The column panels are dropped (one overlay at a time).
+bool spmv_sell_build(Csr *A, const int32_t *rp) {
+  Ctx *c = A->ctx;
+  SpmvPlan &p = A->plan;
+  spmv_sell_free(A);
+  const int64_t rows = A->rows;
+  if (rows <= 0 || A->nnz <= 0) return false;
+  if (p.long_row <= 0) p.long_row = c->spmv_long;
+  const bool f64 = A->dtype == MGX_F64;
+  std::vector<int64_t> soff;
+  std::vector<int32_t> lrows;
+  if (!sliced_offsets(rp, rows, p.long_row, f64 ? 2 : 4, soff, &lrows)) return false;
+  if (f64 && !sliced_offsets(rp, rows, p.long_row, 4, p.soff32, nullptr)) p.soff32.clear();
+  spmv_panels_free(A);
+  p.sell = 1;  // from here on spmv_sell_free releases what was allocated
+  try {
+    p.nslong = (int64_t)lrows.size();
+    if (p.nslong) {
+      p.d_slrows = (int32_t *)dev_alloc(lrows.size() * 4);
+      MGX_HIP(hipMemcpy(p.d_slrows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (f64)
+      sliced_fill<double, 2>(A, p.sl, soff, static_cast<const double *>(A->val));
+    else
+      sliced_fill<float, 4>(A, p.sl, soff, static_cast<const float *>(A->val));
+    if (f64 && A->val32 && !p.soff32.empty()) sliced_fill<float, 4>(A, p.sl32, p.soff32, A->val32);
+  } catch (...) {
+    spmv_sell_free(A);
+    throw;
+  }
+  return true;
+}
+
 void spmv_panels_free(Csr *A) {
   SpmvPlan &p = A->plan;
   if (!p.npanels) return;
@@ -322,6 +545,7 @@ bool spmv_panels_build(Csr *A, int npanels) {
   Ctx *c = A->ctx;
   SpmvPlan &p = A->plan;
   spmv_panels_free(A);
+  spmv_sell_free(A);
   const int64_t n = A->rows, cols = A->cols, nnz = A->nnz;
   if (npanels < 2 || n <= 0 || cols <= 0 || nnz <= 0) return false;
   const size_t es = A->elem();
@@ -458,8 +682,27 @@ static void spmv_panels_dispatch(Csr *A, const T *X, T *Y) {
 #undef MGX_PANEL_U
   }
   if (p.nlong > 0)
-    k_spmv_longrows<T><<<(unsigned)p.nlong, kBlock, 0, c->stream>>>(
-        p.d_lrows, p.nlong, A->rp, A->ci, static_cast<const T *>(A->val), X, Y);
+    k_spmv_longrows<T, false><<<(unsigned)p.nlong, kBlock, 0, c->stream>>>(
+        p.d_lrows, p.nlong, A->rp, A->ci, static_cast<const T *>(A->val), X, Y, T(1), T(0),
+        nullptr);
+  MGX_HIP(hipGetLastError());
+}
+
+template <class T, bool AB>
+static void spmv_sell_dispatch(Csr *A, const SpmvPlan::Sliced &l, const T *V, const T *X, T *Y,
+                               T alpha, T beta, const int32_t *live) {
+  Ctx *c = A->ctx;
+  const SpmvPlan &p = A->plan;
+  const int64_t nb = (A->rows + kBlock - 1) / kBlock;
+  const int remap = p.remap && nb >= 64;
+  const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
+  constexpr int W = sizeof(T) == 8 ? 2 : 4;
+  k_spmv_sell<T, AB, W><<<grid, kBlock, 0, c->stream>>>(
+      A->rows, nb, A->rp, l.d_soff, l.d_sci, static_cast<const T *>(l.d_val), X, Y, alpha, beta,
+      p.long_row, remap, live);
+  if (p.nslong > 0)
+    k_spmv_longrows<T, AB><<<(unsigned)p.nslong, kBlock, 0, c->stream>>>(
+        p.d_slrows, p.nslong, A->rp, A->ci, V, X, Y, alpha, beta, live);
   MGX_HIP(hipGetLastError());
 }
 
@@ -467,6 +710,18 @@ template <class T, bool AB>
 static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
                           T beta, const int32_t *live) {
   Ctx *c = A->ctx;
+  // sliced mode: A's values or their FP32 copy
+  if (A->plan.sell) {
+    const SpmvPlan::Sliced *l = nullptr;
+    if (static_cast<const void *>(V) == A->val && sizeof(T) == A->elem())
+      l = &A->plan.sl;
+    else if (sizeof(T) == 4 && static_cast<const void *>(V) == A->val32 && A->plan.sl32.d_soff)
+      l = &A->plan.sl32;
+    if (l) {
+      spmv_sell_dispatch<T, AB>(A, *l, V, X, Y, alpha, beta, live);
+      return;
+    }
+  }
   // column-panel mode: plain y = A x with A's own values, outside solver loops
   if (!AB && !live && A->plan.npanels > 0 && V == static_cast<const T *>(A->val)) {
     spmv_panels_dispatch<T>(A, X, Y);
@@ -656,16 +911,67 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
     else
       spmv_panels_free(A);
   }
+  // the sliced layout, where its padding is small
+  int best_sell = 0;
+  {
+    x = dev_alloc((size_t)std::max<int64_t>(cols, 1) * es);
+    y = dev_alloc((size_t)std::max<int64_t>(rows, 1) * es);
+    e0 = e1 = nullptr;
+    try {
+      MGX_HIP(hipMemsetAsync(x, 0, (size_t)std::max<int64_t>(cols, 1) * es, c->stream));
+      MGX_HIP(hipEventCreate(&e0));
+      MGX_HIP(hipEventCreate(&e1));
+      // keep the panels while the sliced copy is timed: it drops them
+      const int keep_panels = best_panels;
+      if (spmv_sell_build(A, rp.data())) {
+        int sell_remap = best.remap;
+        for (int rr : {best.remap, 1 - best.remap}) {
+          A->plan.remap = rr;
+          ncand++;
+          for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+          MGX_HIP(hipEventRecord(e0, c->stream));
+          for (int i = 0; i < reps; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+          MGX_HIP(hipEventRecord(e1, c->stream));
+          MGX_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+          const double us = 1e3 * ms / reps;
+          if (us < best_us) {
+            best_us = us;
+            best_sell = 1;
+            best_panels = 0;
+            sell_remap = rr;
+          }
+        }
+        A->plan.remap = best_sell ? sell_remap : best.remap;
+        if (!best_sell) {
+          spmv_sell_free(A);
+          if (keep_panels) spmv_panels_build(A, keep_panels);
+        }
+      }
+    } catch (...) {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      dev_free(x);
+      dev_free(y);
+      spmv_sell_free(A);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(x);
+    dev_free(y);
+  }
   A->plan.tuned_us = best_us;
   if (out) {
     out->rows_per_block = best.rows_max;
     out->nnz_per_block = best.nnz_max;
     out->long_row = long_t;
-    out->xcd_remap = best.remap;
+    out->xcd_remap = A->plan.remap;
     out->us = best_us;
     out->candidates = (int32_t)ncand;
     out->column_panels = best_panels;
-    out->row_walk = best.walk;
+    out->row_walk = best_sell ? 2 : best.walk;
   }
 }
 

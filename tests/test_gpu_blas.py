@@ -151,7 +151,7 @@ def test_matvec_autotune(m, ctx, oracle, dtype):
     before = Y.download()
     t = A.autotune(reps=3)
     assert t["rows_per_block"] in (64, 128, 256) and t["nnz_per_block"] in (1024, 2048, 4096)
-    assert t["candidates"] >= 26 and t["us"] > 0 and t["long_row"] == 256
+    assert t["candidates"] >= 27 and t["us"] > 0 and t["long_row"] == 256
     Y.fill(0.0)
     m.matvec(A, X, Y)
     after = Y.download()
@@ -219,14 +219,23 @@ def test_matvec_alpha_beta(m, ctx, oracle):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_matvec_transpose(m, ctx, oracle, dtype):
-    say("matvec_T", dtype)
+@pytest.mark.parametrize("layout", ["sorted", "unsorted_dup"])
+def test_matvec_transpose(m, ctx, oracle, dtype, layout):
+    """A^T is built on the device by a stable sort of the entries by column;
+    each column's products are added in ascending row and, within a row, in
+    storage order (unsorted and repeated columns included), as the CPU
+    scatter loop does."""
+    say("matvec_T", dtype, layout)
     rng = np.random.default_rng(1)
     rows, cols = 3000, 1700
     lens = rng.integers(0, 30, rows)
     rp = np.zeros(rows + 1, np.int32)
     rp[1:] = np.cumsum(lens)
-    ci = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens]).astype(np.int32)
+    if layout == "sorted":
+        ci = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens])
+    else:
+        ci = np.concatenate([rng.integers(0, cols, l) for l in lens])
+    ci = ci.astype(np.int32)
     val = rng.uniform(-1, 1, rp[-1]).astype(dtype)
     x = oracle.random_vector(rows, -1.0, 1.0, 12, dtype=dtype)
     A = m.CRS(ctx, rows, cols, rp, ci, val)
@@ -393,6 +402,78 @@ def test_matvec_plans_bit_identical(m, ctx, oracle, dtype):
         assert bits_equal(Y.download(), base), plan
     t = A.autotune(reps=2)
     assert t["candidates"] >= 26 + 3 * (dtype == np.float64)
+    # this matrix's row lengths vary too much for the sliced layout
+    with pytest.raises(m.MgxError):
+        A.set_plan(row_walk=2)
     Y.fill(0.0)
     m.matvec(A, X, Y)
     assert bits_equal(Y.download(), base), t
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_sliced_layout(m, ctx, oracle, dtype):
+    """The sliced layout (64-row slices, one lane per row) computes the bits
+    of every other plan: 27-point and 7-point stencils, a matrix with rows
+    above the long-row threshold, y = a A x + b y, after update_values, and
+    inside the mixed-precision solver (its FP32 copy of the values)."""
+    say("matvec sliced", dtype.__name__)
+    rng = np.random.default_rng(5)
+    mats = [m.stencil3d(24, 20, 17, 27, 1e-2), m.stencil3d(40, 33, 9, 7)]
+    lens = rng.integers(10, 13, 9000)  # 64-row slices padded by under 10 %
+    lens[[17, 4000, 8999]] = [600, 300, 1000]  # long rows: the block-tree path
+    rp = np.zeros(len(lens) + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    cols = 12000
+    ci = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens])
+    mats.append((rp, ci.astype(np.int32), rng.uniform(-1, 1, rp[-1])))
+    for k, (rp, ci, val) in enumerate(mats):
+        val = val.astype(dtype)
+        rows = len(rp) - 1
+        ncol = cols if k == 2 else rows
+        x = oracle.random_vector(ncol, -1.0, 1.0, 40 + k, dtype=dtype)
+        A = m.CRS(ctx, rows, ncol, rp, ci, val)
+        X = m.Vector(ctx, x, dtype=dtype)
+        Y = m.Vector(ctx, n=rows, dtype=dtype)
+        m.matvec(A, X, Y)
+        base = Y.download()
+        ref = oracle.spmv(rp, ci, val, x)
+        short = np.diff(rp) <= 256
+        assert bits_equal(base[short], ref[short])
+        y0 = oracle.random_vector(rows, -1.0, 1.0, 60 + k, dtype=dtype)
+        Y.upload(y0)
+        m.matvec_ab(0.5, A, X, -2.0, Y)
+        base_ab = Y.download()
+        A.set_plan(rows_per_block=128, nnz_per_block=1024, row_walk=2)
+        Y.fill(0.0)
+        m.matvec(A, X, Y)
+        assert bits_equal(Y.download(), base), k
+        Y.upload(y0)
+        m.matvec_ab(0.5, A, X, -2.0, Y)
+        assert bits_equal(Y.download(), base_ab), k
+        v2 = (val * 3).astype(dtype)
+        A.update_values(v2)
+        B = m.CRS(ctx, rows, ncol, rp, ci, v2)
+        Y2 = m.Vector(ctx, n=rows, dtype=dtype)
+        m.matvec(A, X, Y)
+        m.matvec(B, X, Y2)
+        assert bits_equal(Y.download(), Y2.download()), k
+        for o in (A, B, X, Y, Y2):
+            o.free()
+    if dtype == np.float64:
+        # mixed precision: FP32 iterations through the sliced FP32 values
+        rp, ci, val = m.stencil3d(32, 32, 32, 27, 1e-2)
+        n = len(rp) - 1
+        b = oracle.random_vector(n, -1.0, 1.0, 3001)
+        xs = []
+        for sliced in (False, True):
+            A = m.CRS(ctx, n, n, rp, ci, val)
+            if sliced:
+                A.set_plan(row_walk=2)
+            X = m.Vector(ctx, np.zeros(n))
+            B = m.Vector(ctx, b)
+            r = m.mixed_cg(A, X, B, tol=1e-10, delta=1e-2)
+            assert r.status == 0
+            xs.append(X.download())
+            for o in (A, X, B):
+                o.free()
+        assert bits_equal(xs[0], xs[1])
