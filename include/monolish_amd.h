@@ -66,29 +66,35 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
  * CRS SpMV to cuSPARSE): times A's SpMV under candidate plans -- rows per
  * block {64, 128, 256} x entries per block {1024, 2048, 4096} x XCD-aware
  * order on/off with products staged in LDS in entry order; rows per block
- * {128, 256} x entries {1024, 2048} x XCD order with the row walk -- then, for matrices whose x does not fit an XCD's L2 and
- * whose rows list columns in ascending order, the best of these with 4, 8
- * or 16 column panels (the short rows' entries stored a second time,
- * panel-major, so each pass gathers x from an L2-sized slice) -- `reps`
- * launches each, keeps the fastest for A and reports it (within 2 %, an
- * XCD-ordered plan).  The long-row threshold is unchanged, so results do
- * not depend on the choice.  The panel copy is dropped by
- * mgx_csr_update_values. */
+ * {128, 256} x entries {1024, 2048} x XCD order with the row walk -- then,
+ * for matrices whose x does not fit an XCD's L2 and whose rows list columns
+ * in ascending order, the best of these with 4, 8 or 16 column panels (the
+ * short rows' entries stored a second time, panel-major, so each pass
+ * gathers x from an L2-sized slice), and, where 64-row slices pad the
+ * entries by at most 30 %, the sliced layout (entries stored a second time
+ * in 64-row slices of 16-byte tuples, one lane per row, no LDS) under both
+ * XCD orders -- `reps` launches each; keeps the fastest for A and reports it
+ * (within 2 %, an XCD-ordered block plan).  The long-row threshold is
+ * unchanged, so results do not depend on the choice.  The panel copy is
+ * dropped by mgx_csr_update_values; the sliced copy is rewritten. */
 typedef struct mgx_spmv_tuning {
   int32_t rows_per_block, nnz_per_block, long_row, xcd_remap;
   double us;           /* time of one SpMV under the chosen plan */
   int32_t candidates;  /* plans timed */
   int32_t column_panels; /* 0, or the column panels of the chosen plan */
-  int32_t row_walk;    /* stream blocks: 0 products staged in LDS in entry
-                          order; 1 row walk (lane per row over LDS-staged
-                          val / col_ind; coalesced gathers on banded rows) */
+  int32_t row_walk;    /* 0 products staged in LDS in entry order; 1 row
+                          walk (lane per row over LDS-staged val / col_ind;
+                          coalesced gathers on banded rows); 2 the sliced
+                          layout */
   int32_t pad;
 } mgx_spmv_tuning;
 int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out);
 /* Sets A's SpMV plan explicitly (the candidates the autotuner times):
  * rows_per_block in [1, 256], nnz_per_block in [64, 8192] (<= 4096 with
- * row_walk), xcd_remap and row_walk 0/1, column_panels 0 or 2..64 (needs
- * ascending columns within rows: MGX_ERR_INVALID otherwise).
+ * row_walk), xcd_remap 0/1, row_walk 0/1 (block kind) or 2 (the sliced
+ * layout, over a product-staged block plan kept for other values; refused
+ * when its padding exceeds 30 %), column_panels 0 or 2..64 (needs ascending
+ * columns within rows; not with row_walk 2): MGX_ERR_INVALID otherwise.
  * The long-row threshold stays as it is. */
 int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_remap,
                      int row_walk, int column_panels);
@@ -171,6 +177,10 @@ typedef struct mgx_csr_stats {
   int32_t max_row, min_row;
   double mean_row;
   int64_t stream_blocks, long_rows; /* SpMV plan */
+  /* distinct 128-byte lines of x gathered by 64 consecutive entries
+     (sampled): above 24, fused CG forms p before its SpMV and gathers it
+     once per entry instead of gathering z and p */
+  double gather_lines;
 } mgx_csr_stats;
 int mgx_csr_stats_get(mgx_csr *A, mgx_csr_stats *out);
 

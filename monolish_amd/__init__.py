@@ -60,7 +60,8 @@ class SpmvTuning(C.Structure):
 
 class CsrStats(C.Structure):
     _fields_ = [("rows", _I64), ("cols", _I64), ("nnz", _I64), ("max_row", _I32),
-                ("min_row", _I32), ("mean_row", _D), ("stream_blocks", _I64), ("long_rows", _I64)]
+                ("min_row", _I32), ("mean_row", _D), ("stream_blocks", _I64), ("long_rows", _I64),
+                ("gather_lines", _D)]
 
 
 def _sig(name, args, res=C.c_int):

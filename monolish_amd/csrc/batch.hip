@@ -146,12 +146,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
   if (st->status != kStatRunning) return;
   if (d.split) return;  // split iteration: plan-based SpMV and k_cg_pq instead
   const bool odd = (st->iter & 1) != 0;
-  const T *p_old = odd ? d.pb : d.pa;
+  const bool once = d.once != 0;  // p already formed in pa by k_cg_p
+  const T *p_old = once ? d.pa : (odd ? d.pb : d.pa);
   T *p_new = odd ? d.pa : d.pb;
   const bool first = st->first != 0;
   const T beta = (T)st->beta;
   const T *z = d.z;
   auto pv = [&](int64_t c) -> T {
+    if (once) return p_old[c];
     if (first) return z[c];
     T bp = beta * p_old[c];
     return z[c] + bp;
@@ -168,12 +170,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
     const int64_t i0 = R0 + 2 * t;
     if (i0 < R1) {
       T p = pv(i0);
-      p_new[i0] = p;
+      if (!once) p_new[i0] = p;
       acc0 = p * qloc[2 * t];
     }
     if (i0 + 1 < R1) {
       T p = pv(i0 + 1);
-      p_new[i0 + 1] = p;
+      if (!once) p_new[i0 + 1] = p;
       acc1 = p * qloc[2 * t + 1];
     }
   }
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const bool odd = (st->iter & 1) != 0;
-  const T *p_new = odd || d.split ? d.pa : d.pb;
+  const T *p_new = odd || d.split || d.once ? d.pa : d.pb;
   const T alpha = (T)st->alpha;
   const T nalpha = -alpha;
   const int t = threadIdx.x;
@@ -469,8 +471,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_persist(const SysDesc<T> *sys, Gr
   }
 }
 
-// split iteration, stage 1: p = z in the first iteration, else
-// p = z + beta p (xpay, cg.cpp:76); stage 2 is q = A p by the plan-based SpMV
+// split iteration (and the gather-once two-kernel form), stage 1: p = z in
+// the first iteration, else p = z + beta p (xpay, cg.cpp:76), in pa; stage
+// 2 is q = A p by the plan-based SpMV (k_cg_spmv for the gather-once form)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const TileRef *tiles,
                                                  int64_t nlaunch) {
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const Ti
   if (b >= nlaunch) return;
   int64_t tile;
   const SysDesc<T> &d = locate(sys, tiles, b, tile);
-  if (!d.split) return;
+  if (!d.split && !d.once) return;
   const SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const bool first = st->first != 0;

@@ -181,6 +181,7 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   d.n = n;
   d.ntiles = nt;
   d.split = cfg.method == kMethodCG && n >= kSplitRows ? 1 : 0;
+  d.once = cfg.method == kMethodCG && !d.split && A->gather_lines > kScatteredLines ? 1 : 0;
   d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
   d.method = cfg.method;
   d.tol_scale = cfg.tol_relative ? cfg.tol : 0.0;
@@ -256,6 +257,7 @@ template <class T> void Engine<T>::flush_pending() {
 template <class T> void Engine<T>::rebuild_lists() {
   std::vector<TileRef> lst[2];
   any_fused_ = false;
+  any_once_ = false;
   split_slots_.clear();
   for (int s : active_) {
     if (reported_[s]) continue;
@@ -265,6 +267,7 @@ template <class T> void Engine<T>::rebuild_lists() {
         split_slots_.push_back(s);
       else
         any_fused_ = true;
+      any_once_ = any_once_ || h_sys_[s].once;
     }
     for (int64_t k = 0; k < h_sys_[s].ntiles; k++) lst[m].push_back(TileRef{s, (int32_t)k});
   }
@@ -298,9 +301,11 @@ template <class T> void Engine<T>::rebuild_lists() {
 template <class T> void Engine<T>::launch_iters(int iters) {
   for (int i = 0; i < iters; i++) {
     if (ntiles_[0]) {
-      if (!split_slots_.empty()) {
-        // split form: p, then q = A p per system, then <p, q>
+      // p = z + beta p of the split and gather-once systems
+      if (!split_slots_.empty() || any_once_)
         batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
+      if (!split_slots_.empty()) {
+        // split form: q = A p per system, then <p, q>
         for (int s : split_slots_)
           launch_spmv(csr_[s], val_[s], DTof<T>::v, h_sys_[s].pa, h_sys_[s].q, 1.0, 0.0, false,
                       &d_states_[s].status);
@@ -357,7 +362,7 @@ template <class T> void Engine<T>::step(int iters) {
 template <class T> bool Engine<T>::persistent_ok(int slot) const {
   if (!c_->use_persistent || active_.size() != 1 || active_[0] != slot) return false;
   const SysDesc<T> &d = h_sys_[slot];
-  return method_[slot] == kMethodCG && !d.split;
+  return method_[slot] == kMethodCG && !d.split && !d.once;
 }
 
 template <class T> void Engine<T>::run_persistent(int slot) {

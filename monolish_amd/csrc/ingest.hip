@@ -480,6 +480,12 @@ Csr *csr_from_host_coo(Ctx *c, int64_t rows, int64_t cols, int64_t nnz, const in
     }
     spmv_plan_build(c, A, rp.data());
     if (rows == cols && rows > 0) fused_plan_build(A, rp.data());
+    if (m >= 64) {  // a contiguous sample of the columns from the middle
+      const int64_t k = std::min<int64_t>(m, 4096), e0 = (m - k) / 2;
+      std::vector<int32_t> smp(k);
+      MGX_HIP(hipMemcpy(smp.data(), A->ci + e0, (size_t)k * 4, hipMemcpyDeviceToHost));
+      A->gather_lines = sample_gather_lines(smp.data(), k, sizeof(T));
+    }
   } catch (...) {
     dev_free(d_row);
     dev_free(d_col);

@@ -33,7 +33,9 @@ template <class T> struct SysDesc {
   int32_t split;
   int32_t vec;  // row0 even and x, b 16-byte aligned: vector loads allowed
   int32_t method;  // kMethodCG or kMethodBiCGSTAB
-  int32_t pad_;
+  // CG, two-kernel form: 1 = scattered columns: p = z + beta p is formed by
+  // k_cg_p first and the SpMV gathers p once per entry (instead of z and p)
+  int32_t once;
   double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
 };
 
@@ -47,6 +49,9 @@ enum { kMethodCG = 0, kMethodBiCGSTAB = 1 };
 // overhead is small against their SpMV, and the plan-based SpMV streams
 // better than the fused quarter blocks, which gather two vectors
 constexpr int64_t kSplitRows = 1 << 19;
+// matrices whose 64 consecutive entries gather from more than this many
+// distinct 128-byte lines of x (sampled at upload) use the gather-once form
+constexpr double kScatteredLines = 24.0;
 
 // one CG iteration of the listed systems: the fused SpMV stage (when a
 // listed system uses the two-kernel form) and the update stage; systems of
@@ -192,6 +197,7 @@ template <class T> class Engine {
   TileRef *d_ptiles_ = nullptr;   // tiles of the pending systems
   int64_t ptiles_cap_ = 0;
   bool any_fused_ = false;        // a listed CG system uses the two-kernel form
+  bool any_once_ = false;         // ... in its gather-once variant
   std::vector<int> split_slots_;  // listed CG systems of the split form
   int32_t *d_split_slots_ = nullptr;
   int64_t split_cap_ = 0;

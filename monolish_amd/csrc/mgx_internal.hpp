@@ -152,6 +152,9 @@ struct Csr {
   int32_t fused_nnz = 0;     // entry budget of a chunk
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
   Csr *trans = nullptr;      // A^T, built on first matvec_T
+  // distinct 128-byte lines of x gathered by 64 consecutive entries
+  // (sampled average; 0 when not measured): high for scattered columns
+  double gather_lines = 0;
   size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
 };
 
@@ -252,6 +255,9 @@ void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);
 void fused_plan_build(Csr *A, const int32_t *rp_host = nullptr);
 // col_ind / val of nnz entries plus 4 zeroed padding entries
 void alloc_entries(Csr *A, int64_t nnz);
+// average distinct 128-byte x lines per 64 consecutive entries of col_ind
+// (host array), over up to 64 windows spread across it
+double sample_gather_lines(const int32_t *ci, int64_t nnz, size_t es);
 void csr_free(Csr *A);
 // A^T built on the device (stable sort of the entries by column)
 Csr *csr_transpose(Csr *A);

@@ -502,6 +502,7 @@ int mgx_csr_create(mgx_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
         MGX_HIP(hipMemcpy(A->val, val, (size_t)nnz * A->elem(), hipMemcpyHostToDevice));
       }
       if (raw_differs) A->rp_raw.assign(row_ptr, row_ptr + rows + 1);
+      A->gather_lines = sample_gather_lines(col_ind, used, A->elem());
       spmv_plan_build(ctx, A, rp.data());
       if (rows == cols) fused_plan_build(A, rp.data());
     } catch (...) {
@@ -587,6 +588,7 @@ int mgx_csr_stats_get(mgx_csr *A, mgx_csr_stats *s) {
     s->mean_row = A->plan.mean_row;
     s->stream_blocks = A->plan.nblocks - A->plan.nlong;
     s->long_rows = A->plan.nlong;
+    s->gather_lines = A->gather_lines;
     return MGX_OK;
   });
 }

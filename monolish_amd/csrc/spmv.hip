@@ -407,6 +407,21 @@ static void sliced_free(SpmvPlan::Sliced &l) {
   l = SpmvPlan::Sliced();
 }
 
+double sample_gather_lines(const int32_t *ci, int64_t nnz, size_t es) {
+  if (nnz < 64) return 0;
+  const int64_t per_line = (int64_t)(128 / es);
+  const int64_t nw = std::min<int64_t>(64, nnz / 64);
+  double total = 0;
+  for (int64_t w = 0; w < nw; w++) {
+    const int64_t e0 = (nnz - 64) * w / std::max<int64_t>(nw - 1, 1);
+    int64_t lines[64];
+    for (int k = 0; k < 64; k++) lines[k] = ci[e0 + k] / per_line;
+    std::sort(lines, lines + 64);
+    total += (double)(std::unique(lines, lines + 64) - lines);
+  }
+  return total / (double)nw;
+}
+
 void spmv_sell_free(Csr *A) {
   SpmvPlan &p = A->plan;
   if (!p.sell) return;

@@ -104,8 +104,14 @@ def test_bicgstab_fused_breakdown_and_maxiter(m, ctx, oracle):
 
 
 def test_batch_identical_to_alone_and_oracle(m, ctx, oracle):
+    """One batch mixes scattered M-matrices (p formed first, gathered once
+    per entry) and stencils (z and p gathered, p formed at each gather);
+    every system reproduces the oracle's CG / BiCGSTAB bit for bit."""
     say("batch vs alone")
     syss = systems(m, oracle)
+    scattered = [m.CRS(ctx, len(b), len(b), rp, ci, v).stats()["gather_lines"] > 24
+                 for _, rp, ci, v, b in syss]
+    assert scattered == [True] * 4 + [False] * 5
     for method in ("cg", "bicgstab"):
         group = [s for s in syss if s[0] == method]
         As, Xs, Bs = [], [], []
