@@ -575,7 +575,9 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
     T t2 = beta * p1;
     return r[c] + t2;
   };
+  const bool once = d.once != 0;  // p and phat already formed by k_bcg_p
   auto phv = [&](int64_t c) -> T {
+    if (once) return d.phat[c];
     T p = pval(c);
     return M ? M[c] * p : p;
   };
@@ -591,9 +593,11 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
     for (int e = 0; e < 2; e++) {
       const int64_t i = i0 + e;
       if (i < R1) {
-        T p = pval(i);
-        p_new[i] = p;
-        d.phat[i] = M ? M[i] * p : p;
+        if (!once) {
+          T p = pval(i);
+          p_new[i] = p;
+          d.phat[i] = M ? M[i] * p : p;
+        }
         acc[e] = qloc[2 * t + e] * d.r0[i];
       }
     }
@@ -638,7 +642,9 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
     T t1 = nalpha * v_new[c];
     return t1 + r[c];
   };
+  const bool once = d.once != 0;  // s and shat already formed by k_bcg_s
   auto shv = [&](int64_t c) -> T {
+    if (once) return d.shat[c];
     T s = sval(c);
     return M ? M[c] * s : s;
   };
@@ -654,9 +660,14 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
     for (int e = 0; e < 2; e++) {
       const int64_t i = i0 + e;
       if (i < R1) {
-        T sv = sval(i);
-        d.s[i] = sv;
-        d.shat[i] = M ? M[i] * sv : sv;
+        T sv;
+        if (once) {
+          sv = d.s[i];
+        } else {
+          sv = sval(i);
+          d.s[i] = sv;
+          d.shat[i] = M ? M[i] * sv : sv;
+        }
         T tv = qloc[2 * t + e];
         ts2[e] = tv * sv;
         tt2[e] = tv * tv;
@@ -684,6 +695,64 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
         if (stopped) atomicAdd(stopped, 1);
       }
     }
+  }
+}
+
+// gather-once BiCGSTAB, before K1: p = r + beta (p - omega v) (p = r in the
+// first iteration) and phat = M p over the system's rows, the arithmetic of
+// K1's gathers (bicgstab.cpp:66-77)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bcg_p(const SysDesc<T> *sys, const TileRef *tiles,
+                                                  int64_t nlaunch) {
+  const int64_t b = blockIdx.x;
+  if (b >= nlaunch) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (!d.once) return;
+  const SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  const bool odd = (st->iter & 1) != 0;
+  const T *p_old = odd ? d.pb : d.pa;
+  T *p_new = odd ? d.pa : d.pb;
+  const T *v_old = odd ? d.vb : d.va;
+  const bool first = st->first != 0;
+  const T beta = (T)st->beta;
+  const T nomega = -(T)st->omega;
+  const int64_t R1 = min(d.n, (tile + 1) * kTile);
+  for (int64_t i = tile * kTile + threadIdx.x; i < R1; i += kBlock) {
+    T p = d.r[i];
+    if (!first) {
+      T t1 = nomega * v_old[i];
+      T p1 = t1 + p_old[i];
+      T t2 = beta * p1;
+      p = d.r[i] + t2;
+    }
+    p_new[i] = p;
+    d.phat[i] = d.M ? d.M[i] * p : p;
+  }
+}
+
+// gather-once BiCGSTAB, between K1 and K2: s = r - alpha v and shat = M s
+// (bicgstab.cpp:83-86)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bcg_s(const SysDesc<T> *sys, const TileRef *tiles,
+                                                  int64_t nlaunch) {
+  const int64_t b = blockIdx.x;
+  if (b >= nlaunch) return;
+  int64_t tile;
+  const SysDesc<T> &d = locate(sys, tiles, b, tile);
+  if (!d.once) return;
+  const SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  const bool odd = (st->iter & 1) != 0;
+  const T *v_new = odd ? d.va : d.vb;
+  const T nalpha = -(T)st->alpha;
+  const int64_t R1 = min(d.n, (tile + 1) * kTile);
+  for (int64_t i = tile * kTile + threadIdx.x; i < R1; i += kBlock) {
+    T t1 = nalpha * v_new[i];
+    T s = t1 + d.r[i];
+    d.s[i] = s;
+    d.shat[i] = d.M ? d.M[i] * s : s;
   }
 }
 
@@ -1044,13 +1113,16 @@ void batch_launch_cg_finish(Ctx *c, const SysDesc<T> *sys, const int32_t *slots,
 
 template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
-                                int64_t nlaunch, int32_t *stopped, int lds_entries) {
+                                int64_t nlaunch, int32_t *stopped, int lds_entries,
+                                bool any_once) {
   if (nlaunch <= 0) return;
   const size_t lds = (size_t)std::max(lds_entries, 2 * kBlock) * sizeof(T);
   const int64_t nq = 4 * nlaunch;
   const int remap = nq >= 64;
   const unsigned gq = (unsigned)(remap ? (nq + 7) / 8 * 8 : nq);
+  if (any_once) k_bcg_p<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch);
   k_bcg_spmv1<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap);
+  if (any_once) k_bcg_s<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch);
   k_bcg_spmv2<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap, stopped);
   k_bcg_update<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
 }
@@ -1075,8 +1147,8 @@ template void batch_launch_cg_finish<double>(Ctx *, const SysDesc<double> *, con
 template void batch_launch_cg_finish<float>(Ctx *, const SysDesc<float> *, const int32_t *, int,
                                             int, int32_t *);
 template void batch_launch_bicgstab_iter<double>(Ctx *, const SysDesc<double> *,
-                                                 const TileRef *, int64_t, int32_t *, int);
+                                                 const TileRef *, int64_t, int32_t *, int, bool);
 template void batch_launch_bicgstab_iter<float>(Ctx *, const SysDesc<float> *, const TileRef *,
-                                                int64_t, int32_t *, int);
+                                                int64_t, int32_t *, int, bool);
 
 }  // namespace mgx

@@ -93,6 +93,7 @@ void batch_ir_cg(Ctx *c, int64_t nsys, Csr *const *A, double *const *x, const do
     icfg.precond = p->precond;
     icfg.tol = inner_tol;  // absolute on the unit-norm scaled residual
     icfg.tol_relative = 0;
+    icfg.gather_once = 1;  // every correction of the batch in the same launches
     IrSys *d_ir = (IrSys *)dev_alloc(sizeof(IrSys) * std::max<int64_t>(nsys, 1));
     TileRef *d_tl = nullptr;
     int64_t tl_cap = 0;

@@ -181,7 +181,10 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   d.n = n;
   d.ntiles = nt;
   d.split = cfg.method == kMethodCG && n >= kSplitRows ? 1 : 0;
-  d.once = cfg.method == kMethodCG && !d.split && A->gather_lines > kScatteredLines ? 1 : 0;
+  d.once = cfg.method == kMethodBiCGSTAB ||
+                   (!d.split && (cfg.gather_once || A->gather_lines > kScatteredLines))
+               ? 1
+               : 0;
   d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
   d.method = cfg.method;
   d.tol_scale = cfg.tol_relative ? cfg.tol : 0.0;
@@ -258,6 +261,7 @@ template <class T> void Engine<T>::rebuild_lists() {
   std::vector<TileRef> lst[2];
   any_fused_ = false;
   any_once_ = false;
+  any_bcg_once_ = false;
   split_slots_.clear();
   for (int s : active_) {
     if (reported_[s]) continue;
@@ -268,6 +272,8 @@ template <class T> void Engine<T>::rebuild_lists() {
       else
         any_fused_ = true;
       any_once_ = any_once_ || h_sys_[s].once;
+    } else {
+      any_bcg_once_ = any_bcg_once_ || h_sys_[s].once;
     }
     for (int64_t k = 0; k < h_sys_[s].ntiles; k++) lst[m].push_back(TileRef{s, (int32_t)k});
   }
@@ -321,7 +327,7 @@ template <class T> void Engine<T>::launch_iters(int iters) {
     }
     if (ntiles_[1])
       batch_launch_bicgstab_iter<T>(c_, d_sys_, d_tiles_[1], ntiles_[1], d_stopped_,
-                                    lds_entries_);
+                                    lds_entries_, any_bcg_once_);
   }
 }
 

@@ -33,8 +33,9 @@ template <class T> struct SysDesc {
   int32_t split;
   int32_t vec;  // row0 even and x, b 16-byte aligned: vector loads allowed
   int32_t method;  // kMethodCG or kMethodBiCGSTAB
-  // CG, two-kernel form: 1 = scattered columns: p = z + beta p is formed by
-  // k_cg_p first and the SpMV gathers p once per entry (instead of z and p)
+  // 1 = gather-once form: CG (two-kernel form) forms p = z + beta p in k_cg_p
+  // before its SpMV, BiCGSTAB forms p, phat and s, shat in k_bcg_p / k_bcg_s,
+  // so each SpMV gathers one vector per entry (instead of 2 or 3-4)
   int32_t once;
   double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
 };
@@ -70,9 +71,12 @@ void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int
 template <class T>
 void batch_launch_cg_finish(Ctx *c, const SysDesc<T> *sys, const int32_t *slots, int nslots,
                             int stage, int32_t *stopped);
+// one BiCGSTAB iteration; any_once: a listed system uses the gather-once
+// form (p, phat and s, shat formed by streaming passes before the SpMVs)
 template <class T>
 void batch_launch_bicgstab_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
-                                int64_t nlaunch, int32_t *stopped, int lds_entries);
+                                int64_t nlaunch, int32_t *stopped, int lds_entries,
+                                bool any_once);
 
 // grid barrier state of the persistent kernels; zero before each launch
 struct GridSync {
@@ -125,6 +129,10 @@ struct SolveCfg {
   int64_t miniter = 0, maxiter = INT64_MAX / 4;
   double *rhist = nullptr;  // device array or nullptr
   int64_t rhist_len = 0;
+  // CG: gather-once form for any matrix (multi-system launches, where the
+  // extra p pass costs less than the second gather); otherwise only for
+  // matrices with scattered columns.  BiCGSTAB always uses it.
+  int gather_once = 0;
 };
 
 // A set of systems iterated together.  add() uploads nothing: the matrix and
@@ -198,6 +206,7 @@ template <class T> class Engine {
   int64_t ptiles_cap_ = 0;
   bool any_fused_ = false;        // a listed CG system uses the two-kernel form
   bool any_once_ = false;         // ... in its gather-once variant
+  bool any_bcg_once_ = false;     // a listed BiCGSTAB system in the gather-once form
   std::vector<int> split_slots_;  // listed CG systems of the split form
   int32_t *d_split_slots_ = nullptr;
   int64_t split_cap_ = 0;

@@ -29,6 +29,7 @@ SolveCfg cfg_of(const mgx_solve_params *p, int method) {
   cfg.tol_relative = p->tol_is_relative;
   cfg.miniter = p->miniter;
   cfg.maxiter = p->maxiter <= 0 ? INT64_MAX / 4 : p->maxiter;
+  cfg.gather_once = 1;  // batches and the server run many systems per launch
   return cfg;
 }
 
