@@ -222,6 +222,11 @@ int mgx_vreciprocal(mgx_vec *a, mgx_vec *y);
 #define MGX_PRECOND_DEFAULT 0 /* no preconditioner bound: z = r */
 #define MGX_PRECOND_NONE 1    /* equation::none (src/equation/none.cpp) */
 #define MGX_PRECOND_JACOBI 2  /* equation::Jacobi (src/equation/jacobi.cpp) */
+#define MGX_PRECOND_SOR 3     /* equation::SOR (src/equation/sor.cpp): forward and
+                                 backward sweeps with 1/(omega diag(A)), run as
+                                 level-scheduled kernels with the serial loop's
+                                 bits; solvers using it take the reference call
+                                 sequence in either mode */
 
 /* how the iteration is executed; every mode follows the reference's
  * arithmetic, so FP64 results of REFERENCE and FUSED are bit-identical */
@@ -238,6 +243,7 @@ typedef struct mgx_solve_params {
   int32_t check_every; /* FUSED: iterations per replayed graph (0: auto) */
   double *rhistory;    /* optional host array [rhistory_len] of residuals */
   int64_t rhistory_len;
+  double omega;        /* solver::set_omega: SOR relaxation (default 1.9) */
 } mgx_solve_params;
 
 typedef struct mgx_solve_result {
@@ -261,6 +267,13 @@ int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
 /* equation::Jacobi<CRS<T>,T>::solve (src/equation/jacobi.cpp:83-183) */
 int mgx_jacobi_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                      const mgx_solve_params *p, mgx_solve_result *res);
+/* equation::SOR<CRS<T>,T>::solve (src/equation/sor.cpp:91-164): x += (D/omega
+ * + L)^-1 (b - A M x) with the bound preconditioner M, relative stopping
+ * test ||b - A M x|| / ||b|| < tol; the lower sweep (sor_kernel_lower,
+ * src/equation/kernel/sor_kernel.hpp:5-22) runs level by level on the device
+ * with the serial loop's bits */
+int mgx_sor_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                  const mgx_solve_params *p, mgx_solve_result *res);
 /* util::get_residual_l2 (src/utils/equation_utils.cpp:45-56): ||b - A x|| */
 int mgx_residual_l2(mgx_csr *A, mgx_vec *x, mgx_vec *b, double *out);
 

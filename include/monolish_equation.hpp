@@ -1,6 +1,7 @@
 // monolish_equation.hpp -- drop-in monolish::equation solvers for
-// matrix::CRS<T> on the device: none, Jacobi (solver and preconditioner), CG,
-// BiCGSTAB (include/monolish_equation.hpp:40-257 of monolish 0.17.3).
+// matrix::CRS<T> on the device: none, Jacobi and SOR (solvers and
+// preconditioners), CG, BiCGSTAB (include/monolish_equation.hpp:40-257 of
+// monolish 0.17.3).
 //
 // With none / Jacobi / no preconditioner bound, CG and BiCGSTAB run the fused
 // device loop (mgx_cg_solve / mgx_bicgstab_solve); any other preconditioner
@@ -79,6 +80,41 @@ class Jacobi : public monolish::solver::solver<MATRIX, Float> {
   }
   [[nodiscard]] std::string name() const { return "monolish::equation::Jacobi"; }
   [[nodiscard]] std::string solver_name() const { return "Jacobi"; }
+};
+
+// -------------------------------------------------------------------- SOR
+// As a preconditioner (sor.cpp:12-82) it is bound to CG / BiCGSTAB /
+// Jacobi / SOR solvers, which then run the device sweeps (the forward and
+// backward loops of sor_kernel_precond with M = 1/(omega diag(A))); as a
+// solver (sor.cpp:91-164) it iterates x += (D/omega + L)^-1 (b - A M x) on
+// the device with its own omega (a bound SOR preconditioner uses it too).
+template <typename MATRIX, typename Float>
+class SOR : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  void create_precond(MATRIX &A) {
+    if (A.get_row() != A.get_col()) throw std::runtime_error("error A.row != A.col");
+    this->precond.A = &A;
+  }
+  void apply_precond(const vector<Float> &r, vector<Float> &z) {
+    (void)r, (void)z;
+    throw std::runtime_error(
+        "equation::SOR::apply_precond: bind the SOR object to a solver (the sweeps run there)");
+  }
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    Logger::get_instance().solver_in(monolish_func);
+    int ret = 0;
+    if (this->lib == 0) {
+      detail_eq::check_system(A, x, b, "equation::SOR");
+      int kind = this->device_precond();
+      if (kind >= 0) this->precond.create_precond(A);
+      ret = this->run_device(mgx_sor_solve, A, x, b, kind >= 0 ? kind : MGX_PRECOND_DEFAULT,
+                             MGX_MODE_REFERENCE);
+    }
+    Logger::get_instance().solver_out();
+    return ret;
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::SOR"; }
+  [[nodiscard]] std::string solver_name() const { return "SOR"; }
 };
 
 // --------------------------------------------------------------------- CG

@@ -10,6 +10,7 @@ namespace monolish {
 namespace equation {
 template <typename MATRIX, typename Float> class none;
 template <typename MATRIX, typename Float> class Jacobi;
+template <typename MATRIX, typename Float> class SOR;
 }  // namespace equation
 
 namespace solver {
@@ -26,6 +27,14 @@ template <class M, class F> struct precond_kind<equation::none<M, F>> {
 template <class M, class F> struct precond_kind<equation::Jacobi<M, F>> {
   static constexpr int value = MGX_PRECOND_JACOBI;
 };
+template <class M, class F> struct precond_kind<equation::SOR<M, F>> {
+  static constexpr int value = MGX_PRECOND_SOR;
+};
+// the relaxation coefficient of a bound SOR object (read at solve time)
+template <class P> std::function<double()> omega_of(P &) { return nullptr; }
+template <class M, class F> std::function<double()> omega_of(equation::SOR<M, F> &p) {
+  return [&p] { return (double)p.get_omega(); };
+}
 }  // namespace detail_solver
 
 template <typename MATRIX, typename Float> class precondition {
@@ -36,6 +45,7 @@ template <typename MATRIX, typename Float> class precondition {
   std::function<void(const vector<Float> &r, vector<Float> &z)> apply_precond;
   // MGX_PRECOND_* of the bound create/apply objects (-1: custom callbacks)
   int create_kind = MGX_PRECOND_DEFAULT, apply_kind = MGX_PRECOND_DEFAULT;
+  std::function<double()> omega;  // a bound SOR object's omega
 
   std::function<void(void)> get_precond();
   void set_precond_data(vector<Float> &m) { M = m; }
@@ -92,6 +102,9 @@ template <typename MATRIX, typename Float> class solver {
     p.miniter = (int64_t)miniter;
     p.precond = kind;
     p.mode = mode;
+    // the SOR solver uses its own omega (also for a bound SOR preconditioner)
+    p.omega = fn != mgx_sor_solve && kind == MGX_PRECOND_SOR && precond.omega ? precond.omega()
+                                                                             : (double)omega;
     std::vector<double> hist;
     if (print_rhistory) {
       hist.assign(std::min<size_t>(maxiter, (size_t)1 << 22), 0.0);
@@ -121,6 +134,7 @@ template <typename MATRIX, typename Float> class solver {
   template <class PRECOND> void set_create_precond(PRECOND &p) {
     precond.create_precond = std::bind(&PRECOND::create_precond, &p, std::placeholders::_1);
     precond.create_kind = detail_solver::precond_kind<PRECOND>::value;
+    precond.omega = detail_solver::omega_of(p);
   }
   template <class PRECOND> void set_apply_precond(PRECOND &p) {
     precond.apply_precond = std::bind(&PRECOND::apply_precond, &p, std::placeholders::_1,

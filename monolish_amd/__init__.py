@@ -23,7 +23,7 @@ OK = 0
 SOLVER_SIZE_ERROR, SOLVER_MAXITER, SOLVER_BREAKDOWN, SOLVER_RESIDUAL_NAN, SOLVER_NOT_IMPL = -1, -2, -3, -4, -10
 ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND = -101, -102, -103, -104, -105
 F64, F32 = 0, 1
-PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI = 0, 1, 2
+PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
 MODE_REFERENCE, MODE_FUSED = 0, 1
 
 
@@ -44,7 +44,7 @@ _P, _I64, _I32, _D = C.c_void_p, C.c_int64, C.c_int32, C.c_double
 class SolveParams(C.Structure):
     _fields_ = [("tol", _D), ("maxiter", _I64), ("miniter", _I64), ("precond", _I32),
                 ("mode", _I32), ("tol_is_relative", _I32), ("check_every", _I32),
-                ("rhistory", C.POINTER(_D)), ("rhistory_len", _I64)]
+                ("rhistory", C.POINTER(_D)), ("rhistory_len", _I64), ("omega", _D)]
 
 
 class SolveResultC(C.Structure):
@@ -113,7 +113,7 @@ for _n in ("mgx_vadd", "mgx_vsub", "mgx_vmul"):
     _sig(_n, [_P, _P, _P])
 _sig("mgx_vreciprocal", [_P, _P])
 _sig("mgx_solve_params_default", [C.POINTER(SolveParams)], None)
-for _n in ("mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve"):
+for _n in ("mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve", "mgx_sor_solve"):
     _sig(_n, [_P, _P, _P, C.POINTER(SolveParams), C.POINTER(SolveResultC)])
 _sig("mgx_residual_l2", [_P, _P, _P, C.POINTER(_D)])
 _sig("mgx_ir_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64, C.POINTER(SolveResultC)])
@@ -486,9 +486,11 @@ class SolveResult:
     rhistory: np.ndarray | None = None
 
 
-def _params(tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, history):
+def _params(tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, history,
+            omega=1.9):
     p = SolveParams()
     _lib.mgx_solve_params_default(C.byref(p))
+    p.omega = float(omega)
     p.tol = tol
     p.maxiter = int(maxiter)
     p.miniter = int(miniter)
@@ -504,8 +506,10 @@ def _params(tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, 
     return p, hist
 
 
-def _solve(fn, A, x, b, tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, history):
-    p, hist = _params(tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, history)
+def _solve(fn, A, x, b, tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, history,
+           omega=1.9):
+    p, hist = _params(tol, maxiter, miniter, precond, mode, tol_is_relative, check_every, history,
+                      omega)
     r = SolveResultC()
     rc = check(fn(A.handle, x.handle, b.handle, C.byref(p), C.byref(r)))
     if hist is not None:
@@ -514,24 +518,31 @@ def _solve(fn, A, x, b, tol, maxiter, miniter, precond, mode, tol_is_relative, c
 
 
 def cg(A, x, b, tol=1e-8, maxiter=0, miniter=0, precond=PRECOND_JACOBI, mode=MODE_FUSED,
-       tol_is_relative=False, check_every=0, history=0) -> SolveResult:
-    """equation::CG<CRS<T>,T>::solve (src/equation/cg.cpp)."""
+       tol_is_relative=False, check_every=0, history=0, omega=1.9) -> SolveResult:
+    """equation::CG<CRS<T>,T>::solve (src/equation/cg.cpp); omega: SOR precond."""
     return _solve(_lib.mgx_cg_solve, A, x, b, tol, maxiter, miniter, precond, mode,
-                  tol_is_relative, check_every, history)
+                  tol_is_relative, check_every, history, omega)
 
 
 def bicgstab(A, x, b, tol=1e-8, maxiter=0, miniter=0, precond=PRECOND_JACOBI, mode=MODE_REFERENCE,
-             tol_is_relative=False, check_every=0, history=0) -> SolveResult:
+             tol_is_relative=False, check_every=0, history=0, omega=1.9) -> SolveResult:
     """equation::BiCGSTAB<CRS<T>,T>::solve (src/equation/bicgstab.cpp)."""
     return _solve(_lib.mgx_bicgstab_solve, A, x, b, tol, maxiter, miniter, precond, mode,
-                  tol_is_relative, check_every, history)
+                  tol_is_relative, check_every, history, omega)
 
 
 def jacobi(A, x, b, tol=1e-8, maxiter=0, miniter=0, precond=PRECOND_DEFAULT, mode=MODE_REFERENCE,
-           check_every=0, history=0) -> SolveResult:
+           check_every=0, history=0, omega=1.9) -> SolveResult:
     """equation::Jacobi<CRS<T>,T>::solve (src/equation/jacobi.cpp); tol is relative."""
     return _solve(_lib.mgx_jacobi_solve, A, x, b, tol, maxiter, miniter, precond, mode, False,
-                  check_every, history)
+                  check_every, history, omega)
+
+
+def sor(A, x, b, tol=1e-8, maxiter=0, miniter=0, precond=PRECOND_DEFAULT, omega=1.9,
+        history=0) -> SolveResult:
+    """equation::SOR<CRS<T>,T>::solve (src/equation/sor.cpp); tol is relative."""
+    return _solve(_lib.mgx_sor_solve, A, x, b, tol, maxiter, miniter, precond, MODE_REFERENCE,
+                  False, 0, history, omega)
 
 
 def ir_cg(A, x, b, tol=1e-10, tol_is_relative=True, inner_tol=1e-4, max_outer=100, maxiter=0,

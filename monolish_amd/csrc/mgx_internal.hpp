@@ -93,6 +93,13 @@ struct Vec {
   size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
 };
 
+// SOR sweep schedule: rows grouped by level, level l = rows[ptr[l], ptr[l+1])
+struct SorLevels {
+  int32_t nlev = 0, max_width = 0;
+  int32_t *d_rows = nullptr, *d_ptr = nullptr;
+  std::vector<int32_t> ptr;
+};
+
 struct SpmvPlan {
   int2 *d_blocks = nullptr;  // nblocks+1 descriptors (first row | long flag, first entry)
   int64_t nblocks = 0;
@@ -152,6 +159,7 @@ struct Csr {
   int32_t fused_nnz = 0;     // entry budget of a chunk
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
   Csr *trans = nullptr;      // A^T, built on first matvec_T
+  SorLevels sor[3];          // SOR sweeps: forward, forward (solver), backward
   // distinct 128-byte lines of x gathered by 64 consecutive entries
   // (sampled average; 0 when not measured): high for scattered columns
   double gather_lines = 0;
@@ -261,6 +269,12 @@ double sample_gather_lines(const int32_t *ci, int64_t nnz, size_t es);
 void csr_free(Csr *A);
 // A^T built on the device (stable sort of the entries by column)
 Csr *csr_transpose(Csr *A);
+// one SOR sweep over A (kind 0: forward of the preconditioner, 1: forward
+// of the SOR solver, 2: backward of the preconditioner), levels built on
+// first use; z written in place (r unused by kind 2)
+void sor_sweep(Csr *A, const void *val, mgx_dtype dt, int kind, const void *D, const void *r,
+               void *z);
+void sor_levels_free(Csr *A);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);
 

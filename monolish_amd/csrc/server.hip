@@ -282,6 +282,8 @@ int mgx_batch_solve(mgx_ctx *ctx, int method, int64_t nsys, mgx_csr *const *A,
     MGX_REQUIRE(method == MGX_METHOD_CG || method == MGX_METHOD_BICGSTAB,
                 "mgx_batch_solve: unknown method %d", method);
     MGX_REQUIRE(nsys >= 0 && nsys < (1 << 30), "mgx_batch_solve: bad system count");
+    MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
+                "mgx_batch_solve: preconditioner must be default, none or Jacobi");
     if (nsys == 0) return MGX_OK;
     const mgx_dtype dt = A[0]->dtype;
     for (int64_t i = 0; i < nsys; i++) {
@@ -333,7 +335,7 @@ int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz, const i
                 "mgx_server_submit: unknown method %d", method);
     MGX_REQUIRE(n > 0 && nnz >= 0, "mgx_server_submit: empty system");
     MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
-                "mgx_server_submit: unknown preconditioner");
+                "mgx_server_submit: preconditioner must be default, none or Jacobi");
     auto r = std::make_shared<Request>();
     r->method = method;
     r->n = n;
@@ -372,7 +374,7 @@ int mgx_server_submit_resident(mgx_server *s, int method, mgx_csr *A, mgx_vec *b
     MGX_REQUIRE(A->rows == A->cols && A->rows > 0 && b->n == A->rows && (!x || x->n == A->rows),
                 "mgx_server_submit_resident: size mismatch");
     MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
-                "mgx_server_submit_resident: unknown preconditioner");
+                "mgx_server_submit_resident: preconditioner must be default, none or Jacobi");
     auto r = std::make_shared<Request>();
     r->method = method;
     r->n = A->rows;

@@ -99,23 +99,51 @@ struct SolveOut {
   double resid = 0;
 };
 
-// the preconditioner vector: Jacobi -> 1/diag(A) on the device, else null
+// a preconditioner choice: MGX_PRECOND_* and the SOR relaxation coefficient
+struct PrecSpec {
+  int kind;
+  double omega;
+};
+
+template <class T> struct Prec {
+  int kind = MGX_PRECOND_DEFAULT;
+  T *M = nullptr;  // Jacobi: 1/diag(A); SOR: 1/(omega diag(A))
+  Csr *A = nullptr;
+  const void *val = nullptr;
+};
+
+// Jacobi::create_precond (jacobi.cpp:12-29): M = 1/diag(A); SOR::
+// create_precond (sor.cpp:12-35): M = 1/(omega diag(A)) (omega in T);
+// none / default: nothing
 template <class T>
-T *make_precond(Csr *A, const void *val, int precond) {
-  if (precond != MGX_PRECOND_JACOBI) return nullptr;
+Prec<T> make_precond(Csr *A, const void *val, PrecSpec spec) {
+  Prec<T> P;
+  P.kind = spec.kind;
+  P.A = A;
+  P.val = val;
+  if (spec.kind != MGX_PRECOND_JACOBI && spec.kind != MGX_PRECOND_SOR) return P;
   Ctx *c = A->ctx;
-  T *M = (T *)ws_get(c, (size_t)A->rows * sizeof(T));
-  launch_diag_v(A, val, DT<T>::v, M);
-  launch_reciprocal(c, DT<T>::v, A->rows, M, M);
-  return M;
+  P.M = (T *)ws_get(c, (size_t)A->rows * sizeof(T));
+  launch_diag_v(A, val, DT<T>::v, P.M);
+  if (spec.kind == MGX_PRECOND_SOR)
+    launch_scal(c, DT<T>::v, A->rows, (double)(T)spec.omega, P.M);
+  launch_reciprocal(c, DT<T>::v, A->rows, P.M, P.M);
+  return P;
 }
 
+// apply_precond: Jacobi z = M o r (jacobi.cpp:53-58); SOR the forward and
+// backward sweeps of sor_kernel_precond (sor_kernel.hpp:41-73); none
+// copy(r, z) (none.cpp:40-46); default z = r (monolish_solver.hpp:246-251)
 template <class T>
-void apply_precond(Ctx *c, const T *M, int64_t n, const T *r, T *z) {
-  if (M)
-    launch_vv(c, DT<T>::v, 2, n, M, r, z);
-  else
+void apply_precond(Ctx *c, const Prec<T> &P, int64_t n, const T *r, T *z) {
+  if (P.kind == MGX_PRECOND_JACOBI) {
+    launch_vv(c, DT<T>::v, 2, n, P.M, r, z);
+  } else if (P.kind == MGX_PRECOND_SOR) {
+    sor_sweep(P.A, P.val, DT<T>::v, 0, P.M, r, z);
+    sor_sweep(P.A, P.val, DT<T>::v, 2, P.M, nullptr, z);
+  } else {
     launch_copy(c, DT<T>::v, n, r, z);
+  }
 }
 
 template <class T> T dot_t(Ctx *c, int64_t n, const T *x, const T *y) {
@@ -124,7 +152,7 @@ template <class T> T dot_t(Ctx *c, int64_t n, const T *x, const T *y) {
 
 // equation::CG, reference call sequence (cg.cpp:27-85)
 template <class T>
-SolveOut cg_reference(Csr *A, const void *val, T *x, const T *b, int precond,
+SolveOut cg_reference(Csr *A, const void *val, T *x, const T *b, PrecSpec precond,
                       double tol, int64_t miniter, int64_t maxiter,
                       double *rhist, int64_t rhist_len) {
   Ctx *c = A->ctx;
@@ -134,7 +162,7 @@ SolveOut cg_reference(Csr *A, const void *val, T *x, const T *b, int precond,
   T *r = (T *)ws_get(c, bytes), *p = (T *)ws_get(c, bytes),
     *q = (T *)ws_get(c, bytes), *z = (T *)ws_get(c, bytes);
   for (T *v : {r, p, q, z}) launch_fill(c, dt, v, n, 0.0);
-  T *M = make_precond<T>(A, val, precond);
+  const Prec<T> M = make_precond<T>(A, val, precond);
   SolveOut o;
   launch_spmv(A, val, dt, x, q, 1, 0, false);  // cg.cpp:39
   launch_vv(c, dt, 1, n, b, q, r);             // cg.cpp:40
@@ -220,14 +248,15 @@ SolveOut cg_fused(Csr *A, const void *val, T *x, const T *b, int precond, double
                   int64_t miniter, int64_t maxiter, int check_every, double *rhist_host,
                   int64_t rhist_len) {
   if (A->rows == 0 || maxiter <= 0)
-    return cg_reference<T>(A, val, x, b, precond, tol, miniter, maxiter, rhist_host, rhist_len);
+    return cg_reference<T>(A, val, x, b, PrecSpec{precond, 1.9}, tol, miniter, maxiter,
+                           rhist_host, rhist_len);
   return fused_solve<T>(kMethodCG, A, val, x, b, precond, tol, miniter, maxiter, check_every,
                         rhist_host, rhist_len);
 }
 
 // equation::BiCGSTAB, reference call sequence (bicgstab.cpp:27-134)
 template <class T>
-SolveOut bicgstab_reference(Csr *A, T *x, const T *b, int precond, double tol,
+SolveOut bicgstab_reference(Csr *A, T *x, const T *b, PrecSpec precond, double tol,
                             int64_t miniter, int64_t maxiter, double *rhist,
                             int64_t rhist_len) {
   Ctx *c = A->ctx;
@@ -240,7 +269,7 @@ SolveOut bicgstab_reference(Csr *A, T *x, const T *b, int precond, double tol,
     *v = (T *)ws_get(c, bytes), *t = (T *)ws_get(c, bytes);
   for (T *w : {r, r0, p, phat, s, shat, v, t}) launch_fill(c, dt, w, n, 0.0);
   T rho_old = 1, rho = 1, alpha = 1, beta, omega = 1;
-  T *M = make_precond<T>(A, A->val, precond);
+  const Prec<T> M = make_precond<T>(A, A->val, precond);
   SolveOut o;
   auto nrm = [&](const T *w) { return host_sqrt(dot_t(c, n, w, w)); };
   launch_spmv(A, A->val, dt, x, r, 1, 0, false);  // :47
@@ -303,7 +332,7 @@ SolveOut bicgstab_reference(Csr *A, T *x, const T *b, int precond, double tol,
 
 // equation::Jacobi solver, reference call sequence (jacobi.cpp:96-149)
 template <class T>
-SolveOut jacobi_reference(Csr *A, T *x, const T *b, int precond, double tol,
+SolveOut jacobi_reference(Csr *A, T *x, const T *b, PrecSpec precond, double tol,
                           int64_t miniter, int64_t maxiter, double *rhist,
                           int64_t rhist_len) {
   Ctx *c = A->ctx;
@@ -319,7 +348,7 @@ SolveOut jacobi_reference(Csr *A, T *x, const T *b, int precond, double tol,
   T nrm2 = 0;
   launch_diag(A, d);                      // :108
   launch_reciprocal(c, dt, n, d, d);      // :109
-  T *M = make_precond<T>(A, A->val, precond);  // :111
+  const Prec<T> M = make_precond<T>(A, A->val, precond);  // :111
   SolveOut o;
   for (int64_t iter = 0; iter < maxiter; iter++) {
     apply_precond(c, M, n, x, s);                 // :116
@@ -350,6 +379,57 @@ SolveOut jacobi_reference(Csr *A, T *x, const T *b, int precond, double tol,
   return o;
 }
 
+// equation::SOR solver, reference call sequence (sor.cpp:91-164): d =
+// 1/(diag(A)/omega); per iteration s = M x, t = A s, r = b - t, ||r||, the
+// lower sweep t = (D/omega + L)^-1 r (sor_kernel_lower), x += t; relative test
+template <class T>
+SolveOut sor_reference(Csr *A, T *x, const T *b, PrecSpec precond, double tol, int64_t miniter,
+                       int64_t maxiter, double *rhist, int64_t rhist_len) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const mgx_dtype dt = DT<T>::v;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  T *r = (T *)ws_get(c, bytes), *t = (T *)ws_get(c, bytes), *s = (T *)ws_get(c, bytes),
+    *d = (T *)ws_get(c, bytes);
+  for (T *w : {r, t, s, d}) launch_fill(c, dt, w, n, 0.0);
+  auto nrm = [&](const T *w) { return host_sqrt(dot_t(c, n, w, w)); };
+  const T w = (T)(1.0 / (double)(T)precond.omega);  // :112
+  launch_diag(A, d);                                 // :113
+  launch_scal(c, dt, n, (double)w, d);               // :114
+  launch_reciprocal(c, dt, n, d, d);                 // :115
+  T bnrm2 = nrm(b);                                  // :118
+  bnrm2 = (T)(1.0 / (double)bnrm2);                  // :119
+  T nrm2 = 0;
+  const Prec<T> M = make_precond<T>(A, A->val, precond);  // :122
+  SolveOut o;
+  for (int64_t iter = 0; iter < maxiter; iter++) {
+    apply_precond(c, M, n, x, s);                   // :126
+    launch_spmv(A, A->val, dt, s, t, 1, 0, false);  // :127
+    launch_axpyz(c, dt, n, -1.0, t, b, r);          // :128
+    nrm2 = nrm(r);                                  // :129
+    sor_sweep(A, A->val, dt, 1, d, r, t);           // :133
+    launch_axpy(c, dt, n, 1.0, t, x);               // :137
+    nrm2 = nrm2 * bnrm2;                            // :139
+    if (rhist && iter < rhist_len) rhist[iter] = (double)nrm2;
+    if ((double)nrm2 < tol && miniter <= iter + 1) {  // :145
+      o.status = MGX_OK;
+      o.iters = iter + 1;
+      o.resid = (double)nrm(r);
+      return o;
+    }
+    if (std::isnan(nrm2)) {  // :151
+      o.status = MGX_SOLVER_RESIDUAL_NAN;
+      o.iters = iter + 1;
+      o.resid = (double)nrm(r);
+      return o;
+    }
+  }
+  o.status = MGX_SOLVER_MAXITER;
+  o.iters = maxiter;
+  o.resid = (double)nrm(r);
+  return o;
+}
+
 int64_t maxiter_of(const mgx_solve_params *p) {
   return p->maxiter <= 0 ? INT64_MAX / 4 : p->maxiter;
 }
@@ -364,8 +444,10 @@ void check_solve_args(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params
   MGX_REQUIRE(x->ctx == A->ctx && b->ctx == A->ctx, "%s: context mismatch", w);
   MGX_REQUIRE(x->dtype == A->dtype && b->dtype == A->dtype, "%s: dtype mismatch", w);
   MGX_REQUIRE(x->n == A->rows && b->n == A->rows, "%s: size mismatch", w);
-  MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_JACOBI,
+  MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_SOR,
               "%s: unknown preconditioner %d", w, p->precond);
+  MGX_REQUIRE(p->precond != MGX_PRECOND_SOR || (p->omega > 0 && p->omega < 2),
+              "%s: SOR omega must be in (0, 2)", w);
   MGX_REQUIRE(p->mode == MGX_MODE_REFERENCE || p->mode == MGX_MODE_FUSED,
               "%s: unknown mode %d", w, p->mode);
 }
@@ -612,6 +694,7 @@ void mgx_solve_params_default(mgx_solve_params *p) {
   p->miniter = 0;
   p->precond = MGX_PRECOND_DEFAULT;
   p->mode = MGX_MODE_FUSED;
+  p->omega = 1.9;  // monolish_solver.hpp:48
 }
 
 int mgx_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
@@ -621,21 +704,24 @@ int mgx_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
     activate(A->ctx);
     const double thr = threshold(A->ctx, p, b);
     const int64_t mi = maxiter_of(p);
+    // the SOR preconditioner's sweeps run as their own kernels: the CG with
+    // it follows the reference call sequence in either mode
+    const bool fused = p->mode == MGX_MODE_FUSED && p->precond != MGX_PRECOND_SOR;
+    const PrecSpec ps{p->precond, p->omega};
     return run_solver(A, res, [&] {
       if (A->dtype == MGX_F64) {
-        if (p->mode == MGX_MODE_FUSED)
+        if (fused)
           return cg_fused<double>(A, A->val, (double *)x->d, (const double *)b->d, p->precond,
                                   thr, p->miniter, mi, p->check_every, p->rhistory,
                                   p->rhistory_len);
-        return cg_reference<double>(A, A->val, (double *)x->d, (const double *)b->d,
-                                    p->precond, thr, p->miniter, mi, p->rhistory,
-                                    p->rhistory_len);
+        return cg_reference<double>(A, A->val, (double *)x->d, (const double *)b->d, ps, thr,
+                                    p->miniter, mi, p->rhistory, p->rhistory_len);
       }
-      if (p->mode == MGX_MODE_FUSED)
+      if (fused)
         return cg_fused<float>(A, A->val, (float *)x->d, (const float *)b->d, p->precond, thr,
                                p->miniter, mi, p->check_every, p->rhistory, p->rhistory_len);
-      return cg_reference<float>(A, A->val, (float *)x->d, (const float *)b->d, p->precond,
-                                 thr, p->miniter, mi, p->rhistory, p->rhistory_len);
+      return cg_reference<float>(A, A->val, (float *)x->d, (const float *)b->d, ps, thr,
+                                 p->miniter, mi, p->rhistory, p->rhistory_len);
     });
   });
 }
@@ -647,7 +733,9 @@ int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_param
     activate(A->ctx);
     const double thr = threshold(A->ctx, p, b);
     const int64_t mi = maxiter_of(p);
-    const bool fused = p->mode == MGX_MODE_FUSED && A->rows > 0 && mi > 0;
+    const bool fused =
+        p->mode == MGX_MODE_FUSED && A->rows > 0 && mi > 0 && p->precond != MGX_PRECOND_SOR;
+    const PrecSpec ps{p->precond, p->omega};
     return run_solver(A, res, [&] {
       if (fused && A->dtype == MGX_F64)
         return fused_solve<double>(kMethodBiCGSTAB, A, A->val, (double *)x->d,
@@ -658,9 +746,9 @@ int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_param
                                   (const float *)b->d, p->precond, thr, p->miniter, mi,
                                   p->check_every, p->rhistory, p->rhistory_len);
       if (A->dtype == MGX_F64)
-        return bicgstab_reference<double>(A, (double *)x->d, (const double *)b->d, p->precond,
-                                          thr, p->miniter, mi, p->rhistory, p->rhistory_len);
-      return bicgstab_reference<float>(A, (float *)x->d, (const float *)b->d, p->precond, thr,
+        return bicgstab_reference<double>(A, (double *)x->d, (const double *)b->d, ps, thr,
+                                          p->miniter, mi, p->rhistory, p->rhistory_len);
+      return bicgstab_reference<float>(A, (float *)x->d, (const float *)b->d, ps, thr,
                                        p->miniter, mi, p->rhistory, p->rhistory_len);
     });
   });
@@ -674,12 +762,32 @@ int mgx_jacobi_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params 
     // the Jacobi iteration's own test is relative (jacobi.cpp:124,131)
     const double thr = p->tol;
     const int64_t mi = maxiter_of(p);
+    const PrecSpec ps{p->precond, p->omega};
     return run_solver(A, res, [&] {
       if (A->dtype == MGX_F64)
-        return jacobi_reference<double>(A, (double *)x->d, (const double *)b->d, p->precond,
-                                        thr, p->miniter, mi, p->rhistory, p->rhistory_len);
-      return jacobi_reference<float>(A, (float *)x->d, (const float *)b->d, p->precond, thr,
+        return jacobi_reference<double>(A, (double *)x->d, (const double *)b->d, ps, thr,
+                                        p->miniter, mi, p->rhistory, p->rhistory_len);
+      return jacobi_reference<float>(A, (float *)x->d, (const float *)b->d, ps, thr,
                                      p->miniter, mi, p->rhistory, p->rhistory_len);
+    });
+  });
+}
+
+int mgx_sor_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                  mgx_solve_result *res) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, "mgx_sor_solve");
+    activate(A->ctx);
+    // the SOR iteration's test is relative (sor.cpp:139,145)
+    const double thr = p->tol;
+    const int64_t mi = maxiter_of(p);
+    const PrecSpec ps{p->precond, p->omega};
+    return run_solver(A, res, [&] {
+      if (A->dtype == MGX_F64)
+        return sor_reference<double>(A, (double *)x->d, (const double *)b->d, ps, thr,
+                                     p->miniter, mi, p->rhistory, p->rhistory_len);
+      return sor_reference<float>(A, (float *)x->d, (const float *)b->d, ps, thr,
+                                  p->miniter, mi, p->rhistory, p->rhistory_len);
     });
   });
 }
@@ -706,6 +814,8 @@ int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *
                     double inner_tol, int64_t max_outer, mgx_solve_result *res) {
   return guard([&] {
     check_solve_args(A, x, b, p, res, "mgx_ir_cg_solve");
+    MGX_REQUIRE(p->precond != MGX_PRECOND_SOR,
+                "mgx_ir_cg_solve: Jacobi or none (the SOR sweeps are not part of the fused engine)");
     MGX_REQUIRE(A->dtype == MGX_F64, "mgx_ir_cg_solve: needs an FP64 matrix");
     MGX_REQUIRE(inner_tol > 0 && inner_tol < 1, "mgx_ir_cg_solve: inner_tol in (0,1)");
     activate(A->ctx);
@@ -726,6 +836,8 @@ int mgx_mixed_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_param
                        double delta, int64_t max_updates, mgx_solve_result *res) {
   return guard([&] {
     check_solve_args(A, x, b, p, res, "mgx_mixed_cg_solve");
+    MGX_REQUIRE(p->precond != MGX_PRECOND_SOR,
+                "mgx_mixed_cg_solve: Jacobi or none (the SOR sweeps are not part of the fused engine)");
     MGX_REQUIRE(A->dtype == MGX_F64, "mgx_mixed_cg_solve: needs an FP64 matrix");
     MGX_REQUIRE(delta > 0 && delta < 1, "mgx_mixed_cg_solve: delta in (0,1)");
     activate(A->ctx);

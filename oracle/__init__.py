@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 SUCCESS, SIZE_ERROR, MAXITER, BREAKDOWN, RESIDUAL_NAN, NOT_IMPL = 0, -1, -2, -3, -4, -10
-PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI = 0, 1, 2
+PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
 
 
 def build() -> None:
@@ -59,6 +59,8 @@ def _declare(L):
     L.or_coo_sort_arrays.restype = _I64
     L.or_crs_from_coo.argtypes = [C.POINTER(_COO), _P, _P, _P]
     L.or_num_threads.restype = C.c_int
+    L.or_set_sor_omega.argtypes = [C.c_double]
+    L.or_get_sor_omega.restype = C.c_double
     for s, T in (("d", C.c_double), ("s", C.c_float)):
         getattr(L, f"or_crs_diag_{s}").argtypes = [_I64, _I64, _P, _P, _P, _P]
         getattr(L, f"or_spmv_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
@@ -78,7 +80,8 @@ def _declare(L):
         getattr(L, f"or_vreciprocal_{s}").argtypes = [_I64, _P, _P]
         getattr(L, f"or_residual_l2_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
         getattr(L, f"or_residual_l2_{s}").restype = T
-        for f in ("cg", "bicgstab", "jacobi"):
+        getattr(L, f"or_sor_precond_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
+        for f in ("cg", "bicgstab", "jacobi", "sor"):
             fn = getattr(L, f"or_{f}_{s}")
             fn.argtypes = [_I64, _P, _P, _P, _P, _P, C.c_int, C.c_double, _I64, _I64,
                            C.POINTER(_I64), C.POINTER(C.c_double), _P]
@@ -272,7 +275,7 @@ def residual_l2(rp, ci, val, x, b):
 # --------------------------------------------------------------- solvers ----
 def solve(method: str, rp, ci, val, x0, b, precond=PRECOND_JACOBI, tol=1e-8,
           miniter=0, maxiter=2**62, history=False):
-    """Run equation::{CG,BiCGSTAB,Jacobi}::solve restated.  Returns
+    """Run equation::{CG,BiCGSTAB,Jacobi,SOR}::solve restated.  Returns
     (status, x, final_iter, final_resid, rhistory or None)."""
     n = len(rp) - 1
     dt = val.dtype
@@ -289,6 +292,21 @@ def solve(method: str, rp, ci, val, x0, b, precond=PRECOND_JACOBI, tol=1e-8,
     if hist is not None:
         hist = hist[: min(fi.value, len(hist))]
     return rc, x, fi.value, fr.value, hist
+
+
+def set_sor_omega(w: float):
+    """Relaxation coefficient of the SOR preconditioner / solver (default 1.9)."""
+    lib().or_set_sor_omega(float(w))
+
+
+def sor_precond(rp, ci, val, r):
+    """SOR::create_precond + apply_precond (sor_kernel_precond) on r."""
+    n = len(rp) - 1
+    z = np.zeros(n, val.dtype)
+    getattr(lib(), f"or_sor_precond_{_sfx(val.dtype)}")(n, _ptr(_c(rp, np.int32)),
+                                                        _ptr(_c(ci, np.int32)), _ptr(val),
+                                                        _ptr(_c(r, val.dtype)), _ptr(z))
+    return z
 
 
 def num_threads() -> int:

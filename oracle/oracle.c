@@ -14,6 +14,10 @@
 
 int or_num_threads(void) { return omp_get_max_threads(); }
 
+static double g_sor_omega = 1.9;
+void or_set_sor_omega(double w) { g_sor_omega = w; }
+double or_get_sor_omega(void) { return g_sor_omega; }
+
 /* ---------------------------------------------------------------------------
  * std::mt19937 (the C++ standard's mersenne_twister_engine<uint32_t, 32, 624,
  * 397, 31, 0x9908b0df, 11, 0xffffffff, 7, 0x9d2c5680, 15, 0xefc60000, 18,

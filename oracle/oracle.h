@@ -64,6 +64,12 @@ extern "C" {
 #define OR_PRECOND_DEFAULT 0 /* solver without a bound preconditioner: z = r */
 #define OR_PRECOND_NONE 1    /* equation::none: z = copy(r) */
 #define OR_PRECOND_JACOBI 2  /* equation::Jacobi: z = (1/diag(A)) o r */
+#define OR_PRECOND_SOR 3     /* equation::SOR: forward and backward sweeps */
+
+/* relaxation coefficient of the SOR preconditioner and solver (the
+ * reference's solver default, include/monolish_solver.hpp:48: 1.9) */
+void or_set_sor_omega(double w);
+double or_get_sor_omega(void);
 
 /* ---- random numbers: std::mt19937 + std::uniform_real_distribution<double> */
 typedef struct {
@@ -135,7 +141,13 @@ void or_crs_from_coo(const or_coo *coo, int32_t *row_ptr, int32_t *col_ind,
   int or_jacobi_##S(int64_t n, const int32_t *rp, const int32_t *ci,           \
                     const T *val, T *x, const T *b, int precond, double tol,   \
                     int64_t miniter, int64_t maxiter, int64_t *final_iter,     \
-                    double *final_resid, double *rhist);
+                    double *final_resid, double *rhist);                       \
+  int or_sor_##S(int64_t n, const int32_t *rp, const int32_t *ci,              \
+                 const T *val, T *x, const T *b, int precond, double tol,      \
+                 int64_t miniter, int64_t maxiter, int64_t *final_iter,        \
+                 double *final_resid, double *rhist);                          \
+  void or_sor_precond_##S(int64_t n, const int32_t *rp, const int32_t *ci,     \
+                          const T *val, const T *r, T *z);
 
 OR_DECL(double, d)
 OR_DECL(float, s)

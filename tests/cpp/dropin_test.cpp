@@ -114,6 +114,7 @@ bool solver_harness(const std::string &file, T tol) {
   solver.set_miniter(0);
   solver.set_maxiter(10000);
   PRECOND precond;
+  precond.set_omega(1.9);  // test/equation/iterative/equation_kernel.hpp:29
   solver.set_create_precond(precond);
   solver.set_apply_precond(precond);
   int rc = solver.solve(A, x, b);
@@ -185,6 +186,12 @@ int main(int argc, char **argv) {
   HARNESS(BiCGSTAB, Jacobi, float, 1e-4, mtx2);
   HARNESS(Jacobi, none, double, 1e-8, mtx);
   HARNESS(Jacobi, Jacobi, float, 1e-4, mtx);
+  HARNESS(CG, SOR, double, 1e-8, mtx);
+  HARNESS(CG, SOR, float, 1e-4, mtx);
+  HARNESS(BiCGSTAB, SOR, double, 1e-8, mtx);
+  HARNESS(Jacobi, SOR, double, 1e-8, mtx);
+  HARNESS(SOR, none, double, 1e-8, mtx);
+  HARNESS(SOR, none, float, 1e-4, mtx);
   fused_and_callback_paths_agree<double>();
   fused_and_callback_paths_agree<float>();
   // operands on the host are refused, not computed on the CPU

@@ -44,7 +44,7 @@ PREC = {"default": 0, "none": 1, "jacobi": 2}
 
 
 def gpu_solve(m, ctx, method, rp, ci, val, x0, b, precond, tol, maxiter=0, miniter=0,
-              mode=None, history=0):
+              mode=None, history=0, omega=None):
     n = len(rp) - 1
     A = m.CRS(ctx, n, n, rp, ci, val)
     X = m.Vector(ctx, x0, dtype=val.dtype)
@@ -52,6 +52,8 @@ def gpu_solve(m, ctx, method, rp, ci, val, x0, b, precond, tol, maxiter=0, minit
     kw = dict(tol=tol, maxiter=maxiter, miniter=miniter, precond=precond, history=history)
     if mode is not None:
         kw["mode"] = mode
+    if omega is not None:
+        kw["omega"] = omega
     res = getattr(m, method)(A, X, B, **kw)
     return res, X.download()
 
@@ -314,3 +316,115 @@ def test_mixed_cg_edge_cases(m, ctx, oracle):
     assert abs(res.final_resid - oracle.residual_l2(rp, ci, val, X.download(), b)) <= 1e-12 * oracle.nrm2(b)
     with pytest.raises(m.MgxError):
         m.mixed_cg(A, X, B, delta=1.0)
+
+
+def _block_diag(nblk, bs, seed):
+    """nblk independent symmetric, diagonally dominant bs x bs blocks: a few
+    SOR levels of nblk rows each (the one-launch-per-level path)."""
+    rng = np.random.default_rng(seed)
+    n = nblk * bs
+    rp = np.arange(0, n * bs + 1, bs, dtype=np.int32)
+    ci = np.concatenate([np.tile(np.arange(k * bs, (k + 1) * bs), bs) for k in range(nblk)])
+    blk = rng.uniform(-1, 1, (nblk, bs, bs))
+    blk = blk + blk.transpose(0, 2, 1)
+    blk[:, np.arange(bs), np.arange(bs)] = 2.0 * bs
+    return rp, ci.astype(np.int32), blk.reshape(-1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("omega", [1.9, 1.0])
+def test_sor_preconditioned_solvers_bit_exact(m, ctx, oracle, dtype, omega):
+    """CG and BiCGSTAB with the SOR preconditioner (forward and backward
+    sweeps, level-scheduled on the device) reproduce the oracle's serial
+    sweeps bit for bit: x, iteration count and residual history; the fused
+    mode takes the reference call sequence for it and gives the same bits."""
+    say("sor precond", dtype.__name__, omega)
+    cases = [("cg", m.laplacian2d(48, 40)), ("cg", m.stencil3d(12, 10, 9, 7)),
+             ("bicgstab", m.convdiff2d(40, 20.0)), ("cg", _block_diag(6000, 3, 1)),
+             ("jacobi", m.laplacian2d(16, 16))]
+    oracle.set_sor_omega(omega)
+    for method, (rp, ci, val) in cases:
+        val = val.astype(dtype)
+        n = len(rp) - 1
+        b = oracle.random_vector(n, -1.0, 1.0, 77, dtype=dtype)
+        tol = float((1e-10 if dtype == np.float64 else 1e-5) * oracle.nrm2(b))
+        ref = oracle.solve(method, rp, ci, val, np.zeros(n, dtype), b, precond=3, tol=tol,
+                           maxiter=5000, history=True)
+        for mode in (m.MODE_REFERENCE, m.MODE_FUSED):
+            res, x = gpu_solve(m, ctx, method, rp, ci, val, np.zeros(n, dtype), b,
+                               m.PRECOND_SOR, tol, maxiter=5000, mode=mode, history=5000,
+                               omega=omega)
+            say(method, n, mode, res.final_iter, ref[2])
+            assert res.status == ref[0] == 0
+            assert res.final_iter == ref[2]
+            assert bits_equal(x, ref[1])
+            assert bits_equal(res.rhistory, ref[4])
+    oracle.set_sor_omega(1.9)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_sor_solver_bit_exact(m, ctx, oracle, dtype):
+    """equation::SOR on the device (lower sweep level by level) against the
+    oracle's serial loop: Poisson, a tridiagonal chain (one row per level),
+    independent blocks (wide levels, one launch per level), an unsorted row
+    whose sweep stops at its first upper entry, and the reference harness
+    (test.mtx, b = A 1, x0 random, x -> 1 within 10 tol)."""
+    say("sor solver", dtype.__name__)
+    n3 = 3000
+    rows, cols, vals = [], [], []
+    for i in range(n3):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n3:
+                rows.append(i)
+                cols.append(j)
+                vals.append(4.0 if i == j else -1.0)
+    rp = np.zeros(n3 + 1, np.int32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    trid = (np.cumsum(rp).astype(np.int32), np.array(cols, np.int32), np.array(vals))
+    unsorted = (np.array([0, 2, 4, 7], np.int32), np.array([0, 1, 1, 0, 2, 0, 1], np.int32),
+                np.array([3.0, -1.0, 3.0, -1.0, 3.0, -1.0, -1.0]))
+    cases = [m.laplacian2d(30, 30), trid, _block_diag(5000, 3, 2), unsorted]
+    for omega in (1.9, 1.2):
+        oracle.set_sor_omega(omega)
+        # the unsorted system diverges at 1.9 (the lower sweep skips its rows'
+        # off-diagonal entries): run it where the iteration converges
+        for rp, ci, val in cases if omega < 1.5 else cases[:3]:
+            val = val.astype(dtype)
+            n = len(rp) - 1
+            b = oracle.random_vector(n, -1.0, 1.0, 5, dtype=dtype)
+            x0 = oracle.random_vector(n, 0.0, 1.0, 6, dtype=dtype)
+            tol = 1e-9 if dtype == np.float64 else 1e-4
+            ref = oracle.solve("sor", rp, ci, val, x0, b, precond=1, tol=tol, maxiter=3000,
+                               history=True)
+            res, x = gpu_solve(m, ctx, "sor", rp, ci, val, x0, b, m.PRECOND_NONE, tol,
+                               maxiter=3000, history=3000, omega=omega)
+            say("sor", n, omega, res.final_iter, ref[2], res.status)
+            assert res.status == ref[0] and res.final_iter == ref[2]
+            assert bits_equal(x, ref[1])
+            assert bits_equal(res.rhistory, ref[4])
+    oracle.set_sor_omega(1.9)
+    rp, ci, val = _mtx(oracle, "test.mtx", dtype)
+    n = len(rp) - 1
+    ans = np.ones(n, dtype)
+    b = oracle.spmv(rp, ci, val, ans)
+    x0 = oracle.random_vector(n, 0.0, 1.0, 822569775, dtype=dtype)
+    tol = 1e-8 if dtype == np.float64 else 1e-4
+    res, x = gpu_solve(m, ctx, "sor", rp, ci, val, x0, b, m.PRECOND_NONE, tol, maxiter=10000)
+    assert res.status == 0 and ans_check(x, ans, tol * 10)
+
+
+def test_sor_rejected_where_unsupported(m, ctx, oracle):
+    say("sor rejects")
+    rp, ci, val = m.laplacian2d(8, 8)
+    n = len(rp) - 1
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    X, B = m.Vector(ctx, n=n), m.Vector(ctx, np.ones(n))
+    with pytest.raises(m.MgxError):
+        m.mixed_cg(A, X, B, precond=m.PRECOND_SOR)
+    with pytest.raises(m.MgxError):
+        m.batch_solve(ctx, m.METHOD_CG, [A], [X], [B], precond=m.PRECOND_SOR)
+    with pytest.raises(m.MgxError):
+        m.cg(A, X, B, precond=m.PRECOND_SOR, omega=2.0)
+    A.free()
+    X.free()
+    B.free()

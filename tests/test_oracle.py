@@ -321,18 +321,24 @@ def _mtx_crs(oracle, name, dtype):
 
 
 @pytest.mark.parametrize("method,mtx", [("cg", "test.mtx"), ("bicgstab", "test.mtx"),
-                                        ("bicgstab", "test2.mtx"), ("jacobi", "test.mtx")])
+                                        ("bicgstab", "test2.mtx"), ("jacobi", "test.mtx"),
+                                        ("sor", "test.mtx")])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-8), (np.float32, 1e-4)])
-@pytest.mark.parametrize("precond", [1, 2])
+@pytest.mark.parametrize("precond", [1, 2, 3])
 def test_reference_solver_harness(oracle, method, mtx, dtype, tol, precond):
     # test/equation/iterative/equation_kernel.hpp:4-56: b = A*1, x0 from
     # vector(n, 0, 1, test_random_engine()) with test_random_engine =
     # mt19937(1234), maxiter 10000; pass when x is within 10*tol of 1
+    # (SOR preconditioners with omega = 1.9, equation_kernel.hpp:29; the SOR
+    # solver is run with `none` only, sor.cpp:6 PRECOND 0)
+    if method == "sor" and precond != 1:
+        pytest.skip("the reference runs its SOR solver with none only")
     rp, ci, val = _mtx_crs(oracle, mtx, dtype)
     n = len(rp) - 1
     ans = np.ones(n, dtype)
     b = oracle.spmv(rp, ci, val, ans)
     x0 = oracle.random_vector(n, 0.0, 1.0, 822569775, dtype=dtype)
+    oracle.set_sor_omega(1.9)
     rc, x, it, res, _ = oracle.solve(method, rp, ci, val, x0, b, precond=precond, tol=tol, maxiter=10000)
     assert rc == 0
     assert ans_check(x, ans, tol * 10)
@@ -423,3 +429,87 @@ def test_solver_error_paths(oracle):
     # miniter forces iterations past convergence
     rc, x, it, res, _ = oracle.solve("cg", rp, ci, val, np.zeros(n), b, tol=1.0, miniter=7)
     assert rc == 0 and it == 7
+
+
+# ------------------------------------------------------------------- SOR ----
+def _random_diag_dominant(rng, n, k, sorted_cols=True):
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        cs = rng.choice(n, k, replace=False).tolist()
+        if i not in cs:
+            cs.append(i)
+        if sorted_cols:
+            cs.sort()
+        for c in cs:
+            rows.append(i)
+            cols.append(c)
+            vals.append(float(k + 2) if c == i else float(rng.uniform(-1, 1)))
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    return np.cumsum(rp).astype(np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.9])
+def test_sor_precond_is_the_triangular_product(oracle, omega):
+    """SOR::apply_precond = (wD + U)^-1 (wD) (wD + L)^-1 r, D = diag(A),
+    written with dense triangular solves (scipy) -- an independent
+    formulation of the two sweeps (forward over the strictly lower part,
+    backward over the strictly upper part, both scaled by M = 1/(w diag A)
+    as SOR::create_precond builds it; the SOR solver uses w/diag instead)."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(3)
+    rp, ci, val = _random_diag_dominant(rng, 40, 5)
+    A = _dense_from_crs(40, 40, rp, ci, val)
+    r = rng.uniform(-1, 1, 40)
+    oracle.set_sor_omega(omega)
+    z = oracle.sor_precond(rp, ci, val, r)
+    Dw = np.diag(np.diag(A) * omega)
+    z1 = sl.solve_triangular(Dw + np.tril(A, -1), r, lower=True)
+    zz = sl.solve_triangular(Dw + np.triu(A, 1), Dw @ z1, lower=False)
+    assert np.allclose(z, zz, rtol=1e-12, atol=1e-14)
+    # FP32: the backward sums in double, so the FP32 result tracks FP64
+    zf = oracle.sor_precond(rp, ci, val.astype(np.float32), r.astype(np.float32))
+    assert np.allclose(zf, zz, rtol=1e-4, atol=1e-5)
+    oracle.set_sor_omega(1.9)
+
+
+def test_sor_solver_is_the_sor_iteration(oracle):
+    """equation::SOR with `none`: x_(k+1) = x_k + (D/w + L)^-1 (b - A x_k),
+    dense and independent of the oracle's loop, for a few iterations; the
+    recorded residual history is ||b - A x_k|| / ||b|| before each update."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(4)
+    n = 30
+    rp, ci, val = _random_diag_dominant(rng, n, 4)
+    A = _dense_from_crs(n, n, rp, ci, val)
+    b = rng.uniform(-1, 1, n)
+    w = 1.3
+    oracle.set_sor_omega(w)
+    x = np.zeros(n)
+    hist = []
+    for _ in range(6):
+        r = b - A @ x
+        hist.append(np.linalg.norm(r) / np.linalg.norm(b))
+        x = x + sl.solve_triangular(np.diag(np.diag(A)) / w + np.tril(A, -1), r, lower=True)
+    rc, xo, it, res, h = oracle.solve("sor", rp, ci, val, np.zeros(n), b, precond=1, tol=1e-30,
+                                      maxiter=6, history=True)
+    oracle.set_sor_omega(1.9)
+    assert rc == -2 and it == 6
+    assert np.allclose(xo, x, rtol=1e-12, atol=1e-14)
+    assert np.allclose(h, hist, rtol=1e-12)
+
+
+def test_sor_lower_sweep_stops_at_the_first_upper_entry(oracle):
+    """sor_kernel_lower leaves a row at its first entry with col >= row
+    (`else break`): with an unsorted row, lower entries after it are not
+    used.  Row 2 stores (2,2) before (2,0): its sweep value ignores A_20."""
+    rp = np.array([0, 1, 2, 4], np.int32)
+    ci = np.array([0, 1, 2, 0], np.int32)
+    val = np.array([2.0, 2.0, 2.0, 1.0])
+    b = np.array([2.0, 2.0, 2.0])
+    oracle.set_sor_omega(1.0)
+    # one iteration from x = 0: r = b, t_i = r_i / A_ii (row 2's A_20 skipped)
+    rc, x, it, res, _ = oracle.solve("sor", rp, ci, val, np.zeros(3), b, precond=1, tol=1e-30,
+                                     maxiter=1)
+    oracle.set_sor_omega(1.9)
+    assert x.tolist() == [1.0, 1.0, 1.0]
