@@ -172,6 +172,61 @@ def bench_krylov(m, ctx, reps_fused, reps_plain):
     return out, xf, tol
 
 
+def bench_sor(m, ctx, reps, with_cpu):
+    """SOR-preconditioned CG (omega 1.9) and the SOR solver on the C1 Poisson
+    system: the sweeps run level by level on the device (511 levels of at
+    most 256 rows for the 256x256 grid)."""
+    rp, ci, val, b = c1_problem(m)
+    n = len(rp) - 1
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    B = m.Vector(ctx, b)
+    X = m.Vector(ctx, n=n)
+    bn = m.nrm2(B)
+    tol = 1e-10 * bn
+    out = {"workload": "SOR-PCG (omega 1.9) and SOR(1.9) iteration, 2D 5-point Poisson "
+                       "256x256, b=U[-1,1) seed 1001, x0=0"}
+    runs = {"sor_pcg": lambda: m.cg(A, X, B, tol=tol, precond=m.PRECOND_SOR, omega=1.9),
+            "sor_solver": lambda: m.sor(A, X, B, tol=1e-6, maxiter=2000, omega=1.9)}
+    xs = {}
+    for name, fn in runs.items():
+        lat = []
+        for i in range(reps + 1):
+            X.fill(0.0)
+            ctx.sync()
+            t0 = time.perf_counter()
+            res = fn()
+            dt = time.perf_counter() - t0
+            if i > 0:
+                lat.append(dt)
+        xs[name] = X.download()
+        med = statistics.median(lat)
+        out[name] = {"solves_per_s": round(1.0 / med, 3), "p50_ms": round(1e3 * med, 2),
+                     "iterations": res.final_iter, "status": res.status,
+                     "ms_per_iteration": round(1e3 * med / max(res.final_iter, 1), 4)}
+        log(f"sor {name}: {out[name]}")
+    if with_cpu:
+        import oracle
+        oracle.set_sor_omega(1.9)
+        t0 = time.perf_counter()
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_SOR, tol=tol)
+        t1 = time.perf_counter()
+        ref2 = oracle.solve("sor", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_NONE,
+                            tol=1e-6, maxiter=2000)
+        t2 = time.perf_counter()
+        out["cpu"] = {"kind": "port", "threads": 1,
+                      "sor_pcg_solves_per_s": round(1.0 / (t1 - t0), 3),
+                      "sor_solver_solves_per_s": round(1.0 / (t2 - t1), 3),
+                      "sor_pcg_gpu_bit_identical": bool(np.array_equal(
+                          ref[1].view(np.uint64), xs["sor_pcg"].view(np.uint64))),
+                      "sor_solver_gpu_bit_identical": bool(np.array_equal(
+                          ref2[1].view(np.uint64), xs["sor_solver"].view(np.uint64)))}
+        log(f"sor cpu: {out['cpu']}")
+    A.free()
+    B.free()
+    X.free()
+    return out
+
+
 def bench_mixed(m, ctx, side, reps):
     rp, ci, val = m.stencil3d(side, side, side, 27, 1e-2)
     n = len(rp) - 1
@@ -527,7 +582,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip the sweep, Krylov and CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--only", default="", help="comma list of legs: cpu,sweep,krylov,mixed,"
-                                                 "batched,served,ingest (default: all)")
+                                                 "batched,served,ingest,sor (default: all)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-nmax", type=int, default=100000,
@@ -596,7 +651,7 @@ def main():
         "roofline": roofline,
     }
     legs = set(args.only.split(",")) if args.only else {"cpu", "sweep", "krylov", "mixed", "batched",
-                                                         "served", "ingest"}
+                                                         "served", "ingest", "sor"}
     if args.quick:
         legs = set()
     if "cpu" in legs:
@@ -630,6 +685,8 @@ def main():
         line["served"] = bench_served(m, ctx, 2000.0, args.serve_seconds, 1000, args.serve_nmax)
     if "ingest" in legs:
         line["ingest"] = bench_ingest(m, ctx, 128, "cpu" in legs)
+    if "sor" in legs:
+        line["sor"] = bench_sor(m, ctx, 3, "cpu" in legs)
     line["device"] = info["name"]
     print(json.dumps(line))
     ctx.close()

@@ -227,6 +227,15 @@ int mgx_vreciprocal(mgx_vec *a, mgx_vec *y);
                                  level-scheduled kernels with the serial loop's
                                  bits; solvers using it take the reference call
                                  sequence in either mode */
+#define MGX_PRECOND_IC 4      /* equation::IC (src/equation/ic.cpp): IC(0) of the
+                                 lower pattern, L L^T; square, sorted columns,
+                                 A(i,i) stored, SPD (MGX_ERR_INVALID naming the
+                                 row otherwise); factor and both triangular
+                                 solves level-scheduled on the device */
+#define MGX_PRECOND_ILU 5     /* equation::ILU (src/equation/ilu.cpp): ILU(0), L U
+                                 on A's pattern; same requirements, a zero pivot
+                                 is an error.  Solvers with IC / ILU take the
+                                 reference call sequence in either mode */
 
 /* how the iteration is executed; every mode follows the reference's
  * arithmetic, so FP64 results of REFERENCE and FUSED are bit-identical */
@@ -273,6 +282,15 @@ int mgx_jacobi_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
  * src/equation/kernel/sor_kernel.hpp:5-22) runs level by level on the device
  * with the serial loop's bits */
 int mgx_sor_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                  const mgx_solve_params *p, mgx_solve_result *res);
+/* equation::IC<CRS<T>,T>::solve / equation::ILU<CRS<T>,T>::solve
+ * (src/equation/ic.cpp:140-235, ilu.cpp:145-240): factor a copy of A's
+ * values (IC(0) / ILU(0), as MGX_PRECOND_IC / _ILU) and apply it once,
+ * x = (L L^T)^-1 b or (L U)^-1 b.  p->precond is ignored; res reports
+ * MGX_OK with final_iter 0.  Errors as for the preconditioners */
+int mgx_ic_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
+                 const mgx_solve_params *p, mgx_solve_result *res);
+int mgx_ilu_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                   const mgx_solve_params *p, mgx_solve_result *res);
 /* util::get_residual_l2 (src/utils/equation_utils.cpp:45-56): ||b - A x|| */
 int mgx_residual_l2(mgx_csr *A, mgx_vec *x, mgx_vec *b, double *out);

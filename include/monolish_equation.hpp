@@ -117,6 +117,65 @@ class SOR : public monolish::solver::solver<MATRIX, Float> {
   [[nodiscard]] std::string solver_name() const { return "SOR"; }
 };
 
+// ------------------------------------------------------------- IC / ILU
+// As preconditioners (ic.cpp:12-140, ilu.cpp:12-145) they are bound to CG /
+// BiCGSTAB / Jacobi / SOR solvers, which factor a copy of A's values on the
+// device (IC(0) of the lower pattern / ILU(0), no fill) and apply the two
+// level-scheduled triangular solves; as solvers (ic.cpp:140-235,
+// ilu.cpp:145-240) they apply the factor once, x = M^-1 b.  The reference
+// requires device-resident data (it throws on the CPU); so does this one.
+template <typename MATRIX, typename Float>
+class IC : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  void create_precond(MATRIX &A) {
+    if (A.get_row() != A.get_col()) throw std::runtime_error("error A.row != A.col");
+    this->precond.A = &A;
+  }
+  void apply_precond(const vector<Float> &r, vector<Float> &z) {
+    (void)r, (void)z;
+    throw std::runtime_error(
+        "equation::IC::apply_precond: bind the IC object to a solver (the solves run there)");
+  }
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    Logger::get_instance().solver_in(monolish_func);
+    int ret = 0;
+    if (this->lib == 0) {
+      detail_eq::check_system(A, x, b, "equation::IC");
+      ret = this->run_device(mgx_ic_solve, A, x, b, MGX_PRECOND_DEFAULT, MGX_MODE_REFERENCE);
+    }
+    Logger::get_instance().solver_out();
+    return ret;
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::IC"; }
+  [[nodiscard]] std::string solver_name() const { return "IC"; }
+};
+
+template <typename MATRIX, typename Float>
+class ILU : public monolish::solver::solver<MATRIX, Float> {
+ public:
+  void create_precond(MATRIX &A) {
+    if (A.get_row() != A.get_col()) throw std::runtime_error("error A.row != A.col");
+    this->precond.A = &A;
+  }
+  void apply_precond(const vector<Float> &r, vector<Float> &z) {
+    (void)r, (void)z;
+    throw std::runtime_error(
+        "equation::ILU::apply_precond: bind the ILU object to a solver (the solves run there)");
+  }
+  [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    Logger::get_instance().solver_in(monolish_func);
+    int ret = 0;
+    if (this->lib == 0) {
+      detail_eq::check_system(A, x, b, "equation::ILU");
+      ret = this->run_device(mgx_ilu_solve, A, x, b, MGX_PRECOND_DEFAULT, MGX_MODE_REFERENCE);
+    }
+    Logger::get_instance().solver_out();
+    return ret;
+  }
+  [[nodiscard]] std::string name() const { return "monolish::equation::ILU"; }
+  [[nodiscard]] std::string solver_name() const { return "ILU"; }
+};
+
 // --------------------------------------------------------------------- CG
 template <typename MATRIX, typename Float>
 class CG : public monolish::solver::solver<MATRIX, Float> {

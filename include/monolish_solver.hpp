@@ -11,6 +11,8 @@ namespace equation {
 template <typename MATRIX, typename Float> class none;
 template <typename MATRIX, typename Float> class Jacobi;
 template <typename MATRIX, typename Float> class SOR;
+template <typename MATRIX, typename Float> class IC;
+template <typename MATRIX, typename Float> class ILU;
 }  // namespace equation
 
 namespace solver {
@@ -29,6 +31,12 @@ template <class M, class F> struct precond_kind<equation::Jacobi<M, F>> {
 };
 template <class M, class F> struct precond_kind<equation::SOR<M, F>> {
   static constexpr int value = MGX_PRECOND_SOR;
+};
+template <class M, class F> struct precond_kind<equation::IC<M, F>> {
+  static constexpr int value = MGX_PRECOND_IC;
+};
+template <class M, class F> struct precond_kind<equation::ILU<M, F>> {
+  static constexpr int value = MGX_PRECOND_ILU;
 };
 // the relaxation coefficient of a bound SOR object (read at solve time)
 template <class P> std::function<double()> omega_of(P &) { return nullptr; }

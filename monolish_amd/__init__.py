@@ -24,6 +24,7 @@ SOLVER_SIZE_ERROR, SOLVER_MAXITER, SOLVER_BREAKDOWN, SOLVER_RESIDUAL_NAN, SOLVER
 ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND = -101, -102, -103, -104, -105
 F64, F32 = 0, 1
 PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
+PRECOND_IC, PRECOND_ILU = 4, 5
 MODE_REFERENCE, MODE_FUSED = 0, 1
 
 
@@ -113,7 +114,8 @@ for _n in ("mgx_vadd", "mgx_vsub", "mgx_vmul"):
     _sig(_n, [_P, _P, _P])
 _sig("mgx_vreciprocal", [_P, _P])
 _sig("mgx_solve_params_default", [C.POINTER(SolveParams)], None)
-for _n in ("mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve", "mgx_sor_solve"):
+for _n in ("mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve", "mgx_sor_solve",
+           "mgx_ic_solve", "mgx_ilu_solve"):
     _sig(_n, [_P, _P, _P, C.POINTER(SolveParams), C.POINTER(SolveResultC)])
 _sig("mgx_residual_l2", [_P, _P, _P, C.POINTER(_D)])
 _sig("mgx_ir_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64, C.POINTER(SolveResultC)])
@@ -543,6 +545,20 @@ def sor(A, x, b, tol=1e-8, maxiter=0, miniter=0, precond=PRECOND_DEFAULT, omega=
     """equation::SOR<CRS<T>,T>::solve (src/equation/sor.cpp); tol is relative."""
     return _solve(_lib.mgx_sor_solve, A, x, b, tol, maxiter, miniter, precond, MODE_REFERENCE,
                   False, 0, history, omega)
+
+
+def ic_solve(A, x, b) -> SolveResult:
+    """equation::IC<CRS<T>,T>::solve (src/equation/ic.cpp): x = (L L^T)^-1 b
+    with the IC(0) factor of A."""
+    return _solve(_lib.mgx_ic_solve, A, x, b, 1e-8, 0, 0, PRECOND_DEFAULT, MODE_REFERENCE,
+                  False, 0, 0)
+
+
+def ilu_solve(A, x, b) -> SolveResult:
+    """equation::ILU<CRS<T>,T>::solve (src/equation/ilu.cpp): x = (L U)^-1 b
+    with the ILU(0) factor of A."""
+    return _solve(_lib.mgx_ilu_solve, A, x, b, 1e-8, 0, 0, PRECOND_DEFAULT, MODE_REFERENCE,
+                  False, 0, 0)
 
 
 def ir_cg(A, x, b, tol=1e-10, tol_is_relative=True, inner_tol=1e-4, max_outer=100, maxiter=0,

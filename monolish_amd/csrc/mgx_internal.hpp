@@ -93,11 +93,23 @@ struct Vec {
   size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
 };
 
-// SOR sweep schedule: rows grouped by level, level l = rows[ptr[l], ptr[l+1])
+// schedule of a triangular sweep (SOR, IC / ILU factorisation and solves):
+// rows grouped by level, level l = rows[ptr[l], ptr[l+1])
 struct SorLevels {
   int32_t nlev = 0, max_width = 0;
   int32_t *d_rows = nullptr, *d_ptr = nullptr;
   std::vector<int32_t> ptr;
+};
+
+// structure-only data of IC(0) / ILU(0) (built on first use): the diagonal
+// position of each row, and for IC the rows of L^T (entries (j, i) of L with
+// j > i as factor positions lt_e and rows lt_c, increasing j) with their
+// backward schedule
+struct IluStruct {
+  bool built = false;
+  int32_t *d_diag = nullptr;
+  int32_t *d_lt_rp = nullptr, *d_lt_e = nullptr, *d_lt_c = nullptr;
+  SorLevels lt;
 };
 
 struct SpmvPlan {
@@ -160,6 +172,7 @@ struct Csr {
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
   Csr *trans = nullptr;      // A^T, built on first matvec_T
   SorLevels sor[3];          // SOR sweeps: forward, forward (solver), backward
+  IluStruct ilu;             // IC / ILU structure
   // distinct 128-byte lines of x gathered by 64 consecutive entries
   // (sampled average; 0 when not measured): high for scattered columns
   double gather_lines = 0;
@@ -275,6 +288,24 @@ Csr *csr_transpose(Csr *A);
 void sor_sweep(Csr *A, const void *val, mgx_dtype dt, int kind, const void *D, const void *r,
                void *z);
 void sor_levels_free(Csr *A);
+// level schedule of a sweep kind over A's pattern (0: lower entries, 1:
+// lower entries up to the first other one, 2: upper entries), built on
+// first use from the device row_ptr / col_ind
+const SorLevels &sor_levels(Csr *A, int kind);
+// schedule of any row structure: kind 2 rows depend on their entries' rows
+// above them (columns > row), else on those below (columns < row; kind 1
+// stops at the first other column)
+void build_levels(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci, int64_t n,
+                  int kind, SorLevels &out);
+void levels_free(SorLevels &L);
+// IC(0) (kind MGX_PRECOND_IC) / ILU(0) (MGX_PRECOND_ILU) of A's pattern:
+// f (A->nnz entries of dt) holds a copy of the values on entry and the
+// factor on return; throws on unsorted columns, a missing diagonal or a
+// zero (IC: non-positive) pivot, naming the row
+void ilu_factor(Csr *A, mgx_dtype dt, int kind, void *f);
+// z = (L U)^-1 r with the factor f (t: n-entry intermediate)
+void ilu_apply(Csr *A, mgx_dtype dt, int kind, const void *f, const void *r, void *t, void *z);
+void ilu_free(Csr *A);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);
 

@@ -150,6 +150,7 @@ void csr_free(Csr *A) {
   spmv_panels_free(A);
   spmv_sell_free(A);
   sor_levels_free(A);
+  ilu_free(A);
   dev_free(A->d_chunks);
   dev_free(A->d_tile_chunk);
   if (A->trans) csr_free(A->trans);

@@ -107,22 +107,31 @@ struct PrecSpec {
 
 template <class T> struct Prec {
   int kind = MGX_PRECOND_DEFAULT;
-  T *M = nullptr;  // Jacobi: 1/diag(A); SOR: 1/(omega diag(A))
+  T *M = nullptr;  // Jacobi: 1/diag(A); SOR: 1/(omega diag(A)); IC / ILU: the factor (nnz)
+  T *t = nullptr;  // IC / ILU: intermediate vector of the two triangular solves
   Csr *A = nullptr;
   const void *val = nullptr;
 };
 
 // Jacobi::create_precond (jacobi.cpp:12-29): M = 1/diag(A); SOR::
 // create_precond (sor.cpp:12-35): M = 1/(omega diag(A)) (omega in T);
-// none / default: nothing
+// IC / ILU::create_precond (ic.cpp:12-80, ilu.cpp:12-85): M = a copy of
+// the values, factored in place; none / default: nothing
 template <class T>
 Prec<T> make_precond(Csr *A, const void *val, PrecSpec spec) {
   Prec<T> P;
   P.kind = spec.kind;
   P.A = A;
   P.val = val;
-  if (spec.kind != MGX_PRECOND_JACOBI && spec.kind != MGX_PRECOND_SOR) return P;
   Ctx *c = A->ctx;
+  if (spec.kind == MGX_PRECOND_IC || spec.kind == MGX_PRECOND_ILU) {
+    P.M = (T *)ws_get(c, (size_t)std::max<int64_t>(A->nnz, 1) * sizeof(T));
+    P.t = (T *)ws_get(c, (size_t)std::max<int64_t>(A->rows, 1) * sizeof(T));
+    launch_copy(c, DT<T>::v, A->nnz, val, P.M);
+    ilu_factor(A, DT<T>::v, spec.kind, P.M);
+    return P;
+  }
+  if (spec.kind != MGX_PRECOND_JACOBI && spec.kind != MGX_PRECOND_SOR) return P;
   P.M = (T *)ws_get(c, (size_t)A->rows * sizeof(T));
   launch_diag_v(A, val, DT<T>::v, P.M);
   if (spec.kind == MGX_PRECOND_SOR)
@@ -132,7 +141,8 @@ Prec<T> make_precond(Csr *A, const void *val, PrecSpec spec) {
 }
 
 // apply_precond: Jacobi z = M o r (jacobi.cpp:53-58); SOR the forward and
-// backward sweeps of sor_kernel_precond (sor_kernel.hpp:41-73); none
+// backward sweeps of sor_kernel_precond (sor_kernel.hpp:41-73); IC / ILU
+// the two triangular solves (ic.cpp:100-140, ilu.cpp:105-145); none
 // copy(r, z) (none.cpp:40-46); default z = r (monolish_solver.hpp:246-251)
 template <class T>
 void apply_precond(Ctx *c, const Prec<T> &P, int64_t n, const T *r, T *z) {
@@ -141,6 +151,8 @@ void apply_precond(Ctx *c, const Prec<T> &P, int64_t n, const T *r, T *z) {
   } else if (P.kind == MGX_PRECOND_SOR) {
     sor_sweep(P.A, P.val, DT<T>::v, 0, P.M, r, z);
     sor_sweep(P.A, P.val, DT<T>::v, 2, P.M, nullptr, z);
+  } else if (P.kind == MGX_PRECOND_IC || P.kind == MGX_PRECOND_ILU) {
+    ilu_apply(P.A, DT<T>::v, P.kind, P.M, r, P.t, z);
   } else {
     launch_copy(c, DT<T>::v, n, r, z);
   }
@@ -444,7 +456,7 @@ void check_solve_args(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params
   MGX_REQUIRE(x->ctx == A->ctx && b->ctx == A->ctx, "%s: context mismatch", w);
   MGX_REQUIRE(x->dtype == A->dtype && b->dtype == A->dtype, "%s: dtype mismatch", w);
   MGX_REQUIRE(x->n == A->rows && b->n == A->rows, "%s: size mismatch", w);
-  MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_SOR,
+  MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_ILU,
               "%s: unknown preconditioner %d", w, p->precond);
   MGX_REQUIRE(p->precond != MGX_PRECOND_SOR || (p->omega > 0 && p->omega < 2),
               "%s: SOR omega must be in (0, 2)", w);
@@ -704,9 +716,9 @@ int mgx_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
     activate(A->ctx);
     const double thr = threshold(A->ctx, p, b);
     const int64_t mi = maxiter_of(p);
-    // the SOR preconditioner's sweeps run as their own kernels: the CG with
-    // it follows the reference call sequence in either mode
-    const bool fused = p->mode == MGX_MODE_FUSED && p->precond != MGX_PRECOND_SOR;
+    // the SOR / IC / ILU preconditioners' sweeps run as their own kernels:
+    // the CG with them follows the reference call sequence in either mode
+    const bool fused = p->mode == MGX_MODE_FUSED && p->precond <= MGX_PRECOND_JACOBI;
     const PrecSpec ps{p->precond, p->omega};
     return run_solver(A, res, [&] {
       if (A->dtype == MGX_F64) {
@@ -734,7 +746,7 @@ int mgx_bicgstab_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_param
     const double thr = threshold(A->ctx, p, b);
     const int64_t mi = maxiter_of(p);
     const bool fused =
-        p->mode == MGX_MODE_FUSED && A->rows > 0 && mi > 0 && p->precond != MGX_PRECOND_SOR;
+        p->mode == MGX_MODE_FUSED && A->rows > 0 && mi > 0 && p->precond <= MGX_PRECOND_JACOBI;
     const PrecSpec ps{p->precond, p->omega};
     return run_solver(A, res, [&] {
       if (fused && A->dtype == MGX_F64)
@@ -771,6 +783,43 @@ int mgx_jacobi_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params 
                                      p->miniter, mi, p->rhistory, p->rhistory_len);
     });
   });
+}
+
+namespace {
+
+// IC::solve / ILU::solve (ic.cpp:140-235, ilu.cpp:145-240): factor a copy of
+// the values and apply the preconditioner once, x = (L U)^-1 b
+int factor_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                 mgx_solve_result *res, int kind, const char *who) {
+  return guard([&] {
+    check_solve_args(A, x, b, p, res, who);
+    activate(A->ctx);
+    return run_solver(A, res, [&] {
+      const PrecSpec ps{kind, p->omega};
+      if (A->dtype == MGX_F64) {
+        const Prec<double> P = make_precond<double>(A, A->val, ps);
+        apply_precond(A->ctx, P, A->rows, (const double *)b->d, (double *)x->d);
+      } else {
+        const Prec<float> P = make_precond<float>(A, A->val, ps);
+        apply_precond(A->ctx, P, A->rows, (const float *)b->d, (float *)x->d);
+      }
+      SolveOut o;
+      o.status = MGX_OK;
+      return o;
+    });
+  });
+}
+
+}  // namespace
+
+int mgx_ic_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                 mgx_solve_result *res) {
+  return factor_solve(A, x, b, p, res, MGX_PRECOND_IC, "mgx_ic_solve");
+}
+
+int mgx_ilu_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
+                  mgx_solve_result *res) {
+  return factor_solve(A, x, b, p, res, MGX_PRECOND_ILU, "mgx_ilu_solve");
 }
 
 int mgx_sor_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
@@ -814,8 +863,9 @@ int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *
                     double inner_tol, int64_t max_outer, mgx_solve_result *res) {
   return guard([&] {
     check_solve_args(A, x, b, p, res, "mgx_ir_cg_solve");
-    MGX_REQUIRE(p->precond != MGX_PRECOND_SOR,
-                "mgx_ir_cg_solve: Jacobi or none (the SOR sweeps are not part of the fused engine)");
+    MGX_REQUIRE(p->precond <= MGX_PRECOND_JACOBI,
+                "mgx_ir_cg_solve: Jacobi or none (the SOR / IC / ILU sweeps are not part of "
+                "the fused engine)");
     MGX_REQUIRE(A->dtype == MGX_F64, "mgx_ir_cg_solve: needs an FP64 matrix");
     MGX_REQUIRE(inner_tol > 0 && inner_tol < 1, "mgx_ir_cg_solve: inner_tol in (0,1)");
     activate(A->ctx);
@@ -836,8 +886,9 @@ int mgx_mixed_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_param
                        double delta, int64_t max_updates, mgx_solve_result *res) {
   return guard([&] {
     check_solve_args(A, x, b, p, res, "mgx_mixed_cg_solve");
-    MGX_REQUIRE(p->precond != MGX_PRECOND_SOR,
-                "mgx_mixed_cg_solve: Jacobi or none (the SOR sweeps are not part of the fused engine)");
+    MGX_REQUIRE(p->precond <= MGX_PRECOND_JACOBI,
+                "mgx_mixed_cg_solve: Jacobi or none (the SOR / IC / ILU sweeps are not part "
+                "of the fused engine)");
     MGX_REQUIRE(A->dtype == MGX_F64, "mgx_mixed_cg_solve: needs an FP64 matrix");
     MGX_REQUIRE(delta > 0 && delta < 1, "mgx_mixed_cg_solve: delta in (0,1)");
     activate(A->ctx);

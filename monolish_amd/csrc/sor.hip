@@ -1,25 +1,20 @@
 // sor.hip -- SOR sweeps on the device, bit-identical to the reference's
-// serial loops (src/equation/kernel/sor_kernel.hpp:5-73).
+// serial loops (src/equation/kernel/sor_kernel.hpp:5-73), and the level
+// schedules of every triangular sweep (SOR, IC / ILU).
 //
-// A sweep computes each row from rows computed before it.  Rows are grouped
-// into levels once per matrix (level of a row = 1 + the largest level of
-// the rows it reads); rows of one level are independent, so a level runs in
-// parallel and each row still adds its entries in storage order from the
-// values the serial loop would have seen -- the same bits.  Levels narrower
-// than kWgRows run inside one 1024-thread workgroup with a barrier between
-// levels (one launch per sweep); wider schedules launch once per level.
+// Each sweep runs level by level (mgx_sweep.hpp): rows of one level are
+// independent, and each row adds its entries in storage order from the
+// values the serial loop would have seen -- the same bits.
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "mgx_device.hpp"
+#include "mgx_sweep.hpp"
 
 namespace mgx {
 
 namespace {
-
-constexpr int kWgThreads = 1024;
-constexpr int kWgRows = 4096;  // widest level the one-workgroup form takes
 
 // kind 0: forward, precond (sor_kernel_precond's first loop: every entry
 //         left of the diagonal, z_i = (r_i - sum A_ij z_j) D_i)
@@ -27,59 +22,44 @@ constexpr int kWgRows = 4096;  // widest level the one-workgroup form takes
 //         column is left of the row; the first other one ends the row)
 // kind 2: backward, precond (second loop: entries right of the diagonal,
 //         the sum kept in double, z_i -= sum * D_i)
-template <class T, int KIND>
-__device__ __forceinline__ void sor_row(int32_t i, const int32_t *__restrict__ rp,
-                                        const int32_t *__restrict__ ci, const T *__restrict__ val,
-                                        const T *__restrict__ D, const T *__restrict__ r, T *z) {
-  const int32_t e0 = rp[i], e1 = rp[i + 1];
-  if (KIND == 2) {
-    double tmp = 0.0;
-    for (int32_t j = e0; j < e1; j++) {
-      const int32_t c = ci[j];
-      if (i < c) {
-        T p = val[j] * z[c];
-        tmp = tmp + (double)p;
+template <class T, int KIND> struct SorRow {
+  const int32_t *__restrict__ rp;
+  const int32_t *__restrict__ ci;
+  const T *__restrict__ val;
+  const T *__restrict__ D;
+  const T *__restrict__ r;
+  T *z;
+  __device__ __forceinline__ void operator()(int32_t i) const {
+    const int32_t e0 = rp[i], e1 = rp[i + 1];
+    if (KIND == 2) {
+      double tmp = 0.0;
+      for (int32_t j = e0; j < e1; j++) {
+        const int32_t c = ci[j];
+        if (i < c) {
+          T p = val[j] * z[c];
+          tmp = tmp + (double)p;
+        }
       }
-    }
-    z[i] = (T)((double)z[i] - tmp * (double)D[i]);
-  } else {
-    T tmp = r[i];
-    for (int32_t j = e0; j < e1; j++) {
-      const int32_t c = ci[j];
-      if (i > c) {
-        T p = val[j] * z[c];
-        tmp = tmp - p;
-      } else if (KIND == 1) {
-        break;
+      z[i] = (T)((double)z[i] - tmp * (double)D[i]);
+    } else {
+      T tmp = r[i];
+      for (int32_t j = e0; j < e1; j++) {
+        const int32_t c = ci[j];
+        if (i > c) {
+          T p = val[j] * z[c];
+          tmp = tmp - p;
+        } else if (KIND == 1) {
+          break;
+        }
       }
+      z[i] = tmp * D[i];
     }
-    z[i] = tmp * D[i];
   }
-}
+};
 
-template <class T, int KIND>
-__global__ __launch_bounds__(kWgThreads) void k_sor_wg(
-    const int32_t *__restrict__ lptr, int32_t nlev, const int32_t *__restrict__ rows,
-    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, const T *__restrict__ val,
-    const T *__restrict__ D, const T *__restrict__ r, T *z) {
-  for (int32_t L = 0; L < nlev; L++) {
-    const int32_t k1 = lptr[L + 1];
-    for (int32_t k = lptr[L] + (int32_t)threadIdx.x; k < k1; k += kWgThreads)
-      sor_row<T, KIND>(rows[k], rp, ci, val, D, r, z);
-    __syncthreads();  // the level's rows are written before the next reads them
-  }
-}
+}  // namespace
 
-template <class T, int KIND>
-__global__ __launch_bounds__(kBlock) void k_sor_level(
-    const int32_t *__restrict__ rows, int32_t count, const int32_t *__restrict__ rp,
-    const int32_t *__restrict__ ci, const T *__restrict__ val, const T *__restrict__ D,
-    const T *__restrict__ r, T *z) {
-  const int32_t k = (int32_t)(blockIdx.x * kBlock + threadIdx.x);
-  if (k < count) sor_row<T, KIND>(rows[k], rp, ci, val, D, r, z);
-}
-
-// levels of one sweep kind from the host row_ptr / col_ind
+// levels of one sweep kind from a host row structure
 void build_levels(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci, int64_t n,
                   int kind, SorLevels &out) {
   std::vector<int32_t> lev(n, 0);
@@ -120,7 +100,7 @@ void build_levels(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci
   MGX_HIP(hipMemcpy(out.d_ptr, out.ptr.data(), (size_t)(nlev + 1) * 4, hipMemcpyHostToDevice));
 }
 
-const SorLevels &levels_of(Csr *A, int kind) {
+const SorLevels &sor_levels(Csr *A, int kind) {
   if (!A->sor[kind].d_ptr) {
     const int64_t n = A->rows;
     std::vector<int32_t> rp(n + 1), ci;
@@ -134,22 +114,11 @@ const SorLevels &levels_of(Csr *A, int kind) {
   return A->sor[kind];
 }
 
+namespace {
+
 template <class T, int KIND>
 void sweep_t(Csr *A, const T *val, const T *D, const T *r, T *z) {
-  Ctx *c = A->ctx;
-  const SorLevels &L = levels_of(A, KIND);
-  if (L.nlev == 0) return;
-  if (L.max_width <= kWgRows) {
-    k_sor_wg<T, KIND><<<1, kWgThreads, 0, c->stream>>>(L.d_ptr, L.nlev, L.d_rows, A->rp, A->ci,
-                                                       val, D, r, z);
-  } else {
-    for (int32_t l = 0; l < L.nlev; l++) {
-      const int32_t cnt = L.ptr[l + 1] - L.ptr[l];
-      k_sor_level<T, KIND><<<(unsigned)((cnt + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
-          L.d_rows + L.ptr[l], cnt, A->rp, A->ci, val, D, r, z);
-    }
-  }
-  MGX_HIP(hipGetLastError());
+  run_sweep(A->ctx, sor_levels(A, KIND), SorRow<T, KIND>{A->rp, A->ci, val, D, r, z});
 }
 
 template <class T> void sweep_kind(Csr *A, int kind, const T *val, const T *D, const T *r, T *z) {
@@ -174,12 +143,14 @@ void sor_sweep(Csr *A, const void *val, mgx_dtype dt, int kind, const void *D, c
                       static_cast<const float *>(r), static_cast<float *>(z));
 }
 
+void levels_free(SorLevels &L) {
+  dev_free(L.d_rows);
+  dev_free(L.d_ptr);
+  L = SorLevels();
+}
+
 void sor_levels_free(Csr *A) {
-  for (SorLevels &L : A->sor) {
-    dev_free(L.d_rows);
-    dev_free(L.d_ptr);
-    L = SorLevels();
-  }
+  for (SorLevels &L : A->sor) levels_free(L);
 }
 
 }  // namespace mgx

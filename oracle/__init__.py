@@ -17,7 +17,9 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 SUCCESS, SIZE_ERROR, MAXITER, BREAKDOWN, RESIDUAL_NAN, NOT_IMPL = 0, -1, -2, -3, -4, -10
+FACTOR_ERROR = -11
 PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
+PRECOND_IC, PRECOND_ILU = 4, 5
 
 
 def build() -> None:
@@ -61,6 +63,7 @@ def _declare(L):
     L.or_num_threads.restype = C.c_int
     L.or_set_sor_omega.argtypes = [C.c_double]
     L.or_get_sor_omega.restype = C.c_double
+    L.or_factor_bad_row.restype = _I64
     for s, T in (("d", C.c_double), ("s", C.c_float)):
         getattr(L, f"or_crs_diag_{s}").argtypes = [_I64, _I64, _P, _P, _P, _P]
         getattr(L, f"or_spmv_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
@@ -81,6 +84,10 @@ def _declare(L):
         getattr(L, f"or_residual_l2_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
         getattr(L, f"or_residual_l2_{s}").restype = T
         getattr(L, f"or_sor_precond_{s}").argtypes = [_I64, _P, _P, _P, _P, _P]
+        getattr(L, f"or_ilu_factor_{s}").argtypes = [_I64, _P, _P, _P, C.c_int, _P]
+        getattr(L, f"or_ilu_factor_{s}").restype = C.c_int
+        getattr(L, f"or_ilu_precond_{s}").argtypes = [_I64, _P, _P, _P, C.c_int, _P, _P]
+        getattr(L, f"or_ilu_precond_{s}").restype = C.c_int
         for f in ("cg", "bicgstab", "jacobi", "sor"):
             fn = getattr(L, f"or_{f}_{s}")
             fn.argtypes = [_I64, _P, _P, _P, _P, _P, C.c_int, C.c_double, _I64, _I64,
@@ -311,3 +318,30 @@ def sor_precond(rp, ci, val, r):
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+def ilu_factor(rp, ci, val, kind):
+    """IC::create_precond (kind PRECOND_IC) / ILU::create_precond
+    (PRECOND_ILU): the factored copy of val.  Returns (status, f); status
+    FACTOR_ERROR for unsorted columns, a missing diagonal or a zero pivot
+    (factor_bad_row() names the row)."""
+    n = len(rp) - 1
+    f = np.zeros(max(int(rp[-1]) if n > 0 else 0, 1), val.dtype)
+    rc = getattr(lib(), f"or_ilu_factor_{_sfx(val.dtype)}")(
+        n, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)), _ptr(val), int(kind), _ptr(f))
+    return rc, f[: int(rp[-1]) if n > 0 else 0]
+
+
+def ilu_precond(rp, ci, val, kind, r):
+    """create_precond + apply_precond of IC / ILU on r.  Returns (status, z)."""
+    n = len(rp) - 1
+    z = np.zeros(n, val.dtype)
+    rc = getattr(lib(), f"or_ilu_precond_{_sfx(val.dtype)}")(
+        n, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)), _ptr(val), int(kind),
+        _ptr(_c(r, val.dtype)), _ptr(z))
+    return rc, z
+
+
+def factor_bad_row() -> int:
+    """Row at which the last IC / ILU set-up failed (-1: none)."""
+    return int(lib().or_factor_bad_row())

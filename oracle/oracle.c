@@ -17,6 +17,8 @@ int or_num_threads(void) { return omp_get_max_threads(); }
 static double g_sor_omega = 1.9;
 void or_set_sor_omega(double w) { g_sor_omega = w; }
 double or_get_sor_omega(void) { return g_sor_omega; }
+static int64_t g_factor_bad_row = -1;
+int64_t or_factor_bad_row(void) { return g_factor_bad_row; }
 
 /* ---------------------------------------------------------------------------
  * std::mt19937 (the C++ standard's mersenne_twister_engine<uint32_t, 32, 624,

@@ -60,16 +60,23 @@ extern "C" {
 #define OR_BREAKDOWN -3
 #define OR_RESIDUAL_NAN -4
 #define OR_NOT_IMPL -10
+/* IC / ILU set-up failed: unsorted or duplicate columns, a missing
+ * diagonal entry or a zero (IC: non-positive) pivot; the reference throws */
+#define OR_FACTOR_ERROR -11
 
 #define OR_PRECOND_DEFAULT 0 /* solver without a bound preconditioner: z = r */
 #define OR_PRECOND_NONE 1    /* equation::none: z = copy(r) */
 #define OR_PRECOND_JACOBI 2  /* equation::Jacobi: z = (1/diag(A)) o r */
 #define OR_PRECOND_SOR 3     /* equation::SOR: forward and backward sweeps */
+#define OR_PRECOND_IC 4      /* equation::IC: IC(0), L L^T */
+#define OR_PRECOND_ILU 5     /* equation::ILU: ILU(0), L U */
 
 /* relaxation coefficient of the SOR preconditioner and solver (the
  * reference's solver default, include/monolish_solver.hpp:48: 1.9) */
 void or_set_sor_omega(double w);
 double or_get_sor_omega(void);
+/* row at which the last IC / ILU set-up failed (-1: none) */
+int64_t or_factor_bad_row(void);
 
 /* ---- random numbers: std::mt19937 + std::uniform_real_distribution<double> */
 typedef struct {
@@ -147,7 +154,11 @@ void or_crs_from_coo(const or_coo *coo, int32_t *row_ptr, int32_t *col_ind,
                  int64_t miniter, int64_t maxiter, int64_t *final_iter,        \
                  double *final_resid, double *rhist);                          \
   void or_sor_precond_##S(int64_t n, const int32_t *rp, const int32_t *ci,     \
-                          const T *val, const T *r, T *z);
+                          const T *val, const T *r, T *z);                     \
+  int or_ilu_factor_##S(int64_t n, const int32_t *rp, const int32_t *ci,       \
+                        const T *val, int kind, T *f);                         \
+  int or_ilu_precond_##S(int64_t n, const int32_t *rp, const int32_t *ci,      \
+                         const T *val, int kind, const T *r, T *z);
 
 OR_DECL(double, d)
 OR_DECL(float, s)

@@ -192,6 +192,17 @@ int main(int argc, char **argv) {
   HARNESS(Jacobi, SOR, double, 1e-8, mtx);
   HARNESS(SOR, none, double, 1e-8, mtx);
   HARNESS(SOR, none, float, 1e-4, mtx);
+  HARNESS(CG, IC, double, 1e-8, mtx);
+  HARNESS(CG, IC, float, 1e-4, mtx);
+  HARNESS(CG, ILU, double, 1e-8, mtx);
+  HARNESS(CG, ILU, float, 1e-4, mtx);
+  HARNESS(BiCGSTAB, ILU, double, 1e-8, mtx);
+  HARNESS(BiCGSTAB, IC, float, 1e-4, mtx);
+  HARNESS(Jacobi, ILU, double, 1e-8, mtx);
+  HARNESS(IC, none, double, 1e-8, mtx);
+  HARNESS(IC, none, float, 1e-4, mtx);
+  HARNESS(ILU, none, double, 1e-8, mtx);
+  HARNESS(ILU, none, float, 1e-4, mtx);
   fused_and_callback_paths_agree<double>();
   fused_and_callback_paths_agree<float>();
   // operands on the host are refused, not computed on the CPU
