@@ -173,9 +173,9 @@ def bench_krylov(m, ctx, reps_fused, reps_plain):
 
 
 def bench_sor(m, ctx, reps, with_cpu):
-    """SOR-preconditioned CG (omega 1.9) and the SOR solver on the C1 Poisson
-    system: the sweeps run level by level on the device (511 levels of at
-    most 256 rows for the 256x256 grid)."""
+    """SOR-, IC(0)- and ILU(0)-preconditioned CG and the SOR solver on the C1
+    Poisson system: the sweeps and the factorisation run level by level on
+    the device (511 levels of at most 256 rows for the 256x256 grid)."""
     rp, ci, val, b = c1_problem(m)
     n = len(rp) - 1
     A = m.CRS(ctx, n, n, rp, ci, val)
@@ -183,10 +183,12 @@ def bench_sor(m, ctx, reps, with_cpu):
     X = m.Vector(ctx, n=n)
     bn = m.nrm2(B)
     tol = 1e-10 * bn
-    out = {"workload": "SOR-PCG (omega 1.9) and SOR(1.9) iteration, 2D 5-point Poisson "
-                       "256x256, b=U[-1,1) seed 1001, x0=0"}
+    out = {"workload": "SOR-PCG (omega 1.9), IC(0)-PCG, ILU(0)-PCG and SOR(1.9) iteration, "
+                       "2D 5-point Poisson 256x256, b=U[-1,1) seed 1001, x0=0"}
     runs = {"sor_pcg": lambda: m.cg(A, X, B, tol=tol, precond=m.PRECOND_SOR, omega=1.9),
-            "sor_solver": lambda: m.sor(A, X, B, tol=1e-6, maxiter=2000, omega=1.9)}
+            "sor_solver": lambda: m.sor(A, X, B, tol=1e-6, maxiter=2000, omega=1.9),
+            "ic_pcg": lambda: m.cg(A, X, B, tol=tol, precond=m.PRECOND_IC),
+            "ilu_pcg": lambda: m.cg(A, X, B, tol=tol, precond=m.PRECOND_ILU)}
     xs = {}
     for name, fn in runs.items():
         lat = []
@@ -213,9 +215,14 @@ def bench_sor(m, ctx, reps, with_cpu):
         ref2 = oracle.solve("sor", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_NONE,
                             tol=1e-6, maxiter=2000)
         t2 = time.perf_counter()
+        ref3 = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=oracle.PRECOND_IC, tol=tol)
+        t3 = time.perf_counter()
         out["cpu"] = {"kind": "port", "threads": 1,
                       "sor_pcg_solves_per_s": round(1.0 / (t1 - t0), 3),
                       "sor_solver_solves_per_s": round(1.0 / (t2 - t1), 3),
+                      "ic_pcg_solves_per_s": round(1.0 / (t3 - t2), 3),
+                      "ic_pcg_gpu_bit_identical": bool(np.array_equal(
+                          ref3[1].view(np.uint64), xs["ic_pcg"].view(np.uint64))),
                       "sor_pcg_gpu_bit_identical": bool(np.array_equal(
                           ref[1].view(np.uint64), xs["sor_pcg"].view(np.uint64))),
                       "sor_solver_gpu_bit_identical": bool(np.array_equal(

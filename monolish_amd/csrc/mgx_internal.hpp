@@ -101,15 +101,44 @@ struct SorLevels {
   std::vector<int32_t> ptr;
 };
 
-// structure-only data of IC(0) / ILU(0) (built on first use): the diagonal
-// position of each row, and for IC the rows of L^T (entries (j, i) of L with
-// j > i as factor positions lt_e and rows lt_c, increasing j) with their
-// backward schedule
+// plan of a triangular sweep (SOR sweeps, IC / ILU solves): rows in level
+// order, rec[k] = {row, first, end, 0} of its dependency entries, packed in
+// the row's order: dcol = the column read, dent = the entry (position in the
+// values / factor) whose coefficient multiplies it
+struct TriPlan {
+  bool built = false;
+  SorLevels lev;  // d_rows = the rows in level order
+  int4 *d_rec = nullptr;
+  // dpos: level-order position of the row read, or -1 when the sweep's LDS
+  // copy of it may already be overwritten
+  int32_t *d_dcol = nullptr, *d_dent = nullptr, *d_dpos = nullptr;
+  int64_t ndeps = 0;
+  // blocks of whole levels staged through LDS: {first level, first
+  // position, first dependency entry}, nblk + 1 entries (0: not staged)
+  int4 *d_blk = nullptr;
+  int32_t nblk = 0;
+};
+// plans of a matrix: entries left of the diagonal; left of it up to the
+// first other entry (the SOR solver's sweep); right of it; rows of L^T (IC,
+// built by the IC set-up)
+enum { kTriLower = 0, kTriLowerBreak = 1, kTriUpper = 2, kTriLt = 3 };
+// how a row ends: out = acc * sc; the SOR backward form (acc in double
+// from 0, out -= acc * sc); out = acc / sc; out = acc
+enum { kTriMul = 0, kTriSorBack = 1, kTriDiv = 2, kTriUnit = 3 };
+// a plan with its coefficients and per-row scale gathered in level order
+struct TriOp {
+  const TriPlan *plan = nullptr;
+  void *coef = nullptr, *sc = nullptr;
+  void *ginit = nullptr;  // staged plans: the initial values in level order
+  int fin = kTriMul;
+};
+
+// structure-only data of IC(0) / ILU(0) (built on first use): the checks on
+// the pattern and the diagonal position of each row (the L^T plan goes to
+// Csr::tri[kTriLt])
 struct IluStruct {
   bool built = false;
   int32_t *d_diag = nullptr;
-  int32_t *d_lt_rp = nullptr, *d_lt_e = nullptr, *d_lt_c = nullptr;
-  SorLevels lt;
 };
 
 struct SpmvPlan {
@@ -171,7 +200,7 @@ struct Csr {
   int32_t fused_nnz = 0;     // entry budget of a chunk
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
   Csr *trans = nullptr;      // A^T, built on first matvec_T
-  SorLevels sor[3];          // SOR sweeps: forward, forward (solver), backward
+  TriPlan tri[4];            // triangular sweep plans (kTriLower ... kTriLt)
   IluStruct ilu;             // IC / ILU structure
   // distinct 128-byte lines of x gathered by 64 consecutive entries
   // (sampled average; 0 when not measured): high for scattered columns
@@ -282,29 +311,29 @@ double sample_gather_lines(const int32_t *ci, int64_t nnz, size_t es);
 void csr_free(Csr *A);
 // A^T built on the device (stable sort of the entries by column)
 Csr *csr_transpose(Csr *A);
-// one SOR sweep over A (kind 0: forward of the preconditioner, 1: forward
-// of the SOR solver, 2: backward of the preconditioner), levels built on
-// first use; z written in place (r unused by kind 2)
-void sor_sweep(Csr *A, const void *val, mgx_dtype dt, int kind, const void *D, const void *r,
-               void *z);
-void sor_levels_free(Csr *A);
-// level schedule of a sweep kind over A's pattern (0: lower entries, 1:
-// lower entries up to the first other one, 2: upper entries), built on
-// first use from the device row_ptr / col_ind
-const SorLevels &sor_levels(Csr *A, int kind);
-// schedule of any row structure: kind 2 rows depend on their entries' rows
-// above them (columns > row), else on those below (columns < row; kind 1
-// stops at the first other column)
-void build_levels(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci, int64_t n,
-                  int kind, SorLevels &out);
+// schedule of any row structure (ptr, col; ent = entry positions, nullptr =
+// the identity): filter 2 rows depend on their entries right of the
+// diagonal, else on those left of it (filter 1 stops at the first other one)
+void build_tri_plan(const std::vector<int32_t> &ptr, const std::vector<int32_t> &col,
+                    const std::vector<int32_t> *ent, int64_t n, int filter, TriPlan &P);
+// plan kTriLower / kTriLowerBreak / kTriUpper of A's pattern, built on first
+// use from the device row_ptr / col_ind
+const TriPlan &tri_plan(Csr *A, int which);
+// gather coef[d] = vals[dent[d]] and sc[k] = scale[scale_pos ? scale_pos[row]
+// : row] for the plan (buffers from the context's scratch)
+TriOp tri_prepare(Csr *A, const TriPlan &P, mgx_dtype dt, int fin, const void *vals,
+                  const void *scale, const int32_t *scale_pos);
+// one sweep: out_row = fin(in_row - sum coef * out_col) in level order
+void tri_run(Csr *A, const TriOp &op, mgx_dtype dt, const void *in, void *out);
 void levels_free(SorLevels &L);
+void tri_free(Csr *A);
 // IC(0) (kind MGX_PRECOND_IC) / ILU(0) (MGX_PRECOND_ILU) of A's pattern:
 // f (A->nnz entries of dt) holds a copy of the values on entry and the
 // factor on return; throws on unsorted columns, a missing diagonal or a
-// zero (IC: non-positive) pivot, naming the row
+// zero (IC: non-positive) pivot, naming the row.  Builds the structure
+// (diagonal positions, the kTriLt plan for IC) on first use
 void ilu_factor(Csr *A, mgx_dtype dt, int kind, void *f);
-// z = (L U)^-1 r with the factor f (t: n-entry intermediate)
-void ilu_apply(Csr *A, mgx_dtype dt, int kind, const void *f, const void *r, void *t, void *z);
+const int32_t *ilu_diag(Csr *A);
 void ilu_free(Csr *A);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);

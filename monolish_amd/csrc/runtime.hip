@@ -149,7 +149,7 @@ void csr_free(Csr *A) {
   dev_free(A->plan.d_blocks);
   spmv_panels_free(A);
   spmv_sell_free(A);
-  sor_levels_free(A);
+  tri_free(A);
   ilu_free(A);
   dev_free(A->d_chunks);
   dev_free(A->d_tile_chunk);

@@ -110,7 +110,7 @@ template <class T> struct Prec {
   T *M = nullptr;  // Jacobi: 1/diag(A); SOR: 1/(omega diag(A)); IC / ILU: the factor (nnz)
   T *t = nullptr;  // IC / ILU: intermediate vector of the two triangular solves
   Csr *A = nullptr;
-  const void *val = nullptr;
+  TriOp fwd, bwd;  // SOR / IC / ILU: the forward and backward sweeps
 };
 
 // Jacobi::create_precond (jacobi.cpp:12-29): M = 1/diag(A); SOR::
@@ -122,13 +122,21 @@ Prec<T> make_precond(Csr *A, const void *val, PrecSpec spec) {
   Prec<T> P;
   P.kind = spec.kind;
   P.A = A;
-  P.val = val;
   Ctx *c = A->ctx;
+  const mgx_dtype dt = DT<T>::v;
   if (spec.kind == MGX_PRECOND_IC || spec.kind == MGX_PRECOND_ILU) {
     P.M = (T *)ws_get(c, (size_t)std::max<int64_t>(A->nnz, 1) * sizeof(T));
     P.t = (T *)ws_get(c, (size_t)std::max<int64_t>(A->rows, 1) * sizeof(T));
-    launch_copy(c, DT<T>::v, A->nnz, val, P.M);
-    ilu_factor(A, DT<T>::v, spec.kind, P.M);
+    launch_copy(c, dt, A->nnz, val, P.M);
+    ilu_factor(A, dt, spec.kind, P.M);
+    const int32_t *diag = ilu_diag(A);
+    if (spec.kind == MGX_PRECOND_IC) {
+      P.fwd = tri_prepare(A, tri_plan(A, kTriLower), dt, kTriDiv, P.M, P.M, diag);
+      P.bwd = tri_prepare(A, A->tri[kTriLt], dt, kTriDiv, P.M, P.M, diag);
+    } else {
+      P.fwd = tri_prepare(A, tri_plan(A, kTriLower), dt, kTriUnit, P.M, P.M, diag);
+      P.bwd = tri_prepare(A, tri_plan(A, kTriUpper), dt, kTriDiv, P.M, P.M, diag);
+    }
     return P;
   }
   if (spec.kind != MGX_PRECOND_JACOBI && spec.kind != MGX_PRECOND_SOR) return P;
@@ -137,6 +145,10 @@ Prec<T> make_precond(Csr *A, const void *val, PrecSpec spec) {
   if (spec.kind == MGX_PRECOND_SOR)
     launch_scal(c, DT<T>::v, A->rows, (double)(T)spec.omega, P.M);
   launch_reciprocal(c, DT<T>::v, A->rows, P.M, P.M);
+  if (spec.kind == MGX_PRECOND_SOR) {
+    P.fwd = tri_prepare(A, tri_plan(A, kTriLower), dt, kTriMul, val, P.M, nullptr);
+    P.bwd = tri_prepare(A, tri_plan(A, kTriUpper), dt, kTriSorBack, val, P.M, nullptr);
+  }
   return P;
 }
 
@@ -149,10 +161,11 @@ void apply_precond(Ctx *c, const Prec<T> &P, int64_t n, const T *r, T *z) {
   if (P.kind == MGX_PRECOND_JACOBI) {
     launch_vv(c, DT<T>::v, 2, n, P.M, r, z);
   } else if (P.kind == MGX_PRECOND_SOR) {
-    sor_sweep(P.A, P.val, DT<T>::v, 0, P.M, r, z);
-    sor_sweep(P.A, P.val, DT<T>::v, 2, P.M, nullptr, z);
+    tri_run(P.A, P.fwd, DT<T>::v, r, z);
+    tri_run(P.A, P.bwd, DT<T>::v, z, z);
   } else if (P.kind == MGX_PRECOND_IC || P.kind == MGX_PRECOND_ILU) {
-    ilu_apply(P.A, DT<T>::v, P.kind, P.M, r, P.t, z);
+    tri_run(P.A, P.fwd, DT<T>::v, r, P.t);
+    tri_run(P.A, P.bwd, DT<T>::v, P.t, z);
   } else {
     launch_copy(c, DT<T>::v, n, r, z);
   }
@@ -413,13 +426,14 @@ SolveOut sor_reference(Csr *A, T *x, const T *b, PrecSpec precond, double tol, i
   bnrm2 = (T)(1.0 / (double)bnrm2);                  // :119
   T nrm2 = 0;
   const Prec<T> M = make_precond<T>(A, A->val, precond);  // :122
+  const TriOp lower = tri_prepare(A, tri_plan(A, kTriLowerBreak), dt, kTriMul, A->val, d, nullptr);
   SolveOut o;
   for (int64_t iter = 0; iter < maxiter; iter++) {
     apply_precond(c, M, n, x, s);                   // :126
     launch_spmv(A, A->val, dt, s, t, 1, 0, false);  // :127
     launch_axpyz(c, dt, n, -1.0, t, b, r);          // :128
     nrm2 = nrm(r);                                  // :129
-    sor_sweep(A, A->val, dt, 1, d, r, t);           // :133
+    tri_run(A, lower, dt, r, t);                    // :133
     launch_axpy(c, dt, n, 1.0, t, x);               // :137
     nrm2 = nrm2 * bnrm2;                            // :139
     if (rhist && iter < rhist_len) rhist[iter] = (double)nrm2;

@@ -344,6 +344,7 @@ def test_sor_preconditioned_solvers_bit_exact(m, ctx, oracle, dtype, omega):
     say("sor precond", dtype.__name__, omega)
     cases = [("cg", m.laplacian2d(48, 40)), ("cg", m.stencil3d(12, 10, 9, 7)),
              ("bicgstab", m.convdiff2d(40, 20.0)), ("cg", _block_diag(6000, 3, 1)),
+             ("cg", _block_diag(2000, 3, 8)), ("bicgstab", _arrow(10000, sym=False)),
              ("jacobi", m.laplacian2d(16, 16))]
     oracle.set_sor_omega(omega)
     for method, (rp, ci, val) in cases:
@@ -386,12 +387,13 @@ def test_sor_solver_bit_exact(m, ctx, oracle, dtype):
     trid = (np.cumsum(rp).astype(np.int32), np.array(cols, np.int32), np.array(vals))
     unsorted = (np.array([0, 2, 4, 7], np.int32), np.array([0, 1, 1, 0, 2, 0, 1], np.int32),
                 np.array([3.0, -1.0, 3.0, -1.0, 3.0, -1.0, -1.0]))
-    cases = [m.laplacian2d(30, 30), trid, _block_diag(5000, 3, 2), unsorted]
+    cases = [m.laplacian2d(30, 30), trid, _block_diag(5000, 3, 2), _block_diag(1500, 3, 5),
+             _arrow(12000, sym=False), unsorted]
     for omega in (1.9, 1.2):
         oracle.set_sor_omega(omega)
         # the unsorted system diverges at 1.9 (the lower sweep skips its rows'
         # off-diagonal entries): run it where the iteration converges
-        for rp, ci, val in cases if omega < 1.5 else cases[:3]:
+        for rp, ci, val in cases if omega < 1.5 else cases[:-1]:
             val = val.astype(dtype)
             n = len(rp) - 1
             b = oracle.random_vector(n, -1.0, 1.0, 5, dtype=dtype)
@@ -443,9 +445,28 @@ def _random_sorted(n, density, seed, diag=6.0, sym=False):
     return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy()
 
 
+def _arrow(n, eps=1e-4, sym=True):
+    """tridiag(-1, 4, -1) with a coupling eps to row / column 0 (sym) or, with
+    rows of at most four entries, to column 0 below the diagonal and column
+    n-1 above it: rows far down the level order read a row more than 8192
+    positions back (the sweeps' reads outside their LDS copy), in the
+    forward and in the backward sweeps; n levels of one row each."""
+    import scipy.sparse as sp
+    M = sp.diags([-np.ones(n - 1), 4.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tolil()
+    M[1:, 0] = eps
+    if sym:
+        M[0, 1:] = eps
+    else:
+        M[:-1, n - 1] = eps
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy()
+
+
 def _factor_cases(m, kind):
     cases = [("poisson", m.laplacian2d(48, 40)), ("stencil27", m.stencil3d(14, 12, 10, 27)),
-             ("blocks", _block_diag(6000, 3, 3)),
+             ("blocks", _block_diag(6000, 3, 3)), ("blocks2k", _block_diag(2000, 3, 7)),
+             ("arrow", _arrow(20000)), ("arrow_short_rows", _arrow(20000, 1e-3, sym=False)),
              ("random_spd", _random_sorted(3000, 0.002, 5, diag=12.0, sym=True))]
     if kind == 5:
         cases += [("convdiff", m.convdiff2d(40, 20.0)),
