@@ -292,6 +292,19 @@ int mgx_ic_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                  const mgx_solve_params *p, mgx_solve_result *res);
 int mgx_ilu_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                   const mgx_solve_params *p, mgx_solve_result *res);
+/* standard_eigen::LOBPCG<CRS<T>,T>::solve (src/eigen/lobpcg_standard.cpp:
+ * 10-300) with initvec_scheme::USER: the m smallest eigenpairs of symmetric
+ * A (1 <= m <= 8, 3m <= n).  X holds m * n entries, row i = vector i: the
+ * initial vectors on entry, the eigenvectors on return; lambda (m host
+ * doubles) receives the eigenvalues.  Stops when the largest residual norm
+ * ||M (A x_i - lambda_i x_i)|| (M the preconditioner p->precond) is below
+ * p->tol (absolute) after p->miniter iterations; p->rhistory receives that
+ * largest residual per iteration.  The Gram matrices and the basis updates
+ * run on the device, the 3m x 3m eigenproblem on the host (the reference
+ * calls LAPACK sygvd there).  A Gram matrix that is not positive definite
+ * within the {w, x} block is an error (the reference loops) */
+int mgx_lobpcg_solve(mgx_csr *A, mgx_vec *X, int64_t m, double *lambda,
+                     const mgx_solve_params *p, mgx_solve_result *res);
 /* util::get_residual_l2 (src/utils/equation_utils.cpp:45-56): ||b - A x|| */
 int mgx_residual_l2(mgx_csr *A, mgx_vec *x, mgx_vec *b, double *out);
 

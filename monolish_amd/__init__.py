@@ -118,6 +118,7 @@ for _n in ("mgx_cg_solve", "mgx_bicgstab_solve", "mgx_jacobi_solve", "mgx_sor_so
            "mgx_ic_solve", "mgx_ilu_solve"):
     _sig(_n, [_P, _P, _P, C.POINTER(SolveParams), C.POINTER(SolveResultC)])
 _sig("mgx_residual_l2", [_P, _P, _P, C.POINTER(_D)])
+_sig("mgx_lobpcg_solve", [_P, _P, _I64, _P, C.POINTER(SolveParams), C.POINTER(SolveResultC)])
 _sig("mgx_ir_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64, C.POINTER(SolveResultC)])
 _sig("mgx_mixed_cg_solve", [_P, _P, _P, C.POINTER(SolveParams), _D, _I64,
                             C.POINTER(SolveResultC)])
@@ -559,6 +560,22 @@ def ilu_solve(A, x, b) -> SolveResult:
     with the ILU(0) factor of A."""
     return _solve(_lib.mgx_ilu_solve, A, x, b, 1e-8, 0, 0, PRECOND_DEFAULT, MODE_REFERENCE,
                   False, 0, 0)
+
+
+def lobpcg(A, X, m, tol=1e-8, maxiter=0, miniter=0, precond=PRECOND_DEFAULT, omega=1.9,
+           history=0):
+    """standard_eigen::LOBPCG<CRS<T>,T>::solve (src/eigen/lobpcg_standard.cpp)
+    with initvec_scheme::USER: X (a Vector of m * n entries, row i = vector
+    i) holds the initial vectors and receives the eigenvectors.  Returns
+    (SolveResult, eigenvalues as float64)."""
+    p, hist = _params(tol, maxiter, miniter, precond, MODE_REFERENCE, False, 0, history, omega)
+    lam = np.zeros(int(m))
+    r = SolveResultC()
+    rc = check(_lib.mgx_lobpcg_solve(A.handle, X.handle, int(m), lam.ctypes.data_as(C.c_void_p),
+                                     C.byref(p), C.byref(r)))
+    if hist is not None:
+        hist = hist[: min(r.final_iter, hist.size)]
+    return SolveResult(rc, r.final_iter, r.final_resid, r.seconds, 0, 0, hist), lam
 
 
 def ir_cg(A, x, b, tol=1e-10, tol_is_relative=True, inner_tol=1e-4, max_outer=100, maxiter=0,

@@ -1,0 +1,132 @@
+// lobpcg.hip -- device kernels of the LOBPCG eigensolver
+// (src/eigen/lobpcg_standard.cpp:10-300): the Gram matrices of the basis
+// and the basis update.
+//
+// The basis {w, x, p} (k = 2m or 3m vectors of n entries, one vector per
+// row, as the reference's Dense wxp) and its image {W, X, P} = A{w, x, p}
+// are read once per iteration by k_gram, which forms the lower triangles of
+// Sa = {W,X,P}{w,x,p}^T and Sb = {w,x,p}{w,x,p}^T (the reference's
+// matmul(WXP, twxp) and matmul(wxp, twxp)) from per-tile partial sums in a
+// fixed order.  k_lobpcg_combine forms the next x, p, X, P of all m pairs
+// from the previous basis in one pass (the reference's chain of axpy calls,
+// lobpcg_standard.cpp:220-238, in its order).
+#include "mgx_device.hpp"
+
+namespace mgx {
+
+namespace {
+
+constexpr int kGramTile = 128;  // entries of each vector per block
+
+// lower triangle of Sa (pairs 0..npair-1) and Sb (npair..2 npair-1), pair
+// (i, j), i >= j, numbered i (i + 1) / 2 + j
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gram(int64_t n, int k, const T *__restrict__ wxp,
+                                                 const T *__restrict__ WXP, double *part) {
+  extern __shared__ unsigned char smem[];
+  T *sv = reinterpret_cast<T *>(smem);  // k rows of wxp, then k rows of WXP
+  const int64_t e0 = (int64_t)blockIdx.x * kGramTile;
+  const int len = (int)(n - e0 < kGramTile ? n - e0 : kGramTile);
+  for (int q = threadIdx.x; q < 2 * k * kGramTile; q += kBlock) {
+    const int r = q / kGramTile, e = q % kGramTile;
+    const T *src = r < k ? wxp + (int64_t)r * n : WXP + (int64_t)(r - k) * n;
+    sv[q] = e < len ? src[e0 + e] : (T)0;
+  }
+  __syncthreads();
+  const int npair = k * (k + 1) / 2;
+  for (int o = threadIdx.x; o < 2 * npair; o += kBlock) {
+    const int pr = o < npair ? o : o - npair;
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= pr) i++;
+    const int j = pr - i * (i + 1) / 2;
+    const T *a = o < npair ? sv + (k + i) * kGramTile : sv + i * kGramTile;
+    const T *b = sv + j * kGramTile;
+    double s = 0.0;
+    for (int e = 0; e < len; e++) s += (double)(a[e] * b[e]);
+    part[(int64_t)blockIdx.x * 2 * npair + o] = s;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(int64_t nblk, int nout,
+                                                        const double *__restrict__ part,
+                                                        double *out) {
+  const int o = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (o >= nout) return;
+  double s = 0.0;
+  for (int64_t b = 0; b < nblk; b++) s += part[b * nout + o];
+  out[o] = s;
+}
+
+// coef[i * k + j] (k = the row stride, 3m): coefficient j (of w_j,
+// x_{j-m}, p_{j-2m}) of pair i; has_p: the basis holds p
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_lobpcg_combine(
+    int64_t n, int m, int k, bool has_p, const T *__restrict__ coef, const T *__restrict__ wxp_p,
+    const T *__restrict__ WXP_p, T *wxp, T *WXP) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  for (int i = 0; i < m; i++) {
+    const T *b = coef + (int64_t)i * k;
+    T p = 0, x = 0, P = 0, X = 0;
+    for (int j = 0; j < m; j++) {
+      const T bw = b[j], bx = b[m + j];
+      const T wj = wxp_p[(int64_t)j * n + e], Wj = WXP_p[(int64_t)j * n + e];
+      const T xj = wxp_p[(int64_t)(m + j) * n + e], Xj = WXP_p[(int64_t)(m + j) * n + e];
+      p = p + bw * wj;
+      x = x + bw * wj;
+      P = P + bw * Wj;
+      X = X + bw * Wj;
+      if (has_p) {
+        const T bp = b[2 * m + j];
+        const T pj = wxp_p[(int64_t)(2 * m + j) * n + e], Pj = WXP_p[(int64_t)(2 * m + j) * n + e];
+        p = p + bp * pj;
+        x = x + bp * pj;
+        P = P + bp * Pj;
+        X = X + bp * Pj;
+      }
+      x = x + bx * xj;
+      X = X + bx * Xj;
+    }
+    wxp[(int64_t)(m + i) * n + e] = x;
+    wxp[(int64_t)(2 * m + i) * n + e] = p;
+    WXP[(int64_t)(m + i) * n + e] = X;
+    WXP[(int64_t)(2 * m + i) * n + e] = P;
+  }
+}
+
+}  // namespace
+
+void lobpcg_gram(Ctx *c, mgx_dtype dt, int64_t n, int k, const void *wxp, const void *WXP,
+                 double *part, double *out) {
+  const int64_t nblk = (n + kGramTile - 1) / kGramTile;
+  const size_t es = dt == MGX_F64 ? 8 : 4;
+  const size_t lds = (size_t)2 * k * kGramTile * es;
+  const int nout = k * (k + 1);
+  if (dt == MGX_F64)
+    k_gram<double><<<(unsigned)nblk, kBlock, lds, c->stream>>>(n, k, (const double *)wxp,
+                                                               (const double *)WXP, part);
+  else
+    k_gram<float><<<(unsigned)nblk, kBlock, lds, c->stream>>>(n, k, (const float *)wxp,
+                                                              (const float *)WXP, part);
+  k_gram_reduce<double><<<(unsigned)((nout + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
+      nblk, nout, part, out);
+  MGX_HIP(hipGetLastError());
+}
+
+void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,
+                    const void *coef, const void *wxp_p, const void *WXP_p, void *wxp,
+                    void *WXP) {
+  const unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
+  if (dt == MGX_F64)
+    k_lobpcg_combine<double><<<g, kBlock, 0, c->stream>>>(
+        n, m, stride, has_p, (const double *)coef, (const double *)wxp_p,
+        (const double *)WXP_p, (double *)wxp, (double *)WXP);
+  else
+    k_lobpcg_combine<float><<<g, kBlock, 0, c->stream>>>(
+        n, m, stride, has_p, (const float *)coef, (const float *)wxp_p, (const float *)WXP_p,
+        (float *)wxp, (float *)WXP);
+  MGX_HIP(hipGetLastError());
+}
+
+}  // namespace mgx

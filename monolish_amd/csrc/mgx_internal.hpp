@@ -335,6 +335,17 @@ void tri_free(Csr *A);
 void ilu_factor(Csr *A, mgx_dtype dt, int kind, void *f);
 const int32_t *ilu_diag(Csr *A);
 void ilu_free(Csr *A);
+// LOBPCG: out (k (k + 1) doubles) = the lower triangles of Sa = WXP wxp^T
+// and Sb = wxp wxp^T, row-major by pair (i, j), i >= j (Sa first); part
+// holds ceil(n / 128) * k (k + 1) doubles of tile partials
+void lobpcg_gram(Ctx *c, mgx_dtype dt, int64_t n, int k, const void *wxp, const void *WXP,
+                 double *part, double *out);
+// rows m..2m-1 (x) and 2m..3m-1 (p) of wxp and WXP from the previous basis
+// wxp_p, WXP_p with coef (m rows of `stride` coefficients, device); has_p:
+// the previous basis holds p
+void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,
+                    const void *coef, const void *wxp_p, const void *WXP_p, void *wxp,
+                    void *WXP);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);
 

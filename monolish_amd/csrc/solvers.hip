@@ -710,6 +710,240 @@ SolveOut mp_cg(Csr *A, double *x, const double *b, int precond, double thr,
 
 using namespace mgx;
 
+namespace {
+
+// ---- LOBPCG (src/eigen/lobpcg_standard.cpp:10-300) ------------------------
+// The small generalised eigenproblem Sa v = lambda Sb v (k <= 24) of each
+// iteration is solved on the host, as the reference does with LAPACK's
+// sygvd (jobz 'V', uplo 'L', itype 1): Cholesky Sb = L L^T, the cyclic
+// Jacobi method on C = L^-1 Sa L^-T (Golub & Van Loan, Matrix
+// Computations, 4th ed., Alg. 8.5.3), eigenvalues ascending, v = L^-T q so
+// that v^T Sb v = 1.  Each eigenvector is signed so that its largest
+// component is positive.
+
+// in-place Cholesky of the lower triangle of B (row-major k x k); returns 0,
+// or the order of the first leading minor that is not positive definite
+int chol_lower(int k, std::vector<double> &B) {
+  for (int j = 0; j < k; j++) {
+    double d = B[j * k + j];
+    for (int q = 0; q < j; q++) d -= B[j * k + q] * B[j * k + q];
+    if (!(d > 0.0)) return j + 1;
+    d = std::sqrt(d);
+    B[j * k + j] = d;
+    for (int i = j + 1; i < k; i++) {
+      double s = B[i * k + j];
+      for (int q = 0; q < j; q++) s -= B[i * k + q] * B[j * k + q];
+      B[i * k + j] = s / d;
+    }
+  }
+  return 0;
+}
+
+// eigenpairs of the symmetric C (full, row-major); w ascending, V row i =
+// eigenvector i
+void jacobi_eig(int k, std::vector<double> C, std::vector<double> &w, std::vector<double> &V) {
+  std::vector<double> Q(k * k, 0.0);
+  for (int i = 0; i < k; i++) Q[i * k + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < k; i++)
+      for (int j = 0; j < k; j++) {
+        tot += C[i * k + j] * C[i * k + j];
+        if (i != j) off += C[i * k + j] * C[i * k + j];
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < k - 1; p++)
+      for (int q = p + 1; q < k; q++) {
+        const double apq = C[p * k + q];
+        if (apq == 0.0) continue;
+        const double theta = (C[q * k + q] - C[p * k + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int r = 0; r < k; r++) {  // C = J^T C J
+          const double crp = C[r * k + p], crq = C[r * k + q];
+          C[r * k + p] = cs * crp - sn * crq;
+          C[r * k + q] = sn * crp + cs * crq;
+        }
+        for (int r = 0; r < k; r++) {
+          const double cpr = C[p * k + r], cqr = C[q * k + r];
+          C[p * k + r] = cs * cpr - sn * cqr;
+          C[q * k + r] = sn * cpr + cs * cqr;
+        }
+        for (int r = 0; r < k; r++) {  // Q = Q J
+          const double qrp = Q[r * k + p], qrq = Q[r * k + q];
+          Q[r * k + p] = cs * qrp - sn * qrq;
+          Q[r * k + q] = sn * qrp + cs * qrq;
+        }
+      }
+  }
+  std::vector<int> idx(k);
+  for (int i = 0; i < k; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int a, int b) { return C[a * k + a] < C[b * k + b]; });
+  w.resize(k);
+  V.assign(k * k, 0.0);
+  for (int i = 0; i < k; i++) {
+    w[i] = C[idx[i] * k + idx[i]];
+    for (int r = 0; r < k; r++) V[i * k + r] = Q[r * k + idx[i]];
+  }
+}
+
+// Sa v = lambda Sb v from the lower triangles; returns 0 or the Cholesky
+// failure order of Sb
+int sygv_lower(int k, const std::vector<double> &Sa, std::vector<double> Sb,
+               std::vector<double> &w, std::vector<double> &V) {
+  const int q = chol_lower(k, Sb);
+  if (q) return q;
+  // C = L^-1 Sa L^-T with Sa symmetric from its lower triangle
+  std::vector<double> S(k * k), Y(k * k), C(k * k);
+  for (int i = 0; i < k; i++)
+    for (int j = 0; j < k; j++) S[i * k + j] = i >= j ? Sa[i * k + j] : Sa[j * k + i];
+  for (int c = 0; c < k; c++)  // Y = L^-1 S (column by column)
+    for (int i = 0; i < k; i++) {
+      double s = S[i * k + c];
+      for (int r = 0; r < i; r++) s -= Sb[i * k + r] * Y[r * k + c];
+      Y[i * k + c] = s / Sb[i * k + i];
+    }
+  for (int r = 0; r < k; r++)  // C^T = L^-1 Y^T, C symmetric
+    for (int i = 0; i < k; i++) {
+      double s = Y[r * k + i];
+      for (int q2 = 0; q2 < i; q2++) s -= Sb[i * k + q2] * C[r * k + q2];
+      C[r * k + i] = s / Sb[i * k + i];
+    }
+  for (int i = 0; i < k; i++)
+    for (int j = 0; j < i; j++) C[i * k + j] = C[j * k + i] = 0.5 * (C[i * k + j] + C[j * k + i]);
+  std::vector<double> Qv;
+  jacobi_eig(k, C, w, Qv);
+  V.assign(k * k, 0.0);
+  for (int e = 0; e < k; e++) {  // v = L^-T q
+    for (int i = k - 1; i >= 0; i--) {
+      double s = Qv[e * k + i];
+      for (int r = i + 1; r < k; r++) s -= Sb[r * k + i] * V[e * k + r];
+      V[e * k + i] = s / Sb[i * k + i];
+    }
+    int big = 0;
+    for (int i = 1; i < k; i++)
+      if (std::fabs(V[e * k + i]) > std::fabs(V[e * k + big])) big = i;
+    if (V[e * k + big] < 0)
+      for (int i = 0; i < k; i++) V[e * k + i] = -V[e * k + i];
+  }
+  return 0;
+}
+
+template <class T>
+SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond, double tol,
+                    int64_t miniter, int64_t maxiter, double *rhist, int64_t rhist_len) {
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const mgx_dtype dt = DT<T>::v;
+  const int k3 = (int)(3 * m);
+  const size_t blk = (size_t)k3 * n * sizeof(T);
+  T *wxp = (T *)ws_get(c, blk), *WXP = (T *)ws_get(c, blk), *wxp_p = (T *)ws_get(c, blk),
+    *WXP_p = (T *)ws_get(c, blk);
+  T *v1 = (T *)ws_get(c, (size_t)n * sizeof(T)), *v2 = (T *)ws_get(c, (size_t)n * sizeof(T));
+  const int64_t ntile = (n + 127) / 128;
+  double *part = (double *)ws_get(c, (size_t)ntile * k3 * (k3 + 1) * 8);
+  double *gout = (double *)ws_get(c, (size_t)k3 * (k3 + 1) * 8);
+  T *coef = (T *)ws_get(c, (size_t)m * k3 * sizeof(T));
+  for (T *b : {wxp, WXP, wxp_p, WXP_p}) launch_fill(c, dt, b, (int64_t)k3 * n, 0.0);
+  const Prec<T> M = make_precond<T>(A, A->val, precond);  // :24
+  auto nrm = [&](const T *v) { return host_sqrt(dot_t(c, n, v, v)); };
+  auto row = [&](T *base, int64_t r) { return base + r * n; };
+  // initvec_scheme::USER: x = the caller's vectors (:93-97)
+  launch_copy(c, dt, m * n, Xio, row(wxp, m));
+  for (int64_t i = 0; i < m; i++) {  // :103-113
+    T *x = row(wxp, m + i), *X = row(WXP, m + i), *w = row(wxp, i), *W = row(WXP, i);
+    launch_spmv(A, A->val, dt, x, X, 1, 0, false);
+    const T mu = dot_t(c, n, x, X);
+    launch_axpyz(c, dt, n, -(double)mu, x, X, w);
+    const T norm = nrm(w);
+    launch_scal(c, dt, n, (double)(T)(1.0 / (double)norm), w);
+    launch_spmv(A, A->val, dt, w, W, 1, 0, false);
+  }
+  auto out_x = [&] { launch_copy(c, dt, m * n, row(wxp, m), Xio); };
+  bool singular = false;
+  std::vector<double> g(k3 * (k3 + 1)), w, V;
+  std::vector<T> hcoef(m * k3);
+  std::vector<T> residual(m);
+  T resid = 0;
+  SolveOut o;
+  for (int64_t iter = 0; iter < maxiter; iter++) {
+    const int k = (iter == 0 || singular) ? (int)(2 * m) : k3;  // :136-149
+    lobpcg_gram(c, dt, n, k, wxp, WXP, part, gout);              // :156-160
+    MGX_HIP(hipMemcpyAsync(g.data(), gout, (size_t)k * (k + 1) * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+    MGX_HIP(hipStreamSynchronize(c->stream));
+    const int npair = k * (k + 1) / 2;
+    std::vector<double> Sa(k * k, 0.0), Sb(k * k, 0.0);
+    for (int i = 0; i < k; i++)
+      for (int j = 0; j <= i; j++) {
+        Sa[i * k + j] = g[i * (i + 1) / 2 + j];
+        Sb[i * k + j] = g[npair + i * (i + 1) / 2 + j];
+      }
+    const int q = sygv_lower(k, Sa, Sb, w, V);  // :170-172
+    if (q > 0) {
+      // a leading minor of Sb past the {w, x} block: the p block lost rank;
+      // restart the step without it (:177-186)
+      MGX_REQUIRE(q > k - m && k == k3,
+                  "LOBPCG: the basis Gram matrix is not positive definite (order %d of %d)", q,
+                  k);
+      singular = true;
+      iter--;
+      continue;
+    }
+    launch_copy(c, dt, (int64_t)k * n, wxp, wxp_p);  // :197-198
+    launch_copy(c, dt, (int64_t)k * n, WXP, WXP_p);
+    for (int64_t i = 0; i < m; i++) {
+      lambda[i] = (double)(T)w[i];  // :203
+      for (int j = 0; j < k3; j++) hcoef[i * k3 + j] = j < k ? (T)V[i * k + j] : (T)0;
+    }
+    MGX_HIP(hipMemcpyAsync(coef, hcoef.data(), hcoef.size() * sizeof(T), hipMemcpyHostToDevice,
+                           c->stream));
+    lobpcg_combine(c, dt, n, (int)m, k3, k == k3, coef, wxp_p, WXP_p, wxp, WXP);  // :220-238
+    for (int64_t i = 0; i < m; i++) {
+      T *x = row(wxp, m + i), *X = row(WXP, m + i), *pv = row(wxp, 2 * m + i),
+        *P = row(WXP, 2 * m + i), *wv = row(wxp, i), *W = row(WXP, i);
+      const T normp = nrm(pv);  // :240-244
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp), pv);
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp), P);
+      const T normx = nrm(x);
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx), x);
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx), X);
+      launch_axpyz(c, dt, n, -(double)(T)lambda[i], x, X, wv);  // :248
+      launch_copy(c, dt, n, wv, v2);                             // :250-252
+      apply_precond(c, M, n, v2, v1);
+      launch_copy(c, dt, n, v1, wv);
+      residual[i] = nrm(wv);  // :255
+      if (std::isnan(residual[i])) {  // :260-270
+        out_x();
+        o.status = MGX_SOLVER_RESIDUAL_NAN;
+        o.iters = iter + 1;
+        o.resid = (double)residual[i];
+        return o;
+      }
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)residual[i]), wv);  // :272
+      launch_spmv(A, A->val, dt, wv, W, 1, 0, false);                      // :274
+    }
+    resid = *std::max_element(residual.begin(), residual.end());  // :277
+    if (rhist && iter < rhist_len) rhist[iter] = (double)resid;
+    if ((double)resid < tol && miniter < iter + 1) {  // :279-287
+      out_x();
+      o.status = MGX_OK;
+      o.iters = iter + 1;
+      o.resid = (double)resid;
+      return o;
+    }
+    singular = false;
+  }
+  out_x();
+  o.status = MGX_SOLVER_MAXITER;
+  o.iters = maxiter;
+  o.resid = (double)resid;
+  return o;
+}
+
+}  // namespace
+
 extern "C" {
 
 void mgx_solve_params_default(mgx_solve_params *p) {
@@ -834,6 +1068,32 @@ int mgx_ic_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
 int mgx_ilu_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
                   mgx_solve_result *res) {
   return factor_solve(A, x, b, p, res, MGX_PRECOND_ILU, "mgx_ilu_solve");
+}
+
+int mgx_lobpcg_solve(mgx_csr *A, mgx_vec *X, int64_t m, double *lambda,
+                     const mgx_solve_params *p, mgx_solve_result *res) {
+  return guard([&] {
+    MGX_REQUIRE(A && X && lambda && p && res, "mgx_lobpcg_solve: NULL argument");
+    MGX_REQUIRE(A->rows == A->cols, "mgx_lobpcg_solve: error, A.row != A.col");
+    MGX_REQUIRE(m >= 1 && m <= 8, "mgx_lobpcg_solve: 1 <= m <= 8 eigenpairs");
+    MGX_REQUIRE(3 * m <= A->rows, "mgx_lobpcg_solve: needs 3 m <= n");
+    MGX_REQUIRE(X->ctx == A->ctx && X->dtype == A->dtype && X->n == m * A->rows,
+                "mgx_lobpcg_solve: X must hold m * n entries of A's dtype");
+    MGX_REQUIRE(p->precond >= MGX_PRECOND_DEFAULT && p->precond <= MGX_PRECOND_ILU,
+                "mgx_lobpcg_solve: unknown preconditioner %d", p->precond);
+    MGX_REQUIRE(p->precond != MGX_PRECOND_SOR || (p->omega > 0 && p->omega < 2),
+                "mgx_lobpcg_solve: SOR omega must be in (0, 2)");
+    activate(A->ctx);
+    const int64_t mi = maxiter_of(p);
+    const PrecSpec ps{p->precond, p->omega};
+    return run_solver(A, res, [&] {
+      if (A->dtype == MGX_F64)
+        return lobpcg_run<double>(A, (double *)X->d, m, lambda, ps, p->tol, p->miniter, mi,
+                                  p->rhistory, p->rhistory_len);
+      return lobpcg_run<float>(A, (float *)X->d, m, lambda, ps, p->tol, p->miniter, mi,
+                               p->rhistory, p->rhistory_len);
+    });
+  });
 }
 
 int mgx_sor_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,

@@ -345,3 +345,136 @@ def ilu_precond(rp, ci, val, kind, r):
 def factor_bad_row() -> int:
     """Row at which the last IC / ILU set-up failed (-1: none)."""
     return int(lib().or_factor_bad_row())
+
+
+# ---------------------------------------------------------------- LOBPCG ----
+def _chol_fail_order(B):
+    """Order (1-based) of the first leading minor of B (lower triangle) that
+    is not positive definite, 0 when B is positive definite."""
+    k = B.shape[0]
+    L = np.zeros_like(B)
+    for j in range(k):
+        d = B[j, j] - np.dot(L[j, :j], L[j, :j])
+        if not d > 0:
+            return j + 1
+        L[j, j] = np.sqrt(d)
+        for i in range(j + 1, k):
+            L[i, j] = (B[i, j] - np.dot(L[i, :j], L[j, :j])) / L[j, j]
+    return 0
+
+
+def lobpcg(rp, ci, val, X0, m, tol=1e-8, maxiter=1000, miniter=0, precond=PRECOND_DEFAULT):
+    """standard_eigen::LOBPCG<CRS<T>,T>::solve (src/eigen/lobpcg_standard.cpp:
+    10-300) with initvec_scheme::USER, restated with numpy: vectors in T,
+    Gram matrices in double, the 3m x 3m problem Sa v = lambda Sb v by
+    scipy.linalg.eigh (LAPACK's sygvd, uplo 'L', as the reference), each
+    eigenvector signed so that its largest component is positive.
+    Preconditioners: default (z = r), none (copy) and Jacobi.  Returns
+    (status, eigenvalues, X (m x n), iterations, max residual, history)."""
+    import scipy.linalg as sla
+    dt = val.dtype
+    n = len(rp) - 1
+    T = dt.type
+    Minv = None
+    if precond == PRECOND_JACOBI:
+        d = np.zeros(n, dt)
+        lib_diag = getattr(lib(), f"or_crs_diag_{_sfx(dt)}")
+        lib_diag(n, n, _ptr(_c(rp, np.int32)), _ptr(_c(ci, np.int32)), _ptr(val), _ptr(d))
+        Minv = (T(1) / d).astype(dt)
+    mv = lambda v: spmv(rp, ci, val, v.astype(dt))
+    nrm = lambda v: T(np.sqrt(T(np.dot(v, v))))
+    wxp = np.zeros((3 * m, n), dt)
+    WXP = np.zeros((3 * m, n), dt)
+    wxp[m:2 * m] = np.asarray(X0, dt).reshape(m, n)
+    for i in range(m):
+        x = wxp[m + i]
+        WXP[m + i] = mv(x)
+        mu = T(np.dot(x, WXP[m + i]))
+        wxp[i] = (WXP[m + i] - mu * x).astype(dt)
+        wxp[i] = (wxp[i] * T(1.0 / float(nrm(wxp[i])))).astype(dt)
+        WXP[i] = mv(wxp[i])
+    singular = False
+    hist = []
+    resid = T(0)
+    lam = np.zeros(m)
+    it = 0
+    while it < maxiter:
+        k = 2 * m if (it == 0 or singular) else 3 * m
+        a = wxp[:k].astype(np.float64)
+        A_ = WXP[:k].astype(np.float64)
+        Sa = A_ @ a.T
+        Sb = a @ a.T
+        Sa = np.tril(Sa) + np.tril(Sa, -1).T
+        Sb = np.tril(Sb) + np.tril(Sb, -1).T
+        q = _chol_fail_order(Sb)
+        if q:
+            if q > k - m and k == 3 * m:
+                singular = True
+                continue
+            raise RuntimeError("LOBPCG: Gram matrix not positive definite")
+        w, V = sla.eigh(Sa, Sb, lower=True)
+        V = V.T.copy()
+        for e in range(k):
+            big = int(np.argmax(np.abs(V[e])))
+            if V[e, big] < 0:
+                V[e] = -V[e]
+        wxp_p = wxp[:k].copy()
+        WXP_p = WXP[:k].copy()
+        residual = np.zeros(m, dt)
+        for i in range(m):
+            lam[i] = float(T(w[i]))
+            b = np.zeros(3 * m, dt)
+            b[:k] = V[i].astype(dt)
+            p = np.zeros(n, dt)
+            x = np.zeros(n, dt)
+            P = np.zeros(n, dt)
+            X = np.zeros(n, dt)
+            for j in range(m):
+                p = p + b[j] * wxp_p[j]
+                x = x + b[j] * wxp_p[j]
+                P = P + b[j] * WXP_p[j]
+                X = X + b[j] * WXP_p[j]
+                if k == 3 * m:
+                    p = p + b[2 * m + j] * wxp_p[2 * m + j]
+                    x = x + b[2 * m + j] * wxp_p[2 * m + j]
+                    P = P + b[2 * m + j] * WXP_p[2 * m + j]
+                    X = X + b[2 * m + j] * WXP_p[2 * m + j]
+                x = x + b[m + j] * wxp_p[m + j]
+                X = X + b[m + j] * WXP_p[m + j]
+            sp = T(1.0 / float(nrm(p)))
+            p, P = (p * sp).astype(dt), (P * sp).astype(dt)
+            sx = T(1.0 / float(nrm(x)))
+            x, X = (x * sx).astype(dt), (X * sx).astype(dt)
+            wv = (X - T(lam[i]) * x).astype(dt)
+            if Minv is not None:
+                wv = (Minv * wv).astype(dt)
+            residual[i] = nrm(wv)
+            wxp[m + i], wxp[2 * m + i], WXP[m + i], WXP[2 * m + i] = x, p, X, P
+            if np.isnan(residual[i]):
+                return RESIDUAL_NAN, lam, wxp[m:2 * m].copy(), it + 1, float(residual[i]), hist
+            wxp[i] = (wv * T(1.0 / float(residual[i]))).astype(dt)
+            WXP[i] = mv(wxp[i])
+        resid = residual.max()
+        hist.append(float(resid))
+        if float(resid) < tol and miniter < it + 1:
+            return SUCCESS, lam, wxp[m:2 * m].copy(), it + 1, float(resid), hist
+        singular = False
+        it += 1
+    return MAXITER, lam, wxp[m:2 * m].copy(), maxiter, float(resid), hist
+
+
+def tridiagonal_toeplitz(n, a, b):
+    """util::tridiagonal_toeplitz_matrix (src/utils/create_matrix/
+    tridiagonal_toeplitz.cpp:6-23) as CRS arrays, and its closed-form
+    eigenvalues (:30-33): a - 2|b| cos(pi (k+1) / (n+1))."""
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                rows.append(i)
+                cols.append(j)
+                vals.append(a if i == j else b)
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    ev = a - 2.0 * abs(b) * np.cos(np.pi * (np.arange(n) + 1) / (n + 1))
+    return np.cumsum(rp).astype(np.int32), np.array(cols, np.int32), np.array(vals), ev

@@ -684,3 +684,58 @@ def test_ilu_preconditioner_cuts_iterations(oracle):
     it = {k: oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=k, tol=1e-10)[2]
           for k in (2, 4, 5)}
     assert it[4] < 0.75 * it[2] and abs(it[4] - it[5]) <= 1
+
+
+# ---- LOBPCG -----------------------------------------------------------------
+def _x0(m, n):
+    # test/eigen/LOBPCG/lobpcg.cpp:97-99: x(i, (7 i) mod n) = 1
+    X0 = np.zeros((m, n))
+    for i in range(m):
+        X0[i, (i * 7) % n] = 1.0
+    return X0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("precond", [0, 2])
+def test_lobpcg_reference_harness(oracle, dtype, precond):
+    """test/eigen/LOBPCG/lobpcg.cpp:268-282: tridiagonal Toeplitz (100, 10,
+    -1), 2 eigenpairs, residual tol 1e-5 * 100, maxiter 100; eigenvalues
+    within 1e-3 (relative) of tridiagonal_toeplitz_matrix_eigenvalue."""
+    rp, ci, val, ev = oracle.tridiagonal_toeplitz(100, 10.0, -1.0)
+    st, lam, X, it, res, _ = oracle.lobpcg(rp, ci, val.astype(dtype), _x0(2, 100), 2,
+                                           tol=1e-5 * 100, maxiter=100, precond=precond)
+    assert st == 0
+    np.testing.assert_allclose(lam, ev[:2], rtol=1e-3)
+
+
+def test_lobpcg_laplacian_closed_form(oracle):
+    """The four smallest eigenvalues of the 2D 5-point Laplacian 16 x 12
+    (4 - 2 cos(i pi / 17) - 2 cos(j pi / 13)) and the eigenvector residuals
+    ||A x - lambda x|| from a tight solve; the vectors are orthonormal."""
+    import monolish_amd as m
+    rp, ci, val = m.laplacian2d(16, 12)
+    n = len(rp) - 1
+    exact = np.sort([4 - 2 * np.cos(i * np.pi / 17) - 2 * np.cos(j * np.pi / 13)
+                     for i in range(1, 17) for j in range(1, 13)])[:4]
+    X0 = np.random.default_rng(3).uniform(0, 1, (4, n))
+    st, lam, X, it, res, hist = oracle.lobpcg(rp, ci, val, X0, 4, tol=1e-9, maxiter=500)
+    assert st == 0 and res < 1e-9
+    np.testing.assert_allclose(lam, exact, rtol=1e-12)
+    for i in range(4):
+        r = oracle.spmv(rp, ci, val, X[i]) - lam[i] * X[i]
+        assert np.linalg.norm(r) < 1e-8
+    np.testing.assert_allclose(X @ X.T, np.eye(4), atol=1e-8)
+    assert hist[-1] == res and len(hist) == it
+
+
+def test_lobpcg_matches_eigsh(oracle):
+    """A random sparse SPD matrix: eigenvalues against scipy's eigsh (ARPACK)."""
+    n = 400
+    M = sp.random(n, n, density=0.02, random_state=11, format="csr")
+    M = sp.csr_matrix(M + M.T + sp.diags(np.linspace(1, 40, n)))
+    M.sort_indices()
+    rp, ci, val = M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data
+    ref = np.sort(spla.eigsh(M, k=3, which="SA")[0])
+    st, lam, X, it, res, _ = oracle.lobpcg(rp, ci, val, _x0(3, n), 3, tol=1e-8, maxiter=2000)
+    assert st == 0
+    np.testing.assert_allclose(lam, ref, rtol=1e-10)
