@@ -234,6 +234,48 @@ def bench_sor(m, ctx, reps, with_cpu):
     return out
 
 
+def bench_eigen(m, ctx, with_cpu):
+    """LOBPCG, 10 smallest eigenpairs of the 2D 5-point Laplacian 128x128
+    (the reference's SEVP benchmark shape: 10 pairs, residual tol 1e-4,
+    x(i, 7i mod n) = 1), without and with the Jacobi preconditioner."""
+    rp, ci, val = m.laplacian2d(128, 128)
+    n = len(rp) - 1
+    k = 10
+    X0 = np.zeros((k, n))
+    for i in range(k):
+        X0[i, (i * 7) % n] = 1.0
+    exact = np.sort([4 - 2 * np.cos(i * np.pi / 129) - 2 * np.cos(j * np.pi / 129)
+                     for i in range(1, 20) for j in range(1, 20)])[:k]
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    X = m.Vector(ctx, n=k * n)
+    out = {"workload": "LOBPCG 10 smallest eigenpairs, 2D 5-point Laplacian 128x128 "
+                       "(n=16384), tol 1e-4, x0(i, 7i mod n) = 1"}
+    for name, prec in (("none", m.PRECOND_DEFAULT), ("jacobi", m.PRECOND_JACOBI)):
+        X.upload(X0.reshape(-1))
+        ctx.sync()
+        t0 = time.perf_counter()
+        res, lam = m.lobpcg(A, X, k, tol=1e-4, maxiter=n, precond=prec)
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": round(dt, 3), "iterations": res.final_iter, "status": res.status,
+                     "ms_per_iteration": round(1e3 * dt / max(res.final_iter, 1), 3),
+                     "max_rel_eig_error": float(np.max(np.abs(lam - exact) / exact))}
+        log(f"eigen {name}: {out[name]}")
+    if with_cpu:
+        import oracle
+        it = 40
+        t0 = time.perf_counter()
+        oracle.lobpcg(rp, ci, val, X0, k, tol=0.0, maxiter=it)
+        dt = time.perf_counter() - t0
+        out["cpu"] = {"kind": "port", "threads": oracle.num_threads(),
+                      "ms_per_iteration": round(1e3 * dt / it, 3),
+                      "sample": f"first {it} iterations of the unpreconditioned solve "
+                                "(numpy restatement; SpMV in the oracle's C loop)"}
+        log(f"eigen cpu: {out['cpu']}")
+    A.free()
+    X.free()
+    return out
+
+
 def bench_mixed(m, ctx, side, reps):
     rp, ci, val = m.stencil3d(side, side, side, 27, 1e-2)
     n = len(rp) - 1
@@ -589,7 +631,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip the sweep, Krylov and CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--only", default="", help="comma list of legs: cpu,sweep,krylov,mixed,"
-                                                 "batched,served,ingest,sor (default: all)")
+                                                 "batched,served,ingest,sor,eigen (default: all)")
     ap.add_argument("--batch-systems", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-nmax", type=int, default=100000,
@@ -658,7 +700,7 @@ def main():
         "roofline": roofline,
     }
     legs = set(args.only.split(",")) if args.only else {"cpu", "sweep", "krylov", "mixed", "batched",
-                                                         "served", "ingest", "sor"}
+                                                         "served", "ingest", "sor", "eigen"}
     if args.quick:
         legs = set()
     if "cpu" in legs:
@@ -694,6 +736,8 @@ def main():
         line["ingest"] = bench_ingest(m, ctx, 128, "cpu" in legs)
     if "sor" in legs:
         line["sor"] = bench_sor(m, ctx, 3, "cpu" in legs)
+    if "eigen" in legs:
+        line["eigen"] = bench_eigen(m, ctx, "cpu" in legs)
     line["device"] = info["name"]
     print(json.dumps(line))
     ctx.close()

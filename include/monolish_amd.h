@@ -294,7 +294,7 @@ int mgx_ilu_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                   const mgx_solve_params *p, mgx_solve_result *res);
 /* standard_eigen::LOBPCG<CRS<T>,T>::solve (src/eigen/lobpcg_standard.cpp:
  * 10-300) with initvec_scheme::USER: the m smallest eigenpairs of symmetric
- * A (1 <= m <= 8, 3m <= n).  X holds m * n entries, row i = vector i: the
+ * A (1 <= m <= 10, 3m <= n).  X holds m * n entries, row i = vector i: the
  * initial vectors on entry, the eigenvectors on return; lambda (m host
  * doubles) receives the eigenvalues.  Stops when the largest residual norm
  * ||M (A x_i - lambda_i x_i)|| (M the preconditioner p->precond) is below

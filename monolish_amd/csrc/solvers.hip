@@ -713,7 +713,7 @@ using namespace mgx;
 namespace {
 
 // ---- LOBPCG (src/eigen/lobpcg_standard.cpp:10-300) ------------------------
-// The small generalised eigenproblem Sa v = lambda Sb v (k <= 24) of each
+// The small generalised eigenproblem Sa v = lambda Sb v (k <= 30) of each
 // iteration is solved on the host, as the reference does with LAPACK's
 // sygvd (jobz 'V', uplo 'L', itype 1): Cholesky Sb = L L^T, the cyclic
 // Jacobi method on C = L^-1 Sa L^-T (Golub & Van Loan, Matrix
@@ -1075,7 +1075,7 @@ int mgx_lobpcg_solve(mgx_csr *A, mgx_vec *X, int64_t m, double *lambda,
   return guard([&] {
     MGX_REQUIRE(A && X && lambda && p && res, "mgx_lobpcg_solve: NULL argument");
     MGX_REQUIRE(A->rows == A->cols, "mgx_lobpcg_solve: error, A.row != A.col");
-    MGX_REQUIRE(m >= 1 && m <= 8, "mgx_lobpcg_solve: 1 <= m <= 8 eigenpairs");
+    MGX_REQUIRE(m >= 1 && m <= 10, "mgx_lobpcg_solve: 1 <= m <= 10 eigenpairs");
     MGX_REQUIRE(3 * m <= A->rows, "mgx_lobpcg_solve: needs 3 m <= n");
     MGX_REQUIRE(X->ctx == A->ctx && X->dtype == A->dtype && X->n == m * A->rows,
                 "mgx_lobpcg_solve: X must hold m * n entries of A's dtype");
