@@ -861,6 +861,21 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
     launch_spmv(A, A->val, dt, w, W, 1, 0, false);
   }
   auto out_x = [&] { launch_copy(c, dt, m * n, row(wxp, m), Xio); };
+  // nrm2 of rows first .. first+m-1 of wxp: m canonical dots, one host read
+  double *d_dots = (double *)ws_get(c, (size_t)m * 8);
+  std::vector<double> h_dots(m);
+  auto norms = [&](int64_t first) {
+    for (int64_t i = 0; i < m; i++) {
+      const T *v = row(wxp, first + i);
+      launch_dot(c, dt, n, v, v, d_dots + i, c->d_tick);
+    }
+    MGX_HIP(hipMemcpyAsync(h_dots.data(), d_dots, (size_t)m * 8, hipMemcpyDeviceToHost,
+                           c->stream));
+    MGX_HIP(hipStreamSynchronize(c->stream));
+    std::vector<T> r(m);
+    for (int64_t i = 0; i < m; i++) r[i] = host_sqrt((T)h_dots[i]);
+    return r;
+  };
   bool singular = false;
   std::vector<double> g(k3 * (k3 + 1)), w, V;
   std::vector<T> hcoef(m * k3);
@@ -900,20 +915,28 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
     MGX_HIP(hipMemcpyAsync(coef, hcoef.data(), hcoef.size() * sizeof(T), hipMemcpyHostToDevice,
                            c->stream));
     lobpcg_combine(c, dt, n, (int)m, k3, k == k3, coef, wxp_p, WXP_p, wxp, WXP);  // :220-238
+    // The pairs are independent within an iteration, so each step below
+    // runs for all m pairs before the next, with one host read per batch of
+    // norms instead of one per norm; every value is the one the reference's
+    // pair-by-pair order computes.  (On a NaN residual the x of later pairs
+    // is already updated; the reference leaves them at the previous step.)
+    const std::vector<T> normp = norms(2 * m);  // :240-244
     for (int64_t i = 0; i < m; i++) {
-      T *x = row(wxp, m + i), *X = row(WXP, m + i), *pv = row(wxp, 2 * m + i),
-        *P = row(WXP, 2 * m + i), *wv = row(wxp, i), *W = row(WXP, i);
-      const T normp = nrm(pv);  // :240-244
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp), pv);
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp), P);
-      const T normx = nrm(x);
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx), x);
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx), X);
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp[i]), row(wxp, 2 * m + i));
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp[i]), row(WXP, 2 * m + i));
+    }
+    const std::vector<T> normx = norms(m);
+    for (int64_t i = 0; i < m; i++) {
+      T *x = row(wxp, m + i), *X = row(WXP, m + i), *wv = row(wxp, i);
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx[i]), x);
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx[i]), X);
       launch_axpyz(c, dt, n, -(double)(T)lambda[i], x, X, wv);  // :248
       launch_copy(c, dt, n, wv, v2);                             // :250-252
       apply_precond(c, M, n, v2, v1);
       launch_copy(c, dt, n, v1, wv);
-      residual[i] = nrm(wv);  // :255
+    }
+    residual = norms(0);  // :255
+    for (int64_t i = 0; i < m; i++) {
       if (std::isnan(residual[i])) {  // :260-270
         out_x();
         o.status = MGX_SOLVER_RESIDUAL_NAN;
@@ -921,8 +944,8 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
         o.resid = (double)residual[i];
         return o;
       }
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)residual[i]), wv);  // :272
-      launch_spmv(A, A->val, dt, wv, W, 1, 0, false);                      // :274
+      launch_scal(c, dt, n, (double)(T)(1.0 / (double)residual[i]), row(wxp, i));  // :272
+      launch_spmv(A, A->val, dt, row(wxp, i), row(WXP, i), 1, 0, false);          // :274
     }
     resid = *std::max_element(residual.begin(), residual.end());  // :277
     if (rhist && iter < rhist_len) rhist[iter] = (double)resid;
