@@ -131,3 +131,15 @@ def test_lobpcg_argument_errors(m, ctx, oracle):
         m.lobpcg(A, m.Vector(ctx, n=39), 2)
     A.free()
     X.free()
+
+
+def test_lobpcg_smallest_problems(m, ctx, oracle):
+    """One pair of a 3x3 matrix (k = 3m = n) and three pairs of a 9x9 one:
+    the eigenvalues of numpy.linalg.eigvalsh."""
+    for n, k in ((3, 1), (9, 3)):
+        say("lobpcg small", n, k)
+        rp, ci, val, ev = oracle.tridiagonal_toeplitz(n, 2.0, -1.0)
+        X0 = oracle.random_vector(k * n, 0.0, 1.0, 9).reshape(k, n)
+        res, lam, _ = gpu_lobpcg(m, ctx, rp, ci, val, X0, k, tol=1e-10, maxiter=200)
+        assert res.status in (0, m.SOLVER_MAXITER)
+        np.testing.assert_allclose(lam, np.sort(ev)[:k], rtol=1e-8)

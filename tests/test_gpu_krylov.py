@@ -559,3 +559,35 @@ def test_ic_ilu_errors(m, ctx, oracle):
             m.batch_solve(ctx, m.METHOD_CG, [A], [X], [B], precond=kind)
     for v in (A, X, B):
         v.free()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_triangular_preconditioners_tiny_systems(m, ctx, oracle, dtype):
+    """1x1, 2x2 and 3x3 systems (one level, one block, no dependency
+    entries in some sweeps) through SOR, IC and ILU: bit-identical to the
+    oracle, and IC/ILU solves equal the direct solution to rounding."""
+    mats = [(np.array([0, 1], np.int32), np.array([0], np.int32), np.array([4.0])),
+            (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32),
+             np.array([4.0, -1.0, -1.0, 3.0])),
+            (np.array([0, 2, 5, 7], np.int32), np.array([0, 1, 0, 1, 2, 1, 2], np.int32),
+             np.array([2.0, -1.0, -1.0, 2.0, -1.0, -1.0, 2.0]))]
+    for rp, ci, val in mats:
+        val = val.astype(dtype)
+        n = len(rp) - 1
+        say("tiny", n, dtype.__name__)
+        b = oracle.random_vector(n, -1.0, 1.0, 3, dtype=dtype)
+        for kind in (4, 5):
+            rc, ref = oracle.ilu_precond(rp, ci, val, kind, b)
+            A = m.CRS(ctx, n, n, rp, ci, val)
+            X, B = m.Vector(ctx, n=n, dtype=dtype), m.Vector(ctx, b, dtype=dtype)
+            res = (m.ic_solve if kind == 4 else m.ilu_solve)(A, X, B)
+            assert res.status == 0 and bits_equal(X.download(), ref)
+            for v in (A, X, B):
+                v.free()
+        for prec in (3, 4, 5):
+            ref = oracle.solve("cg", rp, ci, val, np.zeros(n, dtype), b, precond=prec, tol=1e-30,
+                               maxiter=3, history=True)
+            res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n, dtype), b, prec, 1e-30,
+                               maxiter=3, history=3)
+            assert res.status == ref[0] and res.final_iter == ref[2]
+            assert bits_equal(x, ref[1])
