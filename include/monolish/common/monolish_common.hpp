@@ -29,6 +29,8 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../monolish_amd.h"
@@ -325,6 +327,7 @@ template <typename TYPE, typename Float> class view1D {
   }
   TYPE &get_target() { return *target_; }
   const TYPE &get_target() const { return *target_; }
+  void print_all(bool force_cpu = false) const;
 
  private:
   TYPE *target_;
@@ -336,7 +339,9 @@ namespace detail {
 struct VecArg {
   mgx_vec *h = nullptr;
   bool owned = false;
-  template <class F> explicit VecArg(const vector<F> &v) : h(v.dev()) {}
+  // a vector or a Dense matrix (anything with a device handle)
+  template <class T, class = decltype(std::declval<const T &>().dev())>
+  explicit VecArg(const T &v) : h(v.dev()) {}
   template <class TYPE, class F> explicit VecArg(const view1D<TYPE, F> &v) {
     mgx_vec *base = VecArg(v.get_target()).h;
     check(mgx_vec_view(base, (int64_t)v.get_offset(), (int64_t)v.size(), &h));
@@ -348,6 +353,18 @@ struct VecArg {
   }
 };
 }  // namespace detail
+
+template <typename TYPE, typename Float>
+void view1D<TYPE, Float>::print_all(bool force_cpu) const {
+  if (get_device_mem_stat() && !force_cpu) {
+    std::vector<Float> tmp(size_);
+    detail::VecArg a(*this);
+    detail::check(mgx_vec_download(a.h, tmp.data()));
+    for (auto v : tmp) std::printf("%f\n", (double)v);
+  } else {
+    for (size_t i = 0; i < size_; i++) std::cout << data()[i] << std::endl;
+  }
+}
 
 namespace matrix {
 
@@ -640,6 +657,43 @@ template <typename Float> class CRS {
   void alloc(size_t n) { val.reset(new Float[n > 0 ? n : 1](), std::default_delete<Float[]>()); }
   size_t rowN = 0, colN = 0, nnz_ = 0;
   std::shared_ptr<detail::CsrHandle> dev_;
+};
+
+// ------------------------------------------------------------------ Dense
+// row-major M x N storage (monolish_dense.hpp), enough for the eigensolver's
+// eigenvector block: element access on the host, residency like vector
+// (its device copy is one M*N vector)
+template <typename Float> class Dense {
+ public:
+  using value_type = Float;
+  Dense() = default;
+  Dense(const size_t M, const size_t N) : rowN(M), colN(N), v_(M * N) {}
+  Dense(const size_t M, const size_t N, const Float value) : rowN(M), colN(N), v_(M * N, value) {}
+  [[nodiscard]] size_t get_row() const { return rowN; }
+  [[nodiscard]] size_t get_col() const { return colN; }
+  [[nodiscard]] size_t get_nnz() const { return rowN * colN; }
+  [[nodiscard]] std::string type() const { return "Dense"; }
+  Float *data() { return v_.data(); }
+  const Float *data() const { return v_.data(); }
+  void insert(const size_t i, const size_t j, const Float val) {
+    if (get_device_mem_stat()) throw std::runtime_error("Error, GPU matrix cant use insert");
+    v_.data()[i * colN + j] = val;
+  }
+  [[nodiscard]] Float at(const size_t i, const size_t j) const {
+    if (get_device_mem_stat()) throw std::runtime_error("Error, GPU matrix cant use at");
+    return v_.data()[i * colN + j];
+  }
+  [[nodiscard]] bool get_device_mem_stat() const { return v_.get_device_mem_stat(); }
+  void send() const { v_.send(); }
+  void recv() { v_.recv(); }
+  void nonfree_recv() { v_.nonfree_recv(); }
+  void device_free() const { v_.device_free(); }
+  void print_all(bool force_cpu = false) const { v_.print_all(force_cpu); }
+  mgx_vec *dev() const { return v_.dev(); }
+
+ private:
+  size_t rowN = 0, colN = 0;
+  vector<Float> v_;
 };
 
 }  // namespace matrix

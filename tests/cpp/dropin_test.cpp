@@ -9,6 +9,7 @@
 #include <random>
 
 #include "monolish_blas.hpp"
+#include "monolish_eigen.hpp"
 #include "monolish_equation.hpp"
 #include "monolish_vml.hpp"
 
@@ -166,6 +167,36 @@ template <typename T> void fused_and_callback_paths_agree() {
   CHECK(monolish::util::solver_check(MONOLISH_SOLVER_MAXITER) == false);
 }
 
+// test/eigen/LOBPCG/lobpcg.cpp:91-130, 268-282: tridiagonal Toeplitz (100,
+// 10, -1), 2 pairs, x(i, 7i mod n) = 1 (USER) or random (RANDOM), residual
+// tol 1e-5 * 100, maxiter 100, eigenvalues within 1e-3 of the closed form
+template <typename T, typename PRECOND> bool lobpcg_harness(bool user) {
+  using MAT = monolish::matrix::CRS<T>;
+  const int dim = 100;
+  auto coo = monolish::util::tridiagonal_toeplitz_matrix<T>(dim, 10.0, -1.0);
+  MAT A(coo);
+  monolish::vector<T> lambda(2);
+  monolish::matrix::Dense<T> x(2, A.get_row());
+  for (size_t i = 0; i < 2; i++) x.insert(i, (i * 7) % A.get_row(), 1.0);
+  monolish::util::send(A);
+  monolish::eigen::LOBPCG<MAT, T> solver;
+  solver.set_tol(1.0e-5 * dim);
+  solver.set_lib(0);
+  solver.set_miniter(0);
+  solver.set_maxiter(dim);
+  if (user) solver.set_initvec_scheme(monolish::solver::initvec_scheme::USER);
+  PRECOND precond;
+  solver.set_create_precond(precond);
+  solver.set_apply_precond(precond);
+  if (monolish::util::solver_check(solver.solve(A, lambda, x))) return false;
+  for (int i = 0; i < 2; i++) {
+    const double ex = monolish::util::tridiagonal_toeplitz_matrix_eigenvalue<T>(dim, i, 10.0, -1.0);
+    if (std::abs((lambda.data()[i] - ex) / ex) >= 1e-3) return false;
+  }
+  monolish::util::recv(x);
+  return x.get_row() == 2 && !x.get_device_mem_stat();
+}
+
 int main(int argc, char **argv) {
   const std::string dir = argc > 1 ? argv[1] : ".";
   using namespace monolish;
@@ -203,6 +234,11 @@ int main(int argc, char **argv) {
   HARNESS(IC, none, float, 1e-4, mtx);
   HARNESS(ILU, none, double, 1e-8, mtx);
   HARNESS(ILU, none, float, 1e-4, mtx);
+  CHECK((lobpcg_harness<double, equation::none<matrix::CRS<double>, double>>(true)));
+  CHECK((lobpcg_harness<float, equation::none<matrix::CRS<float>, float>>(true)));
+  CHECK((lobpcg_harness<double, equation::Jacobi<matrix::CRS<double>, double>>(true)));
+  CHECK((lobpcg_harness<float, equation::Jacobi<matrix::CRS<float>, float>>(true)));
+  CHECK((lobpcg_harness<double, equation::none<matrix::CRS<double>, double>>(false)));
   fused_and_callback_paths_agree<double>();
   fused_and_callback_paths_agree<float>();
   // operands on the host are refused, not computed on the CPU

@@ -28,8 +28,12 @@ def test_dropin_program_compiles_and_links(tmp_path):
     build_dropin(str(tmp_path / "dropin"))
 
 
+# examples/eigen/lobpcg/lobpcg.cpp compiles; it is not run: its view1D
+# windows (start i*n, size (i+1)*n) reach past the 2 x n eigenvector block
+# for i = 1, which this library refuses and the reference reads out of bounds
 REF_EXAMPLES = ["examples/equation/cg/cg.cpp", "examples/blas/cg_impl/cg.cpp",
-                "examples/blas/innerproduct/innerproduct.cpp"]
+                "examples/blas/innerproduct/innerproduct.cpp",
+                "examples/eigen/lobpcg/lobpcg.cpp"]
 
 
 @pytest.mark.parametrize("example", REF_EXAMPLES)
