@@ -344,7 +344,8 @@ def test_sor_preconditioned_solvers_bit_exact(m, ctx, oracle, dtype, omega):
     say("sor precond", dtype.__name__, omega)
     cases = [("cg", m.laplacian2d(48, 40)), ("cg", m.stencil3d(12, 10, 9, 7)),
              ("bicgstab", m.convdiff2d(40, 20.0)), ("cg", _block_diag(6000, 3, 1)),
-             ("cg", _block_diag(2000, 3, 8)), ("bicgstab", _arrow(10000, sym=False)),
+             ("cg", _block_diag(2000, 3, 8)), ("cg", _block_diag(600, 3, 10)),
+             ("bicgstab", _arrow(10000, sym=False)),
              ("jacobi", m.laplacian2d(16, 16))]
     oracle.set_sor_omega(omega)
     for method, (rp, ci, val) in cases:
@@ -466,6 +467,7 @@ def _arrow(n, eps=1e-4, sym=True):
 def _factor_cases(m, kind):
     cases = [("poisson", m.laplacian2d(48, 40)), ("stencil27", m.stencil3d(14, 12, 10, 27)),
              ("blocks", _block_diag(6000, 3, 3)), ("blocks2k", _block_diag(2000, 3, 7)),
+             ("blocks600", _block_diag(600, 3, 9)),
              ("arrow", _arrow(20000)), ("arrow_short_rows", _arrow(20000, 1e-3, sym=False)),
              ("random_spd", _random_sorted(3000, 0.002, 5, diag=12.0, sym=True))]
     if kind == 5:
