@@ -732,6 +732,12 @@ def main():
         line["batched"] = bench_batched(m, ctx, args.batch_systems)
     if "served" in legs:
         line["served"] = bench_served(m, ctx, 2000.0, args.serve_seconds, 1000, args.serve_nmax)
+        # the same mix at about 70 % of the measured capacity: latency without
+        # the queue that an overloaded arrival rate builds
+        cap = line["served"]["solves_per_s"]
+        line["served"]["below_capacity"] = bench_served(m, ctx, round(0.7 * cap, 1),
+                                                        args.serve_seconds, 1000,
+                                                        args.serve_nmax, seed=5002)
     if "ingest" in legs:
         line["ingest"] = bench_ingest(m, ctx, 128, "cpu" in legs)
     if "sor" in legs:
