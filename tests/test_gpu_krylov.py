@@ -593,3 +593,50 @@ def test_triangular_preconditioners_tiny_systems(m, ctx, oracle, dtype):
                                maxiter=3, history=3)
             assert res.status == ref[0] and res.final_iter == ref[2]
             assert bits_equal(x, ref[1])
+
+
+def test_triangular_plans_follow_updated_values(m, ctx, oracle):
+    """The sweep plans and IC/ILU structure are built once per matrix
+    structure; the coefficients are gathered per solve.  After
+    CRS::operator= (new values, same pattern) SOR-, IC- and ILU-PCG give
+    the oracle's results for the new values, bit for bit."""
+    rp, ci, val = m.laplacian2d(40, 30)
+    n = len(rp) - 1
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    b = oracle.random_vector(n, -1.0, 1.0, 12)
+    B = m.Vector(ctx, b)
+    X = m.Vector(ctx, n=n)
+    val2 = val * oracle.random_vector(len(val), 0.5, 1.5, 13)
+    val2 = 0.5 * (val2 + _transpose_values(rp, ci, val2))  # keep it symmetric
+    val2[_diag_pos(rp, ci)] = val[_diag_pos(rp, ci)] * 1.3
+    for vals in (val, val2, val):
+        A.update_values(vals)
+        for prec in (3, 4, 5):
+            say("updated values", prec)
+            ref = oracle.solve("cg", rp, ci, vals, np.zeros(n), b, precond=prec, tol=1e-10,
+                               maxiter=2000)
+            X.fill(0.0)
+            res = m.cg(A, X, B, tol=1e-10, maxiter=2000, precond=prec)
+            assert res.status == ref[0] == 0 and res.final_iter == ref[2]
+            assert bits_equal(X.download(), ref[1])
+    for v in (A, B, X):
+        v.free()
+
+
+def _diag_pos(rp, ci):
+    return np.array([rp[i] + int(np.nonzero(ci[rp[i]:rp[i + 1]] == i)[0][0])
+                     for i in range(len(rp) - 1)])
+
+
+def _transpose_values(rp, ci, val):
+    """values of A^T at A's positions (A's pattern is symmetric)"""
+    n = len(rp) - 1
+    pos = {}
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            pos[(i, int(ci[e]))] = e
+    out = np.empty_like(val)
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            out[e] = val[pos[(int(ci[e]), i)]]
+    return out
