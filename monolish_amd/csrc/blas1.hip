@@ -99,6 +99,38 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *x, const T *
     if (threadIdx.x == 0) *out = (double)res[0];
 }
 
+// m canonical dots <x_r, x_r> of the rows of a row-major block (row r at
+// X + r ld), one per blockIdx.y, each exactly as k_dot computes it
+template <class T, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_mdot(int64_t n, const T *X, int64_t ld, T *P,
+                                                 int64_t pst, uint32_t *tick, double *out) {
+  __shared__ T sh[8];
+  __shared__ T tp[kBlock];
+  __shared__ T res[2];
+  __shared__ int flag;
+  const int64_t r = blockIdx.y;
+  const T *x = X + r * ld;
+  T *Pr = P + r * pst;
+  const int64_t nt = ntiles_of(n);
+  const int64_t tile = blockIdx.x;
+  const int64_t i0 = tile * kTile + 2 * threadIdx.x;
+  T p0 = 0, p1 = 0;
+  if (VEC && i0 + 1 < n) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    const V2 xv = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(x + i0));
+    p0 = xv[0] * xv[0];
+    p1 = xv[1] * xv[1];
+  } else {
+    if (i0 < n) p0 = x[i0] * x[i0];
+    if (i0 + 1 < n) p1 = x[i0 + 1] * x[i0 + 1];
+  }
+  const T s = block_tree(p0 + p1, sh);
+  if (threadIdx.x == 0) publish_partial(Pr, tile, s);
+  if (arrive_reduce<T, 1, 1>(tick + r * kTickWords, Pr, 0, Pr + nt, nt, tile, tp, sh, res,
+                             &flag))
+    if (threadIdx.x == 0) out[r] = (double)res[0];
+}
+
 template <class T> T *tp(void *p) { return static_cast<T *>(p); }
 template <class T> const T *tp(const void *p) {
   return static_cast<const T *>(p);
@@ -221,6 +253,33 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
     else
       k_dot<float, false><<<g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y), P, S,
                                                        d_ticket, d_out);
+  }
+  MGX_HIP(hipGetLastError());
+}
+
+void launch_mnrm2sq(Ctx *c, mgx_dtype dt, int64_t n, int m, const void *X, int64_t ld,
+                    void *partials, uint32_t *ticks, double *d_out) {
+  if (n <= 0) {
+    launch_fill(c, MGX_F64, d_out, m, 0.0);
+    return;
+  }
+  const int64_t nt = ntiles_of(n), pst = nt + 2 * kRedGroups;
+  const bool vec = aligned16(X) && (ld % 2 == 0);
+  const dim3 g((unsigned)nt, (unsigned)m);
+  if (dt == MGX_F64) {
+    if (vec)
+      k_mdot<double, true><<<g, kBlock, 0, c->stream>>>(n, tp<double>(X), ld,
+                                                        tp<double>(partials), pst, ticks, d_out);
+    else
+      k_mdot<double, false><<<g, kBlock, 0, c->stream>>>(n, tp<double>(X), ld,
+                                                         tp<double>(partials), pst, ticks, d_out);
+  } else {
+    if (vec && ld % 4 == 0)
+      k_mdot<float, true><<<g, kBlock, 0, c->stream>>>(n, tp<float>(X), ld, tp<float>(partials),
+                                                       pst, ticks, d_out);
+    else
+      k_mdot<float, false><<<g, kBlock, 0, c->stream>>>(n, tp<float>(X), ld,
+                                                        tp<float>(partials), pst, ticks, d_out);
   }
   MGX_HIP(hipGetLastError());
 }

@@ -17,6 +17,12 @@ namespace mgx {
 namespace {
 
 constexpr int kGramTile = 128;  // entries of each vector per block
+constexpr int kMaxPairs = 10;   // eigenpairs per solve (the entry point's limit)
+
+// per-row factors passed by value (at most 10 pairs)
+template <class T> struct RowScal {
+  T v[10];
+};
 
 // lower triangle of Sa (pairs 0..npair-1) and Sb (npair..2 npair-1), pair
 // (i, j), i >= j, numbered i (i + 1) / 2 + j
@@ -24,13 +30,17 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_gram(int64_t n, int k, const T *__restrict__ wxp,
                                                  const T *__restrict__ WXP, double *part) {
   extern __shared__ unsigned char smem[];
-  T *sv = reinterpret_cast<T *>(smem);  // k rows of wxp, then k rows of WXP
+  // k rows of wxp, then k rows of WXP; rows padded by one value so the
+  // lanes of a wave, which read different rows at the same entry, fall in
+  // different LDS banks
+  constexpr int kLd = kGramTile + 1;
+  T *sv = reinterpret_cast<T *>(smem);
   const int64_t e0 = (int64_t)blockIdx.x * kGramTile;
   const int len = (int)(n - e0 < kGramTile ? n - e0 : kGramTile);
   for (int q = threadIdx.x; q < 2 * k * kGramTile; q += kBlock) {
     const int r = q / kGramTile, e = q % kGramTile;
     const T *src = r < k ? wxp + (int64_t)r * n : WXP + (int64_t)(r - k) * n;
-    sv[q] = e < len ? src[e0 + e] : (T)0;
+    sv[r * kLd + e] = e < len ? src[e0 + e] : (T)0;
   }
   __syncthreads();
   const int npair = k * (k + 1) / 2;
@@ -39,8 +49,8 @@ __global__ __launch_bounds__(kBlock) void k_gram(int64_t n, int k, const T *__re
     int i = 0;
     while ((i + 1) * (i + 2) / 2 <= pr) i++;
     const int j = pr - i * (i + 1) / 2;
-    const T *a = o < npair ? sv + (k + i) * kGramTile : sv + i * kGramTile;
-    const T *b = sv + j * kGramTile;
+    const T *a = o < npair ? sv + (k + i) * kLd : sv + i * kLd;
+    const T *b = sv + j * kLd;
     double s = 0.0;
     for (int e = 0; e < len; e++) s += (double)(a[e] * b[e]);
     part[(int64_t)blockIdx.x * 2 * npair + o] = s;
@@ -54,7 +64,15 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce(int64_t nblk, int nout,
   const int o = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (o >= nout) return;
   double s = 0.0;
-  for (int64_t b = 0; b < nblk; b++) s += part[b * nout + o];
+  int64_t b = 0;
+  for (; b + 8 <= nblk; b += 8) {  // eight loads in flight, added in order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = part[(b + u) * nout + o];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; b < nblk; b++) s += part[b * nout + o];
   out[o] = s;
 }
 
@@ -66,42 +84,119 @@ __global__ __launch_bounds__(kBlock) void k_lobpcg_combine(
     const T *__restrict__ WXP_p, T *wxp, T *WXP) {
   const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (e >= n) return;
-  for (int i = 0; i < m; i++) {
-    const T *b = coef + (int64_t)i * k;
-    T p = 0, x = 0, P = 0, X = 0;
-    for (int j = 0; j < m; j++) {
-      const T bw = b[j], bx = b[m + j];
-      const T wj = wxp_p[(int64_t)j * n + e], Wj = WXP_p[(int64_t)j * n + e];
-      const T xj = wxp_p[(int64_t)(m + j) * n + e], Xj = WXP_p[(int64_t)(m + j) * n + e];
-      p = p + bw * wj;
-      x = x + bw * wj;
-      P = P + bw * Wj;
-      X = X + bw * Wj;
-      if (has_p) {
-        const T bp = b[2 * m + j];
-        const T pj = wxp_p[(int64_t)(2 * m + j) * n + e], Pj = WXP_p[(int64_t)(2 * m + j) * n + e];
-        p = p + bp * pj;
-        x = x + bp * pj;
-        P = P + bp * Pj;
-        X = X + bp * Pj;
-      }
-      x = x + bx * xj;
-      X = X + bx * Xj;
+  // basis vector j outermost: each previous value is loaded once, and every
+  // output still adds its terms in the order j = 0, 1, ...
+  T p[kMaxPairs], x[kMaxPairs], P[kMaxPairs], X[kMaxPairs];
+#pragma unroll
+  for (int i = 0; i < kMaxPairs; i++) p[i] = x[i] = P[i] = X[i] = 0;
+  for (int j = 0; j < m; j++) {
+    const T wj = wxp_p[(int64_t)j * n + e], Wj = WXP_p[(int64_t)j * n + e];
+    const T xj = wxp_p[(int64_t)(m + j) * n + e], Xj = WXP_p[(int64_t)(m + j) * n + e];
+    T pj = 0, Pj = 0;
+    if (has_p) {
+      pj = wxp_p[(int64_t)(2 * m + j) * n + e];
+      Pj = WXP_p[(int64_t)(2 * m + j) * n + e];
     }
-    wxp[(int64_t)(m + i) * n + e] = x;
-    wxp[(int64_t)(2 * m + i) * n + e] = p;
-    WXP[(int64_t)(m + i) * n + e] = X;
-    WXP[(int64_t)(2 * m + i) * n + e] = P;
+#pragma unroll
+    for (int i = 0; i < kMaxPairs; i++) {
+      if (i < m) {
+        const T *b = coef + (int64_t)i * k;
+        const T bw = b[j], bx = b[m + j];
+        p[i] = p[i] + bw * wj;
+        x[i] = x[i] + bw * wj;
+        P[i] = P[i] + bw * Wj;
+        X[i] = X[i] + bw * Wj;
+        if (has_p) {
+          const T bp = b[2 * m + j];
+          p[i] = p[i] + bp * pj;
+          x[i] = x[i] + bp * pj;
+          P[i] = P[i] + bp * Pj;
+          X[i] = X[i] + bp * Pj;
+        }
+        x[i] = x[i] + bx * xj;
+        X[i] = X[i] + bx * Xj;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxPairs; i++) {
+    if (i < m) {
+      wxp[(int64_t)(m + i) * n + e] = x[i];
+      wxp[(int64_t)(2 * m + i) * n + e] = p[i];
+      WXP[(int64_t)(m + i) * n + e] = X[i];
+      WXP[(int64_t)(2 * m + i) * n + e] = P[i];
+    }
+  }
+}
+
+// rows r0 .. r0+m-1 of A and of B scaled by s[r] (blas::scal of each)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rows_scale2(int64_t n, int m, RowScal<T> s, T *A,
+                                                        T *B) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  for (int r = 0; r < m; r++) {
+    A[(int64_t)r * n + e] = s.v[r] * A[(int64_t)r * n + e];
+    if (B) B[(int64_t)r * n + e] = s.v[r] * B[(int64_t)r * n + e];
+  }
+}
+
+// pair r: x *= s, X *= s, w = -lam x + X, then w = M o w (Jacobi) when M
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_lobpcg_xw(int64_t n, int m, RowScal<T> s,
+                                                      RowScal<T> nlam, T *x, T *X, T *w,
+                                                      const T *__restrict__ M) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  for (int r = 0; r < m; r++) {
+    const int64_t o = (int64_t)r * n + e;
+    const T xv = s.v[r] * x[o], Xv = s.v[r] * X[o];
+    x[o] = xv;
+    X[o] = Xv;
+    const T t = nlam.v[r] * xv + Xv;
+    w[o] = M ? M[e] * t : t;
   }
 }
 
 }  // namespace
 
+void lobpcg_scale_rows(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, void *A,
+                       void *B) {
+  const unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
+  if (dt == MGX_F64) {
+    RowScal<double> v;
+    for (int r = 0; r < m; r++) v.v[r] = s[r];
+    k_rows_scale2<double><<<g, kBlock, 0, c->stream>>>(n, m, v, (double *)A, (double *)B);
+  } else {
+    RowScal<float> v;
+    for (int r = 0; r < m; r++) v.v[r] = (float)s[r];
+    k_rows_scale2<float><<<g, kBlock, 0, c->stream>>>(n, m, v, (float *)A, (float *)B);
+  }
+  MGX_HIP(hipGetLastError());
+}
+
+void lobpcg_xw(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, const double *nlam,
+               void *x, void *X, void *w, const void *M) {
+  const unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
+  if (dt == MGX_F64) {
+    RowScal<double> a, b;
+    for (int r = 0; r < m; r++) a.v[r] = s[r], b.v[r] = nlam[r];
+    k_lobpcg_xw<double><<<g, kBlock, 0, c->stream>>>(n, m, a, b, (double *)x, (double *)X,
+                                                     (double *)w, (const double *)M);
+  } else {
+    RowScal<float> a, b;
+    for (int r = 0; r < m; r++) a.v[r] = (float)s[r], b.v[r] = (float)nlam[r];
+    k_lobpcg_xw<float><<<g, kBlock, 0, c->stream>>>(n, m, a, b, (float *)x, (float *)X,
+                                                    (float *)w, (const float *)M);
+  }
+  MGX_HIP(hipGetLastError());
+}
+
 void lobpcg_gram(Ctx *c, mgx_dtype dt, int64_t n, int k, const void *wxp, const void *WXP,
                  double *part, double *out) {
   const int64_t nblk = (n + kGramTile - 1) / kGramTile;
   const size_t es = dt == MGX_F64 ? 8 : 4;
-  const size_t lds = (size_t)2 * k * kGramTile * es;
+  const size_t lds = (size_t)2 * k * (kGramTile + 1) * es;
   const int nout = k * (k + 1);
   if (dt == MGX_F64)
     k_gram<double><<<(unsigned)nblk, kBlock, lds, c->stream>>>(n, k, (const double *)wxp,

@@ -280,6 +280,11 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
                 double *d_out, uint32_t *d_ticket);
 double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y);
+// d_out[r] = <x_r, x_r> for the m rows x_r = X + r ld, each the canonical
+// dot of launch_dot; partials: m (ntiles(n) + 2 kRedGroups) values of dt,
+// ticks: m kTickWords zeroed counters (zero again afterwards)
+void launch_mnrm2sq(Ctx *c, mgx_dtype dt, int64_t n, int m, const void *X, int64_t ld,
+                    void *partials, uint32_t *ticks, double *d_out);
 
 void spmv_plan_build(Ctx *c, Csr *A, const int32_t *rp_host);
 // plan with explicit budgets (the context's defaults otherwise)
@@ -343,6 +348,13 @@ void lobpcg_gram(Ctx *c, mgx_dtype dt, int64_t n, int k, const void *wxp, const 
 // rows m..2m-1 (x) and 2m..3m-1 (p) of wxp and WXP from the previous basis
 // wxp_p, WXP_p with coef (m rows of `stride` coefficients, device); has_p:
 // the previous basis holds p
+// rows r of A (and of B when not null), r < m <= 10, scaled by (T)s[r]
+void lobpcg_scale_rows(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, void *A,
+                       void *B);
+// pair r <= 10: x_r *= s[r], X_r *= s[r], w_r = nlam[r] x_r + X_r, then
+// w_r = M o w_r when M (Jacobi) is not null; factors rounded to dt
+void lobpcg_xw(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, const double *nlam,
+               void *x, void *X, void *w, const void *M);
 void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,
                     const void *coef, const void *wxp_p, const void *WXP_p, void *wxp,
                     void *WXP);

@@ -715,11 +715,13 @@ namespace {
 // ---- LOBPCG (src/eigen/lobpcg_standard.cpp:10-300) ------------------------
 // The small generalised eigenproblem Sa v = lambda Sb v (k <= 30) of each
 // iteration is solved on the host, as the reference does with LAPACK's
-// sygvd (jobz 'V', uplo 'L', itype 1): Cholesky Sb = L L^T, the cyclic
-// Jacobi method on C = L^-1 Sa L^-T (Golub & Van Loan, Matrix
-// Computations, 4th ed., Alg. 8.5.3), eigenvalues ascending, v = L^-T q so
-// that v^T Sb v = 1.  Each eigenvector is signed so that its largest
-// component is positive.
+// sygvd (jobz 'V', uplo 'L', itype 1): Cholesky Sb = L L^T, then the
+// eigenpairs of C = L^-1 Sa L^-T by Householder tridiagonalisation and the
+// implicit QL method (Wilkinson & Reinsch, Handbook for Automatic
+// Computation II, tred2 / tql2; the cyclic Jacobi method, Golub & Van Loan,
+// Matrix Computations, 4th ed., Alg. 8.5.3, if QL does not converge),
+// eigenvalues ascending, v = L^-T q so that v^T Sb v = 1.  Each eigenvector
+// is signed so that its largest component is positive.
 
 // in-place Cholesky of the lower triangle of B (row-major k x k); returns 0,
 // or the order of the first leading minor that is not positive definite
@@ -788,6 +790,139 @@ void jacobi_eig(int k, std::vector<double> C, std::vector<double> &w, std::vecto
   }
 }
 
+// Householder reduction of the symmetric a (row-major k x k, overwritten by
+// the orthogonal Q) to tridiagonal d (diagonal), e (e[i]: entry (i, i-1))
+void householder_tridiag(int k, std::vector<double> &a, std::vector<double> &d,
+                         std::vector<double> &e) {
+  auto A = [&](int i, int j) -> double & { return a[i * k + j]; };
+  d.assign(k, 0.0);
+  e.assign(k, 0.0);
+  for (int i = k - 1; i > 0; i--) {
+    const int l = i - 1;
+    double h = 0.0;
+    if (l > 0) {
+      double scale = 0.0;
+      for (int q = 0; q <= l; q++) scale += std::fabs(A(i, q));
+      if (scale == 0.0) {
+        e[i] = A(i, l);
+      } else {
+        for (int q = 0; q <= l; q++) {
+          A(i, q) /= scale;
+          h += A(i, q) * A(i, q);
+        }
+        double f = A(i, l);
+        double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+        e[i] = scale * g;
+        h -= f * g;
+        A(i, l) = f - g;
+        f = 0.0;
+        for (int j = 0; j <= l; j++) {
+          A(j, i) = A(i, j) / h;
+          g = 0.0;
+          for (int q = 0; q <= j; q++) g += A(j, q) * A(i, q);
+          for (int q = j + 1; q <= l; q++) g += A(q, j) * A(i, q);
+          e[j] = g / h;
+          f += e[j] * A(i, j);
+        }
+        const double hh = f / (h + h);
+        for (int j = 0; j <= l; j++) {
+          f = A(i, j);
+          e[j] = g = e[j] - hh * f;
+          for (int q = 0; q <= j; q++) A(j, q) -= f * e[q] + g * A(i, q);
+        }
+      }
+    } else {
+      e[i] = A(i, l);
+    }
+    d[i] = h;
+  }
+  d[0] = 0.0;
+  e[0] = 0.0;
+  for (int i = 0; i < k; i++) {
+    const int l = i - 1;
+    if (d[i] != 0.0) {
+      for (int j = 0; j <= l; j++) {
+        double g = 0.0;
+        for (int q = 0; q <= l; q++) g += A(i, q) * A(q, j);
+        for (int q = 0; q <= l; q++) A(q, j) -= g * A(q, i);
+      }
+    }
+    d[i] = A(i, i);
+    A(i, i) = 1.0;
+    for (int j = 0; j <= l; j++) A(j, i) = A(i, j) = 0.0;
+  }
+}
+
+// implicit QL on the tridiagonal (d, e from householder_tridiag), the
+// rotations accumulated into z (columns: eigenvectors); false if an
+// eigenvalue needs more than 60 iterations
+bool tridiag_ql(int k, std::vector<double> &d, std::vector<double> &e, std::vector<double> &z) {
+  auto Z = [&](int i, int j) -> double & { return z[i * k + j]; };
+  for (int i = 1; i < k; i++) e[i - 1] = e[i];
+  e[k - 1] = 0.0;
+  for (int l = 0; l < k; l++) {
+    int it = 0, m;
+    do {
+      for (m = l; m < k - 1; m++) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-16 * dd) break;
+      }
+      if (m != l) {
+        if (it++ == 60) return false;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; i--) {
+          double f = s * e[i];
+          const double b = c * e[i];
+          e[i + 1] = (r = std::hypot(f, g));
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          d[i + 1] = g + (p = s * r);
+          g = c * r - b;
+          for (int q = 0; q < k; q++) {
+            f = Z(q, i + 1);
+            Z(q, i + 1) = s * Z(q, i) + c * f;
+            Z(q, i) = c * Z(q, i) - s * f;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  return true;
+}
+
+// eigenpairs of the symmetric C (full, row-major); w ascending, V row i =
+// eigenvector i
+bool sym_eig(int k, std::vector<double> C, std::vector<double> &w, std::vector<double> &V) {
+  std::vector<double> d, e;
+  householder_tridiag(k, C, d, e);
+  if (!tridiag_ql(k, d, e, C)) return false;
+  std::vector<int> idx(k);
+  for (int i = 0; i < k; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+  w.resize(k);
+  V.assign(k * k, 0.0);
+  for (int i = 0; i < k; i++) {
+    w[i] = d[idx[i]];
+    for (int r = 0; r < k; r++) V[i * k + r] = C[r * k + idx[i]];
+  }
+  return true;
+}
+
 // Sa v = lambda Sb v from the lower triangles; returns 0 or the Cholesky
 // failure order of Sb
 int sygv_lower(int k, const std::vector<double> &Sa, std::vector<double> Sb,
@@ -813,7 +948,7 @@ int sygv_lower(int k, const std::vector<double> &Sa, std::vector<double> Sb,
   for (int i = 0; i < k; i++)
     for (int j = 0; j < i; j++) C[i * k + j] = C[j * k + i] = 0.5 * (C[i * k + j] + C[j * k + i]);
   std::vector<double> Qv;
-  jacobi_eig(k, C, w, Qv);
+  if (!sym_eig(k, C, w, Qv)) jacobi_eig(k, C, w, Qv);
   V.assign(k * k, 0.0);
   for (int e = 0; e < k; e++) {  // v = L^-T q
     for (int i = k - 1; i >= 0; i--) {
@@ -861,21 +996,26 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
     launch_spmv(A, A->val, dt, w, W, 1, 0, false);
   }
   auto out_x = [&] { launch_copy(c, dt, m * n, row(wxp, m), Xio); };
-  // nrm2 of rows first .. first+m-1 of wxp: m canonical dots, one host read
-  double *d_dots = (double *)ws_get(c, (size_t)m * 8);
-  std::vector<double> h_dots(m);
-  auto norms = [&](int64_t first) {
-    for (int64_t i = 0; i < m; i++) {
-      const T *v = row(wxp, first + i);
-      launch_dot(c, dt, n, v, v, d_dots + i, c->d_tick);
-    }
-    MGX_HIP(hipMemcpyAsync(h_dots.data(), d_dots, (size_t)m * 8, hipMemcpyDeviceToHost,
+  // nrm2 of rows first .. first+cnt-1 of wxp: the canonical dots of those
+  // rows in one launch and one host read
+  const int64_t pst = ntiles_of(n) + 2 * kRedGroups;
+  double *d_dots = (double *)ws_get(c, (size_t)2 * m * 8);
+  void *d_part = ws_get(c, (size_t)2 * m * pst * sizeof(T));
+  uint32_t *d_ticks = (uint32_t *)ws_get(c, (size_t)2 * m * kTickWords * 4);
+  MGX_HIP(hipMemsetAsync(d_ticks, 0, (size_t)2 * m * kTickWords * 4, c->stream));
+  std::vector<double> h_dots(2 * m);
+  auto norms = [&](int64_t first, int64_t cnt) {
+    launch_mnrm2sq(c, dt, n, (int)cnt, row(wxp, first), n, d_part, d_ticks, d_dots);
+    MGX_HIP(hipMemcpyAsync(h_dots.data(), d_dots, (size_t)cnt * 8, hipMemcpyDeviceToHost,
                            c->stream));
     MGX_HIP(hipStreamSynchronize(c->stream));
-    std::vector<T> r(m);
-    for (int64_t i = 0; i < m; i++) r[i] = host_sqrt((T)h_dots[i]);
+    std::vector<T> r(cnt);
+    for (int64_t i = 0; i < cnt; i++) r[i] = host_sqrt((T)h_dots[i]);
     return r;
   };
+  // the Jacobi / none / default preconditioners are applied inside the
+  // fused x, w pass; the triangular ones run their sweeps per pair
+  const bool fused_prec = precond.kind <= MGX_PRECOND_JACOBI;
   bool singular = false;
   std::vector<double> g(k3 * (k3 + 1)), w, V;
   std::vector<T> hcoef(m * k3);
@@ -916,26 +1056,33 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
                            c->stream));
     lobpcg_combine(c, dt, n, (int)m, k3, k == k3, coef, wxp_p, WXP_p, wxp, WXP);  // :220-238
     // The pairs are independent within an iteration, so each step below
-    // runs for all m pairs before the next, with one host read per batch of
-    // norms instead of one per norm; every value is the one the reference's
-    // pair-by-pair order computes.  (On a NaN residual the x of later pairs
-    // is already updated; the reference leaves them at the previous step.)
-    const std::vector<T> normp = norms(2 * m);  // :240-244
+    // runs for all m pairs at once (one launch, one host read per batch of
+    // norms); every value is the one the reference's pair-by-pair order
+    // computes.  (On a NaN residual the x of later pairs is already
+    // updated; the reference leaves them at the previous step.)
+    const std::vector<T> nxp = norms(m, 2 * m);  // x rows, then p rows (:240-246)
+    std::vector<double> sp(m), sx(m), nl(m), sw(m);
     for (int64_t i = 0; i < m; i++) {
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp[i]), row(wxp, 2 * m + i));
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normp[i]), row(WXP, 2 * m + i));
+      sx[i] = (double)(T)(1.0 / (double)nxp[i]);
+      sp[i] = (double)(T)(1.0 / (double)nxp[m + i]);
+      nl[i] = -(double)(T)lambda[i];
     }
-    const std::vector<T> normx = norms(m);
-    for (int64_t i = 0; i < m; i++) {
-      T *x = row(wxp, m + i), *X = row(WXP, m + i), *wv = row(wxp, i);
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx[i]), x);
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)normx[i]), X);
-      launch_axpyz(c, dt, n, -(double)(T)lambda[i], x, X, wv);  // :248
-      launch_copy(c, dt, n, wv, v2);                             // :250-252
-      apply_precond(c, M, n, v2, v1);
-      launch_copy(c, dt, n, v1, wv);
+    lobpcg_scale_rows(c, dt, n, (int)m, sp.data(), row(wxp, 2 * m), row(WXP, 2 * m));
+    if (fused_prec) {  // :246-253
+      lobpcg_xw(c, dt, n, (int)m, sx.data(), nl.data(), row(wxp, m), row(WXP, m), wxp,
+                precond.kind == MGX_PRECOND_JACOBI ? M.M : nullptr);
+    } else {
+      for (int64_t i = 0; i < m; i++) {
+        T *x = row(wxp, m + i), *X = row(WXP, m + i), *wv = row(wxp, i);
+        launch_scal(c, dt, n, sx[i], x);
+        launch_scal(c, dt, n, sx[i], X);
+        launch_axpyz(c, dt, n, nl[i], x, X, wv);  // :248
+        launch_copy(c, dt, n, wv, v2);            // :250-252
+        apply_precond(c, M, n, v2, v1);
+        launch_copy(c, dt, n, v1, wv);
+      }
     }
-    residual = norms(0);  // :255
+    residual = norms(0, m);  // :255
     for (int64_t i = 0; i < m; i++) {
       if (std::isnan(residual[i])) {  // :260-270
         out_x();
@@ -944,9 +1091,11 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
         o.resid = (double)residual[i];
         return o;
       }
-      launch_scal(c, dt, n, (double)(T)(1.0 / (double)residual[i]), row(wxp, i));  // :272
-      launch_spmv(A, A->val, dt, row(wxp, i), row(WXP, i), 1, 0, false);          // :274
+      sw[i] = (double)(T)(1.0 / (double)residual[i]);
     }
+    lobpcg_scale_rows(c, dt, n, (int)m, sw.data(), wxp, nullptr);  // :272
+    for (int64_t i = 0; i < m; i++)
+      launch_spmv(A, A->val, dt, row(wxp, i), row(WXP, i), 1, 0, false);  // :274
     resid = *std::max_element(residual.begin(), residual.end());  // :277
     if (rhist && iter < rhist_len) rhist[iter] = (double)resid;
     if ((double)resid < tol && miniter < iter + 1) {  // :279-287
