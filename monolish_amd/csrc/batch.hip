@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq(const SysDesc<T> *sys, const T
   if (b >= nlaunch) return;
   int64_t tile;
   const SysDesc<T> &d = locate(sys, tiles, b, tile);
-  if (!d.split) return;
+  if (!d.split || d.pq_fused) return;  // pq_fused: written by its SpMV
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const int64_t R1 = min(d.n, (tile + 1) * kTile);

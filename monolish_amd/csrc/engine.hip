@@ -185,6 +185,8 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
                    (!d.split && (cfg.gather_once || A->gather_lines > kScatteredLines))
                ? 1
                : 0;
+  d.pq_fused = d.split && spmv_sell_pq_ok(A, val, DTof<T>::v) ? 1 : 0;
+  d.pad2_ = 0;
   d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
   d.method = cfg.method;
   d.tol_scale = cfg.tol_relative ? cfg.tol : 0.0;
@@ -262,14 +264,16 @@ template <class T> void Engine<T>::rebuild_lists() {
   any_fused_ = false;
   any_once_ = false;
   any_bcg_once_ = false;
+  any_pq_unfused_ = false;
   split_slots_.clear();
   for (int s : active_) {
     if (reported_[s]) continue;
     const int m = method_[s];
     if (m == kMethodCG) {
-      if (h_sys_[s].split)
+      if (h_sys_[s].split) {
         split_slots_.push_back(s);
-      else
+        any_pq_unfused_ = any_pq_unfused_ || !h_sys_[s].pq_fused;
+      } else
         any_fused_ = true;
       any_once_ = any_once_ || h_sys_[s].once;
     } else {
@@ -312,10 +316,20 @@ template <class T> void Engine<T>::launch_iters(int iters) {
         batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
       if (!split_slots_.empty()) {
         // split form: q = A p per system, then <p, q>
-        for (int s : split_slots_)
-          launch_spmv(csr_[s], val_[s], DTof<T>::v, h_sys_[s].pa, h_sys_[s].q, 1.0, 0.0, false,
-                      &d_states_[s].status);
-        batch_launch_cg_pq<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
+        for (int s : split_slots_) {
+          const SysDesc<T> &h = h_sys_[s];
+          if (h.pq_fused) {
+            // the sliced layout was there when the system was added; the
+            // stage-3 kernel skips this system, so the fused SpMV must run
+            MGX_REQUIRE(spmv_sell_pq(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, h.P,
+                                     &d_states_[s].status),
+                        "solver: the matrix's SpMV plan changed during a solve");
+          } else {
+            launch_spmv(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, 1.0, 0.0, false,
+                        &d_states_[s].status);
+          }
+        }
+        if (any_pq_unfused_) batch_launch_cg_pq<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
         batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 0,
                                   d_stopped_);
       }

@@ -37,6 +37,10 @@ template <class T> struct SysDesc {
   // before its SpMV, BiCGSTAB forms p, phat and s, shat in k_bcg_p / k_bcg_s,
   // so each SpMV gathers one vector per entry (instead of 2 or 3-4)
   int32_t once;
+  // split form only: its SpMV also writes the <p, q> tile partials (sliced
+  // layout, no long rows), so the separate stage-3 kernel skips it
+  int32_t pq_fused;
+  int32_t pad2_;
   double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
 };
 
@@ -207,6 +211,7 @@ template <class T> class Engine {
   bool any_fused_ = false;        // a listed CG system uses the two-kernel form
   bool any_once_ = false;         // ... in its gather-once variant
   bool any_bcg_once_ = false;     // a listed BiCGSTAB system in the gather-once form
+  bool any_pq_unfused_ = false;   // a listed split system needs the stage-3 kernel
   std::vector<int> split_slots_;  // listed CG systems of the split form
   int32_t *d_split_slots_ = nullptr;
   int64_t split_cap_ = 0;

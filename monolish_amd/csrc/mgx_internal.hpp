@@ -355,6 +355,12 @@ void lobpcg_scale_rows(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, 
 // w_r = M o w_r when M (Jacobi) is not null; factors rounded to dt
 void lobpcg_xw(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, const double *nlam,
                void *x, void *X, void *w, const void *M);
+// q = A p in the sliced layout with the tile partials of <p, q> written to
+// P (the CG engine's split-form stage 3), for matrices whose values val
+// have a sliced layout and no long rows; false (nothing launched) otherwise
+bool spmv_sell_pq_ok(const Csr *A, const void *val, mgx_dtype dt);
+bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
+                  const int32_t *live);
 void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,
                     const void *coef, const void *wxp_p, const void *WXP_p, void *wxp,
                     void *WXP);

@@ -245,19 +245,12 @@ __global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
   }
 }
 
-template <class T, bool AB, int W>
-__global__ __launch_bounds__(kBlock) void k_spmv_sell(
-    int64_t rows, int64_t nblocks, const int32_t *__restrict__ rp,
-    const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
-    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
-    int long_t, int remap, const int32_t *live) {
-  const int64_t b = xcd_block(remap);
-  if (b >= nblocks) return;
-  if (live && *live != kStatRunning) return;
-  const int64_t r = b * kBlock + threadIdx.x;
-  if (r >= rows) return;
-  const int32_t ra = rp[r], len = rp[r + 1] - ra;
-  if (len > long_t) return;  // a long row: k_spmv_longrows
+// one row of the sliced layout: its entries eight at a time (W-entry
+// tuples, nontemporal), x gathered, the products added in storage order
+template <class T, int W>
+__device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__restrict__ soff,
+                                      const int32_t *__restrict__ sci,
+                                      const T *__restrict__ sval, const T *__restrict__ x) {
   constexpr int QT = 8 / W;  // tuples per step: 8 entries
   typedef T VT __attribute__((ext_vector_type(W)));
   typedef int32_t VI __attribute__((ext_vector_type(W)));
@@ -287,6 +280,63 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
           s = s + p;
         }
   }
+  return s;
+}
+
+// q = A p in the sliced layout for a matrix without long rows, one block per
+// 512-row tile (each lane two rows), and the tile partial of <p, q> exactly
+// as the CG engine's stage-3 kernel forms it (lane t: p[2t] q[2t] +
+// p[2t+1] q[2t+1], then the block tree), so that kernel can be skipped
+template <class T, int W>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell_pq(
+    int64_t rows, int64_t ntiles, const int32_t *__restrict__ rp,
+    const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
+    const T *__restrict__ sval, const T *__restrict__ p, T *__restrict__ q, T *P, int remap,
+    const int32_t *live) {
+  __shared__ T qs[kTile];
+  __shared__ T sh[8];
+  const int64_t tile = xcd_block(remap);
+  if (tile >= ntiles) return;
+  if (live && *live != kStatRunning) return;
+  const int64_t r0 = tile * kTile;
+#pragma unroll
+  for (int h = 0; h < kTile / kBlock; h++) {
+    const int64_t r = r0 + h * kBlock + threadIdx.x;
+    T s = 0;
+    if (r < rows) {
+      s = sell_row<T, W>(r, rp[r + 1] - rp[r], soff, sci, sval, p);
+      q[r] = s;
+    }
+    qs[h * kBlock + threadIdx.x] = s;
+  }
+  __syncthreads();
+  const int64_t R1 = rows < r0 + kTile ? rows : r0 + kTile;
+  const int64_t i0 = r0 + 2 * threadIdx.x;
+  T p0 = 0, p1 = 0;
+  if (i0 + 1 < R1) {
+    p0 = p[i0] * qs[2 * threadIdx.x];
+    p1 = p[i0 + 1] * qs[2 * threadIdx.x + 1];
+  } else if (i0 < R1) {
+    p0 = p[i0] * qs[2 * threadIdx.x];
+  }
+  const T t = block_tree(p0 + p1, sh);
+  if (threadIdx.x == 0) publish_partial(P, tile, t);
+}
+
+template <class T, bool AB, int W>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell(
+    int64_t rows, int64_t nblocks, const int32_t *__restrict__ rp,
+    const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
+    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
+    int long_t, int remap, const int32_t *live) {
+  const int64_t b = xcd_block(remap);
+  if (b >= nblocks) return;
+  if (live && *live != kStatRunning) return;
+  const int64_t r = b * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  const int32_t ra = rp[r], len = rp[r + 1] - ra;
+  if (len > long_t) return;  // a long row: k_spmv_longrows
+  const T s = sell_row<T, W>(r, len, soff, sci, sval, x);
   if (AB) {
     T t = alpha * s;
     if (beta != T(0)) {
@@ -721,6 +771,14 @@ static void spmv_sell_dispatch(Csr *A, const SpmvPlan::Sliced &l, const T *V, co
   MGX_HIP(hipGetLastError());
 }
 
+// the sliced layout of the values V (A's own, or their FP32 copy), or null
+static const SpmvPlan::Sliced *sliced_of(const Csr *A, const void *V, size_t es) {
+  if (!A->plan.sell) return nullptr;
+  if (V == A->val && es == A->elem()) return &A->plan.sl;
+  if (es == 4 && V == A->val32 && A->plan.sl32.d_soff) return &A->plan.sl32;
+  return nullptr;
+}
+
 template <class T, bool AB>
 static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
                           T beta, const int32_t *live) {
@@ -988,6 +1046,34 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
     out->column_panels = best_panels;
     out->row_walk = best_sell ? 2 : best.walk;
   }
+}
+
+bool spmv_sell_pq_ok(const Csr *A, const void *val, mgx_dtype dt) {
+  return A->plan.nslong == 0 && sliced_of(A, val, dt == MGX_F64 ? 8 : 4) != nullptr;
+}
+
+bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
+                  const int32_t *live) {
+  const size_t es = dt == MGX_F64 ? 8 : 4;
+  const SpmvPlan::Sliced *l = sliced_of(A, val, es);
+  if (!l || A->plan.nslong > 0) return false;
+  Ctx *c = A->ctx;
+  const int64_t nt = (A->rows + kTile - 1) / kTile;
+  if (nt == 0) return true;
+  const int remap = A->plan.remap && nt >= 64;
+  const unsigned grid = (unsigned)(remap ? ((nt + 7) / 8) * 8 : nt);
+  if (dt == MGX_F64)
+    k_spmv_sell_pq<double, 2><<<grid, kBlock, 0, c->stream>>>(
+        A->rows, nt, A->rp, l->d_soff, l->d_sci, static_cast<const double *>(l->d_val),
+        static_cast<const double *>(p), static_cast<double *>(q), static_cast<double *>(P),
+        remap, live);
+  else
+    k_spmv_sell_pq<float, 4><<<grid, kBlock, 0, c->stream>>>(
+        A->rows, nt, A->rp, l->d_soff, l->d_sci, static_cast<const float *>(l->d_val),
+        static_cast<const float *>(p), static_cast<float *>(q), static_cast<float *>(P), remap,
+        live);
+  MGX_HIP(hipGetLastError());
+  return true;
 }
 
 }  // namespace mgx

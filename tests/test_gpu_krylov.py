@@ -640,3 +640,37 @@ def _transpose_values(rp, ci, val):
         for e in range(rp[i], rp[i + 1]):
             out[e] = val[pos[(int(ci[e]), i)]]
     return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_split_iteration_with_sliced_spmv(m, ctx, oracle, dtype):
+    """The split CG iteration with the sliced-layout SpMV, which also forms
+    the <p, q> tile partials (the stage-3 kernel then skips the system):
+    bit-identical to the oracle, as with the row-block plan; and the
+    mixed-precision CG gives the same bits with either plan."""
+    say("split sliced", dtype.__name__)
+    rp, ci, val = m.stencil3d(96, 96, 64, 27, 0.01)
+    val = val.astype(dtype)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3003, dtype=dtype)
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n, dtype), b, precond=2, tol=1e-30,
+                       maxiter=30, history=True)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    A.set_plan(row_walk=2)
+    X = m.Vector(ctx, n=n, dtype=dtype)
+    B = m.Vector(ctx, b, dtype=dtype)
+    res = m.cg(A, X, B, tol=1e-30, maxiter=30, precond=2, mode=m.MODE_FUSED, history=30)
+    assert res.final_iter == 30 and bits_equal(X.download(), ref[1])
+    assert bits_equal(res.rhistory, ref[4])
+    if dtype == np.float64:
+        bn = float(np.linalg.norm(b))
+        xs = []
+        for walk in (2, 0):
+            A.set_plan(row_walk=walk)
+            X.fill(0.0)
+            r = m.mixed_cg(A, X, B, tol=1e-9 * bn, tol_is_relative=False, delta=1e-3)
+            assert r.status == 0
+            xs.append(X.download())
+        assert bits_equal(xs[0], xs[1])
+    for v in (A, X, B):
+        v.free()
