@@ -283,6 +283,61 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
   return s;
 }
 
+// two rows of the sliced layout at once (their loads interleaved), each
+// summed exactly as sell_row sums it; a row of length 0 gives 0
+template <class T, int W>
+__device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, int32_t lb,
+                                          const int64_t *__restrict__ soff,
+                                          const int32_t *__restrict__ sci,
+                                          const T *__restrict__ sval, const T *__restrict__ x,
+                                          T &sa, T &sb) {
+  constexpr int QT = 8 / W;
+  typedef T VT __attribute__((ext_vector_type(W)));
+  typedef int32_t VI __attribute__((ext_vector_type(W)));
+  const VT *va = reinterpret_cast<const VT *>(sval + (la ? soff[ra / kSlice] + (ra % kSlice) * W : 0));
+  const VI *ca = reinterpret_cast<const VI *>(sci + (la ? soff[ra / kSlice] + (ra % kSlice) * W : 0));
+  const VT *vb = reinterpret_cast<const VT *>(sval + (lb ? soff[rb / kSlice] + (rb % kSlice) * W : 0));
+  const VI *cb = reinterpret_cast<const VI *>(sci + (lb ? soff[rb / kSlice] + (rb % kSlice) * W : 0));
+  sa = 0;
+  sb = 0;
+  const int32_t L = la > lb ? la : lb;
+  for (int32_t j = 0; j < L; j += QT * W) {
+    VT wa[QT], wb[QT];
+    T xa[QT * W], xb[QT * W];
+#pragma unroll
+    for (int q = 0; q < QT; q++) {
+      const int64_t o = (int64_t)(j / W + q) * kSlice;
+      if (j + q * W < la) {
+        wa[q] = __builtin_nontemporal_load(va + o);
+        const VI cc = __builtin_nontemporal_load(ca + o);
+#pragma unroll
+        for (int e = 0; e < W; e++)
+          if (j + q * W + e < la) xa[q * W + e] = x[cc[e]];
+      }
+      if (j + q * W < lb) {
+        wb[q] = __builtin_nontemporal_load(vb + o);
+        const VI cc = __builtin_nontemporal_load(cb + o);
+#pragma unroll
+        for (int e = 0; e < W; e++)
+          if (j + q * W + e < lb) xb[q * W + e] = x[cc[e]];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QT; q++)
+#pragma unroll
+      for (int e = 0; e < W; e++) {
+        if (j + q * W + e < la) {
+          T p = wa[q][e] * xa[q * W + e];
+          sa = sa + p;
+        }
+        if (j + q * W + e < lb) {
+          T p = wb[q][e] * xb[q * W + e];
+          sb = sb + p;
+        }
+      }
+  }
+}
+
 // q = A p in the sliced layout for a matrix without long rows, one block per
 // 512-row tile (each lane two rows), and the tile partial of <p, q> exactly
 // as the CG engine's stage-3 kernel forms it (lane t: p[2t] q[2t] +
@@ -299,15 +354,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_pq(
   if (tile >= ntiles) return;
   if (live && *live != kStatRunning) return;
   const int64_t r0 = tile * kTile;
-#pragma unroll
-  for (int h = 0; h < kTile / kBlock; h++) {
-    const int64_t r = r0 + h * kBlock + threadIdx.x;
-    T s = 0;
-    if (r < rows) {
-      s = sell_row<T, W>(r, rp[r + 1] - rp[r], soff, sci, sval, p);
-      q[r] = s;
-    }
-    qs[h * kBlock + threadIdx.x] = s;
+  static_assert(kTile == 2 * kBlock, "two rows per lane");
+  {
+    const int64_t ra = r0 + threadIdx.x, rb = ra + kBlock;
+    const int32_t la = ra < rows ? rp[ra + 1] - rp[ra] : 0;
+    const int32_t lb = rb < rows ? rp[rb + 1] - rp[rb] : 0;
+    T sa, sb;
+    sell_row2<T, W>(ra, la, rb, lb, soff, sci, sval, p, sa, sb);
+    if (ra < rows) q[ra] = sa;
+    if (rb < rows) q[rb] = sb;
+    qs[threadIdx.x] = sa;
+    qs[kBlock + threadIdx.x] = sb;
   }
   __syncthreads();
   const int64_t R1 = rows < r0 + kTile ? rows : r0 + kTile;
