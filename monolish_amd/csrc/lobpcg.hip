@@ -158,7 +158,53 @@ __global__ __launch_bounds__(kBlock) void k_lobpcg_xw(int64_t n, int m, RowScal<
   }
 }
 
+// Y_r = A X_r for the m rows of X (row-major, stride n): one thread per
+// matrix row reads the row's entries once and keeps m sums, each added in
+// storage order (the plan-based SpMV's result for rows of at most 256
+// entries)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spmm_rows(int64_t n, int m,
+                                                      const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci,
+                                                      const T *__restrict__ val,
+                                                      const T *__restrict__ X, T *Y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  T s[kMaxPairs];
+#pragma unroll
+  for (int r = 0; r < kMaxPairs; r++) s[r] = 0;
+  for (int32_t e = rp[i]; e < rp[i + 1]; e++) {
+    const T a = val[e];
+    const int64_t c = ci[e];
+#pragma unroll
+    for (int r = 0; r < kMaxPairs; r++)
+      if (r < m) {
+        const T p = a * X[(int64_t)r * n + c];
+        s[r] = s[r] + p;
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < kMaxPairs; r++)
+    if (r < m) Y[(int64_t)r * n + i] = s[r];
+}
+
 }  // namespace
+
+bool lobpcg_spmm(Csr *A, mgx_dtype dt, int m, const void *X, void *Y) {
+  if (A->plan.max_row > 256 || A->rows != A->cols) return false;
+  Ctx *c = A->ctx;
+  const int64_t n = A->rows;
+  const unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
+  if (n == 0) return true;
+  if (dt == MGX_F64)
+    k_spmm_rows<double><<<g, kBlock, 0, c->stream>>>(n, m, A->rp, A->ci, (const double *)A->val,
+                                                     (const double *)X, (double *)Y);
+  else
+    k_spmm_rows<float><<<g, kBlock, 0, c->stream>>>(n, m, A->rp, A->ci, (const float *)A->val,
+                                                    (const float *)X, (float *)Y);
+  MGX_HIP(hipGetLastError());
+  return true;
+}
 
 void lobpcg_scale_rows(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, void *A,
                        void *B) {

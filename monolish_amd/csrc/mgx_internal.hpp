@@ -348,6 +348,9 @@ void lobpcg_gram(Ctx *c, mgx_dtype dt, int64_t n, int k, const void *wxp, const 
 // rows m..2m-1 (x) and 2m..3m-1 (p) of wxp and WXP from the previous basis
 // wxp_p, WXP_p with coef (m rows of `stride` coefficients, device); has_p:
 // the previous basis holds p
+// Y_r = A X_r for m <= 10 rows (stride n) in one pass over A, when A's rows
+// have at most 256 entries (then the sums equal the SpMV's); false otherwise
+bool lobpcg_spmm(Csr *A, mgx_dtype dt, int m, const void *X, void *Y);
 // rows r of A (and of B when not null), r < m <= 10, scaled by (T)s[r]
 void lobpcg_scale_rows(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, void *A,
                        void *B);

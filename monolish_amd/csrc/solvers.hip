@@ -1046,8 +1046,11 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
       iter--;
       continue;
     }
-    launch_copy(c, dt, (int64_t)k * n, wxp, wxp_p);  // :197-198
-    launch_copy(c, dt, (int64_t)k * n, WXP, WXP_p);
+    // :197-198 copy the basis to the previous-step buffers; every row of
+    // the current buffers is rewritten below (x, p by the combine, w and W
+    // after it), so exchanging the buffers gives the same values
+    std::swap(wxp, wxp_p);
+    std::swap(WXP, WXP_p);
     for (int64_t i = 0; i < m; i++) {
       lambda[i] = (double)(T)w[i];  // :203
       for (int j = 0; j < k3; j++) hcoef[i * k3 + j] = j < k ? (T)V[i * k + j] : (T)0;
@@ -1094,8 +1097,9 @@ SolveOut lobpcg_run(Csr *A, T *Xio, int64_t m, double *lambda, PrecSpec precond,
       sw[i] = (double)(T)(1.0 / (double)residual[i]);
     }
     lobpcg_scale_rows(c, dt, n, (int)m, sw.data(), wxp, nullptr);  // :272
-    for (int64_t i = 0; i < m; i++)
-      launch_spmv(A, A->val, dt, row(wxp, i), row(WXP, i), 1, 0, false);  // :274
+    if (!lobpcg_spmm(A, dt, (int)m, wxp, WXP))                        // :274
+      for (int64_t i = 0; i < m; i++)
+        launch_spmv(A, A->val, dt, row(wxp, i), row(WXP, i), 1, 0, false);
     resid = *std::max_element(residual.begin(), residual.end());  // :277
     if (rhist && iter < rhist_len) rhist[iter] = (double)resid;
     if ((double)resid < tol && miniter < iter + 1) {  // :279-287
