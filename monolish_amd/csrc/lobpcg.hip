@@ -1,6 +1,7 @@
 // lobpcg.hip -- device kernels of the LOBPCG eigensolver
-// (src/eigen/lobpcg_standard.cpp:10-300): the Gram matrices of the basis
-// and the basis update.
+// (src/eigen/lobpcg_standard.cpp:10-300): the Gram matrices of the basis,
+// the basis update, the per-pair scalings and residuals for all pairs in
+// one launch each, and W = A w for all pairs in one pass over A.
 //
 // The basis {w, x, p} (k = 2m or 3m vectors of n entries, one vector per
 // row, as the reference's Dense wxp) and its image {W, X, P} = A{w, x, p}
