@@ -117,6 +117,9 @@ struct TriPlan {
   // position, first dependency entry}, nblk + 1 entries (0: not staged)
   int4 *d_blk = nullptr;
   int32_t nblk = 0;
+  // dependency-driven sweep across the chip (ticket: its work counter)
+  bool flow = false;
+  uint32_t *d_ticket = nullptr;
 };
 // plans of a matrix: entries left of the diagonal; left of it up to the
 // first other entry (the SOR solver's sweep); right of it; rows of L^T (IC,
@@ -129,7 +132,7 @@ enum { kTriMul = 0, kTriSorBack = 1, kTriDiv = 2, kTriUnit = 3 };
 struct TriOp {
   const TriPlan *plan = nullptr;
   void *coef = nullptr, *sc = nullptr;
-  void *ginit = nullptr;  // staged plans: the initial values in level order
+  void *ginit = nullptr;  // staged and flow plans: the initial values in level order
   int fin = kTriMul;
 };
 

@@ -364,12 +364,158 @@ __global__ __launch_bounds__(kSweepWgThreads) void k_tri_staged(
   }
 }
 
+// The dependency-driven form, for plans too large or too wide for one
+// workgroup: persistent waves take 64 consecutive level-order positions at a
+// time from a ticket counter and each lane computes its row as soon as the
+// rows it reads are written, with no barrier between levels.  Readiness
+// travels with the value: before the sweep every out_i holds kEmpty, a
+// signalling NaN that no arithmetic produces, and a row publishes its value
+// with one device-coherent atomic exchange, so a reader that finds
+// something else has the final value.  A lane loads its entries eight at a
+// time and consumes them strictly in the row's order; at the first one
+// still kEmpty it polls that one entry with a device-coherent atomic read
+// (a plain load may keep seeing a cached copy) until it is written, then
+// goes on, so the arithmetic is the serial sweep's.  Rows only read rows of earlier
+// levels, which sit at earlier positions, so their tickets were taken by
+// waves that are already running: the sweep cannot deadlock.  A row that
+// still waits after kFlowSpinTicks of the 100 MHz wall clock gives up and
+// publishes NaN, so the kernel always ends and the solver reports a breakdown.
+constexpr int kFlowThreads = 256;
+constexpr int kFlowBlocks = 256;
+constexpr uint64_t kFlowSpinTicks = 20000000;  // 0.2 s
+// plans of at least kFlowMinRows rows whose levels hold kFlowMinWidth rows
+// on average take the dependency-driven form
+constexpr int64_t kFlowMinRows = 1 << 17, kFlowMinWidth = 256;
+
+template <class T> struct FlowBits;
+template <> struct FlowBits<double> {
+  using U = uint64_t;
+  static constexpr U kEmpty = 0x7ff0deadbeef0001ull, kNan = 0x7ff8000000000000ull;
+};
+template <> struct FlowBits<float> {
+  using U = uint32_t;
+  static constexpr U kEmpty = 0x7f80dea1u, kNan = 0x7fc00000u;
+};
+
+// ginit[k] = the initial value of row rec[k].x (out for the SOR backward
+// sweep, in otherwise), then out of that row = kEmpty; in may alias out
 template <class T, int FIN>
-__global__ __launch_bounds__(kBlock) void k_tri_level(
-    int32_t k0, int32_t k1, const int4 *__restrict__ rec, const int32_t *__restrict__ dcol,
-    const T *__restrict__ dcoef, const T *__restrict__ sc, const T *__restrict__ in, T *out) {
-  const int32_t k = k0 + (int32_t)(blockIdx.x * kBlock + threadIdx.x);
-  if (k < k1) tri_row<T, FIN>(rec[k], dcol, dcoef, sc[k], in, out);
+__global__ __launch_bounds__(kBlock) void k_flow_init(int64_t n, const int4 *__restrict__ rec,
+                                                      const T *in, T *out, T *ginit,
+                                                      uint32_t *ticket) {
+  using U = typename FlowBits<T>::U;
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k == 0) *ticket = 0u;
+  if (k < n) {
+    const int32_t i = rec[k].x;
+    ginit[k] = FIN == 1 ? out[i] : in[i];
+    reinterpret_cast<U *>(out)[i] = FlowBits<T>::kEmpty;
+  }
+}
+
+template <class T, int FIN>
+__global__ __launch_bounds__(kFlowThreads) void k_tri_flow(
+    int32_t n, const int4 *__restrict__ rec, const int32_t *__restrict__ dcol,
+    const T *__restrict__ dcoef, const T *__restrict__ sc, const T *__restrict__ ginit, T *out,
+    uint32_t *ticket) {
+  using U = typename FlowBits<T>::U;
+  using Acc = typename std::conditional<FIN == 1, double, T>::type;
+  U *ob = reinterpret_cast<U *>(out);
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t nchunk = (uint32_t)((n + 63) / 64);
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0)
+      t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= nchunk) break;
+    const int32_t k = (int32_t)(t * 64u) + lane;
+    const bool has = k < n;
+    const int4 rc = has ? rec[k] : make_int4(0, 0, 0, 0);
+    int32_t d = rc.y;
+    const int32_t d1 = has ? rc.z : rc.y;
+    const T init = has ? ginit[k] : (T)0;
+    Acc acc = FIN == 1 ? (Acc)0 : (Acc)init;
+    const uint64_t t0 = wall_clock64();
+    int32_t wc = -1;  // >= 0: column of entry d, which was still kEmpty
+    T wa = (T)0;
+    // a lane publishes its row as soon as its last entry is consumed: a row
+    // later in the same 64 may read it
+    bool pending = has;
+    while (__any(pending)) {
+      if (pending && d < d1) {
+        if (wc >= 0) {
+          const U z = __hip_atomic_fetch_or(ob + wc, (U)0, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+          if (z != FlowBits<T>::kEmpty) {
+            const T p = wa * __builtin_bit_cast(T, z);
+            if (FIN == 1)
+              acc = acc + (double)p;
+            else
+              acc = acc - p;
+            d++;
+            wc = -1;
+          }
+        } else {
+          int32_t c[kTriChunk];
+          T a[kTriChunk];
+          U zb[kTriChunk];
+#pragma unroll
+          for (int u = 0; u < kTriChunk; u++) {
+            const int32_t e = d + u < d1 ? d + u : d;
+            c[u] = dcol[e];
+            a[u] = dcoef[e];
+          }
+#pragma unroll
+          for (int u = 0; u < kTriChunk; u++)
+            zb[u] = d + u < d1 ? __hip_atomic_load(ob + c[u], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                               : (U)0;
+          bool stop = false;
+          int32_t taken = 0;
+#pragma unroll
+          for (int u = 0; u < kTriChunk; u++) {
+            if (!stop && d + u < d1) {
+              if (zb[u] == FlowBits<T>::kEmpty) {
+                stop = true;
+                wc = c[u];
+                wa = a[u];
+              } else {
+                const T p = a[u] * __builtin_bit_cast(T, zb[u]);
+                if (FIN == 1)
+                  acc = acc + (double)p;
+                else
+                  acc = acc - p;
+                taken++;
+              }
+            }
+          }
+          d += taken;
+        }
+        if (d < d1 && wc >= 0 && wall_clock64() - t0 > kFlowSpinTicks) {
+          acc = (Acc)__builtin_bit_cast(T, FlowBits<T>::kNan);
+          d = d1;
+        }
+      }
+      if (pending && d >= d1) {
+        const T s = sc[k];
+        T v;
+        if (FIN == 0)
+          v = acc * s;
+        else if (FIN == 1)
+          v = (T)((double)init - acc * (double)s);
+        else if (FIN == 2)
+          v = acc / s;
+        else
+          v = acc;
+        U vb = __builtin_bit_cast(U, v);
+        if (vb == FlowBits<T>::kEmpty) vb = FlowBits<T>::kNan;
+        __hip_atomic_exchange(ob + rc.x, vb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pending = false;
+      }
+      if (__any(pending && wc >= 0)) __builtin_amdgcn_s_sleep(2);
+    }
+  }
 }
 
 // coef[d] = vals[dent[d]]
@@ -397,7 +543,15 @@ void tri_run_t(Ctx *c, const TriPlan &P, const T *coef, const T *sc, const T *in
                T *ginit) {
   const SorLevels &L = P.lev;
   if (L.nlev == 0) return;
-  if (P.nblk > 0) {
+  if (P.flow) {
+    const int64_t n = L.ptr[L.nlev];
+    k_flow_init<T, FIN><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
+        n, P.d_rec, in, out, ginit, P.d_ticket);
+    const int64_t want = (n + 63) / 64 / (kFlowThreads / 64) + 1;
+    k_tri_flow<T, FIN><<<(unsigned)std::min<int64_t>(want, kFlowBlocks), kFlowThreads, 0,
+                         c->stream>>>((int32_t)n, P.d_rec, P.d_dcol, coef, sc, ginit, out,
+                                      P.d_ticket);
+  } else if (P.nblk > 0) {
     const int64_t n = L.ptr[L.nlev];
     k_tri_gather<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
         n, P.d_rec, FIN == 1 ? out : in, ginit);
@@ -406,15 +560,9 @@ void tri_run_t(Ctx *c, const TriPlan &P, const T *coef, const T *sc, const T *in
   } else if (L.max_width <= kSweepWgThreads) {
     k_tri_pipe<T, FIN><<<1, kSweepWgThreads, 0, c->stream>>>(
         L.d_ptr, L.nlev, P.d_rec, P.d_dcol, P.d_dpos, coef, sc, in, out);
-  } else if (L.max_width <= kSweepWgRows) {
+  } else {
     k_tri_wg<T, FIN><<<1, kSweepWgThreads, 0, c->stream>>>(L.d_ptr, L.nlev, P.d_rec, P.d_dcol,
                                                            coef, sc, in, out);
-  } else {
-    for (int32_t l = 0; l < L.nlev; l++) {
-      const int32_t k0 = L.ptr[l], k1 = L.ptr[l + 1];
-      k_tri_level<T, FIN><<<(unsigned)((k1 - k0 + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
-          k0, k1, P.d_rec, P.d_dcol, coef, sc, in, out);
-    }
   }
   MGX_HIP(hipGetLastError());
 }
@@ -501,10 +649,13 @@ void build_tri_plan(const std::vector<int32_t> &ptr, const std::vector<int32_t> 
   P.d_dent = (int32_t *)dev_alloc(dent.size() * 4);
   P.d_dpos = (int32_t *)dev_alloc(dpos.size() * 4);
   MGX_HIP(hipMemcpy(P.d_dpos, dpos.data(), dpos.size() * 4, hipMemcpyHostToDevice));
-  // blocks of whole levels for the staged sweep (none when a level alone
-  // exceeds a block)
+  // large plans with wide levels sweep dependency-driven across the chip;
+  // the others in one workgroup, staged in blocks of whole levels when no
+  // level alone exceeds a block
+  P.flow = L.max_width > kSweepWgRows || (n >= kFlowMinRows && n >= kFlowMinWidth * (int64_t)nlev);
+  if (P.flow) P.d_ticket = (uint32_t *)dev_alloc(256);
   std::vector<int4> blk;
-  bool fits = true;
+  bool fits = !P.flow;
   for (int32_t l = 0; l < nlev && fits; l++) {
     const int32_t p0 = L.ptr[l], p1 = L.ptr[l + 1];
     const int32_t e0 = rec[p0].y, e1 = rec[p1 - 1].z;
@@ -555,7 +706,7 @@ TriOp tri_prepare(Csr *A, const TriPlan &P, mgx_dtype dt, int fin, const void *v
   op.fin = fin;
   op.coef = ws_get(c, (size_t)std::max<int64_t>(P.ndeps, 1) * es);
   op.sc = ws_get(c, (size_t)std::max<int64_t>(n, 1) * es);
-  op.ginit = P.nblk > 0 ? ws_get(c, (size_t)std::max<int64_t>(n, 1) * es) : nullptr;
+  op.ginit = P.nblk > 0 || P.flow ? ws_get(c, (size_t)std::max<int64_t>(n, 1) * es) : nullptr;
   const unsigned gd = (unsigned)((P.ndeps + kBlock - 1) / kBlock);
   const unsigned gn = (unsigned)((n + kBlock - 1) / kBlock);
   if (dt == MGX_F64) {
@@ -598,6 +749,7 @@ void tri_free(Csr *A) {
     dev_free(P.d_dent);
     dev_free(P.d_dpos);
     dev_free(P.d_blk);
+    dev_free(P.d_ticket);
     P = TriPlan();
   }
 }

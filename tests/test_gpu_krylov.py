@@ -371,7 +371,7 @@ def test_sor_preconditioned_solvers_bit_exact(m, ctx, oracle, dtype, omega):
 def test_sor_solver_bit_exact(m, ctx, oracle, dtype):
     """equation::SOR on the device (lower sweep level by level) against the
     oracle's serial loop: Poisson, a tridiagonal chain (one row per level),
-    independent blocks (wide levels, one launch per level), an unsorted row
+    independent blocks (levels wider than one workgroup's sweep), an unsorted row
     whose sweep stops at its first upper entry, and the reference harness
     (test.mtx, b = A 1, x0 random, x -> 1 within 10 tol)."""
     say("sor solver", dtype.__name__)
@@ -480,7 +480,7 @@ def _factor_cases(m, kind):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_ic_ilu_factor_and_apply_bit_exact(m, ctx, oracle, kind, dtype):
     """IC::solve / ILU::solve on the device (level-scheduled factorisation
-    and triangular solves, one launch per sweep or one per level for the
+    and triangular solves in one workgroup, or across the chip for the
     independent-blocks matrix, whose levels are 6000 rows wide) reproduce the
     oracle's serial factor-and-apply bit for bit."""
     for name, (rp, ci, val) in _factor_cases(m, kind):
@@ -522,6 +522,40 @@ def test_ic_ilu_preconditioned_solvers_bit_exact(m, ctx, oracle, dtype):
             assert res.status == 0 and res.final_iter == ref[2]
             assert bits_equal(x, ref[1])
             assert bits_equal(res.rhistory, ref[4])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_large_wide_level_sweeps_bit_exact(m, ctx, oracle, dtype):
+    """A 27-point 64x64x48 system (196608 rows, 378 levels of about 520
+    rows) takes the dependency-driven sweep across the whole chip: IC(0) and
+    ILU(0) factor-and-apply, and 12 iterations of SOR-, IC- and
+    ILU-preconditioned CG, bit-identical to the oracle's serial sweeps."""
+    rp, ci, val = m.stencil3d(64, 64, 48, 27, 1e-2)
+    val = val.astype(dtype)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 57, dtype=dtype)
+    for kind in (4, 5):
+        say("wide factor", kind, dtype.__name__)
+        rc, ref = oracle.ilu_precond(rp, ci, val, kind, b)
+        assert rc == 0
+        A = m.CRS(ctx, n, n, rp, ci, val)
+        X = m.Vector(ctx, n=n, dtype=dtype)
+        B = m.Vector(ctx, b, dtype=dtype)
+        res = (m.ic_solve if kind == 4 else m.ilu_solve)(A, X, B)
+        assert res.status == 0
+        assert bits_equal(X.download(), ref)
+        for v in (A, X, B):
+            v.free()
+    tol = float(1e-30 * oracle.nrm2(b))
+    for kind in (m.PRECOND_SOR, 4, 5):
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n, dtype), b, precond=kind, tol=tol,
+                           maxiter=12, history=True)
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n, dtype), b, kind, tol,
+                           maxiter=12, mode=m.MODE_REFERENCE, history=12)
+        say("wide cg", kind, dtype.__name__, res.status, res.final_iter, ref[0], ref[2])
+        assert res.status == ref[0] and res.final_iter == ref[2]
+        assert bits_equal(x, ref[1])
+        assert bits_equal(res.rhistory, ref[4])
 
 
 def test_ic_ilu_errors(m, ctx, oracle):
