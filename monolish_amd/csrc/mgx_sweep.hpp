@@ -8,7 +8,8 @@
 // loop's order and sees the values the serial loop would have seen.
 // Schedules whose levels are at most kSweepWgRows wide run inside one
 // 1024-thread workgroup with a barrier between levels (one launch per
-// sweep); wider schedules launch once per level.
+// sweep); wider schedules, and large ones with wide levels on average,
+// launch once per level.
 #pragma once
 
 #include "mgx_device.hpp"
@@ -38,9 +39,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep_level(const int32_t *__restric
 }
 
 // run op(row) over every row of the schedule L, level by level, on c's stream
+// schedules of at least kSweepChipRows rows whose levels average
+// kSweepChipWidth rows or more launch once per level even when every level
+// fits the workgroup: one CU would carry all of their rows
+constexpr int64_t kSweepChipRows = 1 << 17, kSweepChipWidth = 256;
+
 template <class Op> void run_sweep(Ctx *c, const SorLevels &L, const Op &op) {
   if (L.nlev == 0) return;
-  if (L.max_width <= kSweepWgRows) {
+  const int64_t n = L.ptr[L.nlev];
+  const bool chip = n >= kSweepChipRows && n >= kSweepChipWidth * (int64_t)L.nlev;
+  if (L.max_width <= kSweepWgRows && !chip) {
     k_sweep_wg<Op><<<1, kSweepWgThreads, 0, c->stream>>>(L.d_ptr, L.nlev, L.d_rows, op);
   } else {
     for (int32_t l = 0; l < L.nlev; l++) {
