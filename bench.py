@@ -96,13 +96,19 @@ def cpu_info():
     return model
 
 
-def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup, autotune=True):
+def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup, plan_spec=None):
     rows = len(rp) - 1
     A = m.CRS(ctx, rows, len(x), rp, ci, val)
     X = m.Vector(ctx, x, dtype=val.dtype)
     Y = m.Vector(ctx, n=rows, dtype=val.dtype)
-    # the library's autotuner picks the block plan (outside the timed region)
-    plan = A.autotune(reps=10) if autotune else None
+    if plan_spec:
+        # a fixed plan (profiling runs: every dispatch of the kernel is timed work)
+        plan = dict(parse_plan(plan_spec), long_row=256, candidates=0)
+        A.set_plan(**{k: plan[k] for k in ("rows_per_block", "nnz_per_block", "xcd_remap",
+                                           "row_walk", "column_panels")})
+    else:
+        # the library's autotuner picks the plan (outside the timed region)
+        plan = A.autotune(reps=20)
     ms, wall = time_spmv(m, ev, ctx, A, X, Y, steps, warmup)
     nbytes = spmv_bytes(len(val), rows, len(x), val.dtype.itemsize)
     gbs = nbytes / (ms * 1e-3) / 1e9
@@ -112,11 +118,10 @@ def bench_spmv_matrix(m, ev, ctx, name, rp, ci, val, x, steps, warmup, autotune=
     Y.free()
     out = {"us_per_spmv": round(ms * 1e3, 2), "GB_s": round(gbs, 1),
            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes, "nnz": int(len(val)),
-           "rows": rows}
-    if plan:
-        out["plan"] = {k: plan[k] for k in ("rows_per_block", "nnz_per_block", "long_row",
-                                            "xcd_remap", "row_walk", "column_panels",
-                                            "candidates")}
+           "rows": rows, "kernel": spmv_kernel_name(plan, val.dtype)}
+    out["plan"] = {k: plan[k] for k in ("rows_per_block", "nnz_per_block", "long_row",
+                                        "xcd_remap", "row_walk", "column_panels", "candidates")
+                   if k in plan}
     return out, ms, wall
 
 
@@ -640,21 +645,53 @@ def cpu_baseline_cg(rp, ci, val, b, tol, x_gpu):
                                                       x_gpu.view(np.uint64)))}
 
 
-def cpu_baseline_spmv(rp, ci, val, x, budget_s):
-    import oracle
-    oracle.spmv(rp, ci, val, x)  # warm-up
-    ts = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end and len(ts) < 200:
-        t0 = time.perf_counter()
-        oracle.spmv(rp, ci, val, x)
-        ts.append(time.perf_counter() - t0)
-    med = statistics.median(ts)
-    gbs = spmv_bytes(len(val), len(rp) - 1, len(x), 8) / med / 1e9
-    return {"value": round(gbs, 2), "unit": "GB/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"oracle (CPU restatement of the reference OpenMP CRS row loop) on the full "
-                      f"192^3 7-point FP64 matrix, median of {len(ts)} SpMVs "
-                      f"({med * 1e3:.1f} ms each), CPU: {cpu_info()}"}
+def cpu_baseline_spmv(budget_s):
+    """The oracle's OpenMP SpMV (the reference's CRS row loop) on the full
+    192^3 7-point FP64 matrix, in a child process that pins its threads
+    (physical cores, OMP_PROC_BIND=close, OMP_PLACES=cores) before libgomp
+    loads; median of >= 5 SpMVs after one warm-up."""
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--matrix", "7pt192",
+                        "--seconds", str(budget_s)], capture_output=True, text=True, cwd=ROOT,
+                       timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {r.stderr[-2000:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    log(f"cpu baseline: {out}")
+    return out
+
+
+def spmv_kernel_name(plan, dtype) -> str:
+    """The kernel that runs the SpMV under an autotuned plan, as rocprofv3
+    names it (without the argument list)."""
+    f64 = np.dtype(dtype) == np.float64
+    t = "double" if f64 else "float"
+    w = 16 // np.dtype(dtype).itemsize
+    if plan["row_walk"] == 2:
+        return f"k_spmv_sell<{t}, false, {w}>"
+    if plan["column_panels"]:
+        return f"k_spmv_panel<{t}"
+    u = (plan["nnz_per_block"] + w - 1 + 256 * w - 1) // (256 * w)
+    u = next(k for k in (1, 2, 3, 4, 5, 6, 8, 9, 12, 17) if u <= k)
+    return f"k_spmv<{t}, false, {u}, {plan['row_walk']}>"
+
+
+def parse_plan(spec: str):
+    """'row_walk,rows_per_block,nnz_per_block,xcd_remap,column_panels'."""
+    w, r, z, x, p = (int(v) for v in spec.split(","))
+    return dict(row_walk=w, rows_per_block=r, nnz_per_block=z, xcd_remap=x, column_panels=p)
+
+
+def traffic_for(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary, or None when that kernel was not profiled."""
+    path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for e in json.load(f).get("entries", []):
+            if e.get("workload") == workload and e.get("kernel") == kernel:
+                return e.get("hbm_bytes_per_launch")
+    return None
 
 
 def main():
@@ -670,10 +707,14 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-nmax", type=int, default=100000,
                     help="largest served system (rows); the full stream uses 1e6")
-    ap.add_argument("--traffic-probe", action="store_true",
-                    help="only run 5 vadd launches of known size (counter calibration) and, "
-                         "after the autotuner, 5 SpMVs on the 192^3 matrix (the last 5 SpMV "
-                         "dispatches), for rocprofv3 --pmc runs")
+    ap.add_argument("--plan", default="",
+                    help="fix the headline SpMV plan instead of autotuning: "
+                         "'row_walk,rows_per_block,nnz_per_block,xcd_remap,column_panels' "
+                         "(row_walk 2 = sliced layout); for profiling runs")
+    ap.add_argument("--traffic-probe", default="",
+                    help="c2a|c2b[:f32]: only run 5 vadd launches of known size (counter "
+                         "calibration), then 5 SpMVs on that matrix under --plan (else the "
+                         "autotuned plan), for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
@@ -695,33 +736,47 @@ def main():
         for _ in range(5):
             m.vadd(a, b, y)
         ctx.sync()
-        rp, ci, val = m.stencil3d(192, 192, 192, 7)
-        A = m.CRS(ctx, len(rp) - 1, len(rp) - 1, rp, ci, val)
-        X = m.Vector(ctx, np.ones(len(rp) - 1))
-        Y = m.Vector(ctx, n=len(rp) - 1)
-        A.autotune(reps=10)  # the plan the headline measures
+        for v in (a, b, y):
+            v.free()
+        mat, _, prec = args.traffic_probe.partition(":")
+        dt = np.float32 if prec == "f32" else np.float64
+        if mat == "c2a":
+            rp, ci, val = m.stencil3d(192, 192, 192, 7)
+        else:
+            rp, ci, val = m.random_lognormal(4_000_000, 4_000_000, 32.0, 1.0, 2048, seed=2002)
+        val = val.astype(dt)
+        rows = len(rp) - 1
+        A = m.CRS(ctx, rows, rows, rp, ci, val)
+        X = m.Vector(ctx, m.random_vector(rows, -1.0, 1.0, 2001).astype(dt), dtype=dt)
+        Y = m.Vector(ctx, n=rows, dtype=dt)
+        if args.plan:
+            plan = parse_plan(args.plan)
+            A.set_plan(**plan)
+        else:
+            plan = A.autotune(reps=20)
         ctx.sync()
         for _ in range(5):
             m.matvec(A, X, Y)
         ctx.sync()
         print(json.dumps({"vadd_read_bytes": 2 * n * 8, "vadd_write_bytes": n * 8,
-                          "spmv_alg_bytes": spmv_bytes(len(val), len(rp) - 1, len(rp) - 1, 8)}))
+                          "spmv_alg_bytes": spmv_bytes(len(val), rows, rows, dt().itemsize),
+                          "kernel": spmv_kernel_name(plan, dt), "plan": plan}))
         ctx.close()
         return
 
     # ---- headline: SpMV, 3D 7-point 192^3, FP64 ---------------------------
     rp, ci, val = m.stencil3d(192, 192, 192, 7)
     x = m.random_vector(len(rp) - 1, -1.0, 1.0, 2001)
-    head, ms, wall = bench_spmv_matrix(m, ev, ctx, "C2a fp64", rp, ci, val, x, args.steps, args.warmup)
+    head, ms, wall = bench_spmv_matrix(m, ev, ctx, "C2a fp64", rp, ci, val, x, args.steps,
+                                       args.warmup, plan_spec=args.plan or None)
     sweep = {"c2a_7pt_192_fp64": head}
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "spmv_c2a_fp64_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # the kernel the chosen plan runs, and its HBM traffic from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that same kernel
+    traffic = traffic_for("c2a_fp64", head["kernel"])
     roofline = {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_spmv<double,false>", "bytes_per_launch": head["bytes"]}
+                "kernel": head["kernel"], "plan": head["plan"],
+                "bytes_per_launch": head["bytes"]}
     line = {
         "metric": "CG solves/s @1e-10 + p99 latency vs plain FP64 GPU CG; SpMV HBM GB/s",
         "value": head["GB_s"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -738,7 +793,7 @@ def main():
     if args.quick:
         legs = set()
     if "cpu" in legs:
-        line["cpu_baseline"] = cpu_baseline_spmv(rp, ci, val, x, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline_spmv(args.cpu_seconds)
     if "sweep" in legs:
         r32, _, _ = bench_spmv_matrix(m, ev, ctx, "C2a fp32", rp, ci, val.astype(np.float32),
                                       x.astype(np.float32), args.steps, args.warmup)
@@ -753,6 +808,9 @@ def main():
                                       x2.astype(np.float32), max(args.steps // 4, 10), args.warmup)
         sweep["c2b_lognormal_4M_fp64"] = s64
         sweep["c2b_lognormal_4M_fp32"] = s32
+        for key, wl in (("c2a_7pt_192_fp32", "c2a_fp32"), ("c2b_lognormal_4M_fp64", "c2b_fp64"),
+                        ("c2b_lognormal_4M_fp32", "c2b_fp32")):
+            sweep[key]["traffic"] = traffic_for(wl, sweep[key]["kernel"])
         del rp2, ci2, v2, x2
         line["spmv_sweep"] = sweep
     if "krylov" in legs:

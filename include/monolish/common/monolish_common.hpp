@@ -48,9 +48,13 @@
 namespace monolish {
 
 // ------------------------------------------------------------------ Logger
-// Level-gated timing log with the reference's hooks (monolish_logger.hpp:
-// 19-92).  Level 0 (default) records nothing.  Level >= 1 logs solvers,
-// >= 2 BLAS functions, >= 3 utilities, one YAML-like line per finished call.
+// Level-gated timing log with the reference's hooks and output format
+// (monolish_logger.hpp:19-92, logger_utils.cpp:7-178).  Level 0 (default)
+// records nothing.  Level >= 1 logs solvers: an "IN" line when the solver
+// starts and an "OUT" line with its wall time when it returns.  Level >= 2
+// adds one line per finished BLAS call, level >= 3 one per utility call
+// (send, recv, ...).  The name is the path of nested calls, each followed by
+// '/'.  Each hook first waits for the device, so the times are device times.
 class Logger {
  public:
   size_t LogLevel = 0;
@@ -65,36 +69,55 @@ class Logger {
     if (3 < L) throw std::runtime_error("error bad LogLevel");
     LogLevel = L;
   }
+  // an unopenable file leaves the log on std::cout
   void set_log_filename(const std::string file) {
     out_.reset(new std::ofstream(file));
     if (out_->fail()) out_.reset();
   }
-  void solver_in(const std::string name) { in(1, name); }
-  void solver_out() { out(1, "solver"); }
-  void func_in(const std::string name) { in(2, name); }
-  void func_out() { out(2, "func"); }
-  void util_in(const std::string name) { in(3, name); }
-  void util_out() { out(3, "util"); }
+  void solver_in(const std::string func_name) {
+    if (LogLevel < 1) return;
+    push(func_name);
+    stream() << "- {type : solver, name : " << path() << ", stat : IN}" << std::endl;
+  }
+  void solver_out() { pop(1, "solver", true); }
+  void func_in(const std::string func_name) {
+    if (LogLevel >= 2) push(func_name);
+  }
+  void func_out() { pop(2, "func", false); }
+  void util_in(const std::string func_name) {
+    if (LogLevel >= 3) push(func_name);
+  }
+  void util_out() { pop(3, "util", false); }
 
  private:
   Logger() = default;
-  void in(size_t lvl, const std::string &name) {
-    if (LogLevel < lvl) return;
+  std::ostream &stream() { return out_ ? *out_ : std::cout; }
+  std::string path() const {
+    std::string p;
+    for (auto &c : calls_) p += c + "/";
+    return p;
+  }
+  void push(const std::string &name) {
+    sync_device();
     calls_.push_back(name);
     times_.push_back(std::chrono::system_clock::now());
   }
-  void out(size_t lvl, const char *type) {
+  void pop(size_t lvl, const char *type, bool stat) {
     if (LogLevel < lvl || calls_.empty()) return;
-    mgx_sync_default();
-    const double t = std::chrono::duration<double>(std::chrono::system_clock::now() - times_.back()).count();
-    std::string path;
-    for (auto &c : calls_) path += c + "/";
-    std::ostream &os = out_ ? *out_ : std::cout;
-    os << "- {type : " << type << ", name : " << path << ", time : " << t << "}" << std::endl;
+    sync_device();
+    const double sec =
+        std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::system_clock::now() -
+                                                             times_.back())
+            .count() /
+        1.0e+9;
+    std::ostream &os = stream();
+    os << "- {type : " << type << ", name : " << path() << ", ";
+    if (stat) os << "stat : OUT, ";
+    os << "time : " << sec << "}" << std::endl;
     calls_.pop_back();
     times_.pop_back();
   }
-  static void mgx_sync_default();
+  static void sync_device();
   std::vector<std::string> calls_;
   std::vector<std::chrono::system_clock::time_point> times_;
   std::unique_ptr<std::ostream> out_;
@@ -109,7 +132,10 @@ class Context {
     static Context c;
     return c.ctx_;
   }
+  // the context if one was created, else nullptr (creates nothing)
+  static mgx_ctx *existing() { return live(); }
   ~Context() {
+    live() = nullptr;
     if (ctx_) mgx_finalize(ctx_);
   }
 
@@ -117,6 +143,11 @@ class Context {
   Context() {
     if (mgx_init(0, &ctx_) != MGX_OK)
       throw std::runtime_error(std::string("monolish_amd: no usable GPU: ") + mgx_last_error());
+    live() = ctx_;
+  }
+  static mgx_ctx *&live() {
+    static mgx_ctx *p = nullptr;
+    return p;
   }
   mgx_ctx *ctx_ = nullptr;
 };
@@ -151,7 +182,35 @@ struct CsrHandle {
 
 }  // namespace detail
 
-inline void Logger::mgx_sync_default() { mgx_sync(detail::Context::get()); }
+// waits for the library's device work, if a device context exists yet
+inline void Logger::sync_device() {
+  if (mgx_ctx *c = detail::Context::existing()) mgx_sync(c);
+}
+
+namespace detail {
+// Scoped Logger::func_in/func_out and util_in/util_out pairs.  BLAS calls
+// are logged under the reference's kernel names (Ddot_core, Smatvec_core,
+// copy_core, vvsub_core, ...), so nested paths read as the reference's do.
+class FuncLog {
+ public:
+  explicit FuncLog(const std::string &name) { Logger::get_instance().func_in(name); }
+  ~FuncLog() { Logger::get_instance().func_out(); }
+  FuncLog(const FuncLog &) = delete;
+  FuncLog &operator=(const FuncLog &) = delete;
+};
+class UtilLog {
+ public:
+  explicit UtilLog(const std::string &name) { Logger::get_instance().util_in(name); }
+  ~UtilLog() { Logger::get_instance().util_out(); }
+  UtilLog(const UtilLog &) = delete;
+  UtilLog &operator=(const UtilLog &) = delete;
+};
+// "D" or "S" + op + "_core"
+template <class F> std::string core_name(const char *op) {
+  return std::string(std::is_same<typename std::remove_cv<F>::type, float>::value ? "S" : "D") +
+         op + "_core";
+}
+}  // namespace detail
 
 // ------------------------------------------------------------------ vector
 template <typename Float> class vector {
@@ -250,6 +309,7 @@ template <typename Float> class vector {
   // ---- residency (src/utils/system/gpu_comm.cpp:8-73)
   [[nodiscard]] bool get_device_mem_stat() const { return dev_ && dev_->h; }
   void send() const {
+    detail::UtilLog log_("send");
     auto *self = const_cast<vector *>(this);
     if (get_device_mem_stat()) {
       detail::check(mgx_vec_upload(dev_->h, data()));
@@ -261,14 +321,19 @@ template <typename Float> class vector {
     }
   }
   void recv() {
+    detail::UtilLog log_("recv");
     if (!get_device_mem_stat()) return;
-    nonfree_recv();
-    device_free();
+    detail::check(mgx_vec_download(dev_->h, data()));
+    dev_.reset();
   }
   void nonfree_recv() {
+    detail::UtilLog log_("nonfree_recv");
     if (get_device_mem_stat()) detail::check(mgx_vec_download(dev_->h, data()));
   }
-  void device_free() const { const_cast<vector *>(this)->dev_.reset(); }
+  void device_free() const {
+    detail::UtilLog log_("device_free");
+    const_cast<vector *>(this)->dev_.reset();
+  }
 
   void print_all(bool force_cpu = false) const {
     if (get_device_mem_stat() && !force_cpu) {
@@ -616,6 +681,7 @@ template <typename Float> class CRS {
   // ---- residency (gpu_comm.cpp:89-171)
   [[nodiscard]] bool get_device_mem_stat() const { return dev_ && dev_->h; }
   void send() const {
+    detail::UtilLog log_("send");
     auto *self = const_cast<CRS *>(this);
     if (get_device_mem_stat()) {
       detail::check(mgx_csr_update_values(dev_->h, data()));
@@ -628,15 +694,20 @@ template <typename Float> class CRS {
     }
   }
   void recv() {
+    detail::UtilLog log_("recv");
     if (!get_device_mem_stat()) return;
-    nonfree_recv();
-    device_free();
+    detail::check(mgx_csr_download(dev_->h, nullptr, nullptr, data()));
+    dev_.reset();
   }
   void nonfree_recv() {
+    detail::UtilLog log_("nonfree_recv");
     if (get_device_mem_stat())
       detail::check(mgx_csr_download(dev_->h, nullptr, nullptr, data()));
   }
-  void device_free() const { const_cast<CRS *>(this)->dev_.reset(); }
+  void device_free() const {
+    detail::UtilLog log_("device_free");
+    const_cast<CRS *>(this)->dev_.reset();
+  }
 
   void print_all(bool force_cpu = false) const {
     std::vector<Float> v(data(), data() + nnz_);
@@ -738,6 +809,52 @@ template <typename T, typename U> bool is_same_size(const T &x, const U &y) {
 template <typename T, typename U, typename... Types>
 bool is_same_size(const T &x, const U &y, const Types &...args) {
   return x.size() == y.size() && is_same_size(y, args...);
+}
+// util::is_same_structure (monolish_common.hpp:397-470, compare_crs.cpp:
+// 97-115).  Vectors and views: same size.  CRS: same rows and columns and
+// the same structure hash; the reference's hash (vhash, integer_math.cpp:
+// 12-39) mixes only the array lengths, never their contents, so equal
+// (rows, cols, nnz) is what it compares, and so does this.
+template <typename T, typename U> [[nodiscard]] bool is_same_structure(const T &, const U &) {
+  return false;
+}
+template <typename T> [[nodiscard]] bool is_same_structure(const vector<T> &x, const vector<T> &y) {
+  return x.size() == y.size();
+}
+template <typename T, typename U>
+[[nodiscard]] bool is_same_structure(const view1D<T, U> &x, const vector<U> &y) {
+  return x.size() == y.size();
+}
+template <typename T, typename U>
+[[nodiscard]] bool is_same_structure(const vector<U> &x, const view1D<T, U> &y) {
+  return x.size() == y.size();
+}
+template <typename T, typename U, typename V>
+[[nodiscard]] bool is_same_structure(const view1D<T, V> &x, const view1D<U, V> &y) {
+  return x.size() == y.size();
+}
+template <typename T>
+[[nodiscard]] bool is_same_structure(const matrix::COO<T> &A, const matrix::COO<T> &B) {
+  return A.get_row() == B.get_row() && A.get_col() == B.get_col();
+}
+template <typename T>
+[[nodiscard]] bool is_same_structure(const matrix::CRS<T> &A, const matrix::CRS<T> &B) {
+  detail::UtilLog log_("is_same_structure");
+  return A.get_row() == B.get_row() && A.get_col() == B.get_col() && A.get_nnz() == B.get_nnz();
+}
+template <typename T, typename U, typename... Types>
+[[nodiscard]] bool is_same_structure(const T &A, const U &B, const Types &...args) {
+  return is_same_structure(A, B) && is_same_structure(A, args...);
+}
+// util::is_same_size for matrices (compare_crs.cpp:122-136): rows and columns
+template <typename T>
+[[nodiscard]] bool is_same_size(const matrix::CRS<T> &A, const matrix::CRS<T> &B) {
+  detail::UtilLog log_("is_same_size");
+  return A.get_row() == B.get_row() && A.get_col() == B.get_col();
+}
+template <typename T>
+[[nodiscard]] bool is_same_size(const matrix::COO<T> &A, const matrix::COO<T> &B) {
+  return A.get_row() == B.get_row() && A.get_col() == B.get_col();
 }
 template <typename T, typename U> bool is_same_device_mem_stat(const T &x, const U &y) {
   return x.get_device_mem_stat() == y.get_device_mem_stat();

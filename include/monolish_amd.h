@@ -36,6 +36,7 @@ extern "C" {
 #define MGX_ERR_NOMEM -103    /* device or host allocation failed */
 #define MGX_ERR_BUDGET -104   /* request exceeds the configured HBM budget */
 #define MGX_ERR_NOT_FOUND -105
+#define MGX_ERR_TIMEOUT -106   /* a device wait gave up (no progress within its time limit) */
 
 typedef enum { MGX_F64 = 0, MGX_F32 = 1 } mgx_dtype;
 

@@ -22,17 +22,17 @@ template <class V> void need(const V &v, const char *what) {
 // matvec(A, x, y): y = A x  (matvec_blas.cpp:3849-3860, crs_matvec.hpp:8-114)
 template <typename Float, class V1, class V2>
 void matvec(const matrix::CRS<Float> &A, const V1 &x, V2 &y) {
-  Logger::get_instance().func_in(monolish_func);
+  detail::FuncLog log_(detail::core_name<Float>("matvec"));
   assert(A.get_row() == y.size() && A.get_col() == x.size());
   detail_blas::need(x, "matvec");
   detail_blas::need(y, "matvec");
   detail::VecArg X(x), Y(y);
   detail::check(mgx_matvec(A.dev(), X.h, Y.h));
-  Logger::get_instance().func_out();
 }
 // matvec(a, A, x, b, y): y = a A x + b y (the device path's meaning)
 template <typename Float, class V1, class V2>
 void matvec(const Float &a, const matrix::CRS<Float> &A, const V1 &x, const Float &b, V2 &y) {
+  detail::FuncLog log_(detail::core_name<Float>("matvec"));
   detail_blas::need(x, "matvec");
   detail_blas::need(y, "matvec");
   detail::VecArg X(x), Y(y);
@@ -49,6 +49,7 @@ void matvec_N(const Float &a, const matrix::CRS<Float> &A, const V1 &x, const Fl
 // matvec_T(A, x, y): y = A^T x (crs_matvec.hpp:88-98)
 template <typename Float, class V1, class V2>
 void matvec_T(const matrix::CRS<Float> &A, const V1 &x, V2 &y) {
+  detail::FuncLog log_(detail::core_name<Float>("matvec"));
   assert(A.get_row() == x.size() && A.get_col() == y.size());
   detail_blas::need(x, "matvec_T");
   detail_blas::need(y, "matvec_T");
@@ -58,14 +59,13 @@ void matvec_T(const matrix::CRS<Float> &A, const V1 &x, V2 &y) {
 
 // dot (src/blas/vector/dot.hpp:6-78): blocking, scalar to the host
 template <class V1, class V2> detail_blas::elem_t<V1> dot(const V1 &x, const V2 &y) {
-  Logger::get_instance().func_in(monolish_func);
+  detail::FuncLog log_(detail::core_name<detail_blas::elem_t<V1>>("dot"));
   assert(util::is_same_size(x, y));
   detail_blas::need(x, "dot");
   detail_blas::need(y, "dot");
   detail::VecArg X(x), Y(y);
   double out = 0;
   detail::check(mgx_dot(X.h, Y.h, &out));
-  Logger::get_instance().func_out();
   return (detail_blas::elem_t<V1>)out;
 }
 template <class V1, class V2, class Float> void dot(const V1 &x, const V2 &y, Float &ans) {
@@ -74,6 +74,7 @@ template <class V1, class V2, class Float> void dot(const V1 &x, const V2 &y, Fl
 
 // nrm2 (src/blas/vector/nrm2.hpp:6-67): sqrt(dot(x, x))
 template <class V> detail_blas::elem_t<V> nrm2(const V &x) {
+  detail::FuncLog log_(detail::core_name<detail_blas::elem_t<V>>("nrm2"));
   detail_blas::need(x, "nrm2");
   detail::VecArg X(x);
   double out = 0;
@@ -84,6 +85,7 @@ template <class V, class Float> void nrm2(const V &x, Float &ans) { ans = nrm2(x
 
 // axpy (axpy.hpp:7-61): y = a x + y
 template <class Float, class V1, class V2> void axpy(const Float alpha, const V1 &x, V2 &y) {
+  detail::FuncLog log_(detail::core_name<detail_blas::elem_t<V1>>("axpy"));
   assert(util::is_same_size(x, y));
   detail_blas::need(x, "axpy");
   detail_blas::need(y, "axpy");
@@ -92,6 +94,7 @@ template <class Float, class V1, class V2> void axpy(const Float alpha, const V1
 }
 // xpay (xpay.hpp:7-66): y = x + a y
 template <class Float, class V1, class V2> void xpay(const Float alpha, const V1 &x, V2 &y) {
+  detail::FuncLog log_(detail::core_name<detail_blas::elem_t<V1>>("xpay"));
   assert(util::is_same_size(x, y));
   detail_blas::need(x, "xpay");
   detail_blas::need(y, "xpay");
@@ -100,12 +103,14 @@ template <class Float, class V1, class V2> void xpay(const Float alpha, const V1
 }
 // scal (scal.hpp:6-49): x = a x
 template <class Float, class V> void scal(const Float alpha, V &x) {
+  detail::FuncLog log_(detail::core_name<detail_blas::elem_t<V>>("scal"));
   detail_blas::need(x, "scal");
   detail::VecArg X(x);
   detail::check(mgx_scal((double)alpha, X.h));
 }
 // copy (copy.hpp:5-13): y = x
 template <class V1, class V2> void copy(const V1 &x, V2 &y) {
+  detail::FuncLog log_("copy_core");
   assert(util::is_same_size(x, y));
   detail_blas::need(x, "copy");
   detail_blas::need(y, "copy");
@@ -115,6 +120,7 @@ template <class V1, class V2> void copy(const V1 &x, V2 &y) {
 // axpyz (axpyz.hpp:7-66): z = a x + y
 template <class Float, class V1, class V2, class V3>
 void axpyz(const Float alpha, const V1 &x, const V2 &y, V3 &z) {
+  detail::FuncLog log_(detail::core_name<detail_blas::elem_t<V1>>("axpyz"));
   assert(util::is_same_size(x, y, z));
   detail_blas::need(x, "axpyz");
   detail_blas::need(y, "axpyz");

@@ -19,6 +19,14 @@ namespace monolish {
 namespace equation {
 
 namespace detail_eq {
+// scoped Logger::solver_in / solver_out (an "IN" line now, "OUT" with the time at scope exit)
+class SolverLog {
+ public:
+  explicit SolverLog(const std::string &name) { Logger::get_instance().solver_in(name); }
+  ~SolverLog() { Logger::get_instance().solver_out(); }
+  SolverLog(const SolverLog &) = delete;
+  SolverLog &operator=(const SolverLog &) = delete;
+};
 template <class MATRIX, class Float>
 void check_system(const MATRIX &A, const vector<Float> &x, const vector<Float> &b,
                   const char *who) {
@@ -54,6 +62,7 @@ class Jacobi : public monolish::solver::solver<MATRIX, Float> {
  public:
   // jacobi.cpp:12-29: M = 1 / diag(A), resident when A is
   void create_precond(MATRIX &A) {
+    detail_eq::SolverLog log_(monolish_func);
     if (A.get_row() != A.get_col()) throw std::runtime_error("error A.row != A.col");
     auto &M = this->precond.M;
     M.device_free();
@@ -63,7 +72,10 @@ class Jacobi : public monolish::solver::solver<MATRIX, Float> {
     vml::reciprocal(M, M);
   }
   // jacobi.cpp:53-58: z = M o r
-  void apply_precond(const vector<Float> &r, vector<Float> &z) { vml::mul(this->precond.M, r, z); }
+  void apply_precond(const vector<Float> &r, vector<Float> &z) {
+    detail_eq::SolverLog log_(monolish_func);
+    vml::mul(this->precond.M, r, z);
+  }
   // jacobi.cpp:83-150 (relative stopping test)
   [[nodiscard]] int solve(MATRIX &A, vector<Float> &x, vector<Float> &b) {
     Logger::get_instance().solver_in(monolish_func);
@@ -200,6 +212,7 @@ class CG : public monolish::solver::solver<MATRIX, Float> {
 
  private:
   int monolish_CG(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    detail_eq::SolverLog log_(monolish_func);
     detail_eq::check_system(A, x, b, "equation::CG");
     if (this->resid_method != 0) throw std::runtime_error("error vector size is not same");
     const int kind = this->device_precond();
@@ -269,6 +282,7 @@ class BiCGSTAB : public monolish::solver::solver<MATRIX, Float> {
 
  private:
   int monolish_BiCGSTAB(MATRIX &A, vector<Float> &x, vector<Float> &b) {
+    detail_eq::SolverLog log_(monolish_func);
     detail_eq::check_system(A, x, b, "equation::BiCGSTAB");
     if (this->resid_method != 0) throw std::runtime_error("error vector size is not same");
     const int kind = this->device_precond();

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libmonolish_amd.so")
 
 OK = 0
 SOLVER_SIZE_ERROR, SOLVER_MAXITER, SOLVER_BREAKDOWN, SOLVER_RESIDUAL_NAN, SOLVER_NOT_IMPL = -1, -2, -3, -4, -10
-ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND = -101, -102, -103, -104, -105
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND, ERR_TIMEOUT = -101, -102, -103, -104, -105, -106
 F64, F32 = 0, 1
 PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
 PRECOND_IC, PRECOND_ILU = 4, 5

@@ -335,6 +335,9 @@ TriOp tri_prepare(Csr *A, const TriPlan &P, mgx_dtype dt, int fin, const void *v
 void tri_run(Csr *A, const TriOp &op, mgx_dtype dt, const void *in, void *out);
 void levels_free(SorLevels &L);
 void tri_free(Csr *A);
+// throws MGX_ERR_TIMEOUT (and clears the flag) when a dependency-driven sweep
+// of one of A's plans gave up since the last check
+void tri_check_stall(Csr *A);
 // IC(0) (kind MGX_PRECOND_IC) / ILU(0) (MGX_PRECOND_ILU) of A's pattern:
 // f (A->nnz entries of dt) holds a copy of the values on entry and the
 // factor on return; throws on unsorted columns, a missing diagonal or a

@@ -498,6 +498,7 @@ int run_solver(mgx_csr *A, mgx_solve_result *res, F body) {
   }
   ws_release_all(c);
   MGX_HIP(hipStreamSynchronize(c->stream));
+  tri_check_stall(A);
   res->status = o.status;
   res->final_iter = o.iters;
   res->final_resid = o.resid;

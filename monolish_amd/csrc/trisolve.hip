@@ -377,14 +377,20 @@ __global__ __launch_bounds__(kSweepWgThreads) void k_tri_staged(
 // (a plain load may keep seeing a cached copy) until it is written, then
 // goes on, so the arithmetic is the serial sweep's.  Rows only read rows of earlier
 // levels, which sit at earlier positions, so their tickets were taken by
-// waves that are already running: the sweep cannot deadlock.  A row that
-// still waits after kFlowSpinTicks of the 100 MHz wall clock gives up and
-// publishes NaN, so the kernel always ends and the solver reports a breakdown.
+// waves that are already running: the sweep cannot deadlock.  A lane that
+// has consumed none of its row's entries for kFlowSpinTicks of the 100 MHz
+// wall clock gives up: it publishes NaN and raises the plan's stall flag
+// (ticket[1]), so the kernel always ends and the solver returns
+// MGX_ERR_TIMEOUT.  At most 1024 waves x 64 rows are in flight, so a valid
+// plan's longest wait is 65536 dependency hops (measured 4-7 us each), far
+// inside the 2 s limit.
 constexpr int kFlowThreads = 256;
 constexpr int kFlowBlocks = 256;
-constexpr uint64_t kFlowSpinTicks = 20000000;  // 0.2 s
-// plans of at least kFlowMinRows rows whose levels hold kFlowMinWidth rows
-// on average take the dependency-driven form
+constexpr uint64_t kFlowSpinTicks = 200000000;  // 2 s without progress
+// plans whose levels hold kFlowMinWidth rows on average take the
+// dependency-driven form when they have at least kFlowMinRows rows or a
+// level wider than one workgroup's block; deep plans (a long chain beside
+// wide levels) stay in one workgroup, which sweeps any width level by level
 constexpr int64_t kFlowMinRows = 1 << 17, kFlowMinWidth = 256;
 
 template <class T> struct FlowBits;
@@ -436,7 +442,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_tri_flow(
     const int32_t d1 = has ? rc.z : rc.y;
     const T init = has ? ginit[k] : (T)0;
     Acc acc = FIN == 1 ? (Acc)0 : (Acc)init;
-    const uint64_t t0 = wall_clock64();
+    uint64_t t0 = wall_clock64();  // the lane's last progress
     int32_t wc = -1;  // >= 0: column of entry d, which was still kEmpty
     T wa = (T)0;
     // a lane publishes its row as soon as its last entry is consumed: a row
@@ -455,6 +461,7 @@ __global__ __launch_bounds__(kFlowThreads) void k_tri_flow(
               acc = acc - p;
             d++;
             wc = -1;
+            t0 = wall_clock64();
           }
         } else {
           int32_t c[kTriChunk];
@@ -491,10 +498,12 @@ __global__ __launch_bounds__(kFlowThreads) void k_tri_flow(
             }
           }
           d += taken;
+          if (taken > 0) t0 = wall_clock64();
         }
         if (d < d1 && wc >= 0 && wall_clock64() - t0 > kFlowSpinTicks) {
           acc = (Acc)__builtin_bit_cast(T, FlowBits<T>::kNan);
           d = d1;
+          __hip_atomic_store(ticket + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
       if (pending && d >= d1) {
@@ -652,8 +661,12 @@ void build_tri_plan(const std::vector<int32_t> &ptr, const std::vector<int32_t> 
   // large plans with wide levels sweep dependency-driven across the chip;
   // the others in one workgroup, staged in blocks of whole levels when no
   // level alone exceeds a block
-  P.flow = L.max_width > kSweepWgRows || (n >= kFlowMinRows && n >= kFlowMinWidth * (int64_t)nlev);
-  if (P.flow) P.d_ticket = (uint32_t *)dev_alloc(256);
+  P.flow = (L.max_width > kSweepWgRows || n >= kFlowMinRows) &&
+           n >= kFlowMinWidth * (int64_t)nlev;
+  if (P.flow) {
+    P.d_ticket = (uint32_t *)dev_alloc(256);
+    MGX_HIP(hipMemset(P.d_ticket, 0, 256));  // [0] ticket, [1] stall flag
+  }
   std::vector<int4> blk;
   bool fits = !P.flow;
   for (int32_t l = 0; l < nlev && fits; l++) {
@@ -733,6 +746,20 @@ void tri_run(Csr *A, const TriOp &op, mgx_dtype dt, const void *in, void *out) {
     tri_run_fin<double>(A->ctx, op, in, out);
   else
     tri_run_fin<float>(A->ctx, op, in, out);
+}
+
+void tri_check_stall(Csr *A) {
+  for (TriPlan &P : A->tri) {
+    if (!P.flow || !P.d_ticket) continue;
+    uint32_t flag = 0;
+    MGX_HIP(hipMemcpy(&flag, P.d_ticket + 1, 4, hipMemcpyDeviceToHost));
+    if (flag) {
+      MGX_HIP(hipMemset(P.d_ticket + 1, 0, 4));
+      set_error("triangular sweep: a row waited more than 2 s for the rows it reads; the "
+                "sweep gave up (result NaN)");
+      throw Error{MGX_ERR_TIMEOUT};
+    }
+  }
 }
 
 void levels_free(SorLevels &L) {

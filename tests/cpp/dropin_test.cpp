@@ -197,9 +197,64 @@ template <typename T, typename PRECOND> bool lobpcg_harness(bool user) {
   return x.get_row() == 2 && !x.get_device_mem_stat();
 }
 
+// The Logger hooks alone, with no device work: the nesting, the level gate
+// and the line format of logger_utils.cpp:7-178.  Also util::is_same_structure
+// and is_same_size on host objects.
+int log_hooks(size_t level, const std::string &file) {
+  monolish::util::set_log_level(level);
+  monolish::util::set_log_filename(file);
+  monolish::Logger &l = monolish::Logger::get_instance();
+  l.solver_in("solve");
+  l.solver_in("monolish_CG");
+  l.func_in("Ddot_core");
+  l.util_in("send");
+  l.util_out();
+  l.func_out();
+  l.solver_out();
+  l.solver_out();
+  monolish::util::set_log_level(0);
+  using MAT = monolish::matrix::CRS<double>;
+  auto ca = monolish::util::laplacian_matrix_2D_5p<double>(4, 4);
+  auto cc = monolish::util::laplacian_matrix_2D_5p<double>(2, 8);  // 16 rows, fewer entries
+  MAT A(ca), B(ca), C(cc);
+  monolish::vector<double> u(16), v(16), w(15);
+  CHECK(monolish::util::is_same_structure(A, B));
+  CHECK(!monolish::util::is_same_structure(A, C));
+  CHECK(monolish::util::is_same_size(A, C));
+  CHECK(monolish::util::is_same_structure(u, v));
+  CHECK(!monolish::util::is_same_structure(u, v, w));
+  CHECK(!monolish::util::is_same_structure(A, u));
+  return failures == 0 ? 0 : 1;
+}
+
+// The Logger's output for one drop-in CG solve with a Jacobi preconditioner
+// on test.mtx (device resident, as the reference's GPU runs).
+int log_cg_solve(const std::string &dir, size_t level, const std::string &file) {
+  monolish::util::set_log_level(level);
+  monolish::util::set_log_filename(file);
+  using MAT = monolish::matrix::CRS<double>;
+  monolish::matrix::COO<double> coo(dir + "/test.mtx");
+  MAT A(coo);
+  monolish::vector<double> x(A.get_row(), 0.0), b(A.get_row(), 1.0);
+  monolish::util::send(A, x, b);
+  monolish::equation::CG<MAT, double> solver;
+  monolish::equation::Jacobi<MAT, double> precond;
+  solver.set_create_precond(precond);
+  solver.set_apply_precond(precond);
+  solver.set_tol(1e-12);
+  const int rc = solver.solve(A, x, b);
+  monolish::util::recv(x);
+  monolish::util::set_log_level(0);
+  return rc;
+}
+
 int main(int argc, char **argv) {
   const std::string dir = argc > 1 ? argv[1] : ".";
   using namespace monolish;
+  if (argc > 4 && std::string(argv[2]) == "loghooks")
+    return log_hooks(std::stoul(argv[3]), argv[4]);
+  if (argc > 4 && std::string(argv[2]) == "logcg")
+    return log_cg_solve(dir, std::stoul(argv[3]), argv[4]);
   matvec_known_answers<double>(dir);
   matvec_known_answers<float>(dir);
   vector_blas<double>(4321);

@@ -53,3 +53,85 @@ def test_dropin_program_runs_on_gpu():
     r = subprocess.run([exe, GOLDEN], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0 and "PASS" in r.stdout
+
+
+# --- Logger (monolish_logger.hpp:19-92, logger_utils.cpp:7-178) -------------
+import re  # noqa: E402
+
+_LINE = re.compile(r"^- \{type : (solver|func|util), name : ((?:[A-Za-z_0-9]+/)+), "
+                   r"(?:stat : (IN)|(?:stat : (OUT), )?time : ([0-9.e+-]+))\}$")
+
+
+def parse_log(text):
+    """Each line as (type, name path, stat, seconds); stat is IN, OUT or None."""
+    out = []
+    for line in text.splitlines():
+        m = _LINE.match(line)
+        assert m, f"line not in the reference's format: {line!r}"
+        typ, name, st_in, st_out, sec = m.groups()
+        out.append((typ, name, st_in or st_out, float(sec) if sec is not None else None))
+    return out
+
+
+def _exe():
+    return EXE if os.path.exists(EXE) else build_dropin()
+
+
+@pytest.fixture(scope="module")
+def fresh_exe(tmp_path_factory):
+    return build_dropin(str(tmp_path_factory.mktemp("dropin") / "dropin"))
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+def test_logger_format_and_levels(level, tmp_path, fresh_exe):
+    """The hooks alone (no device): solver IN/OUT lines from level 1, func
+    lines from 2, util lines from 3, nested names joined by '/'.  Also
+    util::is_same_structure / is_same_size on host objects."""
+    f = tmp_path / "log.yaml"
+    r = subprocess.run([fresh_exe, GOLDEN, "loghooks", str(level), str(f)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = [(t, n, s) for t, n, s, _ in parse_log(f.read_text())]
+    want = [("solver", "solve/", "IN"), ("solver", "solve/monolish_CG/", "IN"),
+            ("util", "solve/monolish_CG/Ddot_core/send/", None),
+            ("func", "solve/monolish_CG/Ddot_core/", None),
+            ("solver", "solve/monolish_CG/", "OUT"), ("solver", "solve/", "OUT")]
+    keep = {0: set(), 1: {"solver"}, 2: {"solver", "func"}, 3: {"solver", "func", "util"}}[level]
+    assert got == [w for w in want if w[0] in keep]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_logger_on_device_cg_solve(level, tmp_path):
+    """A drop-in Jacobi-PCG solve on test.mtx logged at each level: solver
+    IN/OUT pairs nest and balance, every timed line has a time >= 0, the
+    Jacobi set-up is logged inside monolish_CG, and each level adds exactly
+    its own line type."""
+    f = tmp_path / "log.yaml"
+    r = subprocess.run([_exe(), GOLDEN, "logcg", str(level), str(f)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = parse_log(f.read_text())
+    types = {t for t, _, _, _ in lines}
+    assert types == {1: {"solver"}, 2: {"solver", "func"}, 3: {"solver", "func", "util"}}[level]
+    stack = []
+    for typ, name, stat, sec in lines:
+        if stat == "IN":
+            assert name == "".join(stack) + name.split("/")[-2] + "/"
+            stack.append(name.split("/")[-2] + "/")
+        else:
+            assert sec is not None and sec >= 0.0
+            if stat == "OUT":
+                assert name == "".join(stack)
+                stack.pop()
+            else:  # func / util lines name the enclosing solvers first
+                assert name.startswith("".join(stack))
+    assert stack == []
+    solver_lines = [(n, s) for t, n, s, _ in lines if t == "solver"]
+    assert solver_lines[0] == ("solve/", "IN") and solver_lines[-1] == ("solve/", "OUT")
+    assert ("solve/monolish_CG/create_precond/", "IN") in solver_lines
+    if level >= 2:
+        names = [n for t, n, _, _ in lines if t == "func"]
+        assert "solve/monolish_CG/create_precond/vreciprocal_core/" in names
+    if level >= 3:
+        assert ("util", "send/") in [(t, n) for t, n, _, _ in lines]

@@ -194,28 +194,89 @@ def test_server_resident_requests(m, ctx, oracle):
     for meth, rp, ci, v, b in syss:
         n = len(b)
         res_sys.append((meth, m.CRS(c2, n, n, rp, ci, v), m.Vector(c2, b), rp, ci, v, b))
+    # the solution vectors exist before the server and are read after it
+    # closes: while a server runs, its context takes no other calls
+    xs = [m.Vector(c2, n=len(b)) for *_, b in res_sys]
     srv = m.Server(c2, max_active=5, iters_per_poll=3)
     tickets = []
     for rep in range(3):
         for i, (meth, A, B, rp, ci, v, b) in enumerate(res_sys):
             code = m.METHOD_CG if meth == "cg" else m.METHOD_BICGSTAB
-            X = m.Vector(c2, n=len(b)) if rep == 1 else None
+            X = xs[i] if rep == 1 else None
             tickets.append((srv.submit_resident(code, A, B, X, tol=1e-10, tol_is_relative=True),
                             i, X))
-    for t, i, X in tickets:
-        r, _ = srv.wait(t, timeout=120.0)
+    results = [(srv.wait(t, timeout=120.0)[0], i, X) for t, i, X in tickets]
+    st = srv.stats()
+    srv.close()
+    for r, i, X in results:
         meth, A, B, rp, ci, v, b = res_sys[i]
         ref = oracle_solve(oracle, meth, rp, ci, v, b)
         assert r.status == 0 and r.final_iter == ref[2]
         if X is not None:
             assert np.array_equal(bits(X.download()), bits(ref[1]))
-    st = srv.stats()
     assert st["completed"] == len(tickets) and st["queued"] == 0
-    srv.close()
-    for t, i, X in tickets:
-        if X is not None:
-            X.free()
+    for X in xs:
+        X.free()
     for _, A, B, *_ in res_sys:
         A.free()
         B.free()
     c2.close()
+
+
+def test_graph_replay_bit_identical(m, oracle):
+    """mgx_set_graphs(ctx, 1): the enqueued iterations are captured once per
+    tile-list change and replayed, with host polls (eager work) between the
+    replays.  A lone fused CG (two-kernel form), a lone split-form CG
+    (n >= 2^19, plan-based SpMV in the graph), a CG batch and a BiCGSTAB
+    batch give the same bits as with graphs off and as the oracle, on two
+    solves in a row."""
+    say("graphs")
+    c = m.Context(0)
+    try:
+        syss = systems(m, oracle)
+
+        def run_batch(method):
+            group = [s for s in syss if s[0] == method]
+            As = [m.CRS(c, len(b), len(b), rp, ci, v) for _, rp, ci, v, b in group]
+            Xs = [m.Vector(c, n=len(b)) for *_, b in group]
+            Bs = [m.Vector(c, b) for *_, b in group]
+            code = m.METHOD_CG if method == "cg" else m.METHOD_BICGSTAB
+            res = m.batch_solve(c, code, As, Xs, Bs, tol=1e-10, tol_is_relative=True, precond=2)
+            return [(r.status, r.final_iter, bits(X.download()).copy()) for r, X in zip(res, Xs)]
+
+        def run_single(rp, ci, v, b, every, tol, maxiter):
+            n = len(b)
+            A = m.CRS(c, n, n, rp, ci, v)
+            X = m.Vector(c, n=n, dtype=v.dtype)
+            r = m.cg(A, X, m.Vector(c, b, dtype=v.dtype), tol=tol, maxiter=maxiter, precond=2,
+                     mode=m.MODE_FUSED, check_every=every)
+            return r.status, r.final_iter, bits(X.download()).copy()
+
+        rp1, ci1, v1 = m.laplacian2d(128, 128)
+        rp2, ci2, v2 = m.stencil3d(96, 96, 64, 7, 0.0)
+        singles = []
+        for k in range(2):
+            b = oracle.random_vector(128 * 128, -1.0, 1.0, 1001 + k)
+            singles.append((rp1, ci1, v1, b, 8 + 9 * k, 1e-10 * oracle.nrm2(b), 0))
+        for k in range(2):  # 40 iterations of the split form, replayed 16 at a time
+            b = oracle.random_vector(len(rp2) - 1, -1.0, 1.0, 9 + k, dtype=np.float32)
+            singles.append((rp2, ci2, v2.astype(np.float32), b, 16, 1e-30, 40))
+        out = {}
+        for g in (0, 1):
+            m.set_graphs(c, bool(g))
+            out[g] = ([run_single(*s) for s in singles], run_batch("cg"), run_batch("bicgstab"))
+        m.set_graphs(c, False)
+        for a, b_ in zip(out[0], out[1]):
+            for (s0, i0, x0), (s1, i1, x1) in zip(a, b_):
+                assert s0 == s1 and i0 == i1 and np.array_equal(x0, x1)
+        for (rp, ci, v, b, _, tol, maxiter), (st, it, x) in zip(singles, out[1][0]):
+            kw = {"maxiter": maxiter} if maxiter else {}
+            ref = oracle.solve("cg", rp, ci, v, np.zeros(len(b), v.dtype), b, precond=2, tol=tol,
+                               **kw)
+            assert st == ref[0] and it == ref[2] and np.array_equal(x, bits(ref[1]))
+        for method, got in (("cg", out[1][1]), ("bicgstab", out[1][2])):
+            for (meth, rp, ci, v, b), (st, it, x) in zip([s for s in syss if s[0] == method], got):
+                ref = oracle_solve(oracle, meth, rp, ci, v, b)
+                assert st == ref[0] and it == ref[2] and np.array_equal(x, bits(ref[1]))
+    finally:
+        c.close()

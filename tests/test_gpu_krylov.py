@@ -114,6 +114,44 @@ def test_cg_poisson256_bit_exact(m, ctx, oracle, mode):
     assert bits_equal(x, ref[1])
     assert bits_equal(res.rhistory, ref[4])
     assert oracle.residual_l2(rp, ci, val, x, b) < 2 * tol
+    # util::get_residual_l2 of the converged iterate, on the device
+    A, X, B = m.CRS(ctx, n, n, rp, ci, val), m.Vector(ctx, x), m.Vector(ctx, b)
+    assert m.residual_l2(A, X, B) == oracle.residual_l2(rp, ci, val, x, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", ["poisson256", "stencil27_48", "rect_lognormal"])
+def test_residual_l2_bit_exact(m, ctx, oracle, case, dtype):
+    """util::get_residual_l2 (equation_utils.cpp:45-56): ||b - A x||_2 through
+    mgx_residual_l2 equals the oracle's tmp = A x; tmp = b - tmp; nrm2(tmp)
+    bit for bit: the C1 matrix at full size, the 27-point stencil, and a
+    rectangular random matrix (x sized by the columns, b by the rows)."""
+    say("residual_l2", case, dtype.__name__)
+    if case == "poisson256":
+        rp, ci, val = _poisson(oracle, 256, dtype)
+        cols = len(rp) - 1
+    elif case == "stencil27_48":
+        rp, ci, val = m.stencil3d(48, 48, 48, 27, 1e-2)
+        val = val.astype(dtype)
+        cols = len(rp) - 1
+    else:
+        cols = 7001
+        rp, ci, val = m.random_lognormal(5003, cols, 12.0, 1.0, 300, seed=77)
+        val = val.astype(dtype)
+    rows = len(rp) - 1
+    x = oracle.random_vector(cols, -1.0, 1.0, 11, dtype=dtype)
+    b = oracle.random_vector(rows, -1.0, 1.0, 12, dtype=dtype)
+    A = m.CRS(ctx, rows, cols, rp, ci, val)
+    X, B = m.Vector(ctx, x, dtype=dtype), m.Vector(ctx, b, dtype=dtype)
+    got = m.residual_l2(A, X, B)
+    want = float(oracle.residual_l2(rp, ci, val, x, b))
+    assert got == want, (got, want)
+    # b := A x from the device SpMV (bit-exact to the row loop) gives exactly 0
+    Y = m.Vector(ctx, n=rows, dtype=dtype)
+    m.matvec(A, X, Y)
+    assert m.residual_l2(A, X, Y) == 0.0
+    for h in (A, X, B, Y):
+        h.free()
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -556,6 +594,55 @@ def test_large_wide_level_sweeps_bit_exact(m, ctx, oracle, dtype):
         assert res.status == ref[0] and res.final_iter == ref[2]
         assert bits_equal(x, ref[1])
         assert bits_equal(res.rhistory, ref[4])
+
+
+def _wide_then_chain(n_free, n_chain):
+    """n_free independent rows (diagonal 4), then a tridiagonal chain
+    (4 on the diagonal, -1 beside it): level 0 is wider than a workgroup's
+    block and the chain adds n_chain - 1 levels of one row."""
+    n = n_free + n_chain
+    rp = [0]
+    ci, val = [], []
+    for i in range(n):
+        if i >= n_free + 1:
+            ci.append(i - 1)
+            val.append(-1.0)
+        ci.append(i)
+        val.append(4.0)
+        if n_free <= i < n - 1:
+            ci.append(i + 1)
+            val.append(-1.0)
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(val)
+
+
+def test_deep_plan_with_a_wide_level_bit_exact(m, ctx, oracle):
+    """5000 independent rows plus a 100000-row chain: one level wider than a
+    workgroup's block, then 99999 levels of one row.  Such deep plans sweep
+    in one workgroup (every wait there is a barrier); IC(0) / ILU(0) apply and
+    SOR-, IC-preconditioned CG stay bit-identical to the oracle, with no
+    give-up of the dependency-driven form."""
+    rp, ci, val = _wide_then_chain(5000, 100000)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 58)
+    for kind in (4, 5):
+        say("deep plan factor", kind)
+        rc, ref = oracle.ilu_precond(rp, ci, val, kind, b)
+        assert rc == 0
+        A = m.CRS(ctx, n, n, rp, ci, val)
+        X = m.Vector(ctx, n=n)
+        res = (m.ic_solve if kind == 4 else m.ilu_solve)(A, X, m.Vector(ctx, b))
+        assert res.status == 0
+        assert bits_equal(X.download(), ref)
+    tol = float(1e-30 * oracle.nrm2(b))
+    for kind in (m.PRECOND_SOR, 4):
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=kind, tol=tol, maxiter=3,
+                           history=True)
+        res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, kind, tol, maxiter=3,
+                           mode=m.MODE_REFERENCE, history=3)
+        say("deep plan cg", kind, res.status, res.final_iter)
+        assert res.status == ref[0] == m.SOLVER_MAXITER and res.final_iter == 3
+        assert bits_equal(x, ref[1]) and bits_equal(res.rhistory, ref[4])
 
 
 def test_ic_ilu_errors(m, ctx, oracle):
