@@ -76,8 +76,9 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
  * in 64-row slices of 16-byte tuples, one lane per row, no LDS) under both
  * XCD orders -- `reps` launches each; keeps the fastest for A and reports it
  * (within 2 %, an XCD-ordered block plan).  The long-row threshold is
- * unchanged, so results do not depend on the choice.  The panel copy is
- * dropped by mgx_csr_update_values; the sliced copy is rewritten. */
+ * unchanged, so results do not depend on the choice.  The column-panel and
+ * L2-panel copies are dropped by mgx_csr_update_values; the sliced copy is
+ * rewritten. */
 typedef struct mgx_spmv_tuning {
   int32_t rows_per_block, nnz_per_block, long_row, xcd_remap;
   double us;           /* time of one SpMV under the chosen plan */
@@ -86,16 +87,20 @@ typedef struct mgx_spmv_tuning {
   int32_t row_walk;    /* 0 products staged in LDS in entry order; 1 row
                           walk (lane per row over LDS-staged val / col_ind;
                           coalesced gathers on banded rows); 2 the sliced
-                          layout */
+                          layout; 3 L2 panels (row sums kept in LDS while the
+                          blocks walk column panels of about 0.5-2 MiB of x,
+                          column_panels of them) */
   int32_t pad;
 } mgx_spmv_tuning;
 int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out);
 /* Sets A's SpMV plan explicitly (the candidates the autotuner times):
  * rows_per_block in [1, 256], nnz_per_block in [64, 8192] (<= 4096 with
- * row_walk), xcd_remap 0/1, row_walk 0/1 (block kind) or 2 (the sliced
+ * row_walk), xcd_remap 0/1, row_walk 0/1 (block kind), 2 (the sliced
  * layout, over a product-staged block plan kept for other values; refused
- * when its padding exceeds 30 %), column_panels 0 or 2..64 (needs ascending
- * columns within rows; not with row_walk 2): MGX_ERR_INVALID otherwise.
+ * when its padding exceeds 30 %) or 3 (L2 panels: column_panels in
+ * [1, 65536], at most 2^20 columns each, ascending columns within rows),
+ * column_panels 0 or 2..64 otherwise (needs ascending columns within rows;
+ * not with row_walk 2): MGX_ERR_INVALID otherwise.
  * The long-row threshold stays as it is. */
 int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_remap,
                      int row_walk, int column_panels);

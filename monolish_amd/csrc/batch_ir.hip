@@ -69,6 +69,22 @@ void batch_ir_cg(Ctx *c, int64_t nsys, Csr *const *A, double *const *x, const do
     owned.clear();
   };
   try {
+    {
+      // the FP32 vectors and both engines' work vectors of every system from
+      // one slab when the pool lacks them
+      SolveCfg r0, i0;
+      r0.method = i0.method = kMethodCG;
+      r0.precond = MGX_PRECOND_NONE;
+      i0.precond = p->precond;
+      std::vector<size_t> sizes;
+      for (int64_t i = 0; i < nsys; i++) {
+        sizes.push_back((size_t)A[i]->rows * sizeof(float));
+        sizes.push_back((size_t)A[i]->rows * sizeof(float));
+        Engine<double>::sizes_of(A[i], r0, sizes);
+        Engine<float>::sizes_of(A[i], i0, sizes);
+      }
+      (void)pool.reserve(sizes);
+    }
     for (int64_t i = 0; i < nsys; i++) {
       const size_t bytes = (size_t)A[i]->rows * sizeof(float);
       r32[i] = (float *)pool.get(bytes);

@@ -21,8 +21,40 @@ size_t DevicePool::round(size_t bytes) {
 
 DevicePool::~DevicePool() {
   for (auto &f : free_)
-    for (void *p : f.second) dev_free(p);
-  for (auto &l : live_) dev_free(l.first);
+    for (void *p : f.second)
+      if (!carved_.count(p)) dev_free(p);
+  for (auto &l : live_)
+    if (!carved_.count(l.first)) dev_free(l.first);
+  for (void *s : slabs_) dev_free(s);
+}
+
+bool DevicePool::reserve(const std::vector<size_t> &bytes) {
+  // what the free lists already hold, per rounded size
+  std::unordered_map<size_t, size_t> have, want;
+  for (size_t b : bytes) want[round(b)]++;
+  size_t total = 0;
+  for (auto &w : want) {
+    auto it = free_.find(w.first);
+    const size_t h = it == free_.end() ? 0 : it->second.size();
+    if (w.second > h) total += (w.second - h) * w.first;
+  }
+  if (total == 0) return true;
+  if (in_use_ + total > budget_) return false;
+  void *slab = nullptr;
+  if (hipMalloc(&slab, total) != hipSuccess) return false;
+  slabs_.push_back(slab);
+  reserved_ += total;
+  char *p = static_cast<char *>(slab);
+  for (auto &w : want) {
+    auto &fl = free_[w.first];
+    const size_t h = fl.size();
+    for (size_t k = h; k < w.second; k++) {
+      fl.push_back(p);
+      carved_.insert(p);
+      p += w.first;
+    }
+  }
+  return true;
 }
 
 // free blocks are kept per rounded size; lookups and returns are O(1)
@@ -39,9 +71,11 @@ void *DevicePool::get(size_t bytes) {
   if (in_use_ + sz > budget_) return nullptr;
   // drop cached blocks of other sizes while they keep us over budget
   for (auto f = free_.begin(); reserved_ + sz > budget_ && f != free_.end(); ++f) {
-    while (!f->second.empty() && reserved_ + sz > budget_) {
-      dev_free(f->second.back());
-      f->second.pop_back();
+    auto &fl = f->second;
+    for (size_t k = fl.size(); k-- > 0 && reserved_ + sz > budget_;) {
+      if (carved_.count(fl[k])) continue;  // slab blocks go with their slab
+      dev_free(fl[k]);
+      fl.erase(fl.begin() + (std::ptrdiff_t)k);
       reserved_ -= f->first;
     }
   }
@@ -108,6 +142,16 @@ template <class T> size_t Engine<T>::bytes_needed(const Csr *A, int method) cons
   const size_t v = DevicePool::round((size_t)n * sizeof(T));
   const int nvec = method == kMethodCG ? 6 : 11;
   return nvec * v + DevicePool::round(((size_t)8 * ntiles_of(n) + 2 * kRedGroups) * sizeof(T));
+}
+
+template <class T>
+void Engine<T>::sizes_of(const Csr *A, const SolveCfg &cfg, std::vector<size_t> &out) {
+  const int64_t n = A->rows;
+  const size_t vb = (size_t)n * sizeof(T);
+  const int nvec = 3 + (cfg.precond == MGX_PRECOND_JACOBI ? 1 : 0) +
+                   (cfg.method == kMethodCG ? 2 : 7);
+  for (int k = 0; k < nvec; k++) out.push_back(vb);
+  out.push_back((size_t)(8 * ntiles_of(n) + 2 * kRedGroups) * sizeof(T));
 }
 
 template <class T>

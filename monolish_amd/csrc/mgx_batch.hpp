@@ -115,6 +115,13 @@ class DevicePool {
   ~DevicePool();
   void *get(size_t bytes);  // nullptr when the budget would be exceeded
   void put(void *p);
+  // Makes the next get() of each listed size (one entry per call) a hit:
+  // the blocks not already cached come from ONE hipMalloc'd slab, carved
+  // into the free lists (a batch of thousands of systems then makes one
+  // allocation instead of one per work vector).  Carved blocks return to
+  // the free lists like any other and are released with their slab when
+  // the pool is destroyed.  False (nothing allocated) past the budget.
+  bool reserve(const std::vector<size_t> &bytes);
   size_t in_use() const { return in_use_; }
   size_t budget() const { return budget_; }
   static size_t round(size_t bytes);
@@ -123,6 +130,8 @@ class DevicePool {
   size_t budget_, in_use_ = 0, reserved_ = 0;
   std::unordered_map<size_t, std::vector<void *>> free_;  // by rounded size
   std::unordered_map<void *, size_t> live_;
+  std::vector<void *> slabs_;
+  std::unordered_set<void *> carved_;  // blocks inside a slab (never freed alone)
 };
 
 struct SolveCfg {
@@ -152,6 +161,9 @@ template <class T> class Engine {
   // returns the slot, or -1 when the pool budget or capacity is exhausted
   int add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cfg);
   size_t bytes_needed(const Csr *A, int method) const;
+  // the device blocks add() takes from the pool for this system (for
+  // DevicePool::reserve ahead of a batch)
+  static void sizes_of(const Csr *A, const SolveCfg &cfg, std::vector<size_t> &out);
   void step(int iters);
   int running() const { return host_running_; }
   // synchronises; returns slots that stopped since the last poll

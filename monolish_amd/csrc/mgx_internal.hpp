@@ -185,6 +185,23 @@ struct SpmvPlan {
   std::vector<int64_t> soff32;  // host offsets of sl32, kept until it is built
   int32_t *d_slrows = nullptr;
   int64_t nslong = 0;
+  // L2-panel mode (0: off): a block owns lrows consecutive rows and keeps
+  // their sums in LDS while it walks lpanels column panels narrow enough for
+  // one XCD's L2; segment (block, panel) lists the block's short-row entries
+  // of that panel grouped by their rank within the row's run in the panel
+  // (rank 0 of every row, then rank 1, ...), each group in row order, so a
+  // barrier between groups adds every row's products in storage order.
+  // Entry index = (row - first row of the block) << 20 | (col - panel start).
+  int32_t lpanels = 0, lrows = 0;
+  int64_t lpw = 0, lnblk = 0;
+  int32_t *d_lsegoff = nullptr;  // lnblk * lpanels + 1 entry offsets
+  int32_t *d_lrgoff = nullptr;   // lnblk * lpanels + 1 offsets into d_lrbound
+  int32_t *d_lrbound = nullptr;  // end of each rank group, from its segment start
+  uint32_t *d_lidx = nullptr;
+  void *d_lval = nullptr;
+  uint32_t *d_llong = nullptr;   // bit per row: a long row (written by the long-row kernel)
+  int32_t *d_llrows = nullptr;   // the long rows
+  int64_t nllong = 0;
 };
 
 struct Csr {
@@ -305,6 +322,21 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out);
 bool spmv_panels_build(Csr *A, int npanels);
 void spmv_panels_free(Csr *A);
 bool spmv_sell_build(Csr *A, const int32_t *rp_host);
+// build / drop the L2-panel mode (false: not applicable -- columns not
+// ascending within a short row, or a panel wider than 2^20 columns)
+bool spmv_lpanel_build(Csr *A, int npanels);
+void spmv_lpanel_free(Csr *A);
+// host arrays of the L2-panel layout (built from host CRS arrays of es-byte
+// values); false when not applicable
+struct LpanelLayout {
+  int P = 0, R = 0;
+  int64_t pw = 0, nblk = 0;
+  std::vector<int32_t> segoff, rgoff, rbound, lrows;
+  std::vector<uint32_t> idx, lng;
+  std::vector<char> val;
+};
+bool lpanel_layout(const int32_t *rp, const int32_t *ci, const char *val, size_t es, int64_t n,
+                   int64_t cols, int long_t, int npanels, LpanelLayout &L);
 void spmv_sell_free(Csr *A);
 void spmv_sell_values(Csr *A);  // refresh the sliced values after a change
 void launch_diag_v(Csr *A, const void *val, mgx_dtype dt, void *d);

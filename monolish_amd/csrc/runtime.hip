@@ -149,6 +149,7 @@ void csr_free(Csr *A) {
   dev_free(A->plan.d_blocks);
   spmv_panels_free(A);
   spmv_sell_free(A);
+  spmv_lpanel_free(A);
   tri_free(A);
   ilu_free(A);
   dev_free(A->d_chunks);
@@ -305,12 +306,13 @@ int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_
     MGX_REQUIRE(A, "mgx_csr_set_plan: NULL matrix");
     MGX_REQUIRE(rows_per_block >= 1 && rows_per_block <= kBlock,
                 "rows_per_block must be in [1, %d]", kBlock);
-    MGX_REQUIRE(row_walk >= 0 && row_walk <= 2, "row_walk must be 0, 1 or 2");
+    MGX_REQUIRE(row_walk >= 0 && row_walk <= 3, "row_walk must be 0, 1, 2 or 3");
     // the row walk stages 12 bytes per FP64 entry in LDS: 4096 entries fit 64 KiB
-    MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk ? 4096 : 8192),
-                "nnz_per_block must be in [64, %d]", row_walk ? 4096 : 8192);
-    MGX_REQUIRE(column_panels == 0 || (column_panels >= 2 && column_panels <= 64),
-                "column_panels must be 0 or in [2, 64]");
+    MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk == 1 ? 4096 : 8192),
+                "nnz_per_block must be in [64, %d]", row_walk == 1 ? 4096 : 8192);
+    MGX_REQUIRE(row_walk == 3 ? column_panels >= 1 && column_panels <= 65536
+                              : column_panels == 0 || (column_panels >= 2 && column_panels <= 64),
+                "column_panels must be 0 or in [2, 64] (in [1, 65536] with row_walk 3)");
     activate(A->ctx);
     MGX_HIP(hipStreamSynchronize(A->ctx->stream));
     std::vector<int32_t> rp(A->rows + 1);
@@ -322,7 +324,11 @@ int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_
                 "mgx_csr_set_plan: the sliced layout and column panels exclude each other");
     spmv_plan_build_with(A, rp.data(), rows_per_block, nnz_per_block, long_t, xcd_remap,
                          row_walk == 1);
-    if (column_panels)
+    if (row_walk == 3)
+      MGX_REQUIRE(spmv_lpanel_build(A, column_panels),
+                  "mgx_csr_set_plan: L2 panels need ascending columns within rows and at "
+                  "most 2^20 columns per panel");
+    else if (column_panels)
       MGX_REQUIRE(spmv_panels_build(A, column_panels),
                   "mgx_csr_set_plan: column panels need ascending columns within rows");
     if (row_walk == 2)
@@ -527,7 +533,8 @@ int mgx_csr_update_values(mgx_csr *A, const void *val) {
     MGX_HIP(hipStreamSynchronize(A->ctx->stream));
     if (A->nnz) MGX_HIP(hipMemcpy(A->val, val, (size_t)A->nnz * A->elem(), hipMemcpyHostToDevice));
     if (A->val32) launch_convert(A->ctx, A->dtype, MGX_F32, A->nnz, A->val, A->val32);
-    spmv_panels_free(A);  // its value copy would be stale
+    spmv_panels_free(A);  // their value copies would be stale
+    spmv_lpanel_free(A);
     spmv_sell_values(A);
     if (A->trans) {
       csr_free(A->trans);

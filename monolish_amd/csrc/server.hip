@@ -39,6 +39,12 @@ int batch_solve_t(Ctx *c, int method, int64_t nsys, mgx_csr *const *A, mgx_vec *
   const double t0 = seconds_now();
   Engine<T> eng(c, &ctx_pool(c), (int)nsys);
   const SolveCfg cfg = cfg_of(p, method);
+  {
+    // every system's work vectors from one slab when the pool lacks them
+    std::vector<size_t> sizes;
+    for (int64_t i = 0; i < nsys; i++) Engine<T>::sizes_of(A[i], cfg, sizes);
+    (void)ctx_pool(c).reserve(sizes);  // on failure add() allocates per block
+  }
   std::vector<int> slot_of(nsys, -1);
   std::vector<int64_t> sys_of(nsys, -1);
   for (int64_t i = 0; i < nsys; i++) {

@@ -406,6 +406,98 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell(
   }
 }
 
+// L2-panel SpMV: block b owns rows [b R, b R + R) and keeps their sums in
+// LDS.  It walks the column panels in order; in panel k it reads segment
+// (b, k): the entries of its short rows whose columns lie in the panel,
+// grouped by rank (the entry's index within its row's run of the panel),
+// each group in row order.  All entries of a pass are loaded and multiplied
+// first (x gathered from the current panel, which the XCD's L2 holds while
+// the blocks on it walk the same panel); then group by group, with a
+// barrier between groups, each lane adds its products into the row sums.
+// Within a group the rows are distinct, and a row's rank-r entry is added
+// after its rank-(r-1) one and after every entry of earlier panels: the
+// products of a row are added from 0 in storage order, the CPU row loop's
+// order (crs_matvec.hpp:100-108), without y or row pointers leaving the
+// chip between panels.  Long rows are skipped (bit set in `lng`) and
+// written by k_spmv_longrows.
+constexpr int kLpanelE = 8;  // entries per lane per pass
+
+template <class T, bool AB>
+__global__ __launch_bounds__(kBlock) void k_spmv_lpanel(
+    int64_t rows, int R, int P, int64_t pw, int64_t nblk, const int32_t *__restrict__ segoff,
+    const int32_t *__restrict__ rgoff, const int32_t *__restrict__ rbound,
+    const uint32_t *__restrict__ pidx, const T *__restrict__ pval,
+    const uint32_t *__restrict__ lng, const T *__restrict__ x, T *__restrict__ y, T alpha,
+    T beta, const int32_t *live) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *acc = reinterpret_cast<T *>(smem_raw);
+  const int64_t b = blockIdx.x;
+  if (b >= nblk) return;
+  if (live && *live != kStatRunning) return;
+  const int64_t r0 = b * (int64_t)R;
+  const int nr = (int)(rows - r0 < R ? rows - r0 : R);
+  for (int i = (int)threadIdx.x; i < nr; i += kBlock) acc[i] = T(0);
+  __syncthreads();
+  constexpr int E = kLpanelE;
+  constexpr int32_t kWin = E * kBlock;
+  for (int k = 0; k < P; k++) {
+    const int64_t seg = b * (int64_t)P + k;
+    const int32_t e0 = segoff[seg], e1 = segoff[seg + 1];
+    const int32_t g0 = rgoff[seg], g1 = rgoff[seg + 1];
+    const T *xk = x + (int64_t)k * pw;
+    for (int32_t base = e0; base < e1; base += kWin) {
+      uint32_t id[E];
+      T v[E], p[E];
+#pragma unroll
+      for (int u = 0; u < E; u++) {
+        const int32_t e = base + u * kBlock + (int32_t)threadIdx.x;
+        if (e < e1) {
+          id[u] = __builtin_nontemporal_load(pidx + e);
+          v[u] = __builtin_nontemporal_load(pval + e);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < E; u++) {
+        const int32_t e = base + u * kBlock + (int32_t)threadIdx.x;
+        if (e < e1) p[u] = v[u] * xk[id[u] & 0xFFFFFu];
+      }
+      // the rank groups overlapping this pass, in order
+      int32_t ga = e0;
+      for (int32_t g = g0; g < g1; g++) {
+        const int32_t gb = e0 + rbound[g];
+        if (gb > base) {
+          if (ga >= base + kWin) break;
+#pragma unroll
+          for (int u = 0; u < E; u++) {
+            const int32_t e = base + u * kBlock + (int32_t)threadIdx.x;
+            if (e >= ga && e < gb && e < e1) {
+              const uint32_t r = id[u] >> 20;
+              acc[r] = acc[r] + p[u];
+            }
+          }
+          __syncthreads();
+        }
+        ga = gb;
+      }
+    }
+  }
+  for (int i = (int)threadIdx.x; i < nr; i += kBlock) {
+    const int64_t r = r0 + i;
+    if ((lng[r >> 5] >> (r & 31)) & 1u) continue;
+    const T s = acc[i];
+    if (AB) {
+      T t = alpha * s;
+      if (beta != T(0)) {
+        T u = beta * y[r];
+        t = t + u;
+      }
+      y[r] = t;
+    } else {
+      y[r] = s;
+    }
+  }
+}
+
 // CRS::diag: d[i] = A(i,i), last stored match wins, 0 if absent.  A block
 // owns 256 consecutive rows and scans their entries with consecutive lanes on
 // consecutive entries (coalesced); the owning row of an entry comes from a
@@ -457,6 +549,7 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   Ctx *c = A->ctx;
   spmv_panels_free(A);  // built for the previous block budgets
   spmv_sell_free(A);
+  spmv_lpanel_free(A);
   const int64_t rows = A->rows;
   const int rows_max = std::min(std::max(rows_req, 1), kBlock);
   const int nnz_max = std::max(nnz_req, 64);
@@ -621,6 +714,7 @@ bool spmv_sell_build(Csr *A, const int32_t *rp) {
   const int64_t rows = A->rows;
   if (rows <= 0 || A->nnz <= 0) return false;
   if (p.long_row <= 0) p.long_row = c->spmv_long;
+  spmv_lpanel_free(A);
   const bool f64 = A->dtype == MGX_F64;
   std::vector<int64_t> soff;
   std::vector<int32_t> lrows;
@@ -668,6 +762,7 @@ bool spmv_panels_build(Csr *A, int npanels) {
   SpmvPlan &p = A->plan;
   spmv_panels_free(A);
   spmv_sell_free(A);
+  spmv_lpanel_free(A);
   const int64_t n = A->rows, cols = A->cols, nnz = A->nnz;
   if (npanels < 2 || n <= 0 || cols <= 0 || nnz <= 0) return false;
   const size_t es = A->elem();
@@ -770,6 +865,186 @@ bool spmv_panels_build(Csr *A, int npanels) {
   return true;
 }
 
+void spmv_lpanel_free(Csr *A) {
+  SpmvPlan &p = A->plan;
+  if (!p.lpanels) return;
+  if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
+  dev_free(p.d_lsegoff);
+  dev_free(p.d_lrgoff);
+  dev_free(p.d_lrbound);
+  dev_free(p.d_lidx);
+  dev_free(p.d_lval);
+  dev_free(p.d_llong);
+  dev_free(p.d_llrows);
+  p.d_lsegoff = p.d_lrgoff = p.d_lrbound = p.d_llrows = nullptr;
+  p.d_lidx = p.d_llong = nullptr;
+  p.d_lval = nullptr;
+  p.lpanels = p.lrows = 0;
+  p.lpw = p.lnblk = p.nllong = 0;
+}
+
+// rows per block of the L2-panel mode: about 2048 blocks (8 per CU, all
+// resident at once, so they walk the panels together), a multiple of 64,
+// at most 4096 (the entry index keeps 12 bits for the row)
+static int lpanel_rows(int64_t rows) {
+  int64_t r = (rows + 2047) / 2048;
+  r = (r + 63) / 64 * 64;
+  return (int)std::min<int64_t>(std::max<int64_t>(r, 256), 4096);
+}
+
+bool lpanel_layout(const int32_t *rp, const int32_t *ci, const char *val, size_t es, int64_t n,
+                   int64_t cols, int long_t, int npanels, LpanelLayout &L) {
+  if (npanels < 1 || n <= 0 || cols <= 0) return false;
+  const int64_t pw = (cols + npanels - 1) / npanels;
+  if (pw > (1 << 20)) return false;
+  const int P = (int)((cols + pw - 1) / pw);
+  const int R = lpanel_rows(n);
+  const int64_t nblk = (n + R - 1) / R;
+  // pass 1, per block: entries per segment and rank groups per segment
+  std::vector<int32_t> seg_n((size_t)nblk * P, 0), seg_g((size_t)nblk * P, 0);
+  L.lng.assign((size_t)(n + 31) / 32 + 1, 0u);
+  L.lrows.clear();
+  for (int64_t r = 0; r < n; r++) {
+    const int32_t a = rp[r], e = rp[r + 1];
+    if (e - a > long_t) {
+      L.lng[(size_t)(r >> 5)] |= 1u << (r & 31);
+      L.lrows.push_back((int32_t)r);
+      continue;
+    }
+    const int64_t b = r / R;
+    int run = 0, kprev = -1;
+    for (int32_t j = a; j < e; j++) {
+      if (j > a && ci[j] <= ci[j - 1]) return false;
+      const int k = (int)(ci[j] / pw);
+      run = k == kprev ? run + 1 : 1;
+      kprev = k;
+      int32_t &g = seg_g[(size_t)b * P + k];
+      g = std::max(g, run);
+      seg_n[(size_t)b * P + k]++;
+    }
+  }
+  const size_t nseg = (size_t)nblk * P;
+  L.segoff.assign(nseg + 1, 0);
+  L.rgoff.assign(nseg + 1, 0);
+  for (size_t sgi = 0; sgi < nseg; sgi++) {
+    L.segoff[sgi + 1] = L.segoff[sgi] + seg_n[sgi];
+    L.rgoff[sgi + 1] = L.rgoff[sgi] + seg_g[sgi];
+  }
+  const int64_t nshort = L.segoff[nseg];
+  L.rbound.assign((size_t)std::max<int32_t>(L.rgoff[nseg], 1), 0);
+  L.idx.assign((size_t)nshort + 1, 0u);
+  L.val.assign(((size_t)nshort + 1) * es, 0);
+  // pass 2, per block: group sizes, then the entries in (panel, rank, row)
+  // order (rows are visited in order, so each group comes out row-sorted)
+  std::vector<int32_t> cnt, pos;
+  for (int64_t b = 0; b < nblk; b++) {
+    const size_t s0 = (size_t)b * P;
+    const int32_t gbase = L.rgoff[s0];
+    cnt.assign((size_t)(L.rgoff[s0 + P] - gbase), 0);
+    const int64_t ra = b * R, rb = std::min<int64_t>(n, ra + R);
+    for (int pass = 0; pass < 2; pass++) {
+      if (pass == 1) {
+        // group sizes -> group ends (from the segment start) and positions
+        pos.assign(cnt.size(), 0);
+        for (int k = 0; k < P; k++) {
+          int32_t acc = 0;
+          for (int32_t g = L.rgoff[s0 + k] - gbase; g < L.rgoff[s0 + k + 1] - gbase; g++) {
+            pos[(size_t)g] = L.segoff[s0 + k] + acc;
+            acc += cnt[(size_t)g];
+            L.rbound[(size_t)(gbase + g)] = acc;
+          }
+        }
+      }
+      for (int64_t r = ra; r < rb; r++) {
+        const int32_t a = rp[r], e = rp[r + 1];
+        if (e - a > long_t) continue;
+        int run = 0, kprev = -1;
+        for (int32_t j = a; j < e; j++) {
+          const int k = (int)(ci[j] / pw);
+          run = k == kprev ? run + 1 : 1;
+          kprev = k;
+          const size_t g = (size_t)(L.rgoff[s0 + k] - gbase + run - 1);
+          if (pass == 0) {
+            cnt[g]++;
+          } else {
+            const int32_t d = pos[g]++;
+            L.idx[(size_t)d] = (uint32_t)(r - ra) << 20 | (uint32_t)(ci[j] - (int64_t)k * pw);
+            std::memcpy(&L.val[(size_t)d * es], val + (size_t)j * es, es);
+          }
+        }
+      }
+    }
+  }
+  L.P = P;
+  L.R = R;
+  L.pw = pw;
+  L.nblk = nblk;
+  return true;
+}
+
+bool spmv_lpanel_build(Csr *A, int npanels) {
+  Ctx *c = A->ctx;
+  SpmvPlan &p = A->plan;
+  spmv_lpanel_free(A);
+  spmv_panels_free(A);  // one overlay at a time
+  spmv_sell_free(A);
+  const int64_t n = A->rows, cols = A->cols, nnz = A->nnz;
+  if (npanels < 1 || n <= 0 || cols <= 0 || nnz <= 0) return false;
+  const size_t es = A->elem();
+  std::vector<int32_t> rp(n + 1), ci(nnz);
+  std::vector<char> val((size_t)nnz * es);
+  MGX_HIP(hipStreamSynchronize(c->stream));
+  MGX_HIP(hipMemcpy(rp.data(), A->rp, (size_t)(n + 1) * 4, hipMemcpyDeviceToHost));
+  MGX_HIP(hipMemcpy(ci.data(), A->ci, (size_t)nnz * 4, hipMemcpyDeviceToHost));
+  MGX_HIP(hipMemcpy(val.data(), A->val, (size_t)nnz * es, hipMemcpyDeviceToHost));
+  const int long_t = p.long_row > 0 ? p.long_row : c->spmv_long;
+  LpanelLayout L;
+  if (!lpanel_layout(rp.data(), ci.data(), val.data(), es, n, cols, long_t, npanels, L))
+    return false;
+  p.lpanels = L.P;  // from here on spmv_lpanel_free releases what was allocated
+  try {
+    p.d_lsegoff = (int32_t *)dev_alloc(L.segoff.size() * 4);
+    p.d_lrgoff = (int32_t *)dev_alloc(L.rgoff.size() * 4);
+    p.d_lrbound = (int32_t *)dev_alloc(L.rbound.size() * 4);
+    p.d_lidx = (uint32_t *)dev_alloc(L.idx.size() * 4);
+    p.d_lval = dev_alloc(L.val.size());
+    p.d_llong = (uint32_t *)dev_alloc(L.lng.size() * 4);
+    p.d_llrows = (int32_t *)dev_alloc(std::max<size_t>(L.lrows.size(), 1) * 4);
+    MGX_HIP(hipMemcpy(p.d_lsegoff, L.segoff.data(), L.segoff.size() * 4, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_lrgoff, L.rgoff.data(), L.rgoff.size() * 4, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_lrbound, L.rbound.data(), L.rbound.size() * 4, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_lidx, L.idx.data(), L.idx.size() * 4, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_lval, L.val.data(), L.val.size(), hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(p.d_llong, L.lng.data(), L.lng.size() * 4, hipMemcpyHostToDevice));
+    if (!L.lrows.empty())
+      MGX_HIP(hipMemcpy(p.d_llrows, L.lrows.data(), L.lrows.size() * 4, hipMemcpyHostToDevice));
+  } catch (...) {
+    spmv_lpanel_free(A);
+    throw;
+  }
+  p.lrows = L.R;
+  p.lpw = L.pw;
+  p.lnblk = L.nblk;
+  p.nllong = (int64_t)L.lrows.size();
+  return true;
+}
+
+template <class T, bool AB>
+static void spmv_lpanel_dispatch(Csr *A, const T *X, T *Y, T alpha, T beta,
+                                 const int32_t *live) {
+  Ctx *c = A->ctx;
+  const SpmvPlan &p = A->plan;
+  const size_t lds = (size_t)p.lrows * sizeof(T);
+  k_spmv_lpanel<T, AB><<<(unsigned)p.lnblk, kBlock, lds, c->stream>>>(
+      A->rows, p.lrows, p.lpanels, p.lpw, p.lnblk, p.d_lsegoff, p.d_lrgoff, p.d_lrbound,
+      p.d_lidx, static_cast<const T *>(p.d_lval), p.d_llong, X, Y, alpha, beta, live);
+  if (p.nllong > 0)
+    k_spmv_longrows<T, AB><<<(unsigned)p.nllong, kBlock, 0, c->stream>>>(
+        p.d_llrows, p.nllong, A->rp, A->ci, static_cast<const T *>(A->val), X, Y, alpha, beta,
+        live);
+  MGX_HIP(hipGetLastError());
+}
+
 template <class T>
 static void spmv_panels_dispatch(Csr *A, const T *X, T *Y) {
   Ctx *c = A->ctx;
@@ -851,6 +1126,11 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
       spmv_sell_dispatch<T, AB>(A, *l, V, X, Y, alpha, beta, live);
       return;
     }
+  }
+  // L2-panel mode: A's own values
+  if (A->plan.lpanels > 0 && static_cast<const void *>(V) == A->val && sizeof(T) == A->elem()) {
+    spmv_lpanel_dispatch<T, AB>(A, X, Y, alpha, beta, live);
+    return;
   }
   // column-panel mode: plain y = A x with A's own values, outside solver loops
   if (!AB && !live && A->plan.npanels > 0 && V == static_cast<const T *>(A->val)) {
@@ -1000,7 +1280,7 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
                        best.walk);
   // column panels, when x outgrows an XCD's L2 (4 MiB)
   double best_us = best.us;
-  int best_panels = 0, ncand = (int)cands.size();
+  int best_panels = 0, ncand = (int)cands.size(), lpanel_best = 0;
   if ((size_t)cols * es > ((size_t)4 << 20)) {
     x = dev_alloc((size_t)cols * es);
     y = dev_alloc((size_t)std::max<int64_t>(rows, 1) * es);
@@ -1036,10 +1316,52 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
     (void)hipEventDestroy(e1);
     dev_free(x);
     dev_free(y);
-    if (best_panels)
-      spmv_panels_build(A, best_panels);
-    else
-      spmv_panels_free(A);
+    // L2 panels of about 0.5, 1 and 2 MiB of x
+    int best_lp = 0;
+    x = dev_alloc((size_t)cols * es);
+    y = dev_alloc((size_t)std::max<int64_t>(rows, 1) * es);
+    try {
+      MGX_HIP(hipMemsetAsync(x, 0, (size_t)cols * es, c->stream));
+      MGX_HIP(hipEventCreate(&e0));
+      MGX_HIP(hipEventCreate(&e1));
+      for (size_t pb : {(size_t)1 << 19, (size_t)1 << 20, (size_t)2 << 20}) {
+        const int np = (int)(((size_t)cols * es + pb - 1) / pb);
+        if (!spmv_lpanel_build(A, np)) continue;
+        ncand++;
+        for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+        MGX_HIP(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
+        MGX_HIP(hipEventRecord(e1, c->stream));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / reps;
+        if (us < best_us) {
+          best_us = us;
+          best_lp = A->plan.lpanels;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      dev_free(x);
+      dev_free(y);
+      spmv_lpanel_free(A);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    e0 = e1 = nullptr;
+    dev_free(x);
+    dev_free(y);
+    if (best_lp) {
+      best_panels = 0;
+      if (A->plan.lpanels != best_lp) spmv_lpanel_build(A, best_lp);
+      lpanel_best = best_lp;
+    } else {
+      spmv_lpanel_free(A);
+      if (best_panels) spmv_panels_build(A, best_panels);
+    }
   }
   // the sliced layout, where its padding is small
   int best_sell = 0;
@@ -1052,7 +1374,7 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
       MGX_HIP(hipEventCreate(&e0));
       MGX_HIP(hipEventCreate(&e1));
       // keep the panels while the sliced copy is timed: it drops them
-      const int keep_panels = best_panels;
+      const int keep_panels = best_panels, keep_lp = lpanel_best;
       const auto timed = [&](int n) {
         for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
         MGX_HIP(hipEventRecord(e0, c->stream));
@@ -1075,7 +1397,7 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
             sell_remap = rr;
           }
         }
-        if (!keep_panels && sell_us < 1.05 * best_us && best_us < 1.05 * sell_us) {
+        if (!keep_panels && !keep_lp && sell_us < 1.05 * best_us && best_us < 1.05 * sell_us) {
           // Plans within a few per cent of each other swap places between
           // short timings; settle them by the medians of interleaved rounds.
           std::vector<double> tb, ts;
@@ -1096,11 +1418,13 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
           best_us = sell_us;
           best_sell = 1;
           best_panels = 0;
+          lpanel_best = 0;
         }
         A->plan.remap = best_sell ? sell_remap : best.remap;
         if (!best_sell) {
           spmv_sell_free(A);
           if (keep_panels) spmv_panels_build(A, keep_panels);
+          if (keep_lp) spmv_lpanel_build(A, keep_lp);
         }
       }
     } catch (...) {
@@ -1124,8 +1448,8 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
     out->xcd_remap = A->plan.remap;
     out->us = best_us;
     out->candidates = (int32_t)ncand;
-    out->column_panels = best_panels;
-    out->row_walk = best_sell ? 2 : best.walk;
+    out->column_panels = lpanel_best ? lpanel_best : best_panels;
+    out->row_walk = best_sell ? 2 : (lpanel_best ? 3 : best.walk);
   }
 }
 

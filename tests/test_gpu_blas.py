@@ -395,7 +395,9 @@ def test_matvec_plans_bit_identical(m, ctx, oracle, dtype):
                  dict(rows_per_block=7, nnz_per_block=4096, row_walk=1, xcd_remap=False),
                  dict(rows_per_block=64, nnz_per_block=4096),
                  dict(rows_per_block=256, nnz_per_block=2048, column_panels=4),
-                 dict(rows_per_block=128, nnz_per_block=1024, column_panels=16, xcd_remap=False)]:
+                 dict(rows_per_block=128, nnz_per_block=1024, column_panels=16, xcd_remap=False),
+                 dict(row_walk=3, column_panels=1), dict(row_walk=3, column_panels=6),
+                 dict(row_walk=3, column_panels=64)]:
         A.set_plan(**plan)
         Y.fill(0.0)
         m.matvec(A, X, Y)
@@ -408,6 +410,88 @@ def test_matvec_plans_bit_identical(m, ctx, oracle, dtype):
     Y.fill(0.0)
     m.matvec(A, X, Y)
     assert bits_equal(Y.download(), base), t
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_matvec_l2_panels(m, ctx, oracle, dtype):
+    """The L2-panel plan (row sums in LDS, column panels walked in order,
+    entries grouped by rank within their row's run of a panel): the CPU row
+    loop's bits for every short row -- with empty rows, rows whose entries
+    fall in one panel (deep rank groups), rows above the long-row threshold
+    (the block tree, as every plan) -- for y = A x and y = a A x + b y, for
+    panel counts from 1 to a panel per 1000 columns; update_values drops the
+    layout and the block plan gives the same bits."""
+    say("matvec l2 panels", dtype.__name__)
+    rng = np.random.default_rng(21)
+    n, cols = 30000, 250000
+    lens = np.clip(np.round(np.exp(rng.normal(np.log(24) - 0.5, 1.0, n))), 0, 1500).astype(int)
+    lens[::97] = 0
+    lens[5::1013] = 700
+    rows_ci = []
+    for r, ln in enumerate(lens):
+        if r % 777 == 3:
+            rows_ci.append(np.arange(200, dtype=np.int64) * 3)  # 200 entries, one panel
+        else:
+            rows_ci.append(np.sort(rng.choice(cols, ln, replace=False)))
+    lens = np.array([len(c) for c in rows_ci])
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate(rows_ci).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1]).astype(dtype)
+    x = oracle.random_vector(cols, -1.0, 1.0, 22, dtype=dtype)
+    y0 = oracle.random_vector(n, -1.0, 1.0, 23, dtype=dtype)
+    A = m.CRS(ctx, n, cols, rp, ci, val)
+    X = m.Vector(ctx, x, dtype=dtype)
+    Y = m.Vector(ctx, n=n, dtype=dtype)
+    A.set_plan(rows_per_block=128, nnz_per_block=2048)
+    m.matvec(A, X, Y)
+    base = Y.download()
+    Y.upload(y0)
+    m.matvec_ab(0.75, A, X, -1.5, Y)
+    base_ab = Y.download()
+    ref = oracle.spmv(rp, ci, val, x)
+    short = np.diff(rp) <= 256
+    assert (~short).sum() > 10 and bits_equal(base[short], ref[short])
+    for panels in (1, 3, 17, 250):
+        A.set_plan(row_walk=3, column_panels=panels)
+        Y.fill(np.nan)
+        m.matvec(A, X, Y)
+        assert bits_equal(Y.download(), base), panels
+        Y.upload(y0)
+        m.matvec_ab(0.75, A, X, -1.5, Y)
+        assert bits_equal(Y.download(), base_ab), panels
+    A.update_values((val * 2).astype(dtype))
+    m.matvec(A, X, Y)
+    ref2 = oracle.spmv(rp, ci, (val * 2).astype(dtype), x)
+    assert bits_equal(Y.download()[short], ref2[short])
+    # unsorted columns within a row: not applicable
+    ci_bad = ci.copy()
+    r0 = int(np.argmax(lens > 3))
+    ci_bad[rp[r0]], ci_bad[rp[r0] + 1] = ci[rp[r0] + 1], ci[rp[r0]]
+    B = m.CRS(ctx, n, cols, rp, ci_bad, val)
+    with pytest.raises(m.MgxError):
+        B.set_plan(row_walk=3, column_panels=4)
+    for h in (A, B, X, Y):
+        h.free()
+
+
+def test_split_cg_with_l2_panel_spmv(m, ctx, oracle):
+    """The split CG iteration (n >= 2^19) runs the matrix's SpMV plan with
+    the solver's stop flag; under the L2-panel plan 40 iterations are
+    bit-identical to the oracle."""
+    say("split cg l2 panels")
+    rp, ci, val = m.stencil3d(96, 96, 64, 7, 0.0)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3001)
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=1e-30, maxiter=40,
+                       history=True)
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    A.set_plan(row_walk=3, column_panels=5)
+    X = m.Vector(ctx, n=n)
+    res = m.cg(A, X, m.Vector(ctx, b), tol=1e-30, maxiter=40, precond=2, mode=m.MODE_FUSED,
+               history=40)
+    assert res.status == ref[0] and res.final_iter == 40
+    assert bits_equal(X.download(), ref[1]) and bits_equal(res.rhistory, ref[4])
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
