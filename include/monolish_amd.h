@@ -36,7 +36,9 @@ extern "C" {
 #define MGX_ERR_NOMEM -103    /* device or host allocation failed */
 #define MGX_ERR_BUDGET -104   /* request exceeds the configured HBM budget */
 #define MGX_ERR_NOT_FOUND -105
-#define MGX_ERR_TIMEOUT -106   /* a device wait gave up (no progress within its time limit) */
+#define MGX_ERR_TIMEOUT -106  /* a wait gave up: a device wait without progress, or a
+                                 served request past its deadline */
+#define MGX_ERR_CANCELLED -107 /* a served request cancelled by mgx_server_cancel */
 
 typedef enum { MGX_F64 = 0, MGX_F32 = 1 } mgx_dtype;
 
@@ -378,6 +380,8 @@ typedef struct mgx_server_stats {
   double mean_queue_s;      /* submit -> admission */
   double solves_per_s;      /* completed / (last completion - first submit) */
   size_t peak_bytes;        /* peak resident bytes (matrices + work) */
+  int64_t cancelled;        /* requests ended by mgx_server_cancel */
+  int64_t expired;          /* requests ended by their deadline */
 } mgx_server_stats;
 
 /* A worker thread on the context's stream admits requests in arrival order
@@ -407,6 +411,22 @@ int mgx_server_submit_resident(mgx_server *s, int method, mgx_csr *A, mgx_vec *b
  * the ticket */
 int mgx_server_wait(mgx_server *s, int64_t ticket, double timeout_s,
                     mgx_solve_result *res, double *x);
+/* Per-request control (no monolish counterpart: the reference solves one
+ * system per call).  mgx_server_cancel ends a queued request at once and a
+ * running one at the worker's next poll (status MGX_ERR_CANCELLED; a running
+ * request's x holds its last iterate); a request that finished first keeps
+ * its result.  mgx_server_set_deadline(s, t, sec) ends the request with
+ * MGX_ERR_TIMEOUT if it has not finished sec seconds after its submission
+ * (0: no deadline).  mgx_server_poll gives MGX_REQ_QUEUED / _ACTIVE / _DONE.
+ * Ended requests are still collected with mgx_server_wait; latency
+ * statistics count completed requests only.  Unknown tickets:
+ * MGX_ERR_NOT_FOUND. */
+#define MGX_REQ_QUEUED 0
+#define MGX_REQ_ACTIVE 1
+#define MGX_REQ_DONE 2
+int mgx_server_cancel(mgx_server *s, int64_t ticket);
+int mgx_server_set_deadline(mgx_server *s, int64_t ticket, double seconds);
+int mgx_server_poll(mgx_server *s, int64_t ticket, int32_t *state);
 int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out);
 /* finishes queued and resident requests, stops the worker */
 int mgx_server_destroy(mgx_server *s);

@@ -21,7 +21,9 @@ LIB_PATH = os.path.join(_HERE, "libmonolish_amd.so")
 
 OK = 0
 SOLVER_SIZE_ERROR, SOLVER_MAXITER, SOLVER_BREAKDOWN, SOLVER_RESIDUAL_NAN, SOLVER_NOT_IMPL = -1, -2, -3, -4, -10
-ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND, ERR_TIMEOUT = -101, -102, -103, -104, -105, -106
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_NOT_FOUND, ERR_TIMEOUT, ERR_CANCELLED = (
+    -101, -102, -103, -104, -105, -106, -107)
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE = 0, 1, 2
 F64, F32 = 0, 1
 PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
 PRECOND_IC, PRECOND_ILU = 4, 5
@@ -144,7 +146,7 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", _I64), ("completed", _I64), ("active", _I64), ("queued", _I64),
                 ("p50_latency_s", _D), ("p99_latency_s", _D), ("mean_latency_s", _D),
                 ("max_latency_s", _D), ("mean_queue_s", _D), ("solves_per_s", _D),
-                ("peak_bytes", C.c_size_t)]
+                ("peak_bytes", C.c_size_t), ("cancelled", _I64), ("expired", _I64)]
 
 
 _sig("mgx_server_create", [_P, C.POINTER(ServerConfig), C.POINTER(_P)])
@@ -152,6 +154,9 @@ _sig("mgx_server_submit", [_P, C.c_int, _I64, _I64, _P, _P, _P, _P, _P, C.POINTE
                            C.POINTER(_I64)])
 _sig("mgx_server_submit_resident", [_P, C.c_int, _P, _P, _P, C.POINTER(SolveParams),
                                     C.POINTER(_I64)])
+_sig("mgx_server_cancel", [_P, _I64])
+_sig("mgx_server_set_deadline", [_P, _I64, _D])
+_sig("mgx_server_poll", [_P, _I64, C.POINTER(_I32)])
 _sig("mgx_server_wait", [_P, _I64, _D, C.POINTER(SolveResultC), _P])
 _sig("mgx_server_stats_get", [_P, C.POINTER(ServerStats)])
 _sig("mgx_server_destroy", [_P])
@@ -748,6 +753,22 @@ class Server:
             return None
         check(rc)
         return SolveResult(r.status, r.final_iter, r.final_resid, r.seconds), x
+
+    def cancel(self, ticket: int):
+        """End a queued or running request (status ERR_CANCELLED); a request
+        that already finished keeps its result."""
+        check(_lib.mgx_server_cancel(self._h, ticket))
+
+    def set_deadline(self, ticket: int, seconds: float):
+        """End the request with ERR_TIMEOUT unless it finishes within
+        `seconds` of its submission (0: no deadline)."""
+        check(_lib.mgx_server_set_deadline(self._h, ticket, float(seconds)))
+
+    def poll(self, ticket: int) -> int:
+        """REQ_QUEUED, REQ_ACTIVE or REQ_DONE."""
+        st = _I32(0)
+        check(_lib.mgx_server_poll(self._h, ticket, C.byref(st)))
+        return int(st.value)
 
     def stats(self) -> dict:
         s = ServerStats()

@@ -224,7 +224,10 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   d.row0 = 0;
   d.n = n;
   d.ntiles = nt;
-  d.split = cfg.method == kMethodCG && n >= kSplitRows ? 1 : 0;
+  // A lone large CG system runs the split iteration (its own tuned SpMV
+  // plan); in multi-system launches (gather_once) a large system joins the
+  // shared tile launches instead of adding its own SpMV launch per iteration.
+  d.split = cfg.method == kMethodCG && n >= kSplitRows && !cfg.gather_once ? 1 : 0;
   d.once = cfg.method == kMethodBiCGSTAB ||
                    (!d.split && (cfg.gather_once || A->gather_lines > kScatteredLines))
                ? 1

@@ -165,6 +165,67 @@ __device__ __forceinline__ bool arrive_reduce(uint32_t *tick, const T *P, int64_
                                               T *sh, T *res, int *flag) {
   __builtin_amdgcn_s_waitcnt(0);  // this lane's published stores completed
   __syncthreads();
+  if (ntiles <= kRedGroups) {
+    // Small reductions (each stage-2 lane holds at most one tile): the units
+    // count straight into the supergroups and those into the top counter;
+    // the last block forms every lane sum itself (0 + the tile partial),
+    // two fewer dependent memory round trips than the general path and the
+    // same bits.
+    const int64_t U = Q * ntiles;
+    const int64_t SG = U < kRedSuper ? U : kRedSuper;
+    if (threadIdx.x == 0) {
+      const int64_t sg = unit % SG;
+      const uint32_t members = (uint32_t)((U - sg + SG - 1) / SG);
+      int last = 0;
+      const uint32_t t = __hip_atomic_fetch_add(&tick[1 + sg], 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+      if (t == members - 1) {
+        __hip_atomic_store(&tick[1 + sg], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t t2 = __hip_atomic_fetch_add(&tick[0], 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (t2 == (uint32_t)SG - 1) {
+          __hip_atomic_store(&tick[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          last = 1;
+        }
+      }
+      *flag = last;
+    }
+    __syncthreads();
+    if (!*flag) return false;
+    T lane[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      lane[v] = T(0);
+      if ((int64_t)threadIdx.x < ntiles) {
+        const T *Pv = P + v * pstride;
+        const int64_t k = threadIdx.x;
+        T p;
+        if (Q == 4) {
+          const T w0 = load_partial(Pv, 4 * k), w1 = load_partial(Pv, 4 * k + 1);
+          const T w2 = load_partial(Pv, 4 * k + 2), w3 = load_partial(Pv, 4 * k + 3);
+          const T lo = w0 + w1;
+          const T hi = w2 + w3;
+          p = lo + hi;
+        } else {
+          p = load_partial(Pv, k);
+        }
+        lane[v] = lane[v] + p;
+      }
+    }
+    if (NV == 1) {
+      const T r = block_tree(lane[0], sh);
+      if (threadIdx.x == 0) res[0] = r;
+    } else {
+      T s0 = lane[0], s1 = lane[NV - 1];
+      block_tree2(s0, s1, sh);
+      if (threadIdx.x == 0) {
+        res[0] = s0;
+        res[1] = s1;
+      }
+    }
+    __syncthreads();
+    return true;
+  }
   const int64_t G = ntiles < kRedGroups ? ntiles : kRedGroups;
   const int64_t g = (Q == 4 ? (unit >> 2) : unit) % G;
   const int64_t m = (ntiles - g + G - 1) / G;  // tiles of group g

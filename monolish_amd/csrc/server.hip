@@ -90,6 +90,10 @@ struct Request {
   // submitted by handle: A, b (and x when given) belong to the caller
   bool by_handle = false;
   bool own_x = true;
+  // control: deadline (absolute seconds_now() time, 0: none) and cancel
+  double deadline = 0;
+  bool cancel = false;
+  bool active = false;  // admitted: holds an engine slot
   // outcome
   bool done = false;
   mgx_solve_result res{};
@@ -110,7 +114,7 @@ struct Server {
   std::vector<std::shared_ptr<Request>> by_slot;
   std::thread worker;
   bool stop = false;
-  int64_t next_id = 1, submitted = 0, completed = 0;
+  int64_t next_id = 1, submitted = 0, completed = 0, cancelled = 0, expired = 0;
   size_t resident = 0, peak = 0, budget = 0;
   std::vector<double> lat, qwait;
   double first_submit = -1, last_done = 0;
@@ -172,15 +176,70 @@ struct Server {
 
   void finish(const std::shared_ptr<Request> &r, int status) {
     std::lock_guard<std::mutex> lk(mu);
+    finish_locked(r, status);
+  }
+
+  void finish_locked(const std::shared_ptr<Request> &r, int status) {
     r->t_done = seconds_now();
     r->res.status = status;
     r->res.seconds = r->t_done - r->t_submit;
     r->done = true;
-    completed++;
-    lat.push_back(r->t_done - r->t_submit);
-    qwait.push_back(r->t_admit > 0 ? r->t_admit - r->t_submit : 0);
-    last_done = r->t_done;
+    r->active = false;
+    if (status == MGX_ERR_CANCELLED || status == MGX_ERR_TIMEOUT) {
+      (status == MGX_ERR_CANCELLED ? cancelled : expired)++;
+    } else {
+      completed++;
+      lat.push_back(r->t_done - r->t_submit);
+      qwait.push_back(r->t_admit > 0 ? r->t_admit - r->t_submit : 0);
+      last_done = r->t_done;
+    }
     cv_done.notify_all();
+  }
+
+  // queued requests that were cancelled or whose deadline passed leave the
+  // queue (called with mu held)
+  void drop_queued_locked(double now) {
+    for (auto it = queue.begin(); it != queue.end();) {
+      const auto &r = *it;
+      const bool late = r->deadline > 0 && now >= r->deadline;
+      if (r->cancel || late) {
+        auto keep = r;
+        it = queue.erase(it);
+        finish_locked(keep, keep->cancel ? MGX_ERR_CANCELLED : MGX_ERR_TIMEOUT);
+      } else {
+        ++it;
+      }
+    }
+  }
+
+  // running requests that were cancelled or whose deadline passed give up
+  // their slot; x keeps the last iterate (by-handle requests) or is copied
+  // out (requests submitted with their arrays)
+  void drop_active(double now) {
+    std::vector<std::shared_ptr<Request>> out;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto &r : by_slot)
+        if (r && !r->done && (r->cancel || (r->deadline > 0 && now >= r->deadline)))
+          out.push_back(r);
+    }
+    if (!out.empty()) eng->fetch_states();  // waits for the stream: x is final
+    for (auto &r : out) {
+      const int code = r->cancel ? MGX_ERR_CANCELLED : MGX_ERR_TIMEOUT;
+      eng->release(r->slot);
+      const SolveState st = eng->state(r->slot);
+      r->res.final_iter = st.iter;
+      r->res.final_resid = st.resid;
+      if (!r->by_handle) {
+        r->x_out.resize(r->n);
+        MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
+      }
+      by_slot[r->slot].reset();
+      free_resident(*r);
+      std::lock_guard<std::mutex> lk(mu);
+      resident -= r->bytes;
+      finish_locked(r, code);
+    }
   }
 
   void loop() {
@@ -190,6 +249,7 @@ struct Server {
       std::vector<std::shared_ptr<Request>> admit;
       {
         std::unique_lock<std::mutex> lk(mu);
+        drop_queued_locked(seconds_now());
         while (!queue.empty() && eng->active() + (int)admit.size() < cfg.max_active) {
           auto r = queue.front();
           if (r->bytes > budget) {  // can never fit
@@ -218,6 +278,10 @@ struct Server {
           r->slot = eng->add(r->A, r->A->val, r->x, r->bd, r->cfg);
           if (r->slot < 0) throw Error{MGX_ERR_BUDGET};
           r->t_admit = seconds_now();
+          {
+            std::lock_guard<std::mutex> lk(mu);
+            r->active = true;
+          }
           by_slot[r->slot] = r;
         } catch (const Error &e) {
           free_resident(*r);
@@ -228,6 +292,8 @@ struct Server {
           finish(r, e.code);
         }
       }
+      if (eng->active() == 0) continue;
+      drop_active(seconds_now());
       if (eng->active() == 0) continue;
       eng->step(cfg.iters_per_poll);
       for (int s : eng->poll()) {
@@ -430,6 +496,55 @@ int mgx_server_wait(mgx_server *s, int64_t ticket, double timeout_s, mgx_solve_r
   });
 }
 
+int mgx_server_cancel(mgx_server *s, int64_t ticket) {
+  return guard([&] {
+    MGX_REQUIRE(s, "mgx_server_cancel: NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+      set_error("mgx_server_cancel: unknown ticket %lld", (long long)ticket);
+      return MGX_ERR_NOT_FOUND;
+    }
+    auto r = it->second;
+    if (r->done) return MGX_OK;  // finished first: the result stands
+    r->cancel = true;
+    if (!r->active) s->drop_queued_locked(seconds_now());
+    s->cv_work.notify_one();
+    return MGX_OK;
+  });
+}
+
+int mgx_server_set_deadline(mgx_server *s, int64_t ticket, double seconds) {
+  return guard([&] {
+    MGX_REQUIRE(s, "mgx_server_set_deadline: NULL server");
+    MGX_REQUIRE(seconds >= 0, "mgx_server_set_deadline: negative time");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+      set_error("mgx_server_set_deadline: unknown ticket %lld", (long long)ticket);
+      return MGX_ERR_NOT_FOUND;
+    }
+    it->second->deadline = seconds > 0 ? it->second->t_submit + seconds : 0;
+    s->cv_work.notify_one();
+    return MGX_OK;
+  });
+}
+
+int mgx_server_poll(mgx_server *s, int64_t ticket, int32_t *state) {
+  return guard([&] {
+    MGX_REQUIRE(s && state, "mgx_server_poll: NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) {
+      set_error("mgx_server_poll: unknown ticket %lld", (long long)ticket);
+      return MGX_ERR_NOT_FOUND;
+    }
+    const auto &r = it->second;
+    *state = r->done ? MGX_REQ_DONE : (r->active ? MGX_REQ_ACTIVE : MGX_REQ_QUEUED);
+    return MGX_OK;
+  });
+}
+
 int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out) {
   return guard([&] {
     MGX_REQUIRE(s && out, "mgx_server_stats_get: NULL argument");
@@ -437,7 +552,7 @@ int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out) {
     out->submitted = s->submitted;
     out->completed = s->completed;
     out->queued = (int64_t)s->queue.size();
-    out->active = s->submitted - s->completed - out->queued;
+    out->active = s->submitted - s->completed - s->cancelled - s->expired - out->queued;
     out->p50_latency_s = percentile(s->lat, 0.50);
     out->p99_latency_s = percentile(s->lat, 0.99);
     double sum = 0, mx = 0, qs = 0;
@@ -452,6 +567,8 @@ int mgx_server_stats_get(mgx_server *s, mgx_server_stats *out) {
     const double span = s->last_done - s->first_submit;
     out->solves_per_s = span > 0 ? (double)s->completed / span : 0;
     out->peak_bytes = s->peak;
+    out->cancelled = s->cancelled;
+    out->expired = s->expired;
     return MGX_OK;
   });
 }

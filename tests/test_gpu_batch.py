@@ -280,3 +280,68 @@ def test_graph_replay_bit_identical(m, oracle):
                 assert st == ref[0] and it == ref[2] and np.array_equal(x, bits(ref[1]))
     finally:
         c.close()
+
+
+def test_server_cancel_deadline_poll(m, oracle):
+    """Per-request control: a long request (one slot) runs while others
+    queue; poll reports ACTIVE / QUEUED; a queued request cancelled ends at
+    once with ERR_CANCELLED; the long request ends at its deadline with
+    ERR_TIMEOUT after some iterations; the remaining request completes,
+    bit-identical to the oracle; statistics count each outcome."""
+    say("server control")
+    import time
+    c = m.Context(0)
+    try:
+        rp_l, ci_l, v_l = m.laplacian2d(400, 400)
+        b_l = oracle.random_vector(400 * 400, -1.0, 1.0, 9)
+        rp, ci, v = m.laplacian2d(40, 40)
+        b = oracle.random_vector(1600, -1.0, 1.0, 10)
+        srv = m.Server(c, max_active=1, iters_per_poll=4)
+        t_long = srv.submit(m.METHOD_CG, rp_l, ci_l, v_l, b_l, tol=1e-300, tol_is_relative=False,
+                            maxiter=10 ** 8)
+        t_cancel = srv.submit(m.METHOD_CG, rp, ci, v, b, tol=1e-10, tol_is_relative=True)
+        t_keep = srv.submit(m.METHOD_CG, rp, ci, v, b, tol=1e-10, tol_is_relative=True)
+        t0 = time.time()
+        while srv.poll(t_long) != m.REQ_ACTIVE and time.time() - t0 < 30:
+            time.sleep(0.01)
+        assert srv.poll(t_long) == m.REQ_ACTIVE
+        assert srv.poll(t_cancel) == m.REQ_QUEUED and srv.poll(t_keep) == m.REQ_QUEUED
+        srv.cancel(t_cancel)
+        r, _ = srv.wait(t_cancel, 1600, timeout=10.0)
+        assert r.status == m.ERR_CANCELLED
+        srv.set_deadline(t_long, 0.5)
+        r, x_long = srv.wait(t_long, 400 * 400, timeout=60.0)
+        assert r.status == m.ERR_TIMEOUT and r.final_iter > 0
+        assert np.all(np.isfinite(x_long)) and np.any(x_long != 0)
+        r, x = srv.wait(t_keep, 1600, timeout=60.0)
+        ref = oracle_solve(oracle, "cg", rp, ci, v, b)
+        assert r.status == 0 and r.final_iter == ref[2]
+        assert np.array_equal(bits(x), bits(ref[1]))
+        st = srv.stats()
+        assert (st["completed"], st["cancelled"], st["expired"]) == (1, 1, 1)
+        with pytest.raises(m.MgxError):
+            srv.poll(t_keep)  # collected by wait: unknown now
+        srv.close()
+    finally:
+        c.close()
+
+
+def test_batch_large_system_shares_launches(m, ctx, oracle):
+    """In a batch a CG system of n >= 2^19 rows joins the shared tile
+    launches (p formed first, gathered once) instead of running its own
+    SpMV per iteration; 40 iterations of it and a small system are
+    bit-identical to the oracle."""
+    say("batch large system")
+    rp1, ci1, v1 = m.stencil3d(96, 96, 64, 7, 0.0)
+    rp2, ci2, v2 = m.laplacian2d(50, 50)
+    syss = [(rp1, ci1, v1), (rp2, ci2, v2)]
+    bs = [oracle.random_vector(len(rp) - 1, -1.0, 1.0, 31 + k) for k, (rp, _, _) in
+          enumerate(syss)]
+    As = [m.CRS(ctx, len(b), len(b), rp, ci, v) for (rp, ci, v), b in zip(syss, bs)]
+    Xs = [m.Vector(ctx, n=len(b)) for b in bs]
+    Bs = [m.Vector(ctx, b) for b in bs]
+    res = m.batch_solve(ctx, m.METHOD_CG, As, Xs, Bs, tol=1e-30, maxiter=40, precond=2)
+    for (rp, ci, v), b, r, X in zip(syss, bs, res, Xs):
+        ref = oracle.solve("cg", rp, ci, v, np.zeros(len(b)), b, precond=2, tol=1e-30, maxiter=40)
+        assert r.status == ref[0] == m.SOLVER_MAXITER and r.final_iter == 40
+        assert np.array_equal(bits(X.download()), bits(ref[1]))
