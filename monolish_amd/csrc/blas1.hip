@@ -225,20 +225,19 @@ void launch_convert(Ctx *c, mgx_dtype from, mgx_dtype to, int64_t n,
   MGX_HIP(hipGetLastError());
 }
 
-void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
-                double *d_out, uint32_t *d_ticket) {
+// the canonical dot into *d_out with the tile partials and lane sums in
+// `partials` ((ntiles + 2 * kRedGroups) values) and zeroed counters d_ticket
+static void launch_dot_into(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
+                            void *partials, double *d_out, uint32_t *d_ticket) {
   if (n <= 0) {
     launch_fill(c, MGX_F64, d_out, 1, 0.0);
     return;
   }
   const int64_t nt = ntiles_of(n);
-  const size_t es = dt == MGX_F64 ? 8 : 4;
-  // tile partials, then the 256 lane sums
-  ensure_partials(c, (size_t)(nt + 2 * kRedGroups) * es);
   const bool vec = aligned16(x) && aligned16(y);
   const unsigned g = (unsigned)nt;
   if (dt == MGX_F64) {
-    double *P = tp<double>(c->d_partials), *S = P + nt;
+    double *P = tp<double>(partials), *S = P + nt;
     if (vec)
       k_dot<double, true><<<g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y), P, S,
                                                        d_ticket, d_out);
@@ -246,7 +245,7 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
       k_dot<double, false><<<g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y), P, S,
                                                         d_ticket, d_out);
   } else {
-    float *P = tp<float>(c->d_partials), *S = P + nt;
+    float *P = tp<float>(partials), *S = P + nt;
     if (vec)
       k_dot<float, true><<<g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y), P, S,
                                                       d_ticket, d_out);
@@ -255,6 +254,16 @@ void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
                                                        d_ticket, d_out);
   }
   MGX_HIP(hipGetLastError());
+}
+
+void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
+                double *d_out, uint32_t *d_ticket) {
+  if (n > 0) {
+    // tile partials, then the 256 lane sums
+    const size_t es = dt == MGX_F64 ? 8 : 4;
+    ensure_partials(c, (size_t)(ntiles_of(n) + 2 * kRedGroups) * es);
+  }
+  launch_dot_into(c, dt, n, x, y, c->d_partials, d_out, d_ticket);
 }
 
 void launch_mnrm2sq(Ctx *c, mgx_dtype dt, int64_t n, int m, const void *X, int64_t ld,
