@@ -106,6 +106,164 @@ template <class T> struct IcFactorRow {
   }
 };
 
+// Wave-per-row factorisation for the chip-wide path (one launch per level):
+// the 64 lanes of a wave hold the entries of row i (rows of at most 64
+// entries); the rows k that row i reads (one per entry left of its
+// diagonal, final since an earlier level) are staged in LDS in one
+// coalesced pass; then the k are taken one at a time in column order, each
+// from LDS.  ILU(0): the owner lane of a_ik forms l_ik = a_ik / a_kk, and
+// every later entry p of row i that row k also holds (binary search of its
+// column in row k's staged upper part) does a_ip -= l_ik a_kp.  IC(0): the
+// lanes p < k find their column in row k's staged lower part and form the
+// products l_ip l_kp; the owner lane of l_ik subtracts them in increasing
+// column order, then divides by l_kk; the diagonal lane subtracts the l_ip^2
+// in order and takes the square root.  Every value goes through the same
+// operations, in the same order, as in the one-thread-per-row sweep, so the
+// factor is bit-identical; the serial chain per row shrinks from one global
+// load per merge step to one staging pass.
+constexpr int kFacWaves = 4;       // rows (waves) per block
+constexpr int kFacStage = 1024;    // staged entries of the rows read, per wave
+
+__device__ __forceinline__ void fac_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// position of column c in the sorted scol[a, b), or -1
+__device__ __forceinline__ int32_t fac_find(const int32_t *scol, int32_t a, int32_t b, int32_t c) {
+  while (a < b) {
+    const int32_t mid = (a + b) >> 1;
+    const int32_t v = scol[mid];
+    if (v == c) return mid;
+    if (v < c)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  return -1;
+}
+
+template <class T, bool IC>
+__global__ __launch_bounds__(64 * kFacWaves) void k_factor_level_wave(
+    const int32_t *__restrict__ rows, int32_t count, const int32_t *__restrict__ rp,
+    const int32_t *__restrict__ ci, const int32_t *__restrict__ diag, T *f, int32_t *bad) {
+  __shared__ int32_t scol[kFacWaves][kFacStage];
+  __shared__ T sval[kFacWaves][kFacStage];
+  __shared__ int32_t soff[kFacWaves][65];  // staged row j0 + a at [soff[a], soff[a + 1])
+  __shared__ int32_t ssrc[kFacWaves][64];  // its first staged entry in ci / f
+  __shared__ int32_t spiv[kFacWaves][64];  // the stage position of its diagonal
+  __shared__ T sli[kFacWaves][64];         // IC: products, then row i's final values
+  __shared__ int32_t sm[kFacWaves][64];    // IC: lane p found its column in row k
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int32_t r = (int32_t)blockIdx.x * kFacWaves + w;
+  if (r >= count) return;  // a whole wave; only wave-level syncs follow
+  const int32_t i = rows[r];
+  const int32_t e0 = rp[i], e1 = rp[i + 1], di = diag[i];
+  const int32_t len = e1 - e0, nlow = di - e0;
+  int32_t *col = scol[w];
+  T *val = sval[w];
+  const int32_t mycol = lane < len ? ci[e0 + lane] : INT32_MAX;
+  T v = lane < len ? f[e0 + lane] : T(0);
+  // lane j < nlow reads row k_j = the column of row i's entry j: ILU stages
+  // its pivot and upper part [dk, end), IC its lower part and pivot [start, dk]
+  const bool owns = lane < nlow;
+  const int32_t ka = owns ? rp[mycol] : 0;
+  const int32_t kd = owns ? diag[mycol] - ka : 0;
+  const int32_t klen = owns ? rp[mycol + 1] - ka : 0;
+  const int32_t pa = IC ? 0 : kd;
+  const int32_t pn = owns ? (IC ? kd + 1 : klen - kd) : 0;
+  int32_t incl = pn;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  const int32_t excl = incl - pn;
+  for (int32_t j0 = 0; j0 < nlow;) {
+    // the rows j0 .. j1-1 that fit the stage (a row of <= 64 entries always does)
+    const int32_t base = __shfl(excl, j0, 64);
+    const bool fits = lane >= j0 && lane < nlow && incl - base <= kFacStage;
+    const int32_t j1 = j0 + (int32_t)__popcll(__ballot(fits));
+    if (lane >= j0 && lane < j1) {
+      soff[w][lane - j0] = excl - base;
+      ssrc[w][lane - j0] = ka + pa;
+      spiv[w][lane - j0] = excl - base + (IC ? kd : 0);
+      if (lane == j1 - 1) soff[w][j1 - j0] = incl - base;
+    }
+    fac_wave_sync();
+    const int32_t total = soff[w][j1 - j0];
+    for (int32_t t = lane; t < total; t += 64) {
+      int32_t a = 0, b = j1 - j0;  // staged row holding entry t
+      while (b - a > 1) {
+        const int32_t mid = (a + b) >> 1;
+        if (soff[w][mid] <= t)
+          a = mid;
+        else
+          b = mid;
+      }
+      const int32_t src = ssrc[w][a] + (t - soff[w][a]);
+      col[t] = ci[src];
+      val[t] = f[src];
+    }
+    fac_wave_sync();
+    for (int32_t j = j0; j < j1; j++) {
+      const int32_t sa = soff[w][j - j0], sb = soff[w][j - j0 + 1], pv = spiv[w][j - j0];
+      const T pivot = val[pv];
+      if (!IC) {
+        if (lane == j) v = v / pivot;  // l_ij = a_ij / a_jj
+        const T lik = __shfl(v, j, 64);
+        if (lane > j && lane < len) {
+          const int32_t q = fac_find(col, pv + 1, sb, mycol);
+          if (q >= 0) {
+            const T prod = lik * val[q];
+            v = v - prod;
+          }
+        }
+      } else {
+        T prod = T(0);
+        int32_t hit = 0;
+        if (lane < j) {
+          const int32_t q = fac_find(col, sa, pv, mycol);
+          if (q >= 0) {
+            prod = v * val[q];
+            hit = 1;
+          }
+        }
+        sli[w][lane] = prod;
+        sm[w][lane] = hit;
+        fac_wave_sync();
+        if (lane == j) {
+          T sum = v;
+          for (int32_t p = 0; p < j; p++)
+            if (sm[w][p]) sum = sum - sli[w][p];
+          v = sum / pivot;
+        }
+        fac_wave_sync();
+      }
+    }
+    fac_wave_sync();  // the stage is read before the next group overwrites it
+    j0 = j1;
+  }
+  if (!IC) {
+    if (lane == nlow && (v == T(0) || v != v)) atomicMin(bad, i);
+    if (lane < len) f[e0 + lane] = v;
+  } else {
+    sli[w][lane] = v;
+    fac_wave_sync();
+    if (lane == nlow) {
+      T sum = v;
+      for (int32_t p = 0; p < nlow; p++) {
+        const T prod = sli[w][p] * sli[w][p];
+        sum = sum - prod;
+      }
+      if (!(sum > (T)0)) atomicMin(bad, i);
+      v = sqrt(sum);
+    }
+    if (lane <= nlow) f[e0 + lane] = v;
+  }
+}
+
 __global__ void k_fill_i32(int32_t *p, int32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -140,6 +298,8 @@ void ensure_struct(Csr *A, int kind) {
   }
   if (!S.built) {
     S.d_diag = upload_i32(diag);
+    S.max_len = 0;
+    for (int64_t i = 0; i < n; i++) S.max_len = std::max(S.max_len, rp[i + 1] - rp[i]);
     S.built = true;
   }
   if (kind == MGX_PRECOND_IC && !A->tri[kTriLt].built) {
@@ -164,10 +324,24 @@ template <class T> void factor_t(Csr *A, int kind, T *f) {
   int32_t *bad = (int32_t *)ws_get(c, 4);
   k_fill_i32<<<1, 1, 0, c->stream>>>(bad, INT32_MAX);
   const SorLevels &lev = tri_plan(A, kTriLower).lev;
-  if (kind == MGX_PRECOND_ILU)
+  if (sweep_launches_per_level(lev) && S.max_len <= 64) {
+    // chip-wide levels: a wave per row
+    for (int32_t l = 0; l < lev.nlev; l++) {
+      const int32_t cnt = lev.ptr[l + 1] - lev.ptr[l];
+      const unsigned g = (unsigned)((cnt + kFacWaves - 1) / kFacWaves);
+      if (kind == MGX_PRECOND_ILU)
+        k_factor_level_wave<T, false><<<g, 64 * kFacWaves, 0, c->stream>>>(
+            lev.d_rows + lev.ptr[l], cnt, A->rp, A->ci, S.d_diag, f, bad);
+      else
+        k_factor_level_wave<T, true><<<g, 64 * kFacWaves, 0, c->stream>>>(
+            lev.d_rows + lev.ptr[l], cnt, A->rp, A->ci, S.d_diag, f, bad);
+    }
+    MGX_HIP(hipGetLastError());
+  } else if (kind == MGX_PRECOND_ILU) {
     run_sweep(c, lev, IluFactorRow<T>{A->rp, A->ci, S.d_diag, f, bad});
-  else
+  } else {
     run_sweep(c, lev, IcFactorRow<T>{A->rp, A->ci, S.d_diag, f, bad});
+  }
   int32_t h = INT32_MAX;
   MGX_HIP(hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, c->stream));
   MGX_HIP(hipStreamSynchronize(c->stream));

@@ -142,6 +142,7 @@ struct TriOp {
 struct IluStruct {
   bool built = false;
   int32_t *d_diag = nullptr;
+  int32_t max_len = 0;  // longest row (entries)
 };
 
 struct SpmvPlan {
@@ -230,6 +231,7 @@ struct Csr {
 
 struct Ctx {
   int device = 0;
+  int32_t batch_iters_per_poll = 0;  // schedule of the last batched solve
   hipStream_t stream = nullptr;
   int num_cus = 0;
   size_t hbm_bytes = 0;

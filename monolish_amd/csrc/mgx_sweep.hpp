@@ -44,11 +44,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_level(const int32_t *__restric
 // fits the workgroup: one CU would carry all of their rows
 constexpr int64_t kSweepChipRows = 1 << 17, kSweepChipWidth = 256;
 
-template <class Op> void run_sweep(Ctx *c, const SorLevels &L, const Op &op) {
-  if (L.nlev == 0) return;
+// true when run_sweep launches once per level (else one workgroup sweeps)
+inline bool sweep_launches_per_level(const SorLevels &L) {
+  if (L.nlev == 0) return false;
   const int64_t n = L.ptr[L.nlev];
   const bool chip = n >= kSweepChipRows && n >= kSweepChipWidth * (int64_t)L.nlev;
-  if (L.max_width <= kSweepWgRows && !chip) {
+  return L.max_width > kSweepWgRows || chip;
+}
+
+template <class Op> void run_sweep(Ctx *c, const SorLevels &L, const Op &op) {
+  if (L.nlev == 0) return;
+  if (!sweep_launches_per_level(L)) {
     k_sweep_wg<Op><<<1, kSweepWgThreads, 0, c->stream>>>(L.d_ptr, L.nlev, L.d_rows, op);
   } else {
     for (int32_t l = 0; l < L.nlev; l++) {

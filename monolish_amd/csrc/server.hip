@@ -56,10 +56,43 @@ int batch_solve_t(Ctx *c, int method, int64_t nsys, mgx_csr *const *A, mgx_vec *
     slot_of[i] = s;
     sys_of[s] = i;
   }
-  const int every = p->check_every > 0 ? p->check_every : 8;
+  // iterations per poll: given, or tuned online while the batch runs (the
+  // schedule changes no bits): after one warm-up step, two steps under each
+  // candidate, then the one with the most system-iterations per second
+  int every = p->check_every > 0 ? p->check_every : 8;
+  const bool tune = p->check_every <= 0;
+  static constexpr int kCand[] = {4, 8, 16, 32};
+  constexpr int kNCand = (int)(sizeof kCand / sizeof kCand[0]);
+  int ci = tune ? -1 : kNCand, meas = 0;
+  double acc_t = 0, acc_w = 0, best_rate = -1;
+  int best_every = every;
   while (eng.running() > 0) {
-    eng.step(every);
-    for (int s : eng.poll()) {
+    const int running = eng.running();
+    const int e = ci >= 0 && ci < kNCand ? kCand[ci] : every;
+    const double ts = seconds_now();
+    eng.step(e);
+    const std::vector<int> stopped = eng.poll();  // waits for the step
+    const double dt = seconds_now() - ts;
+    if (ci < kNCand) {
+      if (ci < 0) {
+        ci = 0;  // the warm-up step carried the set-up launches
+      } else {
+        acc_t += dt;
+        acc_w += (double)running * e;
+        if (++meas == 2) {
+          const double rate = acc_w / std::max(acc_t, 1e-9);
+          if (rate > best_rate) {
+            best_rate = rate;
+            best_every = kCand[ci];
+          }
+          ci++;
+          meas = 0;
+          acc_t = acc_w = 0;
+          if (ci == kNCand) every = best_every;
+        }
+      }
+    }
+    for (int s : stopped) {
       const SolveState st = eng.state(s);
       mgx_solve_result &r = res[sys_of[s]];
       r.status = st.status;
@@ -70,6 +103,7 @@ int batch_solve_t(Ctx *c, int method, int64_t nsys, mgx_csr *const *A, mgx_vec *
     }
   }
   for (int64_t i = 0; i < nsys; i++) eng.release(slot_of[i]);
+  c->batch_iters_per_poll = every;
   return MGX_OK;
 }
 

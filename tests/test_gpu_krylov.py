@@ -641,8 +641,11 @@ def test_deep_plan_with_a_wide_level_bit_exact(m, ctx, oracle):
         res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, kind, tol, maxiter=3,
                            mode=m.MODE_REFERENCE, history=3)
         say("deep plan cg", kind, res.status, res.final_iter)
-        assert res.status == ref[0] == m.SOLVER_MAXITER and res.final_iter == 3
-        assert bits_equal(x, ref[1]) and bits_equal(res.rhistory, ref[4])
+        # SOR runs out its 3 iterations; IC(0) of this tridiagonal chain is
+        # exact, so its PCG converges first
+        assert res.status == ref[0] and res.final_iter == ref[2]
+        assert ref[0] == (m.SOLVER_MAXITER if kind == m.PRECOND_SOR else 0)
+        assert bits_equal(x, ref[1]) and bits_equal(res.rhistory[:ref[2]], ref[4][:ref[2]])
 
 
 def test_ic_ilu_errors(m, ctx, oracle):
