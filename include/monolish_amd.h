@@ -58,6 +58,13 @@ int mgx_finalize(mgx_ctx *ctx);
 int mgx_sync(mgx_ctx *ctx);
 /* the context's hipStream_t (as void*) for callers that record events */
 void *mgx_stream(mgx_ctx *ctx);
+/* Queue every later call of the context on the caller's hipStream_t (as
+ * void*; NULL: back to the library's own stream).  Waits for the work queued
+ * so far.  The stream stays the caller's: it must outlive its use and be
+ * replaced (NULL) before it is destroyed; not while a server runs on the
+ * context.  (monolish's GPU calls run on the default stream and synchronise;
+ * this is the per-call stream of the C-ABI, set per context.) */
+int mgx_set_stream(mgx_ctx *ctx, void *stream);
 /* name (NUL-terminated, truncated to len), compute units, HBM bytes */
 int mgx_device_info(mgx_ctx *ctx, char *name, size_t len, int *compute_units,
                     size_t *hbm_bytes);
@@ -208,6 +215,11 @@ int mgx_matvec_t(mgx_csr *A, mgx_vec *x, mgx_vec *y);
 int mgx_dot(mgx_vec *x, mgx_vec *y, double *out);
 /* blas::nrm2 (src/blas/vector/nrm2.hpp:6-67): sqrt(dot(x,x)) */
 int mgx_nrm2(mgx_vec *x, double *out);
+/* blas::dot / blas::nrm2 with the result left in device memory: *d_out (a
+ * double in device memory) receives the same bits mgx_dot / mgx_nrm2 return,
+ * queued on the context's stream without a host round trip. */
+int mgx_dot_device(mgx_vec *x, mgx_vec *y, double *d_out);
+int mgx_nrm2_device(mgx_vec *x, double *d_out);
 /* blas::axpy (axpy.hpp:7-61): y = a x + y */
 int mgx_axpy(double a, mgx_vec *x, mgx_vec *y);
 /* blas::xpay (xpay.hpp:7-66): y = x + a y */
@@ -249,6 +261,10 @@ int mgx_vreciprocal(mgx_vec *a, mgx_vec *y);
  * arithmetic, so FP64 results of REFERENCE and FUSED are bit-identical */
 #define MGX_MODE_REFERENCE 0 /* one kernel per BLAS call, scalars to host */
 #define MGX_MODE_FUSED 1     /* fused kernels, device scalars (CG, BiCGSTAB) */
+#define MGX_MODE_MIXED_IR 2  /* CG of an FP64 system by FP32-refined iterations
+                                (mgx_ir_cg_solve with inner_tol 1e-5, at most 100
+                                refinement steps; served requests: the server's
+                                ir_inner_tol / ir_max_outer) */
 
 typedef struct mgx_solve_params {
   double tol;          /* solver::set_tol (monolish_solver.hpp:86-215) */
@@ -372,6 +388,10 @@ typedef struct mgx_server_config {
   size_t hbm_budget;        /* bytes for resident requests (0: 3/4 of free) */
   int32_t max_active;       /* systems iterated together (0: 1024) */
   int32_t iters_per_poll;   /* fused iterations between status checks (0: 8) */
+  double ir_inner_tol;      /* FP32-refined requests: accuracy of each FP32
+                               correction (0: 1e-5) */
+  int32_t ir_max_outer;     /* FP32-refined requests: refinement steps (0: 100) */
+  int32_t pad_;
 } mgx_server_config;
 
 typedef struct mgx_server_stats {
@@ -391,7 +411,12 @@ typedef struct mgx_server_stats {
 int mgx_server_create(mgx_ctx *ctx, const mgx_server_config *cfg,
                       mgx_server **out);
 /* copies the FP64 system (CRS arrays, b, optional x0) and queues it;
- * method MGX_METHOD_CG or MGX_METHOD_BICGSTAB; *ticket identifies it */
+ * method MGX_METHOD_CG or MGX_METHOD_BICGSTAB; *ticket identifies it.
+ * p->mode MGX_MODE_MIXED_IR (CG only) serves it by FP32-refined iterations:
+ * FP64 residual, stop test and update per refinement step, the FP32
+ * correction iterated together with the other requests' corrections; each
+ * such request follows mgx_ir_cg_solve's arithmetic (result inner_iter /
+ * outer_iter as there). */
 int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz,
                       const int32_t *row_ptr, const int32_t *col_ind,
                       const double *val, const double *b, const double *x0,

@@ -27,7 +27,7 @@ REQ_QUEUED, REQ_ACTIVE, REQ_DONE = 0, 1, 2
 F64, F32 = 0, 1
 PRECOND_DEFAULT, PRECOND_NONE, PRECOND_JACOBI, PRECOND_SOR = 0, 1, 2, 3
 PRECOND_IC, PRECOND_ILU = 4, 5
-MODE_REFERENCE, MODE_FUSED = 0, 1
+MODE_REFERENCE, MODE_FUSED, MODE_MIXED_IR = 0, 1, 2
 
 
 class MgxError(RuntimeError):
@@ -107,6 +107,9 @@ _sig("mgx_matvec_ab", [_D, _P, _P, _D, _P])
 _sig("mgx_matvec_t", [_P, _P, _P])
 _sig("mgx_dot", [_P, _P, C.POINTER(_D)])
 _sig("mgx_nrm2", [_P, C.POINTER(_D)])
+_sig("mgx_dot_device", [_P, _P, _P])
+_sig("mgx_nrm2_device", [_P, _P])
+_sig("mgx_set_stream", [_P, _P])
 _sig("mgx_axpy", [_D, _P, _P])
 _sig("mgx_xpay", [_D, _P, _P])
 _sig("mgx_scal", [_D, _P])
@@ -139,7 +142,8 @@ _sig("mgx_batch_ir_cg_solve", [_P, _I64, C.POINTER(_P), C.POINTER(_P), C.POINTER
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", _I32), ("iters_per_poll", _I32)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", _I32), ("iters_per_poll", _I32),
+                ("ir_inner_tol", _D), ("ir_max_outer", _I32), ("pad_", _I32)]
 
 
 class ServerStats(C.Structure):
@@ -434,6 +438,26 @@ def dot(x: Vector, y: Vector) -> float:
     return out.value
 
 
+def dot_device(x: Vector, y: Vector, out: Vector, index: int = 0):
+    """<x, y> into out[index] (an FP64 vector), queued without a host round trip."""
+    if out.dtype != np.float64 or not 0 <= index < out.size:
+        raise ValueError("out must be an FP64 vector holding index")
+    check(_lib.mgx_dot_device(x.handle, y.handle, out.device_ptr + 8 * index))
+
+
+def nrm2_device(x: Vector, out: Vector, index: int = 0):
+    """||x|| into out[index] (an FP64 vector), queued without a host round trip."""
+    if out.dtype != np.float64 or not 0 <= index < out.size:
+        raise ValueError("out must be an FP64 vector holding index")
+    check(_lib.mgx_nrm2_device(x.handle, out.device_ptr + 8 * index))
+
+
+def set_stream(ctx: Context, stream: int | None):
+    """Queue the context's later calls on a caller's hipStream_t (None: the
+    library's own stream)."""
+    check(_lib.mgx_set_stream(ctx.handle, C.c_void_p(stream) if stream else None))
+
+
 def nrm2(x: Vector) -> float:
     out = _D(0)
     check(_lib.mgx_nrm2(x.handle, C.byref(out)))
@@ -713,13 +737,16 @@ def batch_ir_cg(ctx: Context, As, xs, bs, tol=1e-10, tol_is_relative=True, inner
 class Server:
     """Continuous-batching solve server on one context (see mgx_server_*)."""
 
-    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, iters_per_poll: int = 0):
-        cfg = ServerConfig(hbm_budget, max_active, iters_per_poll)
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, iters_per_poll: int = 0,
+                 ir_inner_tol: float = 0.0, ir_max_outer: int = 0):
+        cfg = ServerConfig(hbm_budget, max_active, iters_per_poll, ir_inner_tol, ir_max_outer, 0)
         self._h = _P()
         check(_lib.mgx_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)))
 
     def submit(self, method, rp, ci, val, b, x0=None, tol=1e-10, tol_is_relative=True, maxiter=0,
-               precond=PRECOND_JACOBI) -> int:
+               precond=PRECOND_JACOBI, mode=MODE_FUSED) -> int:
+        """Queue a solve of a copied FP64 system; mode MODE_MIXED_IR serves a CG
+        request by FP32-refined iterations."""
         rp = np.ascontiguousarray(rp, np.int32)
         ci = np.ascontiguousarray(ci, np.int32)
         val = np.ascontiguousarray(val, np.float64)
@@ -728,17 +755,17 @@ class Server:
         if x0 is not None:
             x0 = np.ascontiguousarray(x0, np.float64)
             x0p = _ptr(x0)
-        p, _ = _params(tol, maxiter, 0, precond, MODE_FUSED, tol_is_relative, 0, 0)
+        p, _ = _params(tol, maxiter, 0, precond, mode, tol_is_relative, 0, 0)
         t = _I64(0)
         check(_lib.mgx_server_submit(self._h, method, len(b), len(val), _ptr(rp), _ptr(ci), _ptr(val),
                                      _ptr(b), x0p, C.byref(p), C.byref(t)))
         return t.value
 
     def submit_resident(self, method, A, b, x=None, tol=1e-10, tol_is_relative=True, maxiter=0,
-                        precond=PRECOND_JACOBI) -> int:
+                        precond=PRECOND_JACOBI, mode=MODE_FUSED) -> int:
         """Queue a solve of a device-resident system (A, b stay the caller's and may be
         shared by many requests); x, if given, is the start and receives the solution."""
-        p, _ = _params(tol, maxiter, 0, precond, MODE_FUSED, tol_is_relative, 0, 0)
+        p, _ = _params(tol, maxiter, 0, precond, mode, tol_is_relative, 0, 0)
         t = _I64(0)
         check(_lib.mgx_server_submit_resident(self._h, method, A.handle, b.handle,
                                               x.handle if x is not None else None, C.byref(p),
@@ -752,7 +779,8 @@ class Server:
         if rc == ERR_NOT_FOUND:
             return None
         check(rc)
-        return SolveResult(r.status, r.final_iter, r.final_resid, r.seconds), x
+        return SolveResult(r.status, r.final_iter, r.final_resid, r.seconds, r.inner_iter,
+                           r.outer_iter), x
 
     def cancel(self, ticket: int):
         """End a queued or running request (status ERR_CANCELLED); a request

@@ -16,15 +16,6 @@ namespace mgx {
 
 namespace {
 
-struct IrSys {
-  const double *r;  // FP64 residual (FP64 engine's work vector)
-  float *r32, *d32;
-  double *x;
-  double scale;  // 1 / ||r||
-  double rn;     // ||r||
-  int64_t n;
-};
-
 // r32 = (float)(r / ||r||) and d32 = 0 over one 512-row tile of a system
 __global__ __launch_bounds__(kBlock) void k_ir_scale(const IrSys *sys, const TileRef *tiles,
                                                      int64_t nlaunch) {
@@ -52,6 +43,18 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const IrSys *sys, const Ti
 }
 
 }  // namespace
+
+void launch_ir_scale(Ctx *c, const IrSys *sys, const TileRef *tiles, int64_t ntiles) {
+  if (ntiles <= 0) return;
+  k_ir_scale<<<(unsigned)ntiles, kBlock, 0, c->stream>>>(sys, tiles, ntiles);
+  MGX_HIP(hipGetLastError());
+}
+
+void launch_ir_update(Ctx *c, const IrSys *sys, const TileRef *tiles, int64_t ntiles) {
+  if (ntiles <= 0) return;
+  k_ir_update<<<(unsigned)ntiles, kBlock, 0, c->stream>>>(sys, tiles, ntiles);
+  MGX_HIP(hipGetLastError());
+}
 
 void batch_ir_cg(Ctx *c, int64_t nsys, Csr *const *A, double *const *x, const double *const *b,
                  const mgx_solve_params *p, double inner_tol, int64_t max_outer,
@@ -180,9 +183,7 @@ void batch_ir_cg(Ctx *c, int64_t nsys, Csr *const *A, double *const *x, const do
         }
         MGX_HIP(hipMemcpy(d_ir, h_ir.data(), sizeof(IrSys) * nsys, hipMemcpyHostToDevice));
         MGX_HIP(hipMemcpy(d_tl, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice));
-        k_ir_scale<<<(unsigned)tl.size(), kBlock, 0, c->stream>>>(d_ir, d_tl,
-                                                                  (int64_t)tl.size());
-        MGX_HIP(hipGetLastError());
+        launch_ir_scale(c, d_ir, d_tl, (int64_t)tl.size());
         // all corrections in one batched FP32 solve
         {
           Engine<float> e32(c, &pool, (int)go.size());
@@ -205,9 +206,7 @@ void batch_ir_cg(Ctx *c, int64_t nsys, Csr *const *A, double *const *x, const do
             e32.release(s32[k]);
           }
         }
-        k_ir_update<<<(unsigned)tl.size(), kBlock, 0, c->stream>>>(d_ir, d_tl,
-                                                                   (int64_t)tl.size());
-        MGX_HIP(hipGetLastError());
+        launch_ir_update(c, d_ir, d_tl, (int64_t)tl.size());
         for (int64_t i : go) outer[i]++;
         for (size_t k = 0; k < act.size(); k++) e64.release(s64[k]);
       }

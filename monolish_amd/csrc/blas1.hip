@@ -73,7 +73,7 @@ __global__ void k_convert(int64_t n, const S *x, D *y) {
 // canonical dot: one block per 512-element tile (lane t holds elements 2t
 // and 2t+1), the tile partial from the block tree; the blocks then meet in
 // arrive_reduce, which adds the tile partials in the canonical lane order
-template <class T, bool VEC>
+template <class T, bool VEC, bool SQ = false>
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *x, const T *y, T *P,
                                                 T *S, uint32_t *tick, double *out) {
   __shared__ T sh[8];
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *x, const T *
   const T s = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(P, tile, s);
   if (arrive_reduce<T, 1, 1>(tick, P, 0, S, ntiles_of(n), tile, tp, sh, res, &flag))
-    if (threadIdx.x == 0) *out = (double)res[0];
+    if (threadIdx.x == 0) *out = SQ ? (double)sqrt(res[0]) : (double)res[0];
 }
 
 // m canonical dots <x_r, x_r> of the rows of a row-major block (row r at
@@ -227,43 +227,51 @@ void launch_convert(Ctx *c, mgx_dtype from, mgx_dtype to, int64_t n,
 
 // the canonical dot into *d_out with the tile partials and lane sums in
 // `partials` ((ntiles + 2 * kRedGroups) values) and zeroed counters d_ticket
+template <class T, bool SQ>
+static void dot_kernel(Ctx *c, int64_t n, const void *x, const void *y, void *partials,
+                       double *d_out, uint32_t *d_ticket) {
+  const int64_t nt = ntiles_of(n);
+  T *P = tp<T>(partials), *S = P + nt;
+  if (aligned16(x) && aligned16(y))
+    k_dot<T, true, SQ><<<(unsigned)nt, kBlock, 0, c->stream>>>(n, tp<T>(x), tp<T>(y), P, S,
+                                                               d_ticket, d_out);
+  else
+    k_dot<T, false, SQ><<<(unsigned)nt, kBlock, 0, c->stream>>>(n, tp<T>(x), tp<T>(y), P, S,
+                                                                d_ticket, d_out);
+}
+
+// the canonical dot (sq: its square root, in the vectors' precision) into
+// *d_out with the tile partials and lane sums in `partials`
+// ((ntiles + 2 * kRedGroups) values) and zeroed counters d_ticket
 static void launch_dot_into(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
-                            void *partials, double *d_out, uint32_t *d_ticket) {
+                            void *partials, double *d_out, uint32_t *d_ticket,
+                            bool sq = false) {
   if (n <= 0) {
     launch_fill(c, MGX_F64, d_out, 1, 0.0);
     return;
   }
-  const int64_t nt = ntiles_of(n);
-  const bool vec = aligned16(x) && aligned16(y);
-  const unsigned g = (unsigned)nt;
   if (dt == MGX_F64) {
-    double *P = tp<double>(partials), *S = P + nt;
-    if (vec)
-      k_dot<double, true><<<g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y), P, S,
-                                                       d_ticket, d_out);
+    if (sq)
+      dot_kernel<double, true>(c, n, x, y, partials, d_out, d_ticket);
     else
-      k_dot<double, false><<<g, kBlock, 0, c->stream>>>(n, tp<double>(x), tp<double>(y), P, S,
-                                                        d_ticket, d_out);
+      dot_kernel<double, false>(c, n, x, y, partials, d_out, d_ticket);
   } else {
-    float *P = tp<float>(partials), *S = P + nt;
-    if (vec)
-      k_dot<float, true><<<g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y), P, S,
-                                                      d_ticket, d_out);
+    if (sq)
+      dot_kernel<float, true>(c, n, x, y, partials, d_out, d_ticket);
     else
-      k_dot<float, false><<<g, kBlock, 0, c->stream>>>(n, tp<float>(x), tp<float>(y), P, S,
-                                                       d_ticket, d_out);
+      dot_kernel<float, false>(c, n, x, y, partials, d_out, d_ticket);
   }
   MGX_HIP(hipGetLastError());
 }
 
 void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
-                double *d_out, uint32_t *d_ticket) {
+                double *d_out, uint32_t *d_ticket, bool sq) {
   if (n > 0) {
     // tile partials, then the 256 lane sums
     const size_t es = dt == MGX_F64 ? 8 : 4;
     ensure_partials(c, (size_t)(ntiles_of(n) + 2 * kRedGroups) * es);
   }
-  launch_dot_into(c, dt, n, x, y, c->d_partials, d_out, d_ticket);
+  launch_dot_into(c, dt, n, x, y, c->d_partials, d_out, d_ticket, sq);
 }
 
 void launch_mnrm2sq(Ctx *c, mgx_dtype dt, int64_t n, int m, const void *X, int64_t ld,

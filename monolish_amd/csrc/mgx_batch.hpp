@@ -148,6 +148,20 @@ struct SolveCfg {
   int gather_once = 0;
 };
 
+// FP32-refined solves: one system's step of iterative refinement
+struct IrSys {
+  const double *r;  // FP64 residual (an FP64 engine's work vector)
+  float *r32, *d32;
+  double *x;
+  double scale;  // 1 / ||r||
+  double rn;     // ||r||
+  int64_t n;
+};
+// over tiles (slot = index into sys): r32 = (float)(r / ||r||), d32 = 0
+void launch_ir_scale(Ctx *c, const IrSys *sys, const TileRef *tiles, int64_t ntiles);
+// over tiles: x += ||r|| d32
+void launch_ir_update(Ctx *c, const IrSys *sys, const TileRef *tiles, int64_t ntiles);
+
 // A set of systems iterated together.  add() uploads nothing: the matrix and
 // x/b are device resident; it allocates the work vectors from the pool and
 // records the system.  The set-up of all systems added since the last launch

@@ -232,7 +232,8 @@ struct Csr {
 struct Ctx {
   int device = 0;
   int32_t batch_iters_per_poll = 0;  // schedule of the last batched solve
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where every call queues its work
+  hipStream_t own_stream = nullptr;  // the library's stream (stream unless replaced)
   int num_cus = 0;
   size_t hbm_bytes = 0;
   char name[256] = {0};
@@ -297,9 +298,10 @@ void launch_reciprocal(Ctx *c, mgx_dtype dt, int64_t n, const void *a,
                        void *y);
 void launch_convert(Ctx *c, mgx_dtype from, mgx_dtype to, int64_t n,
                     const void *x, void *y);
-// dot into a device double slot (result converted to double exactly)
+// dot into a device double slot (result converted to double exactly); sq:
+// its square root in the vectors' precision (nrm2)
 void launch_dot(Ctx *c, mgx_dtype dt, int64_t n, const void *x, const void *y,
-                double *d_out, uint32_t *d_ticket);
+                double *d_out, uint32_t *d_ticket, bool sq = false);
 double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y);
 // d_out[r] = <x_r, x_r> for the m rows x_r = X + r ld, each the canonical

@@ -198,6 +198,7 @@ int mgx_init(int device, mgx_ctx **out) {
     c->hbm_bytes = prop.totalGlobalMem;
     snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
     MGX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = c->stream;
     c->d_state = (SolveState *)dev_alloc(sizeof(SolveState));
     MGX_HIP(hipMemset(c->d_state, 0, sizeof(SolveState)));
     c->d_tick = (uint32_t *)dev_alloc(kTickWords * sizeof(uint32_t));
@@ -233,7 +234,7 @@ int mgx_finalize(mgx_ctx *ctx) {
     dev_free(ctx->d_tick);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->ev) (void)hipEventDestroy(ctx->ev);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return MGX_OK;
   });
@@ -248,6 +249,17 @@ int mgx_sync(mgx_ctx *ctx) {
 }
 
 void *mgx_stream(mgx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int mgx_set_stream(mgx_ctx *ctx, void *stream) {
+  return guard([&] {
+    MGX_REQUIRE(ctx, "mgx_set_stream: NULL ctx");
+    activate(ctx);
+    // work queued so far finishes before later work starts on the new stream
+    MGX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    return MGX_OK;
+  });
+}
 
 int mgx_device_info(mgx_ctx *ctx, char *name, size_t len, int *cus,
                     size_t *hbm) {
@@ -647,6 +659,25 @@ int mgx_dot(mgx_vec *x, mgx_vec *y, double *out) {
     same_kind(x, y, "mgx_dot");
     activate(x->ctx);
     *out = dot_host(x->ctx, x->dtype, x->n, x->d, y->d);
+    return MGX_OK;
+  });
+}
+
+int mgx_dot_device(mgx_vec *x, mgx_vec *y, double *d_out) {
+  return guard([&] {
+    MGX_REQUIRE(x && y && d_out, "mgx_dot_device: NULL argument");
+    same_kind(x, y, "mgx_dot_device");
+    activate(x->ctx);
+    launch_dot(x->ctx, x->dtype, x->n, x->d, y->d, d_out, x->ctx->d_tick);
+    return MGX_OK;
+  });
+}
+
+int mgx_nrm2_device(mgx_vec *x, double *d_out) {
+  return guard([&] {
+    MGX_REQUIRE(x && d_out, "mgx_nrm2_device: NULL argument");
+    activate(x->ctx);
+    launch_dot(x->ctx, x->dtype, x->n, x->d, x->d, d_out, x->ctx->d_tick, true);
     return MGX_OK;
   });
 }

@@ -124,6 +124,15 @@ struct Request {
   // submitted by handle: A, b (and x when given) belong to the caller
   bool by_handle = false;
   bool own_x = true;
+  // FP32-refined request (mgx_ir_cg_solve's arithmetic): phase 0 waits for
+  // its FP64 residual pass, phase 1 runs its FP32 correction (slot32)
+  bool ir = false;
+  int ir_phase = 0;
+  bool ir_first = true;
+  double thr = 0, rn = 0;
+  int64_t outer = 0, inner = 0;
+  float *r32 = nullptr, *d32 = nullptr;
+  int slot32 = -1;
   // control: deadline (absolute seconds_now() time, 0: none) and cancel
   double deadline = 0;
   bool cancel = false;
@@ -141,6 +150,16 @@ struct Server {
   mgx_server_config cfg;
   std::unique_ptr<DevicePool> pool;
   std::unique_ptr<Engine<double>> eng;
+  // FP32-refined requests: FP64 residual passes and FP32 corrections
+  std::unique_ptr<Engine<double>> eres;
+  std::unique_ptr<Engine<float>> e32;
+  std::vector<std::shared_ptr<Request>> by_slot32;
+  std::vector<std::shared_ptr<Request>> ir_wait;  // phase 0, admitted
+  double ir_inner_tol = 1e-5;
+  int64_t ir_max_outer = 100;
+  IrSys *d_ir = nullptr;
+  TileRef *d_irtl = nullptr;
+  int64_t irtl_cap = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> queue;
@@ -157,7 +176,9 @@ struct Server {
   size_t request_bytes(const Request &r) const {
     Csr probe;
     probe.rows = probe.cols = r.n;
-    const size_t work = eng->bytes_needed(&probe, r.method);
+    size_t work = eng->bytes_needed(&probe, r.method);
+    if (r.ir)  // FP32 correction engine, r32 / d32, the FP32 copy of the values
+      work += work / 2 + DevicePool::round((size_t)r.n * 4) * 2 + (size_t)(r.nnz + 4) * 4;
     if (r.by_handle) return work + (r.own_x ? (size_t)r.n * 8 : 0);
     const size_t csr = (size_t)(r.n + 1) * 4 + (size_t)(r.nnz + 4) * 12 +
                        (size_t)(r.n / 256 + 2) * 4 * 2;
@@ -193,6 +214,12 @@ struct Server {
   }
 
   void free_resident(Request &r) {
+    if (r.r32 || r.d32) {
+      (void)hipStreamSynchronize(ctx->stream);
+      pool->put(r.r32);
+      pool->put(r.d32);
+      r.r32 = r.d32 = nullptr;
+    }
     if (r.by_handle) {
       if (r.own_x) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -250,12 +277,33 @@ struct Server {
   // their slot; x keeps the last iterate (by-handle requests) or is copied
   // out (requests submitted with their arrays)
   void drop_active(double now) {
-    std::vector<std::shared_ptr<Request>> out;
+    std::vector<std::shared_ptr<Request>> out, out_ir;
     {
       std::lock_guard<std::mutex> lk(mu);
+      auto ends = [&](const std::shared_ptr<Request> &r) {
+        return r && !r->done && (r->cancel || (r->deadline > 0 && now >= r->deadline));
+      };
       for (auto &r : by_slot)
-        if (r && !r->done && (r->cancel || (r->deadline > 0 && now >= r->deadline)))
-          out.push_back(r);
+        if (ends(r)) out.push_back(r);
+      for (auto &r : by_slot32)
+        if (ends(r)) out_ir.push_back(r);
+      for (auto &r : ir_wait)
+        if (ends(r)) out_ir.push_back(r);
+    }
+    // FP32-refined requests: x holds the last refined iterate
+    for (auto &r : out_ir) {
+      if (r->slot32 >= 0) {
+        e32->release(r->slot32);
+        by_slot32[r->slot32].reset();
+        r->slot32 = -1;
+      } else {
+        ir_wait.erase(std::remove(ir_wait.begin(), ir_wait.end(), r), ir_wait.end());
+      }
+      r->res.final_iter = r->inner;
+      r->res.final_resid = r->rn;
+      r->res.inner_iter = r->inner;
+      r->res.outer_iter = r->outer;
+      retire(r, r->cancel ? MGX_ERR_CANCELLED : MGX_ERR_TIMEOUT);
     }
     if (!out.empty()) eng->fetch_states();  // waits for the stream: x is final
     for (auto &r : out) {
@@ -264,16 +312,134 @@ struct Server {
       const SolveState st = eng->state(r->slot);
       r->res.final_iter = st.iter;
       r->res.final_resid = st.resid;
-      if (!r->by_handle) {
-        r->x_out.resize(r->n);
-        MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
-      }
       by_slot[r->slot].reset();
-      free_resident(*r);
-      std::lock_guard<std::mutex> lk(mu);
-      resident -= r->bytes;
-      finish_locked(r, code);
+      retire(r, code);
     }
+  }
+
+  int ir_active = 0;  // admitted FP32-refined requests not yet finished
+  int busy() const { return eng->active() + ir_active; }
+
+  // uploads an IrSys list and its tiles (waits for the stream first: the
+  // previous list may still be read)
+  int64_t ir_upload(const std::vector<IrSys> &hs, const std::vector<TileRef> &tl) {
+    MGX_HIP(hipStreamSynchronize(ctx->stream));
+    if (!d_ir) d_ir = (IrSys *)dev_alloc(sizeof(IrSys) * (size_t)cfg.max_active);
+    if ((int64_t)tl.size() > irtl_cap) {
+      dev_free(d_irtl);
+      d_irtl = nullptr;
+      irtl_cap = std::max<int64_t>((int64_t)tl.size(), 2 * irtl_cap);
+      d_irtl = (TileRef *)dev_alloc(sizeof(TileRef) * irtl_cap);
+    }
+    if (!hs.empty()) MGX_HIP(hipMemcpy(d_ir, hs.data(), sizeof(IrSys) * hs.size(),
+                                       hipMemcpyHostToDevice));
+    if (!tl.empty()) MGX_HIP(hipMemcpy(d_irtl, tl.data(), sizeof(TileRef) * tl.size(),
+                                       hipMemcpyHostToDevice));
+    return (int64_t)tl.size();
+  }
+
+  // a finished request leaves: x copied out (requests with their arrays),
+  // resident memory returned, outcome recorded
+  void retire(const std::shared_ptr<Request> &r, int status) {
+    if (!r->by_handle) {
+      MGX_HIP(hipStreamSynchronize(ctx->stream));
+      r->x_out.resize(r->n);
+      MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
+    }
+    free_resident(*r);
+    std::lock_guard<std::mutex> lk(mu);
+    resident -= r->bytes;
+    if (r->ir) ir_active--;
+    finish_locked(r, status);
+  }
+
+  // FP32-refined requests waiting for their residual: one FP64 set-up pass
+  // for all of them gives r = b - A x and ||r|| (mgx_ir_cg_solve's steps);
+  // the stop tests; then r32 = r / ||r||, d32 = 0 and the corrections join
+  // the FP32 engine
+  void ir_residuals() {
+    if (ir_wait.empty()) return;
+    std::vector<int> rs(ir_wait.size());
+    for (size_t k = 0; k < ir_wait.size(); k++) {
+      const auto &r = ir_wait[k];
+      SolveCfg rc;
+      rc.method = kMethodCG;
+      rc.precond = MGX_PRECOND_NONE;
+      rc.tol = r->cfg.tol;
+      rc.tol_relative = r->cfg.tol_relative;
+      rs[k] = eres->add(r->A, r->A->val, r->x, r->bd, rc);
+      MGX_REQUIRE(rs[k] >= 0, "server: no device memory for a residual pass");
+    }
+    eres->prepare();
+    eres->fetch_states();
+    std::vector<IrSys> hs;
+    std::vector<TileRef> tl;
+    std::vector<std::shared_ptr<Request>> go;
+    std::vector<std::pair<std::shared_ptr<Request>, int>> done;
+    for (size_t k = 0; k < ir_wait.size(); k++) {
+      const auto &r = ir_wait[k];
+      const SolveState st = eres->state(rs[k]);
+      if (r->ir_first) r->thr = st.tol;
+      r->ir_first = false;
+      r->rn = std::sqrt(st.zr);
+      int status = kStatRunning;
+      if (r->rn < r->thr)
+        status = MGX_OK;
+      else if (std::isnan(r->rn))
+        status = MGX_SOLVER_RESIDUAL_NAN;
+      else if (r->outer >= ir_max_outer || r->inner >= r->cfg.maxiter)
+        status = MGX_SOLVER_MAXITER;
+      if (status != kStatRunning) {
+        r->res.final_iter = r->inner;
+        r->res.final_resid = r->rn;
+        r->res.inner_iter = r->inner;
+        r->res.outer_iter = r->outer;
+        done.emplace_back(r, status);
+        continue;
+      }
+      const int32_t idx = (int32_t)hs.size();
+      hs.push_back(IrSys{eres->desc(rs[k]).r, r->r32, r->d32, r->x, 1.0 / r->rn, r->rn, r->n});
+      for (int64_t t = 0; t < ntiles_of(r->n); t++) tl.push_back(TileRef{idx, (int32_t)t});
+      go.push_back(r);
+    }
+    launch_ir_scale(ctx, d_ir, d_irtl, ir_upload(hs, tl));
+    for (const auto &r : go) {
+      SolveCfg ic;
+      ic.method = kMethodCG;
+      ic.precond = r->cfg.precond;
+      ic.tol = ir_inner_tol;  // absolute, on the unit-norm scaled residual
+      ic.tol_relative = 0;
+      ic.maxiter = r->cfg.maxiter - r->inner;
+      ic.gather_once = 1;
+      r->slot32 = e32->add(r->A, r->A->val32, r->d32, r->r32, ic);
+      MGX_REQUIRE(r->slot32 >= 0, "server: no device memory for an FP32 correction");
+      by_slot32[r->slot32] = r;
+      r->ir_phase = 1;
+    }
+    for (int s : rs) eres->release(s);  // after the scale read their residuals
+    ir_wait.clear();
+    for (auto &d : done) retire(d.first, d.second);
+  }
+
+  // finished FP32 corrections: x += ||r|| d in FP64, then another residual
+  void ir_corrections_done(const std::vector<int> &stopped) {
+    if (stopped.empty()) return;
+    std::vector<IrSys> hs;
+    std::vector<TileRef> tl;
+    for (int s : stopped) {
+      auto r = by_slot32[s];
+      r->inner += e32->state(s).iter;
+      const int32_t idx = (int32_t)hs.size();
+      hs.push_back(IrSys{nullptr, r->r32, r->d32, r->x, 0.0, r->rn, r->n});
+      for (int64_t t = 0; t < ntiles_of(r->n); t++) tl.push_back(TileRef{idx, (int32_t)t});
+      e32->release(s);
+      by_slot32[s].reset();
+      r->slot32 = -1;
+      r->outer++;
+      r->ir_phase = 0;
+      ir_wait.push_back(r);
+    }
+    launch_ir_update(ctx, d_ir, d_irtl, ir_upload(hs, tl));
   }
 
   void loop() {
@@ -284,14 +450,12 @@ struct Server {
       {
         std::unique_lock<std::mutex> lk(mu);
         drop_queued_locked(seconds_now());
-        while (!queue.empty() && eng->active() + (int)admit.size() < cfg.max_active) {
+        while (!queue.empty() && busy() + (int)admit.size() < cfg.max_active) {
           auto r = queue.front();
           if (r->bytes > budget) {  // can never fit
             queue.pop_front();
-            lk.unlock();
             r->res.status = MGX_ERR_BUDGET;
-            finish(r, MGX_ERR_BUDGET);
-            lk.lock();
+            finish_locked(r, MGX_ERR_BUDGET);
             continue;
           }
           if (resident + r->bytes > budget) break;
@@ -300,7 +464,7 @@ struct Server {
           peak = std::max(peak, resident);
           admit.push_back(r);
         }
-        if (admit.empty() && eng->active() == 0) {
+        if (admit.empty() && busy() == 0) {
           if (stop && queue.empty()) return;
           cv_work.wait_for(lk, std::chrono::milliseconds(2));
           continue;
@@ -309,14 +473,22 @@ struct Server {
       for (auto &r : admit) {
         try {
           upload(*r);
-          r->slot = eng->add(r->A, r->A->val, r->x, r->bd, r->cfg);
-          if (r->slot < 0) throw Error{MGX_ERR_BUDGET};
-          r->t_admit = seconds_now();
-          {
-            std::lock_guard<std::mutex> lk(mu);
-            r->active = true;
+          if (r->ir) {
+            ensure_val32(r->A);
+            r->r32 = (float *)pool->get((size_t)r->n * 4);
+            r->d32 = (float *)pool->get((size_t)r->n * 4);
+            if (!r->r32 || !r->d32) throw Error{MGX_ERR_BUDGET};
+            r->ir_phase = 0;
+            ir_wait.push_back(r);
+          } else {
+            r->slot = eng->add(r->A, r->A->val, r->x, r->bd, r->cfg);
+            if (r->slot < 0) throw Error{MGX_ERR_BUDGET};
+            by_slot[r->slot] = r;
           }
-          by_slot[r->slot] = r;
+          r->t_admit = seconds_now();
+          std::lock_guard<std::mutex> lk(mu);
+          r->active = true;
+          if (r->ir) ir_active++;
         } catch (const Error &e) {
           free_resident(*r);
           {
@@ -326,28 +498,24 @@ struct Server {
           finish(r, e.code);
         }
       }
-      if (eng->active() == 0) continue;
+      if (busy() == 0) continue;
       drop_active(seconds_now());
-      if (eng->active() == 0) continue;
-      eng->step(cfg.iters_per_poll);
-      for (int s : eng->poll()) {
-        auto r = by_slot[s];
-        const SolveState st = eng->state(s);
-        r->res.final_iter = st.iter;
-        r->res.final_resid = st.resid;
-        if (!r->by_handle) {
-          r->x_out.resize(r->n);
-          MGX_HIP(hipMemcpy(r->x_out.data(), r->x, (size_t)r->n * 8, hipMemcpyDeviceToHost));
+      ir_residuals();
+      const bool run64 = eng->active() > 0, run32 = e32->active() > 0;
+      if (run64) eng->step(cfg.iters_per_poll);
+      if (run32) e32->step(cfg.iters_per_poll);
+      if (run64) {
+        for (int s : eng->poll()) {
+          auto r = by_slot[s];
+          const SolveState st = eng->state(s);
+          r->res.final_iter = st.iter;
+          r->res.final_resid = st.resid;
+          eng->release(s);
+          by_slot[s].reset();
+          retire(r, st.status);
         }
-        eng->release(s);
-        by_slot[s].reset();
-        free_resident(*r);
-        {
-          std::lock_guard<std::mutex> lk(mu);
-          resident -= r->bytes;
-        }
-        finish(r, st.status);
       }
+      if (run32) ir_corrections_done(e32->poll());
     }
   }
 
@@ -419,12 +587,19 @@ int mgx_server_create(mgx_ctx *ctx, const mgx_server_config *cfg, mgx_server **o
     if (cfg) s->cfg = *cfg;
     if (s->cfg.max_active <= 0) s->cfg.max_active = 1024;
     if (s->cfg.iters_per_poll <= 0) s->cfg.iters_per_poll = 8;
+    MGX_REQUIRE(s->cfg.ir_inner_tol >= 0 && s->cfg.ir_inner_tol < 1,
+                "mgx_server_create: ir_inner_tol in [0, 1)");
+    if (s->cfg.ir_inner_tol > 0) s->ir_inner_tol = s->cfg.ir_inner_tol;
+    if (s->cfg.ir_max_outer > 0) s->ir_max_outer = s->cfg.ir_max_outer;
     size_t freeb = 0, total = 0;
     MGX_HIP(hipMemGetInfo(&freeb, &total));
     s->budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : freeb / 4 * 3;
     s->pool.reset(new DevicePool(s->budget));
     s->eng.reset(new Engine<double>(ctx, s->pool.get(), s->cfg.max_active));
+    s->eres.reset(new Engine<double>(ctx, s->pool.get(), s->cfg.max_active));
+    s->e32.reset(new Engine<float>(ctx, s->pool.get(), s->cfg.max_active));
     s->by_slot.resize(s->cfg.max_active);
+    s->by_slot32.resize(s->cfg.max_active);
     s->worker = std::thread([s] { s->run(); });
     *out = s;
     return MGX_OK;
@@ -452,6 +627,9 @@ int mgx_server_submit(mgx_server *s, int method, int64_t n, int64_t nnz, const i
     r->b.assign(b, b + n);
     if (x0) r->x0.assign(x0, x0 + n);
     r->cfg = cfg_of(p, method);
+    r->ir = p->mode == MGX_MODE_MIXED_IR;
+    MGX_REQUIRE(!r->ir || method == MGX_METHOD_CG,
+                "mgx_server_submit: FP32-refined requests are CG requests");
     r->bytes = s->request_bytes(*r);
     std::lock_guard<std::mutex> lk(s->mu);
     MGX_REQUIRE(!s->stop, "mgx_server_submit: server stopped (%s)", s->fatal.c_str());
@@ -491,6 +669,9 @@ int mgx_server_submit_resident(mgx_server *s, int method, mgx_csr *A, mgx_vec *b
     r->own_x = x == nullptr;
     r->x = x ? static_cast<double *>(x->d) : nullptr;
     r->cfg = cfg_of(p, method);
+    r->ir = p->mode == MGX_MODE_MIXED_IR;
+    MGX_REQUIRE(!r->ir || method == MGX_METHOD_CG,
+                "mgx_server_submit_resident: FP32-refined requests are CG requests");
     r->bytes = s->request_bytes(*r);
     std::lock_guard<std::mutex> lk(s->mu);
     MGX_REQUIRE(!s->stop, "mgx_server_submit_resident: server stopped (%s)", s->fatal.c_str());
@@ -618,7 +799,14 @@ int mgx_server_destroy(mgx_server *s) {
     if (s->worker.joinable()) s->worker.join();
     for (auto &r : s->by_slot)
       if (r) s->free_resident(*r);
+    for (auto &r : s->by_slot32)
+      if (r) s->free_resident(*r);
+    for (auto &r : s->ir_wait) s->free_resident(*r);
     s->eng.reset();
+    s->eres.reset();
+    s->e32.reset();
+    dev_free(s->d_ir);
+    dev_free(s->d_irtl);
     s->pool.reset();
     delete s;
     return MGX_OK;

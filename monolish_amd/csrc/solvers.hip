@@ -1136,6 +1136,7 @@ void mgx_solve_params_default(mgx_solve_params *p) {
 
 int mgx_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b, const mgx_solve_params *p,
                  mgx_solve_result *res) {
+  if (p && p->mode == MGX_MODE_MIXED_IR) return mgx_ir_cg_solve(A, x, b, p, 1e-5, 100, res);
   return guard([&] {
     check_solve_args(A, x, b, p, res, "mgx_cg_solve");
     activate(A->ctx);

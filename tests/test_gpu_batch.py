@@ -345,3 +345,45 @@ def test_batch_large_system_shares_launches(m, ctx, oracle):
         ref = oracle.solve("cg", rp, ci, v, np.zeros(len(b)), b, precond=2, tol=1e-30, maxiter=40)
         assert r.status == ref[0] == m.SOLVER_MAXITER and r.final_iter == 40
         assert np.array_equal(bits(X.download()), bits(ref[1]))
+
+
+def test_server_fp32_refined_requests(m, oracle):
+    """FP32-refined served requests (FP64 residuals and updates, FP32
+    corrections sharing the FP32 engine's launches) mixed with FP64 requests
+    in one server: each refined result equals mgx_ir_cg_solve of that system
+    alone bit for bit (same outer and inner iteration counts) and reaches the
+    FP64 tolerance; each FP64 result equals the oracle's."""
+    say("server fp32-refined")
+    c = m.Context(0)
+    try:
+        mats = [m.mmatrix(n, 5, 15, 0.5, seed=90 + n) for n in (900, 2500, 6000)]
+        mats.append(m.laplacian2d(70, 70))
+        bs = [oracle.random_vector(len(rp) - 1, -1.0, 1.0, 91 + k)
+              for k, (rp, _, _) in enumerate(mats)]
+        srv = m.Server(c, max_active=5, iters_per_poll=4, ir_inner_tol=1e-5)
+        tickets = []
+        for rep in range(2):
+            for k, ((rp, ci, v), b) in enumerate(zip(mats, bs)):
+                mode = m.MODE_MIXED_IR if (k + rep) % 2 == 0 else m.MODE_FUSED
+                tickets.append((srv.submit(m.METHOD_CG, rp, ci, v, b, tol=1e-10,
+                                           tol_is_relative=True, mode=mode), k, mode))
+        results = [(srv.wait(t, len(bs[k]), timeout=120.0), k, mode) for t, k, mode in tickets]
+        srv.close()
+        for (r, x), k, mode in results:
+            rp, ci, v = mats[k]
+            b = bs[k]
+            assert r.status == 0
+            if mode == m.MODE_MIXED_IR:
+                n = len(b)
+                A, X1 = m.CRS(c, n, n, rp, ci, v), m.Vector(c, n=n)
+                r1 = m.ir_cg(A, X1, m.Vector(c, b), tol=1e-10, tol_is_relative=True,
+                             inner_tol=1e-5)
+                say(n, r.outer_iter, r.inner_iter, r1.outer_iter, r1.inner_iter)
+                assert (r.outer_iter, r.inner_iter) == (r1.outer_iter, r1.inner_iter)
+                assert np.array_equal(bits(x), bits(X1.download()))
+                assert oracle.residual_l2(rp, ci, v, x, b) < 1e-10 * oracle.nrm2(b)
+            else:
+                ref = oracle_solve(oracle, "cg", rp, ci, v, b)
+                assert r.final_iter == ref[2] and np.array_equal(bits(x), bits(ref[1]))
+    finally:
+        c.close()

@@ -561,3 +561,65 @@ def test_matvec_sliced_layout(m, ctx, oracle, dtype):
             for o in (A, X, B):
                 o.free()
         assert bits_equal(xs[0], xs[1])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dot_nrm2_device_results(m, ctx, oracle, dtype):
+    """mgx_dot_device / mgx_nrm2_device leave the result in device memory:
+    the same bits mgx_dot / mgx_nrm2 return, and the oracle's."""
+    say("dot/nrm2 device results", dtype.__name__)
+    out = m.Vector(ctx, n=4)
+    for k, n in enumerate((1, 777, 1_000_003)):
+        x = oracle.random_vector(n, -1.0, 1.0, 300 + k, dtype=dtype)
+        y = oracle.random_vector(n, -1.0, 1.0, 400 + k, dtype=dtype)
+        X, Y = m.Vector(ctx, x, dtype=dtype), m.Vector(ctx, y, dtype=dtype)
+        m.dot_device(X, Y, out, 1)
+        m.nrm2_device(X, out, 3)
+        got = out.download()
+        assert got[1] == m.dot(X, Y) == float(oracle.dot(x, y))
+        assert got[3] == m.nrm2(X) == float(oracle.nrm2(x))
+        X.free()
+        Y.free()
+    out.free()
+
+
+def test_caller_stream(m, oracle):
+    """mgx_set_stream: a context queues its work on the caller's stream; an
+    SpMV, a dot and a fused CG solve there give the bits they give on the
+    library's own stream; NULL restores the library's stream."""
+    import ctypes as C
+    say("caller stream")
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    st = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(st)) == 0
+    c = m.Context(0)
+    try:
+        rp, ci, val = m.laplacian2d(60, 60)
+        n = len(rp) - 1
+        b = oracle.random_vector(n, -1.0, 1.0, 5)
+        A, B = m.CRS(c, n, n, rp, ci, val), m.Vector(c, b)
+
+        def run():
+            Y, X = m.Vector(c, n=n), m.Vector(c, n=n)
+            m.matvec(A, B, Y)
+            d = m.dot(Y, B)
+            r = m.cg(A, X, B, tol=1e-10, tol_is_relative=True, precond=m.PRECOND_JACOBI)
+            return Y.download(), d, r.final_iter, X.download()
+
+        own = c.stream
+        base = run()
+        m.set_stream(c, st.value)
+        assert c.stream == st.value
+        on = run()
+        m.set_stream(c, None)
+        assert c.stream == own
+        assert np.array_equal(base[0], on[0]) and base[1] == on[1] and base[2] == on[2]
+        assert np.array_equal(base[3].view(np.uint64), on[3].view(np.uint64))
+        ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2,
+                           tol=1e-10 * oracle.nrm2(b))
+        assert np.array_equal(on[3].view(np.uint64), ref[1].view(np.uint64))
+    finally:
+        c.close()
+        hip.hipStreamDestroy(st)
