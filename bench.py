@@ -470,7 +470,8 @@ def bench_batched(m, ctx, nsys, seed0=4000):
     return out
 
 
-def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10.0):
+def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10.0,
+                 cg_mode=None):
     """Poisson arrivals; 50% 2D/3D Poisson (CG), 50% 2D upwind convection-
     diffusion with Peclet U[1,50] (BiCGSTAB); Jacobi, 1e-10 relative; sizes
     log-uniform in [nmin, nmax].  Requests draw from a pool of systems that
@@ -479,7 +480,11 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
     the arrival rate and queued requests hold no host copies.  Arrivals stop
     after `seconds`; requests still unfinished `drain` seconds later are
     cancelled (a bounded run when the offered load exceeds capacity) and
-    reported as such; latency percentiles are over completed requests."""
+    reported as such; latency percentiles are over completed requests.
+    cg_mode MODE_MIXED_IR serves the CG requests by FP32-refined iterations
+    (the BiCGSTAB requests stay FP64)."""
+    if cg_mode is None:
+        cg_mode = m.MODE_FUSED
     rng = np.random.default_rng(seed)
     systems = []
     for k in range(pool):
@@ -520,7 +525,9 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
         if k not in keep and len(keep) < 4:
             X = m.Vector(ctx, n=nn)
             keep[k] = X
-        tickets.append(srv.submit_resident(meth, A, B, X, tol=1e-10, tol_is_relative=True))
+        mode = cg_mode if meth == m.METHOD_CG else m.MODE_FUSED
+        tickets.append(srv.submit_resident(meth, A, B, X, tol=1e-10, tol_is_relative=True,
+                                           mode=mode))
     t_submitted = time.perf_counter() - t0
     statuses = []
     cutoff = t0 + seconds + drain
@@ -538,8 +545,12 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
     for k, X in keep.items():
         meth, A, B, nn = systems[k]
         X1 = m.Vector(ctx, n=nn)
-        fn = m.cg if meth == m.METHOD_CG else m.bicgstab
-        fn(A, X1, B, tol=1e-10, tol_is_relative=True, precond=m.PRECOND_JACOBI, mode=m.MODE_FUSED)
+        if meth == m.METHOD_CG and cg_mode == m.MODE_MIXED_IR:
+            m.ir_cg(A, X1, B, tol=1e-10, tol_is_relative=True, inner_tol=1e-5)
+        else:
+            fn = m.cg if meth == m.METHOD_CG else m.bicgstab
+            fn(A, X1, B, tol=1e-10, tol_is_relative=True, precond=m.PRECOND_JACOBI,
+               mode=m.MODE_FUSED)
         same = same and bool(np.array_equal(X1.download().view(np.uint64),
                                             X.download().view(np.uint64)))
     out = {"workload": f"Poisson arrivals {rate}/s for {seconds}s, n log-uniform [{nmin},{nmax}], "
@@ -548,6 +559,8 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
                        f"requests unfinished {drain}s after the last arrival are cancelled",
            "requests": len(tickets), "converged": int(sum(s == 0 for s in statuses)),
            "cancelled_at_cutoff": int(sum(s == m.ERR_CANCELLED for s in statuses)),
+           "cg_requests": "fp32-refined (FP64 residual, FP32 corrections)"
+                          if cg_mode == m.MODE_MIXED_IR else "fp64",
            "solves_per_s": round(st["solves_per_s"], 1),
            "offered_per_s": round(len(tickets) / t_submitted, 1),
            "p50_latency_ms": round(1e3 * st["p50_latency_s"], 2),
@@ -845,6 +858,10 @@ def main():
                                                         args.serve_seconds, 1000,
                                                         args.serve_nmax, seed=5002,
                                                         drain=args.serve_drain)
+        # the same stream with the CG requests refined in FP32
+        line["served"]["fp32_refined_cg"] = bench_served(m, ctx, 2000.0, args.serve_seconds, 1000,
+                                                         args.serve_nmax, drain=args.serve_drain,
+                                                         cg_mode=m.MODE_MIXED_IR)
     if "ingest" in legs:
         line["ingest"] = bench_ingest(m, ctx, 128, "cpu" in legs)
     if "sor" in legs:
