@@ -46,12 +46,14 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const IrSys *sys, const Ti
 
 void launch_ir_scale(Ctx *c, const IrSys *sys, const TileRef *tiles, int64_t ntiles) {
   if (ntiles <= 0) return;
+  MGX_REQUIRE(sys && tiles, "refinement: system list not on the device");
   k_ir_scale<<<(unsigned)ntiles, kBlock, 0, c->stream>>>(sys, tiles, ntiles);
   MGX_HIP(hipGetLastError());
 }
 
 void launch_ir_update(Ctx *c, const IrSys *sys, const TileRef *tiles, int64_t ntiles) {
   if (ntiles <= 0) return;
+  MGX_REQUIRE(sys && tiles, "refinement: system list not on the device");
   k_ir_update<<<(unsigned)ntiles, kBlock, 0, c->stream>>>(sys, tiles, ntiles);
   MGX_HIP(hipGetLastError());
 }

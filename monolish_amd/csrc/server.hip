@@ -402,7 +402,9 @@ struct Server {
       for (int64_t t = 0; t < ntiles_of(r->n); t++) tl.push_back(TileRef{idx, (int32_t)t});
       go.push_back(r);
     }
-    launch_ir_scale(ctx, d_ir, d_irtl, ir_upload(hs, tl));
+    // ir_upload may (re)allocate d_ir / d_irtl: read them only after it returns
+    const int64_t nscale = ir_upload(hs, tl);
+    launch_ir_scale(ctx, d_ir, d_irtl, nscale);
     for (const auto &r : go) {
       SolveCfg ic;
       ic.method = kMethodCG;
@@ -439,7 +441,8 @@ struct Server {
       r->ir_phase = 0;
       ir_wait.push_back(r);
     }
-    launch_ir_update(ctx, d_ir, d_irtl, ir_upload(hs, tl));
+    const int64_t nupdate = ir_upload(hs, tl);  // before d_ir / d_irtl are read
+    launch_ir_update(ctx, d_ir, d_irtl, nupdate);
   }
 
   void loop() {
