@@ -216,16 +216,31 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longrows(
 
 // Sliced layout: rows in slices of kSlice = 64 and entries in tuples of
 // W = 16 / sizeof(value) (one 16-byte value load per lane; the FP32 copy of
-// an FP64 matrix gets a layout of its own with W = 4); slice s stores
-// entry j of its row l at soff[s] + ((j / W) * 64 + l) * W + j % W, the
-// slice as wide as its longest short row (rounded up to W).  One lane per
+// an FP64 matrix gets a layout of its own with W = 4).  Slice s has a tuple
+// region at soff[2s], nt_s = (soff[2s + 1] - soff[2s]) / (64 W) tuples deep,
+// and a tail region at soff[2s + 1].  Row l of the slice, len entries, keeps
+// its first t = min(nt_s, len / W) * W entries in tuples and the other
+// len - t in its tail: entry j sits at soff[2s] + ((j / W) * 64 + l) * W +
+// j % W for j < t, else at soff[2s + 1] + (j - t) * 64 + l.  nt_s is the
+// deepest tuple region that keeps the slice as wide as its longest row.  One lane per
 // row walks its row in storage order with no LDS: a wavefront loads 1 KiB
-// of consecutive values per instruction, and on banded and stencil matrices
-// gathers x from consecutive addresses.
+// of consecutive values per tuple instruction, a tail entry is one 8-byte
+// (FP64) or 4-byte (FP32) load, and no lane reads a slot beyond its own
+// row's entries (a 7-entry stencil row reads exactly its 7 values and
+// indices); on banded and stencil matrices x is gathered from consecutive
+// addresses.
 constexpr int kSlice = 64;
+constexpr int kSellTail = 4;  // tail entries loaded with a row's first tuples
 
-__host__ __device__ constexpr int64_t sell_index(int64_t base, int32_t j, int l, int w) {
-  return base + ((int64_t)(j / w) * kSlice + l) * w + j % w;
+// entries of a row kept in tuples: nt_s tuples at most
+__host__ __device__ constexpr int32_t sell_tupled(int32_t len, int32_t nts, int w) {
+  return (len / w < nts ? len / w : nts) * w;
+}
+
+__host__ __device__ constexpr int64_t sell_index(int64_t tup, int64_t tail, int32_t j,
+                                                 int32_t t, int l, int w) {
+  return j < t ? tup + ((int64_t)(j / w) * kSlice + l) * w + j % w
+               : tail + (int64_t)(j - t) * kSlice + l;
 }
 
 template <class T, int W>
@@ -236,17 +251,46 @@ __global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
   if (r >= rows) return;
   const int32_t ra = rp[r], len = rp[r + 1] - ra;
   if (len > long_t) return;
-  const int64_t base = soff[r / kSlice];
+  const int64_t tup = soff[2 * (r / kSlice)], tail = soff[2 * (r / kSlice) + 1];
+  const int32_t t = sell_tupled(len, (int32_t)((tail - tup) / (kSlice * W)), W);
   const int l = (int)(r % kSlice);
   for (int32_t j = 0; j < len; j++) {
-    const int64_t o = sell_index(base, j, l, W);
+    const int64_t o = sell_index(tup, tail, j, t, l, W);
     if (sci) sci[o] = ci[ra + j];
     sval[o] = val[ra + j];
   }
 }
 
-// one row of the sliced layout: its entries eight at a time (W-entry
-// tuples, nontemporal), x gathered, the products added in storage order
+// the tail entries k0 + [0, kSellTail) of a row (those below rem): values and
+// gathered x
+template <class T>
+__device__ __forceinline__ void sell_tail_load(int64_t tb, int32_t k0, int32_t rem,
+                                               const int32_t *__restrict__ sci,
+                                               const T *__restrict__ sval, T *tv,
+                                               int32_t *tc) {
+#pragma unroll
+  for (int k = 0; k < kSellTail; k++)
+    if (k0 + k < rem) {
+      tv[k] = __builtin_nontemporal_load(sval + tb + (int64_t)(k0 + k) * kSlice);
+      tc[k] = __builtin_nontemporal_load(sci + tb + (int64_t)(k0 + k) * kSlice);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void sell_tail_add(int32_t k0, int32_t rem, const T *tv,
+                                              const T *tx, T &s) {
+#pragma unroll
+  for (int k = 0; k < kSellTail; k++)
+    if (k0 + k < rem) {
+      T p = tv[k] * tx[k];
+      s = s + p;
+    }
+}
+
+// one row of the sliced layout: its tuples eight entries at a time
+// (nontemporal 16-byte loads), the first kSellTail tail entries loaded with
+// the first batch, x gathered, the products added in storage order (tuples,
+// then the tail)
 template <class T, int W>
 __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__restrict__ soff,
                                       const int32_t *__restrict__ sci,
@@ -254,31 +298,56 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
   constexpr int QT = 8 / W;  // tuples per step: 8 entries
   typedef T VT __attribute__((ext_vector_type(W)));
   typedef int32_t VI __attribute__((ext_vector_type(W)));
-  const int64_t base = soff[r / kSlice] + (r % kSlice) * W;
+  const int64_t s2 = 2 * (r / kSlice);
+  const int l = (int)(r % kSlice);
+  const int64_t tup = soff[s2], tail = soff[s2 + 1];
+  const int32_t nt = sell_tupled(len, (int32_t)((tail - tup) / (kSlice * W)), W) / W;
+  const int32_t rem = len - nt * W;
+  const int64_t base = tup + (int64_t)l * W, tb = tail + l;
   const VT *v = reinterpret_cast<const VT *>(sval + base);
   const VI *c = reinterpret_cast<const VI *>(sci + base);
+  T tv[kSellTail], tx[kSellTail];
+  int32_t tc[kSellTail];
+  sell_tail_load(tb, 0, rem, sci, sval, tv, tc);
   T s = 0;
-  for (int32_t j = 0; j < len; j += QT * W) {
+  for (int32_t q0 = 0; q0 < nt; q0 += QT) {
     VT vv[QT];
     T xv[QT * W];
 #pragma unroll
     for (int q = 0; q < QT; q++)
-      if (j + q * W < len) {
-        const int64_t o = (int64_t)(j / W + q) * kSlice;
+      if (q0 + q < nt) {
+        const int64_t o = (int64_t)(q0 + q) * kSlice;
         vv[q] = __builtin_nontemporal_load(v + o);
         const VI cc = __builtin_nontemporal_load(c + o);
 #pragma unroll
-        for (int e = 0; e < W; e++)
-          if (j + q * W + e < len) xv[q * W + e] = x[cc[e]];
+        for (int e = 0; e < W; e++) xv[q * W + e] = x[cc[e]];
       }
+    if (q0 == 0) {
+#pragma unroll
+      for (int k = 0; k < kSellTail; k++)
+        if (k < rem) tx[k] = x[tc[k]];
+    }
 #pragma unroll
     for (int q = 0; q < QT; q++)
+      if (q0 + q < nt)
 #pragma unroll
-      for (int e = 0; e < W; e++)
-        if (j + q * W + e < len) {
+        for (int e = 0; e < W; e++) {
           T p = vv[q][e] * xv[q * W + e];
           s = s + p;
         }
+  }
+  if (nt == 0) {
+#pragma unroll
+    for (int k = 0; k < kSellTail; k++)
+      if (k < rem) tx[k] = x[tc[k]];
+  }
+  sell_tail_add(0, rem, tv, tx, s);
+  for (int32_t k0 = kSellTail; k0 < rem; k0 += kSellTail) {  // longer tails
+    sell_tail_load(tb, k0, rem, sci, sval, tv, tc);
+#pragma unroll
+    for (int k = 0; k < kSellTail; k++)
+      if (k0 + k < rem) tx[k] = x[tc[k]];
+    sell_tail_add(k0, rem, tv, tx, s);
   }
   return s;
 }
@@ -294,47 +363,84 @@ __device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, in
   constexpr int QT = 8 / W;
   typedef T VT __attribute__((ext_vector_type(W)));
   typedef int32_t VI __attribute__((ext_vector_type(W)));
-  const VT *va = reinterpret_cast<const VT *>(sval + (la ? soff[ra / kSlice] + (ra % kSlice) * W : 0));
-  const VI *ca = reinterpret_cast<const VI *>(sci + (la ? soff[ra / kSlice] + (ra % kSlice) * W : 0));
-  const VT *vb = reinterpret_cast<const VT *>(sval + (lb ? soff[rb / kSlice] + (rb % kSlice) * W : 0));
-  const VI *cb = reinterpret_cast<const VI *>(sci + (lb ? soff[rb / kSlice] + (rb % kSlice) * W : 0));
+  const int64_t sa2 = 2 * (ra / kSlice), sb2 = 2 * (rb / kSlice);
+  const int64_t tupa = la ? soff[sa2] : 0, taila = la ? soff[sa2 + 1] : 0;
+  const int64_t tupb = lb ? soff[sb2] : 0, tailb = lb ? soff[sb2 + 1] : 0;
+  const int32_t na = sell_tupled(la, (int32_t)((taila - tupa) / (kSlice * W)), W) / W;
+  const int32_t nb = sell_tupled(lb, (int32_t)((tailb - tupb) / (kSlice * W)), W) / W;
+  const int32_t rema = la - na * W, remb = lb - nb * W;
+  const int64_t oa = tupa + (ra % kSlice) * W, ob = tupb + (rb % kSlice) * W;
+  const int64_t ta = taila + ra % kSlice, tb = tailb + rb % kSlice;
+  const VT *va = reinterpret_cast<const VT *>(sval + oa);
+  const VI *ca = reinterpret_cast<const VI *>(sci + oa);
+  const VT *vb = reinterpret_cast<const VT *>(sval + ob);
+  const VI *cb = reinterpret_cast<const VI *>(sci + ob);
+  T tva[kSellTail], txa[kSellTail], tvb[kSellTail], txb[kSellTail];
+  int32_t tca[kSellTail], tcb[kSellTail];
+  sell_tail_load(ta, 0, rema, sci, sval, tva, tca);
+  sell_tail_load(tb, 0, remb, sci, sval, tvb, tcb);
   sa = 0;
   sb = 0;
-  const int32_t L = la > lb ? la : lb;
-  for (int32_t j = 0; j < L; j += QT * W) {
+  const int32_t N = na > nb ? na : nb;
+  for (int32_t q0 = 0; q0 < N; q0 += QT) {
     VT wa[QT], wb[QT];
     T xa[QT * W], xb[QT * W];
 #pragma unroll
     for (int q = 0; q < QT; q++) {
-      const int64_t o = (int64_t)(j / W + q) * kSlice;
-      if (j + q * W < la) {
+      const int64_t o = (int64_t)(q0 + q) * kSlice;
+      if (q0 + q < na) {
         wa[q] = __builtin_nontemporal_load(va + o);
         const VI cc = __builtin_nontemporal_load(ca + o);
 #pragma unroll
-        for (int e = 0; e < W; e++)
-          if (j + q * W + e < la) xa[q * W + e] = x[cc[e]];
+        for (int e = 0; e < W; e++) xa[q * W + e] = x[cc[e]];
       }
-      if (j + q * W < lb) {
+      if (q0 + q < nb) {
         wb[q] = __builtin_nontemporal_load(vb + o);
         const VI cc = __builtin_nontemporal_load(cb + o);
 #pragma unroll
-        for (int e = 0; e < W; e++)
-          if (j + q * W + e < lb) xb[q * W + e] = x[cc[e]];
+        for (int e = 0; e < W; e++) xb[q * W + e] = x[cc[e]];
+      }
+    }
+    if (q0 == 0) {
+#pragma unroll
+      for (int k = 0; k < kSellTail; k++) {
+        if (k < rema) txa[k] = x[tca[k]];
+        if (k < remb) txb[k] = x[tcb[k]];
       }
     }
 #pragma unroll
     for (int q = 0; q < QT; q++)
 #pragma unroll
       for (int e = 0; e < W; e++) {
-        if (j + q * W + e < la) {
+        if (q0 + q < na) {
           T p = wa[q][e] * xa[q * W + e];
           sa = sa + p;
         }
-        if (j + q * W + e < lb) {
+        if (q0 + q < nb) {
           T p = wb[q][e] * xb[q * W + e];
           sb = sb + p;
         }
       }
+  }
+  if (N == 0) {
+#pragma unroll
+    for (int k = 0; k < kSellTail; k++) {
+      if (k < rema) txa[k] = x[tca[k]];
+      if (k < remb) txb[k] = x[tcb[k]];
+    }
+  }
+  sell_tail_add(0, rema, tva, txa, sa);
+  sell_tail_add(0, remb, tvb, txb, sb);
+  for (int32_t k0 = kSellTail; k0 < rema || k0 < remb; k0 += kSellTail) {  // longer tails
+    sell_tail_load(ta, k0, rema, sci, sval, tva, tca);
+    sell_tail_load(tb, k0, remb, sci, sval, tvb, tcb);
+#pragma unroll
+    for (int k = 0; k < kSellTail; k++) {
+      if (k0 + k < rema) txa[k] = x[tca[k]];
+      if (k0 + k < remb) txb[k] = x[tcb[k]];
+    }
+    sell_tail_add(k0, rema, tva, txa, sa);
+    sell_tail_add(k0, remb, tvb, txb, sb);
   }
 }
 
@@ -635,28 +741,50 @@ void spmv_sell_free(Csr *A) {
   p.nslong = 0;
 }
 
-// slice offsets for w-entry tuples; false when the padding exceeds 30 % of
-// the short rows' entries
+// slice offsets for w-entry tuples: soff[2s] the tuple region of slice s,
+// soff[2s + 1] its tail region, soff[2 ns] the entries of all slices (each
+// slice as wide as its longest row, with the deepest tuple region that
+// allows); false when
+// the padding exceeds 30 % of the short rows' entries
 static bool sliced_offsets(const int32_t *rp, int64_t rows, int long_t, int w,
                            std::vector<int64_t> &soff, std::vector<int32_t> *lrows) {
   const int64_t ns = (rows + kSlice - 1) / kSlice;
-  soff.assign(ns + 1, 0);
-  int64_t short_nnz = 0;
+  soff.assign(2 * ns + 1, 0);
+  int64_t short_nnz = 0, at = 0;
+  std::vector<int32_t> lens;
   for (int64_t s = 0; s < ns; s++) {
-    int32_t width = 0;
+    lens.clear();
+    int32_t maxt = 0;
     for (int64_t r = s * kSlice; r < std::min(rows, (s + 1) * kSlice); r++) {
       const int32_t len = rp[r + 1] - rp[r];
       if (len > long_t) {
         if (lrows) lrows->push_back((int32_t)r);
         continue;
       }
-      width = std::max(width, len);
+      lens.push_back(len);
+      maxt = std::max(maxt, len / w);
       short_nnz += len;
     }
-    width = (width + w - 1) / w * w;
-    soff[s + 1] = soff[s] + (int64_t)width * kSlice;
+    // a slice is at least as wide as its longest row; the deepest tuple
+    // region that keeps it that narrow (nt = 0 always does)
+    int32_t maxlen = 0;
+    for (int32_t len : lens) maxlen = std::max(maxlen, len);
+    int32_t best_nt = 0, best_tail = maxlen;
+    for (int32_t nt = maxt; nt > 0; nt--) {
+      int32_t tail = 0;
+      for (int32_t len : lens) tail = std::max(tail, len - sell_tupled(len, nt, w));
+      if ((int64_t)nt * w + tail <= maxlen) {
+        best_nt = nt;
+        best_tail = tail;
+        break;
+      }
+    }
+    soff[2 * s] = at;
+    soff[2 * s + 1] = at + (int64_t)best_nt * w * kSlice;
+    at = soff[2 * s + 1] + (int64_t)best_tail * kSlice;  // a multiple of kSlice, hence of w
   }
-  return (double)soff[ns] <= 1.3 * (double)short_nnz + kSlice * w;
+  soff[2 * ns] = at;
+  return (double)at <= 1.3 * (double)short_nnz + kSlice * w;
 }
 
 // allocates a layout from host offsets and writes its indices and values
