@@ -220,6 +220,15 @@ int mgx_nrm2(mgx_vec *x, double *out);
  * queued on the context's stream without a host round trip. */
 int mgx_dot_device(mgx_vec *x, mgx_vec *y, double *d_out);
 int mgx_nrm2_device(mgx_vec *x, double *d_out);
+/* No counterpart in the reference.  mgx_dot_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc): the same bits, with the reduction's counters and
+ * partials in a static device array of about 1 MiB per device (held until
+ * the process ends) instead of library-allocated buffers; allocates nothing.
+ * At most 2^26 elements (larger inputs: MGX_ERR_INVALID).  Every call on a
+ * device shares that device's array, so calls that overlap on one device
+ * (other streams, other threads) race. */
+int mgx_dot_device_noalloc(mgx_vec *x, mgx_vec *y, double *d_out);
 /* blas::axpy (axpy.hpp:7-61): y = a x + y */
 int mgx_axpy(double a, mgx_vec *x, mgx_vec *y);
 /* blas::xpay (xpay.hpp:7-66): y = x + a y */

@@ -109,6 +109,7 @@ _sig("mgx_dot", [_P, _P, C.POINTER(_D)])
 _sig("mgx_nrm2", [_P, C.POINTER(_D)])
 _sig("mgx_dot_device", [_P, _P, _P])
 _sig("mgx_nrm2_device", [_P, _P])
+_sig("mgx_dot_device_noalloc", [_P, _P, _P])
 _sig("mgx_set_stream", [_P, _P])
 _sig("mgx_axpy", [_D, _P, _P])
 _sig("mgx_xpay", [_D, _P, _P])
@@ -443,6 +444,15 @@ def dot_device(x: Vector, y: Vector, out: Vector, index: int = 0):
     if out.dtype != np.float64 or not 0 <= index < out.size:
         raise ValueError("out must be an FP64 vector holding index")
     check(_lib.mgx_dot_device(x.handle, y.handle, out.device_ptr + 8 * index))
+
+
+def dot_device_noalloc(x: Vector, y: Vector, out: Vector, index: int = 0):
+    """dot_device without device allocations: the reduction's scratch is a
+    static per-device array (at most 2**26 elements; calls overlapping on one
+    device race)."""
+    if out.dtype != np.float64 or not 0 <= index < out.size:
+        raise ValueError("out must be an FP64 vector holding index")
+    check(_lib.mgx_dot_device_noalloc(x.handle, y.handle, out.device_ptr + 8 * index))
 
 
 def nrm2_device(x: Vector, out: Vector, index: int = 0):

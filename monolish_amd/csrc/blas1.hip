@@ -10,6 +10,8 @@
 // 512-element tile, whose partial is the block tree of the tile; the blocks
 // then meet in arrive_reduce, which runs stage 2 (lane sums in order, final
 // block tree) spread over the last-arriving blocks.
+#include <mutex>
+
 #include "mgx_device.hpp"
 
 namespace mgx {
@@ -301,6 +303,52 @@ void launch_mnrm2sq(Ctx *c, mgx_dtype dt, int64_t n, int m, const void *X, int64
   MGX_HIP(hipGetLastError());
 }
 
+// ---- allocation-free dot ------------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops that forbid hipMalloc): the dot's arrival counters,
+// tile partials and lane sums live in one file-scope device array.  Every
+// call on a device uses that device's copy and overwrites what the last
+// call left there, so calls that overlap on one device (two streams, two
+// threads) race.
+constexpr int64_t kNoAllocMaxN = int64_t(1) << 26;  // 67,108,864 elements
+constexpr size_t kNoAllocTickBytes = 2048;          // >= kTickWords counters, 256-aligned
+constexpr size_t kNoAllocBytes =
+    kNoAllocTickBytes + (size_t)(kNoAllocMaxN / kTile + 2 * kRedGroups) * sizeof(double);
+static_assert(kTickWords * sizeof(uint32_t) <= kNoAllocTickBytes, "counter area");
+static_assert(kNoAllocMaxN % kTile == 0, "whole tiles");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+static void *g_scratch_ptr[kNoAllocDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+// zeroes n counters (a kernel rather than a memset node, so the call can be
+// captured into a graph)
+namespace {
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+}  // namespace
+
+// Looks up this device's address of `scratch` (the array's name is only a
+// host-side stand-in).  Called by mgx_init with `device` current.
+void noalloc_scratch_init(int device) {
+  MGX_REQUIRE(device >= 0 && device < kNoAllocDevices, "static scratch: device %d out of range",
+              device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *ptr = nullptr;
+  MGX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  MGX_REQUIRE(ptr != nullptr, "static scratch: no device address on device %d", device);
+  g_scratch_ptr[device] = ptr;
+}
+
+static void *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kNoAllocDevices ? g_scratch_ptr[device] : nullptr;
+}
+
 double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
                 const void *y) {
   double *slot = &c->d_state->rr;
@@ -312,3 +360,31 @@ double dot_host(Ctx *c, mgx_dtype dt, int64_t n, const void *x,
 }
 
 }  // namespace mgx
+
+using namespace mgx;
+
+extern "C" int mgx_dot_device_noalloc(mgx_vec *x, mgx_vec *y, double *d_out) {
+  return guard([&] {
+    MGX_REQUIRE(x && y && d_out, "mgx_dot_device_noalloc: NULL argument");
+    MGX_REQUIRE(x->ctx == y->ctx, "mgx_dot_device_noalloc: vectors of different contexts");
+    MGX_REQUIRE(x->dtype == y->dtype, "mgx_dot_device_noalloc: dtype mismatch");
+    MGX_REQUIRE(x->n == y->n, "mgx_dot_device_noalloc: size mismatch (%lld vs %lld)",
+                (long long)x->n, (long long)y->n);
+    MGX_REQUIRE(x->n <= kNoAllocMaxN,
+                "mgx_dot_device_noalloc: %lld elements exceed the static scratch's limit of "
+                "%lld",
+                (long long)x->n, (long long)kNoAllocMaxN);
+    Ctx *c = x->ctx;
+    activate(c);
+    char *base = static_cast<char *>(noalloc_scratch(c->device));
+    MGX_REQUIRE(base, "mgx_dot_device_noalloc: static scratch not set up on device %d",
+                c->device);
+    uint32_t *ticks = reinterpret_cast<uint32_t *>(base);
+    if (x->n > 0) {
+      k_zero_u32<<<1, 256, 0, c->stream>>>(ticks, kTickWords);
+      MGX_HIP(hipGetLastError());
+    }
+    launch_dot_into(c, x->dtype, x->n, x->d, y->d, base + kNoAllocTickBytes, d_out, ticks);
+    return MGX_OK;
+  });
+}

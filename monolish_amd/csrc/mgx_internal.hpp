@@ -278,6 +278,9 @@ inline void activate(Ctx *c) {
   }
 }
 void ensure_partials(Ctx *c, size_t bytes);
+// looks up the device address of the allocation-free dot's static scratch
+// on `device` (which must be current); called once per context set-up
+void noalloc_scratch_init(int device);
 
 __host__ __device__ inline int64_t ntiles_of(int64_t n) { return (n + kTile - 1) / kTile; }
 

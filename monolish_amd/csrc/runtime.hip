@@ -206,6 +206,7 @@ int mgx_init(int device, mgx_ctx **out) {
     MGX_HIP(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
     MGX_HIP(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming));
     ensure_partials(c, 1 << 20);
+    noalloc_scratch_init(device);
     *out = c;
     return MGX_OK;
   });
