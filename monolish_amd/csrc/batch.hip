@@ -126,18 +126,116 @@ __device__ __forceinline__ void cg_finish(const SysDesc<T> &d, SolveState *st, T
   if (s != kStatRunning && stopped) atomicAdd(stopped, 1);
 }
 
+// Kernel-finished reductions (SysDesc::lf, at most kRedGroups tiles): a
+// kernel publishes its tile partials and ends; every block of the next
+// kernel of that system reads all of them and finishes the reduction in the
+// canonical order (stage-2 lane t holds 0 + P_t, then the block tree, as
+// arrive_reduce's short path), so no block waits for the others and no
+// arrival counter is touched.  The block of the system's first tile writes
+// the solve state; fields that the same kernel's blocks read are never
+// written by it (iteration numbers alternate between ia and ib, zr between
+// zrs[0] and zrs[1]).
+
+// lane t's stage-2 value of one reduction over nt tile partials at P (Q = 4:
+// quarter partials, the tile partial rebuilt as (W0 + W1) + (W2 + W3))
+template <class T, int Q>
+__device__ __forceinline__ T lf_lane(const T *P, int64_t nt) {
+  T lane = T(0);
+  const int64_t k = threadIdx.x;
+  if (k < nt) {
+    T p;
+    if (Q == 4) {
+      const T w0 = load_partial(P, 4 * k), w1 = load_partial(P, 4 * k + 1);
+      const T w2 = load_partial(P, 4 * k + 2), w3 = load_partial(P, 4 * k + 3);
+      const T lo = w0 + w1;
+      const T hi = w2 + w3;
+      p = lo + hi;
+    } else {
+      p = load_partial(P, k);
+    }
+    lane = lane + p;
+  }
+  return lane;
+}
+
+// offset of the <p, q> quarter partials of an lf system in its P (the
+// update kernel's <z,r> and <r,r> tile partials use [0, 2 ntiles))
+template <class T> __device__ __forceinline__ int64_t lf_pq_off(const SysDesc<T> &d) {
+  return 4 * d.ntiles;
+}
+
+// Start of CG iteration `it` (= the iterations completed) of an lf system,
+// run by every block of the kernel that opens the iteration: finishes
+// <z,r> and <r,r> of the last update, then beta, the residual and the stop
+// tests of cg.cpp:64-84 (as cg_finish).  Returns false (in every thread)
+// when the system stops; else beta (pz: p = z, the first iteration) for all
+// threads.  `writer` (the block of tile 0) records the state.
+template <class T>
+__device__ __forceinline__ bool lf_open(const SysDesc<T> &d, SolveState *st, bool writer,
+                                        int32_t *stopped, T *sh, T *bc, int *flag, T &beta,
+                                        bool &pz, int64_t &it) {
+  it = sld(&st->ia);
+  const int32_t fresh = sld(&st->fresh);
+  if (fresh != 0) {
+    pz = fresh == 1;
+    beta = pz ? T(0) : (T)sld(&st->beta);
+    if (writer && threadIdx.x == 0) {
+      sst(&st->zrs[it & 1], sld(&st->zr));
+      sst(&st->ib, it);
+    }
+    return true;
+  }
+  pz = false;
+  T a = lf_lane<T, 1>(d.P, d.ntiles), c = lf_lane<T, 1>(d.P + d.ntiles, d.ntiles);
+  block_tree2(a, c, sh);
+  if (threadIdx.x == 0) {
+    const T zr_new = a, rr = c;
+    const T zr_old = (T)sld(&st->zrs[(it - 1) & 1]);
+    const T b = zr_new / zr_old;
+    const T resid = dsqrt(rr);
+    int32_t s = kStatRunning;
+    if ((double)resid < sld(&st->tol) && sld(&st->miniter) <= it)
+      s = MGX_OK;
+    else if (resid != resid)
+      s = MGX_SOLVER_RESIDUAL_NAN;
+    else if (it >= sld(&st->maxiter))
+      s = MGX_SOLVER_MAXITER;
+    bc[0] = b;
+    *flag = s;
+    if (writer) {
+      if (d.rhist && it - 1 < d.rhist_len) d.rhist[it - 1] = (double)resid;
+      sst(&st->beta, (double)b);
+      sst(&st->zr, (double)zr_new);
+      sst(&st->rr, (double)rr);
+      sst(&st->resid, (double)resid);
+      sst(&st->iter, it);
+      sst(&st->zrs[it & 1], (double)zr_new);
+      sst(&st->ib, it);
+      if (s != kStatRunning) {
+        sst(&st->status, s);
+        if (stopped) atomicAdd(stopped, 1);
+      }
+    }
+  }
+  __syncthreads();
+  beta = bc[0];
+  return *flag == kStatRunning;
+}
+
 // ---------------------------------------------------------------- CG ----
 // p = z + beta p formed at each gather; q = A p; <p, q>; alpha = <z,r>/<p,q>
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
                                                     const TileRef *tiles,
-                                                    int64_t nlaunch, int remap) {
+                                                    int64_t nlaunch, int remap,
+                                                    int32_t *stopped) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *prod = reinterpret_cast<T *>(smem_raw);
   __shared__ T qloc[kQuarter];
   __shared__ T sh[8];
   __shared__ T res[2];
   __shared__ int flag;
+  __shared__ T bc[2];
   const int64_t b = xcd_block(remap);  // consecutive quarters share an XCD's L2
   if (b >= nlaunch) return;
   int64_t tile;
@@ -145,12 +243,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   if (d.split) return;  // split iteration: plan-based SpMV and k_cg_pq instead
-  const bool odd = (st->iter & 1) != 0;
   const bool once = d.once != 0;  // p already formed in pa by k_cg_p
+  bool first;
+  T beta;
+  int64_t iter;
+  if (d.lf && !once) {
+    // this kernel opens the iteration (finishes the last update's reductions)
+    if (!lf_open(d, st, tile == 0 && (b & 3) == 0, stopped, sh, bc, &flag, beta, first, iter))
+      return;
+  } else if (d.lf) {
+    iter = sld(&st->ib);  // k_cg_p opened it
+    first = false;
+    beta = T(0);
+  } else {
+    iter = st->iter;
+    first = st->first != 0;
+    beta = (T)st->beta;
+  }
+  const bool odd = (iter & 1) != 0;
   const T *p_old = once ? d.pa : (odd ? d.pb : d.pa);
   T *p_new = odd ? d.pa : d.pb;
-  const bool first = st->first != 0;
-  const T beta = (T)st->beta;
   const T *z = d.z;
   auto pv = [&](int64_t c) -> T {
     if (once) return p_old[c];
@@ -180,6 +292,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
     }
   }
   T s = wave_tree(acc0 + acc1);
+  if (d.lf) {
+    // k_cg_update finishes <p, q>
+    if (t == 0) publish_partial(d.P, lf_pq_off(d) + qi, s);
+    return;
+  }
   if (t == 0) publish_partial(d.P, qi, s);
   if (arrive_reduce<T, 1, 4>(d.tick, d.P, 0, d.S, d.ntiles, qi, prod, sh, res, &flag)) {
     if (t == 0) {
@@ -202,15 +319,37 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
   __shared__ T tp[2 * kBlock];
   __shared__ T res[2];
   __shared__ int flag;
+  __shared__ T bc[2];
   const int64_t b = blockIdx.x;
   if (b >= nlaunch) return;
   int64_t tile;
   const SysDesc<T> &d = locate(sys, tiles, b, tile);
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
-  const bool odd = (st->iter & 1) != 0;
+  int64_t iter;
+  T alpha;
+  if (d.lf) {
+    // finish <p, q> of the SpMV kernel; alpha = <z,r> / <p,q> (cg.cpp:55-58)
+    iter = sld(&st->ib);
+    const T lane = lf_lane<T, 4>(d.P + lf_pq_off(d), d.ntiles);
+    const T pq = block_tree(lane, sh);
+    if (threadIdx.x == 0) {
+      const T zr = (T)sld(&st->zrs[iter & 1]);
+      const T a = zr / pq;
+      bc[0] = a;
+      if (tile == 0) {
+        sst(&st->pq, (double)pq);
+        sst(&st->alpha, (double)a);
+      }
+    }
+    __syncthreads();
+    alpha = bc[0];
+  } else {
+    iter = st->iter;
+    alpha = (T)st->alpha;
+  }
+  const bool odd = (iter & 1) != 0;
   const T *p_new = odd || d.split || d.once ? d.pa : d.pb;
-  const T alpha = (T)st->alpha;
   const T nalpha = -alpha;
   const int t = threadIdx.x;
   const int64_t R0 = d.row0 + tile * kTile;
@@ -268,6 +407,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
     publish_partial(d.P, d.ntiles + tile, rr_a);
   }
   if (d.split) return;  // k_cg_finish reduces the partials of the split form
+  if (d.lf) {
+    // the next iteration's opening kernel finishes <z,r> and <r,r>
+    if (t == 0 && tile == 0) {
+      sst(&st->ia, iter + 1);
+      sst(&st->fresh, 0);
+    }
+    return;
+  }
   if (arrive_reduce<T, 2, 1>(d.tick, d.P, d.ntiles, d.S, d.ntiles, tile, tp, sh, res,
                              &flag)) {
     if (t == 0) cg_finish<T>(d, st, res[0], res[1], stopped);
@@ -476,16 +623,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_persist(const SysDesc<T> *sys, Gr
 // 2 is q = A p by the plan-based SpMV (k_cg_spmv for the gather-once form)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const TileRef *tiles,
-                                                 int64_t nlaunch) {
+                                                 int64_t nlaunch, int32_t *stopped) {
   const int64_t b = blockIdx.x;
   if (b >= nlaunch) return;
   int64_t tile;
   const SysDesc<T> &d = locate(sys, tiles, b, tile);
   if (!d.split && !d.once) return;
-  const SolveState *st = d.st;
+  SolveState *st = d.st;
   if (st->status != kStatRunning) return;
-  const bool first = st->first != 0;
-  const T beta = (T)st->beta;
+  bool first;
+  T beta;
+  if (d.lf) {
+    // this kernel opens the iteration (finishes the last update's reductions)
+    __shared__ T sh[8];
+    __shared__ T bc[2];
+    __shared__ int flag;
+    int64_t iter;
+    if (!lf_open(d, st, tile == 0, stopped, sh, bc, &flag, beta, first, iter)) return;
+  } else {
+    first = st->first != 0;
+    beta = (T)st->beta;
+  }
   const int64_t R1 = min(d.n, (tile + 1) * kTile);
   const int64_t i0 = tile * kTile + 2 * threadIdx.x;
   typedef T V2 __attribute__((ext_vector_type(2)));
@@ -1066,7 +1224,7 @@ void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles,
     const int64_t nq = 4 * nlaunch;
     const int remap = nq >= 64;
     const unsigned gq = (unsigned)(remap ? (nq + 7) / 8 * 8 : nq);
-    k_cg_spmv<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap);
+    k_cg_spmv<T><<<gq, kBlock, lds, c->stream>>>(sys, tiles, nq, remap, stopped);
   }
   k_cg_update<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
 }
@@ -1094,8 +1252,10 @@ void batch_launch_cg_persist(Ctx *c, const SysDesc<T> *sys, GridSync *gs, int gr
 }
 
 template <class T>
-void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch) {
-  if (nlaunch > 0) k_cg_p<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch);
+void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
+                       int32_t *stopped) {
+  if (nlaunch > 0)
+    k_cg_p<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
 }
 
 template <class T>
@@ -1137,8 +1297,10 @@ template void batch_launch_cg_persist<double>(Ctx *, const SysDesc<double> *, Gr
                                               int32_t *, int, int);
 template void batch_launch_cg_persist<float>(Ctx *, const SysDesc<float> *, GridSync *, int,
                                              int32_t *, int, int);
-template void batch_launch_cg_p<double>(Ctx *, const SysDesc<double> *, const TileRef *, int64_t);
-template void batch_launch_cg_p<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);
+template void batch_launch_cg_p<double>(Ctx *, const SysDesc<double> *, const TileRef *, int64_t,
+                                        int32_t *);
+template void batch_launch_cg_p<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t,
+                                       int32_t *);
 template void batch_launch_cg_pq<double>(Ctx *, const SysDesc<double> *, const TileRef *,
                                          int64_t);
 template void batch_launch_cg_pq<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);

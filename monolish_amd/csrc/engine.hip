@@ -233,7 +233,7 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
                ? 1
                : 0;
   d.pq_fused = d.split && spmv_sell_pq_ok(A, val, DTof<T>::v) ? 1 : 0;
-  d.pad2_ = 0;
+  d.lf = cfg.method == kMethodCG && !d.split && nt <= kRedGroups && !c_->use_persistent ? 1 : 0;
   d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
   d.method = cfg.method;
   d.tol_scale = cfg.tol_relative ? cfg.tol : 0.0;
@@ -247,6 +247,8 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   hs.maxiter = cfg.maxiter;
   hs.status = kStatRunning;
   hs.first = 1;
+  hs.fresh = 1;
+  hs.ia = hs.ib = 0;
   h_sys_[slot] = d;
   rel_[slot] = cfg.tol_relative ? 1 : 0;
   pending_.push_back(slot);
@@ -360,7 +362,7 @@ template <class T> void Engine<T>::launch_iters(int iters) {
     if (ntiles_[0]) {
       // p = z + beta p of the split and gather-once systems
       if (!split_slots_.empty() || any_once_)
-        batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
+        batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_);
       if (!split_slots_.empty()) {
         // split form: q = A p per system, then <p, q>
         for (int s : split_slots_) {
@@ -429,7 +431,7 @@ template <class T> void Engine<T>::step(int iters) {
 template <class T> bool Engine<T>::persistent_ok(int slot) const {
   if (!c_->use_persistent || active_.size() != 1 || active_[0] != slot) return false;
   const SysDesc<T> &d = h_sys_[slot];
-  return method_[slot] == kMethodCG && !d.split && !d.once;
+  return method_[slot] == kMethodCG && !d.split && !d.once && !d.lf;
 }
 
 template <class T> void Engine<T>::run_persistent(int slot) {
@@ -488,6 +490,10 @@ template <class T> void Engine<T>::resume(int slot, const SolveState &s) {
   MGX_REQUIRE(s.status == kStatRunning, "engine: resume needs a running state");
   MGX_HIP(hipStreamSynchronize(c_->stream));
   h_states_[slot] = s;
+  // kernel-finished reductions: the next iteration takes beta and zr from
+  // the state instead of the last update's partials
+  h_states_[slot].fresh = 2;
+  h_states_[slot].ia = s.iter;
   MGX_HIP(hipMemcpy(d_states_ + slot, &h_states_[slot], sizeof(SolveState),
                     hipMemcpyHostToDevice));
   reported_[slot] = 0;

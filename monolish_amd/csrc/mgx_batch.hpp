@@ -40,7 +40,11 @@ template <class T> struct SysDesc {
   // split form only: its SpMV also writes the <p, q> tile partials (sliced
   // layout, no long rows), so the separate stage-3 kernel skips it
   int32_t pq_fused;
-  int32_t pad2_;
+  // CG, two-kernel form, at most kRedGroups tiles: no arrival chain; the
+  // kernel after a reduction's producer reads all its tile partials and
+  // finishes it in every block (the canonical order), the system's first
+  // block writing the solve state (see k_cg_spmv, k_cg_p, k_cg_update)
+  int32_t lf;
   double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
 };
 
@@ -66,7 +70,8 @@ template <class T>
 void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
                           int32_t *stopped, int lds_entries, bool any_fused);
 template <class T>
-void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
+void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
+                       int32_t *stopped);
 template <class T>
 void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
 // split form: stage 2 of <p,q> (stage 0, after batch_launch_cg_pq) or of

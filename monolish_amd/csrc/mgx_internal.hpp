@@ -79,6 +79,14 @@ struct SolveState {
   double tol, bnorm;
   int64_t iter, miniter, maxiter;
   int32_t status, first;
+  // CG systems whose kernels finish the reductions themselves (SysDesc::lf):
+  // zr of iteration i in zrs[i & 1]; ia = iterations completed (written by
+  // the update kernel), ib = the running iteration (written by the kernel
+  // that starts it); fresh: 1 = start (p = z), 2 = resumed with beta and zr
+  // given, 0 = continuing (beta from the last update's partials)
+  double zrs[2];
+  int64_t ia, ib;
+  int32_t fresh, pad_;
 };
 
 // ------------------------------------------------------------ handles --
