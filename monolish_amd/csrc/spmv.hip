@@ -230,7 +230,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longrows(
 // indices); on banded and stencil matrices x is gathered from consecutive
 // addresses.
 constexpr int kSlice = 64;
-constexpr int kSellTail = 4;  // tail entries loaded with a row's first tuples
 
 // entries of a row kept in tuples: nt_s tuples at most
 __host__ __device__ constexpr int32_t sell_tupled(int32_t len, int32_t nts, int w) {
@@ -261,26 +260,26 @@ __global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
   }
 }
 
-// the tail entries k0 + [0, kSellTail) of a row (those below rem): values and
+// the tail entries k0 + [0, KT) of a row (those below rem): values and
 // gathered x
-template <class T>
+template <class T, int KT>
 __device__ __forceinline__ void sell_tail_load(int64_t tb, int32_t k0, int32_t rem,
                                                const int32_t *__restrict__ sci,
                                                const T *__restrict__ sval, T *tv,
                                                int32_t *tc) {
 #pragma unroll
-  for (int k = 0; k < kSellTail; k++)
+  for (int k = 0; k < KT; k++)
     if (k0 + k < rem) {
       tv[k] = __builtin_nontemporal_load(sval + tb + (int64_t)(k0 + k) * kSlice);
       tc[k] = __builtin_nontemporal_load(sci + tb + (int64_t)(k0 + k) * kSlice);
     }
 }
 
-template <class T>
+template <class T, int KT>
 __device__ __forceinline__ void sell_tail_add(int32_t k0, int32_t rem, const T *tv,
                                               const T *tx, T &s) {
 #pragma unroll
-  for (int k = 0; k < kSellTail; k++)
+  for (int k = 0; k < KT; k++)
     if (k0 + k < rem) {
       T p = tv[k] * tx[k];
       s = s + p;
@@ -288,7 +287,7 @@ __device__ __forceinline__ void sell_tail_add(int32_t k0, int32_t rem, const T *
 }
 
 // one row of the sliced layout: its tuples eight entries at a time
-// (nontemporal 16-byte loads), the first kSellTail tail entries loaded with
+// (nontemporal 16-byte loads), the first KT tail entries loaded with
 // the first batch, x gathered, the products added in storage order (tuples,
 // then the tail)
 template <class T, int W>
@@ -296,6 +295,7 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
                                       const int32_t *__restrict__ sci,
                                       const T *__restrict__ sval, const T *__restrict__ x) {
   constexpr int QT = 8 / W;  // tuples per step: 8 entries
+  constexpr int KT = W - 1;  // tail entries loaded with the first tuples
   typedef T VT __attribute__((ext_vector_type(W)));
   typedef int32_t VI __attribute__((ext_vector_type(W)));
   const int64_t s2 = 2 * (r / kSlice);
@@ -306,9 +306,9 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
   const int64_t base = tup + (int64_t)l * W, tb = tail + l;
   const VT *v = reinterpret_cast<const VT *>(sval + base);
   const VI *c = reinterpret_cast<const VI *>(sci + base);
-  T tv[kSellTail], tx[kSellTail];
-  int32_t tc[kSellTail];
-  sell_tail_load(tb, 0, rem, sci, sval, tv, tc);
+  T tv[KT], tx[KT];
+  int32_t tc[KT];
+  sell_tail_load<T, KT>(tb, 0, rem, sci, sval, tv, tc);
   T s = 0;
   for (int32_t q0 = 0; q0 < nt; q0 += QT) {
     VT vv[QT];
@@ -324,7 +324,7 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
       }
     if (q0 == 0) {
 #pragma unroll
-      for (int k = 0; k < kSellTail; k++)
+      for (int k = 0; k < KT; k++)
         if (k < rem) tx[k] = x[tc[k]];
     }
 #pragma unroll
@@ -338,16 +338,17 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
   }
   if (nt == 0) {
 #pragma unroll
-    for (int k = 0; k < kSellTail; k++)
+    for (int k = 0; k < KT; k++)
       if (k < rem) tx[k] = x[tc[k]];
   }
-  sell_tail_add(0, rem, tv, tx, s);
-  for (int32_t k0 = kSellTail; k0 < rem; k0 += kSellTail) {  // longer tails
-    sell_tail_load(tb, k0, rem, sci, sval, tv, tc);
+  sell_tail_add<T, KT>(0, rem, tv, tx, s);
+#pragma unroll 1
+  for (int32_t k0 = KT; k0 < rem; k0 += KT) {  // longer tails
+    sell_tail_load<T, KT>(tb, k0, rem, sci, sval, tv, tc);
 #pragma unroll
-    for (int k = 0; k < kSellTail; k++)
+    for (int k = 0; k < KT; k++)
       if (k0 + k < rem) tx[k] = x[tc[k]];
-    sell_tail_add(k0, rem, tv, tx, s);
+    sell_tail_add<T, KT>(k0, rem, tv, tx, s);
   }
   return s;
 }
@@ -360,7 +361,8 @@ __device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, in
                                           const int32_t *__restrict__ sci,
                                           const T *__restrict__ sval, const T *__restrict__ x,
                                           T &sa, T &sb) {
-  constexpr int QT = 8 / W;
+  constexpr int QT = 4 / W > 0 ? 4 / W : 1;  // tuples per step and row: 4 entries
+  constexpr int KT = W - 1;  // tail entries loaded with the first tuples
   typedef T VT __attribute__((ext_vector_type(W)));
   typedef int32_t VI __attribute__((ext_vector_type(W)));
   const int64_t sa2 = 2 * (ra / kSlice), sb2 = 2 * (rb / kSlice);
@@ -375,10 +377,10 @@ __device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, in
   const VI *ca = reinterpret_cast<const VI *>(sci + oa);
   const VT *vb = reinterpret_cast<const VT *>(sval + ob);
   const VI *cb = reinterpret_cast<const VI *>(sci + ob);
-  T tva[kSellTail], txa[kSellTail], tvb[kSellTail], txb[kSellTail];
-  int32_t tca[kSellTail], tcb[kSellTail];
-  sell_tail_load(ta, 0, rema, sci, sval, tva, tca);
-  sell_tail_load(tb, 0, remb, sci, sval, tvb, tcb);
+  T tva[KT], txa[KT], tvb[KT], txb[KT];
+  int32_t tca[KT], tcb[KT];
+  sell_tail_load<T, KT>(ta, 0, rema, sci, sval, tva, tca);
+  sell_tail_load<T, KT>(tb, 0, remb, sci, sval, tvb, tcb);
   sa = 0;
   sb = 0;
   const int32_t N = na > nb ? na : nb;
@@ -403,7 +405,7 @@ __device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, in
     }
     if (q0 == 0) {
 #pragma unroll
-      for (int k = 0; k < kSellTail; k++) {
+      for (int k = 0; k < KT; k++) {
         if (k < rema) txa[k] = x[tca[k]];
         if (k < remb) txb[k] = x[tcb[k]];
       }
@@ -424,23 +426,24 @@ __device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, in
   }
   if (N == 0) {
 #pragma unroll
-    for (int k = 0; k < kSellTail; k++) {
+    for (int k = 0; k < KT; k++) {
       if (k < rema) txa[k] = x[tca[k]];
       if (k < remb) txb[k] = x[tcb[k]];
     }
   }
-  sell_tail_add(0, rema, tva, txa, sa);
-  sell_tail_add(0, remb, tvb, txb, sb);
-  for (int32_t k0 = kSellTail; k0 < rema || k0 < remb; k0 += kSellTail) {  // longer tails
-    sell_tail_load(ta, k0, rema, sci, sval, tva, tca);
-    sell_tail_load(tb, k0, remb, sci, sval, tvb, tcb);
+  sell_tail_add<T, KT>(0, rema, tva, txa, sa);
+  sell_tail_add<T, KT>(0, remb, tvb, txb, sb);
+#pragma unroll 1
+  for (int32_t k0 = KT; k0 < rema || k0 < remb; k0 += KT) {  // longer tails
+    sell_tail_load<T, KT>(ta, k0, rema, sci, sval, tva, tca);
+    sell_tail_load<T, KT>(tb, k0, remb, sci, sval, tvb, tcb);
 #pragma unroll
-    for (int k = 0; k < kSellTail; k++) {
+    for (int k = 0; k < KT; k++) {
       if (k0 + k < rema) txa[k] = x[tca[k]];
       if (k0 + k < remb) txb[k] = x[tcb[k]];
     }
-    sell_tail_add(k0, rema, tva, txa, sa);
-    sell_tail_add(k0, remb, tvb, txb, sb);
+    sell_tail_add<T, KT>(k0, rema, tva, txa, sa);
+    sell_tail_add<T, KT>(k0, remb, tvb, txb, sb);
   }
 }
 
