@@ -410,7 +410,16 @@ def bench_batched(m, ctx, nsys, seed0=4000):
     del mats
     ctx.sync()
     t_setup = time.perf_counter() - t0 - t_gen
-    # fused batch: every system in shared launches
+    # fused batch: every system in shared launches.  One untimed warm-up
+    # batch sizes the library's device pool (its first call allocates the
+    # work-vector slab); the timed batch then allocates nothing.
+    t1 = time.perf_counter()
+    m.batch_solve(ctx, m.METHOD_CG, As, Xs, Bs, tol=1e-10, tol_is_relative=True,
+                  precond=m.PRECOND_JACOBI)
+    t_cold = time.perf_counter() - t1
+    for X in Xs:
+        X.fill(0.0)
+    ctx.sync()
     t1 = time.perf_counter()
     res = m.batch_solve(ctx, m.METHOD_CG, As, Xs, Bs, tol=1e-10, tol_is_relative=True,
                         precond=m.PRECOND_JACOBI)
@@ -450,6 +459,7 @@ def bench_batched(m, ctx, nsys, seed0=4000):
     out = {"workload": f"{nsys} SPD M-matrices, n log-uniform [500,20000] (sum n = {int(sizes.sum())}, "
                        f"nnz = {nnz}), Jacobi-PCG FP64 to 1e-10 relative",
            "batched_solves_per_s": round(nsys / t_batch, 1), "batched_seconds": round(t_batch, 3),
+           "first_call_seconds": round(t_cold, 3),
            "converged": ok, "iterations_min_max": [int(min(iters)), int(max(iters))],
            "one_at_a_time_fused_solves_per_s_est": round(nsys / t_single, 1),
            "plain_fp64_gpu_cg_solves_per_s_est": round(nsys / t_plain, 1),
@@ -715,15 +725,20 @@ def parse_plan(spec: str):
     return dict(row_walk=w, rows_per_block=r, nnz_per_block=z, xcd_remap=x, column_panels=p)
 
 
-def traffic_for(workload: str, kernel: str):
+def traffic_for(workload: str, kernel: str, plan: dict):
     """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 counter summary, or None when that kernel was not profiled."""
+    rocprofv3 counter summary of the same kernel under the same XCD order and
+    column-panel count, or None when that plan was not profiled."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for e in json.load(f).get("entries", []):
-            if e.get("workload") == workload and e.get("kernel") == kernel:
+            ep = e.get("plan", {})
+            if (e.get("workload") == workload and e.get("kernel") == kernel
+                    and ep.get("xcd_remap") == plan.get("xcd_remap")
+                    and ep.get("column_panels") == plan.get("column_panels")
+                    and ep.get("row_walk") == plan.get("row_walk")):
                 return e.get("hbm_bytes_per_launch")
     return None
 
@@ -809,7 +824,7 @@ def main():
     sweep = {"c2a_7pt_192_fp64": head}
     # the kernel the chosen plan runs, and its HBM traffic from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that same kernel
-    traffic = traffic_for("c2a_fp64", head["kernel"])
+    traffic = traffic_for("c2a_fp64", head["kernel"], head["plan"])
     roofline = {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(head["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": head["kernel"], "plan": head["plan"],
@@ -847,7 +862,7 @@ def main():
         sweep["c2b_lognormal_4M_fp32"] = s32
         for key, wl in (("c2a_7pt_192_fp32", "c2a_fp32"), ("c2b_lognormal_4M_fp64", "c2b_fp64"),
                         ("c2b_lognormal_4M_fp32", "c2b_fp32")):
-            sweep[key]["traffic"] = traffic_for(wl, sweep[key]["kernel"])
+            sweep[key]["traffic"] = traffic_for(wl, sweep[key]["kernel"], sweep[key]["plan"])
         del rp2, ci2, v2, x2
         line["spmv_sweep"] = sweep
     if "krylov" in legs:
