@@ -645,6 +645,37 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const Ti
     beta = (T)st->beta;
   }
   const int64_t R1 = min(d.n, (tile + 1) * kTile);
+  if constexpr (sizeof(T) == 4) {
+    if (d.split) {
+      // four elements per thread (16-byte accesses), threads 0..127
+      typedef T V4 __attribute__((ext_vector_type(4)));
+      const int64_t j0 = tile * kTile + 4 * threadIdx.x;
+      if (threadIdx.x >= 128 || j0 >= R1) return;
+      if (j0 + 3 < R1) {
+        const V4 zv = *reinterpret_cast<const V4 *>(d.z + j0);
+        V4 pv = zv;
+        if (!first) {
+          pv = *reinterpret_cast<const V4 *>(d.pa + j0);
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const T be = beta * pv[e];
+            pv[e] = zv[e] + be;
+          }
+        }
+        *reinterpret_cast<V4 *>(d.pa + j0) = pv;
+      } else {
+        for (int64_t i = j0; i < R1; i++) {
+          T pv = d.z[i];
+          if (!first) {
+            const T bp = beta * d.pa[i];
+            pv = d.z[i] + bp;
+          }
+          d.pa[i] = pv;
+        }
+      }
+      return;
+    }
+  }
   const int64_t i0 = tile * kTile + 2 * threadIdx.x;
   typedef T V2 __attribute__((ext_vector_type(2)));
   if (i0 + 1 < R1) {
