@@ -180,6 +180,8 @@ __global__ __launch_bounds__(64 * kFacWaves) void k_factor_level_wave(
     if (lane >= d) incl += t;
   }
   const int32_t excl = incl - pn;
+  bool rl = false;  // IC: the single-group right-looking form ran
+  T dsum = T(0);    // IC: the diagonal's sum (lane nlow)
   for (int32_t j0 = 0; j0 < nlow;) {
     // the rows j0 .. j1-1 that fit the stage (a row of <= 64 entries always does)
     const int32_t base = __shfl(excl, j0, 64);
@@ -207,6 +209,35 @@ __global__ __launch_bounds__(64 * kFacWaves) void k_factor_level_wave(
       val[t] = f[src];
     }
     fac_wave_sync();
+    if (IC && j0 == 0 && j1 == nlow) {
+      // Every row this row reads is staged: as each l_ip becomes final it is
+      // broadcast, and each later entry j (and the diagonal) subtracts its
+      // product at once.  Each entry still subtracts its products in
+      // increasing column order, the same operations as the loop below,
+      // without the owner lane's serial pass over all earlier entries.
+      T acc = v;
+      for (int32_t p = 0; p < nlow; p++) {
+        const T pivot = val[spiv[w][p]];
+        if (lane == p) v = acc / pivot;  // l_ip
+        const T lip = __shfl(v, p, 64);
+        const int32_t cp = __shfl(mycol, p, 64);
+        if (lane > p && lane < nlow) {
+          const int32_t q = fac_find(col, soff[w][lane], spiv[w][lane], cp);
+          if (q >= 0) {
+            const T prod = lip * val[q];
+            acc = acc - prod;
+          }
+        } else if (lane == nlow) {
+          const T prod = lip * lip;
+          acc = acc - prod;
+        }
+      }
+      dsum = acc;
+      rl = true;
+      fac_wave_sync();
+      j0 = j1;
+      continue;
+    }
     for (int32_t j = j0; j < j1; j++) {
       const int32_t sa = soff[w][j - j0], sb = soff[w][j - j0 + 1], pv = spiv[w][j - j0];
       const T pivot = val[pv];
@@ -249,16 +280,21 @@ __global__ __launch_bounds__(64 * kFacWaves) void k_factor_level_wave(
     if (lane == nlow && (v == T(0) || v != v)) atomicMin(bad, i);
     if (lane < len) f[e0 + lane] = v;
   } else {
-    sli[w][lane] = v;
-    fac_wave_sync();
-    if (lane == nlow) {
-      T sum = v;
-      for (int32_t p = 0; p < nlow; p++) {
-        const T prod = sli[w][p] * sli[w][p];
-        sum = sum - prod;
+    if (!rl) {
+      sli[w][lane] = v;
+      fac_wave_sync();
+      if (lane == nlow) {
+        T sum = v;
+        for (int32_t p = 0; p < nlow; p++) {
+          const T prod = sli[w][p] * sli[w][p];
+          sum = sum - prod;
+        }
+        dsum = sum;
       }
-      if (!(sum > (T)0)) atomicMin(bad, i);
-      v = sqrt(sum);
+    }
+    if (lane == nlow) {
+      if (!(dsum > (T)0)) atomicMin(bad, i);
+      v = sqrt(dsum);
     }
     if (lane <= nlow) f[e0 + lane] = v;
   }
