@@ -137,7 +137,10 @@ __device__ __forceinline__ void cg_finish(const SysDesc<T> &d, SolveState *st, T
 // zrs[0] and zrs[1]).
 
 // lane t's stage-2 value of one reduction over nt tile partials at P (Q = 4:
-// quarter partials, the tile partial rebuilt as (W0 + W1) + (W2 + W3))
+// quarter partials, the tile partial rebuilt as (W0 + W1) + (W2 + W3)).  The
+// partials were written by an earlier kernel of the stream, so plain
+// (cached) loads see them; only partials of the same kernel need the
+// device-coherent loads of load_partial.
 template <class T, int Q>
 __device__ __forceinline__ T lf_lane(const T *P, int64_t nt) {
   T lane = T(0);
@@ -145,13 +148,13 @@ __device__ __forceinline__ T lf_lane(const T *P, int64_t nt) {
   if (k < nt) {
     T p;
     if (Q == 4) {
-      const T w0 = load_partial(P, 4 * k), w1 = load_partial(P, 4 * k + 1);
-      const T w2 = load_partial(P, 4 * k + 2), w3 = load_partial(P, 4 * k + 3);
+      const T w0 = P[4 * k], w1 = P[4 * k + 1];
+      const T w2 = P[4 * k + 2], w3 = P[4 * k + 3];
       const T lo = w0 + w1;
       const T hi = w2 + w3;
       p = lo + hi;
     } else {
-      p = load_partial(P, k);
+      p = P[k];
     }
     lane = lane + p;
   }
