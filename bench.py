@@ -489,7 +489,7 @@ def bench_batched(m, ctx, nsys, seed0=4000):
     return out
 
 
-def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10.0,
+def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60.0,
                  cg_mode=None):
     """Poisson arrivals; 50% 2D/3D Poisson (CG), 50% 2D upwind convection-
     diffusion with Peclet U[1,50] (BiCGSTAB); Jacobi, 1e-10 relative; sizes
@@ -506,6 +506,7 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
         cg_mode = m.MODE_FUSED
     rng = np.random.default_rng(seed)
     systems = []
+    nnzs = []
     for k in range(pool):
         n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
         if k % 2 == 0:
@@ -523,11 +524,13 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
         nn = len(rp) - 1
         b = m.random_vector(nn, -1.0, 1.0, seed + k)
         systems.append((meth, m.CRS(ctx, nn, nn, rp, ci, v), m.Vector(ctx, b), nn))
+        nnzs.append(len(v))
     ctx.sync()
     srv = m.Server(ctx, max_active=1024, iters_per_poll=8)
     # the first request of a few systems keeps its solution, for the check
     # that a result in the mix equals the same system solved alone
     keep = {}
+    sys_of = []
     t0 = time.perf_counter()
     t = 0.0
     tickets = []
@@ -547,8 +550,10 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
         mode = cg_mode if meth == m.METHOD_CG else m.MODE_FUSED
         tickets.append(srv.submit_resident(meth, A, B, X, tol=1e-10, tol_is_relative=True,
                                            mode=mode))
+        sys_of.append(k)
     t_submitted = time.perf_counter() - t0
     statuses = [None] * len(tickets)
+    iters = [0] * len(tickets)
     cutoff = t0 + seconds + drain
     late = []
     for i, tk in enumerate(tickets):
@@ -557,13 +562,24 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
         if got is None:
             late.append(i)
         else:
-            statuses[i] = got[0].status
+            statuses[i], iters[i] = got[0].status, got[0].final_iter
     # past the cutoff: cancel every unfinished request at once, then collect
     for i in late:
         srv.cancel(tickets[i])
     for i in late:
-        statuses[i] = srv.wait(tickets[i], timeout=60.0)[0].status
+        got = srv.wait(tickets[i], timeout=60.0)[0]
+        statuses[i], iters[i] = got.status, got.final_iter
     wall = time.perf_counter() - t0
+    # a floor on the HBM traffic of the work done: the SpMV bytes of every
+    # iteration run (one SpMV per CG iteration, two per BiCGSTAB iteration;
+    # the inner iterations of FP32-refined CG requests at FP32 sizes),
+    # cancelled requests included
+    spmv_bytes_done = 0
+    for i, k in enumerate(sys_of):
+        meth, _, _, nn = systems[k]
+        sv = 4 if meth == m.METHOD_CG and cg_mode == m.MODE_MIXED_IR else 8
+        per = spmv_bytes(nnzs[k], nn, nn, sv) * (1 if meth == m.METHOD_CG else 2)
+        spmv_bytes_done += per * max(iters[i], 0)
     st = srv.stats()
     srv.close()
     # kept requests that converged: the result in the mix equals the lone
@@ -601,6 +617,7 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=10
            "p50_latency_ms": round(1e3 * st["p50_latency_s"], 2),
            "p99_latency_ms": round(1e3 * st["p99_latency_s"], 2),
            "mean_queue_ms": round(1e3 * st["mean_queue_s"], 2), "wall_s": round(wall, 2),
+           "spmv_GB_s_floor": round(spmv_bytes_done / wall / 1e9, 1),
            "peak_resident_MB": round(st["peak_bytes"] / 2 ** 20, 1),
            "result_in_mix_equals_alone": same if compared else None,
            "results_compared": compared}
@@ -771,9 +788,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-nmax", type=int, default=1000000,
                     help="largest served system (rows)")
-    ap.add_argument("--serve-drain", type=float, default=10.0,
+    ap.add_argument("--serve-drain", type=float, default=60.0,
                     help="seconds after the last arrival before unfinished requests are "
-                         "cancelled")
+                         "cancelled (the default lets the overloaded stream finish, so "
+                         "its solves/s is the capacity for the whole mix)")
     ap.add_argument("--plan", default="",
                     help="fix the headline SpMV plan instead of autotuning: "
                          "'row_walk,rows_per_block,nnz_per_block,xcd_remap,column_panels' "
@@ -899,10 +917,10 @@ def main():
                                                         args.serve_seconds, 1000,
                                                         args.serve_nmax, seed=5002,
                                                         drain=args.serve_drain)
-        # the same stream with the CG requests refined in FP32 (a one-second
+        # the same stream with the CG requests refined in FP32 (a half-second
         # window keeps the default run short)
         line["served"]["fp32_refined_cg"] = bench_served(m, ctx, 2000.0,
-                                                         min(args.serve_seconds, 1.0), 1000,
+                                                         min(args.serve_seconds, 0.5), 1000,
                                                          args.serve_nmax, drain=args.serve_drain,
                                                          cg_mode=m.MODE_MIXED_IR)
     if "ingest" in legs:

@@ -82,8 +82,10 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
  * short rows' entries stored a second time, panel-major, so each pass
  * gathers x from an L2-sized slice), and, where 64-row slices pad the
  * entries by at most 30 %, the sliced layout (entries stored a second time
- * in 64-row slices of 16-byte tuples, one lane per row, no LDS) under both
- * XCD orders -- `reps` launches each; keeps the fastest for A and reports it
+ * in 64-row slices of 16-byte tuples, one lane per row, no LDS) in launch
+ * order, with one range of row blocks per XCD, and with runs of 16 row
+ * blocks dealt to the XCDs in turn -- `reps` launches each; keeps the
+ * fastest for A and reports it
  * (within 2 %, an XCD-ordered block plan).  The long-row threshold is
  * unchanged, so results do not depend on the choice.  The column-panel and
  * L2-panel copies are dropped by mgx_csr_update_values; the sliced copy is
@@ -104,7 +106,9 @@ typedef struct mgx_spmv_tuning {
 int mgx_csr_autotune(mgx_csr *A, int reps, mgx_spmv_tuning *out);
 /* Sets A's SpMV plan explicitly (the candidates the autotuner times):
  * rows_per_block in [1, 256], nnz_per_block in [64, 8192] (<= 4096 with
- * row_walk), xcd_remap 0/1, row_walk 0/1 (block kind), 2 (the sliced
+ * row_walk), xcd_remap 0 (launch order), 1 (one range of blocks per XCD)
+ * or a run length in [2, 1024] (runs of that many consecutive blocks dealt
+ * to the XCDs in turn), row_walk 0/1 (block kind), 2 (the sliced
  * layout, over a product-staged block plan kept for other values; refused
  * when its padding exceeds 30 %) or 3 (L2 panels: column_panels in
  * [1, 65536], at most 2^20 columns each, ascending columns within rows),

@@ -397,7 +397,9 @@ class CRS:
     def set_plan(self, rows_per_block=256, nnz_per_block=2048, xcd_remap=True, row_walk=0,
                  column_panels=0):
         """Fix the SpMV block plan (the autotuner's candidates).  row_walk: 0
-        products staged in LDS in entry order, 1 row walk."""
+        products staged in LDS in entry order, 1 row walk, 2 sliced layout, 3
+        L2 panels.  xcd_remap: 0 launch order, 1 one range of blocks per XCD,
+        c in [2, 1024] runs of c blocks dealt to the XCDs in turn."""
         check(_lib.mgx_csr_set_plan(self._h, rows_per_block, nnz_per_block, int(xcd_remap),
                                     int(row_walk), column_panels))
 

@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
   const SysDesc<T> &d = locate(sys, tiles, b >> 2, tile);
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
-  if (d.split) return;  // split iteration: plan-based SpMV and k_cg_pq instead
+  // split-form systems are not listed: batch_launch_split runs them
   const bool once = d.once != 0;  // p already formed in pa by k_cg_p
   bool first;
   T beta;
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
     alpha = (T)st->alpha;
   }
   const bool odd = (iter & 1) != 0;
-  const T *p_new = odd || d.split || d.once ? d.pa : d.pb;
+  const T *p_new = odd || d.once ? d.pa : d.pb;
   const T nalpha = -alpha;
   const int t = threadIdx.x;
   const int64_t R0 = d.row0 + tile * kTile;
@@ -409,7 +409,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const SysDesc<T> *sys,
     publish_partial(d.P, tile, zr_a);
     publish_partial(d.P, d.ntiles + tile, rr_a);
   }
-  if (d.split) return;  // k_cg_finish reduces the partials of the split form
   if (d.lf) {
     // the next iteration's opening kernel finishes <z,r> and <r,r>
     if (t == 0 && tile == 0) {
@@ -621,9 +620,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_persist(const SysDesc<T> *sys, Gr
   }
 }
 
-// split iteration (and the gather-once two-kernel form), stage 1: p = z in
-// the first iteration, else p = z + beta p (xpay, cg.cpp:76), in pa; stage
-// 2 is q = A p by the plan-based SpMV (k_cg_spmv for the gather-once form)
+// gather-once two-kernel form, stage 1: p = z in the first iteration, else
+// p = z + beta p (xpay, cg.cpp:76), in pa; k_cg_spmv then forms q = A p
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const TileRef *tiles,
                                                  int64_t nlaunch, int32_t *stopped) {
@@ -631,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const Ti
   if (b >= nlaunch) return;
   int64_t tile;
   const SysDesc<T> &d = locate(sys, tiles, b, tile);
-  if (!d.split && !d.once) return;
+  if (!d.once) return;
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   bool first;
@@ -648,37 +646,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const Ti
     beta = (T)st->beta;
   }
   const int64_t R1 = min(d.n, (tile + 1) * kTile);
-  if constexpr (sizeof(T) == 4) {
-    if (d.split) {
-      // four elements per thread (16-byte accesses), threads 0..127
-      typedef T V4 __attribute__((ext_vector_type(4)));
-      const int64_t j0 = tile * kTile + 4 * threadIdx.x;
-      if (threadIdx.x >= 128 || j0 >= R1) return;
-      if (j0 + 3 < R1) {
-        const V4 zv = *reinterpret_cast<const V4 *>(d.z + j0);
-        V4 pv = zv;
-        if (!first) {
-          pv = *reinterpret_cast<const V4 *>(d.pa + j0);
-#pragma unroll
-          for (int e = 0; e < 4; e++) {
-            const T be = beta * pv[e];
-            pv[e] = zv[e] + be;
-          }
-        }
-        *reinterpret_cast<V4 *>(d.pa + j0) = pv;
-      } else {
-        for (int64_t i = j0; i < R1; i++) {
-          T pv = d.z[i];
-          if (!first) {
-            const T bp = beta * d.pa[i];
-            pv = d.z[i] + bp;
-          }
-          d.pa[i] = pv;
-        }
-      }
-      return;
-    }
-  }
   const int64_t i0 = tile * kTile + 2 * threadIdx.x;
   typedef T V2 __attribute__((ext_vector_type(2)));
   if (i0 + 1 < R1) {
@@ -702,33 +669,190 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc<T> *sys, const Ti
   }
 }
 
-// split iteration, stage 3: tile partials of <p, q>
+// ------------------------------------------- split form, one system ----
+// The split form's streaming stages launched for one system at a time, its
+// descriptor passed by value (a block starts without loading it through the
+// tile list) and kSplitTiles consecutive tiles per block, every load of
+// those tiles issued before any arithmetic.  Each tile's partials are the
+// ones the tile-list kernels form (lane t: elements 2t and 2t+1, the sum of
+// the two, then the block tree), so the bits do not change.  Work vectors
+// start at row 0 of the system (row0 is 0 for every engine system).
+
+// p = z + beta p (cg.cpp:75-78), 16-byte accesses
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_pq(const SysDesc<T> *sys, const TileRef *tiles,
-                                                  int64_t nlaunch) {
-  __shared__ T sh[8];
-  const int64_t b = blockIdx.x;
-  if (b >= nlaunch) return;
-  int64_t tile;
-  const SysDesc<T> &d = locate(sys, tiles, b, tile);
-  if (!d.split || d.pq_fused) return;  // pq_fused: written by its SpMV
-  SolveState *st = d.st;
+__global__ __launch_bounds__(kBlock) void k_split_p(const SysDesc<T> d) {
+  const SolveState *st = d.st;
   if (st->status != kStatRunning) return;
-  const int64_t R1 = min(d.n, (tile + 1) * kTile);
-  const int64_t i0 = tile * kTile + 2 * threadIdx.x;
-  typedef T V2 __attribute__((ext_vector_type(2)));
-  T p0 = 0, p1 = 0;
-  if (i0 + 1 < R1) {
-    const V2 pv = *reinterpret_cast<const V2 *>(d.pa + i0);
-    const V2 qv = *reinterpret_cast<const V2 *>(d.q + i0);
-    p0 = pv[0] * qv[0];
-    p1 = pv[1] * qv[1];
-  } else if (i0 < R1) {
-    p0 = d.pa[i0] * d.q[i0];
+  const bool first = st->first != 0;
+  const T beta = (T)st->beta;
+  constexpr int V = 16 / (int)sizeof(T);
+  typedef T VT __attribute__((ext_vector_type(V)));
+  constexpr int NV = kSplitTiles * kTile / V;  // vectors per block
+  constexpr int PER = (NV + kBlock - 1) / kBlock;
+  const int64_t e0 = (int64_t)blockIdx.x * kSplitTiles * kTile;
+  VT zv[PER], pv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int v = k * kBlock + (int)threadIdx.x;
+    const int64_t j = e0 + (int64_t)v * V;
+    if (v < NV && j + V <= d.n) {
+      zv[k] = *reinterpret_cast<const VT *>(d.z + j);
+      if (!first) pv[k] = *reinterpret_cast<const VT *>(d.pa + j);
+    }
   }
-  const T s = block_tree(p0 + p1, sh);
-  if (threadIdx.x == 0) publish_partial(d.P, tile, s);
-  // k_cg_finish (stage 0) adds the tile partials and computes alpha
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int v = k * kBlock + (int)threadIdx.x;
+    const int64_t j = e0 + (int64_t)v * V;
+    if (v >= NV || j >= d.n) continue;
+    if (j + V <= d.n) {
+      VT o = zv[k];
+      if (!first) {
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+          const T be = beta * pv[k][e];
+          o[e] = zv[k][e] + be;
+        }
+      }
+      *reinterpret_cast<VT *>(d.pa + j) = o;
+    } else {
+      for (int64_t i = j; i < d.n; i++) {
+        T pi = d.z[i];
+        if (!first) {
+          const T bp = beta * d.pa[i];
+          pi = d.z[i] + bp;
+        }
+        d.pa[i] = pi;
+      }
+    }
+  }
+}
+
+// the per-wave sums of kSplitTiles tiles' lane values (a: slots 0..3, b:
+// slots 4..7 of each tile), then the tiles' block sums published by threads
+// 0..kSplitTiles-1 in block_tree2's order
+template <class T, int NR>
+__device__ __forceinline__ void split_publish(const SysDesc<T> &d, const T (*lanev)[kSplitTiles],
+                                              T *sh) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kSplitTiles; k++)
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const T w = wave_tree(lanev[r][k]);
+      if (lane == 0) sh[k * 8 + 4 * r + wv] = w;
+    }
+  __syncthreads();
+  const int64_t tile = (int64_t)blockIdx.x * kSplitTiles + threadIdx.x;
+  if ((int)threadIdx.x < kSplitTiles && tile < d.ntiles) {
+    const T *w = sh + threadIdx.x * 8;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const T lo = w[4 * r] + w[4 * r + 1], hi = w[4 * r + 2] + w[4 * r + 3];
+      publish_partial(d.P, r * d.ntiles + tile, lo + hi);
+    }
+  }
+}
+
+// x += alpha p, r -= alpha q, z = M r; tile partials of <z, r> and <r, r>
+// (cg.cpp:60-73), as k_cg_update for a split system
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_split_update(const SysDesc<T> d) {
+  __shared__ T sh[kSplitTiles * 8];
+  const SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  const T alpha = (T)st->alpha;
+  const T nalpha = -alpha;
+  const int t = threadIdx.x;
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  V2 pv[kSplitTiles], qv[kSplitTiles], xv[kSplitTiles], rv[kSplitTiles], mv[kSplitTiles];
+  bool full[kSplitTiles];
+#pragma unroll
+  for (int k = 0; k < kSplitTiles; k++) {
+    const int64_t tile = (int64_t)blockIdx.x * kSplitTiles + k;
+    const int64_t i0 = tile * kTile + 2 * t;
+    full[k] = d.vec && tile < d.ntiles && i0 + 1 < d.n;
+    if (full[k]) {
+      pv[k] = *reinterpret_cast<const V2 *>(d.pa + i0);
+      qv[k] = *reinterpret_cast<const V2 *>(d.q + i0);
+      xv[k] = *reinterpret_cast<const V2 *>(d.x + i0);
+      rv[k] = *reinterpret_cast<const V2 *>(d.r + i0);
+      if (d.M) mv[k] = *reinterpret_cast<const V2 *>(d.M + i0);
+    }
+  }
+  T lanev[2][kSplitTiles];
+#pragma unroll
+  for (int k = 0; k < kSplitTiles; k++) {
+    const int64_t tile = (int64_t)blockIdx.x * kSplitTiles + k;
+    const int64_t i0 = tile * kTile + 2 * t;
+    T zr2[2] = {0, 0}, rr2[2] = {0, 0};
+    if (full[k]) {
+      V2 zv;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        T t1 = alpha * pv[k][e];
+        xv[k][e] = t1 + xv[k][e];
+        T t2 = nalpha * qv[k][e];
+        T r_ = t2 + rv[k][e];
+        rv[k][e] = r_;
+        T z_ = d.M ? mv[k][e] * r_ : r_;
+        zv[e] = z_;
+        zr2[e] = z_ * r_;
+        rr2[e] = r_ * r_;
+      }
+      *reinterpret_cast<V2 *>(d.x + i0) = xv[k];
+      *reinterpret_cast<V2 *>(d.r + i0) = rv[k];
+      *reinterpret_cast<V2 *>(d.z + i0) = zv;
+    } else if (tile < d.ntiles) {
+      const int64_t R1 = min(d.n, (tile + 1) * kTile);
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int64_t i = i0 + e;
+        if (i < R1) {
+          T pi = d.pa[i], qi = d.q[i];
+          T t1 = alpha * pi;
+          d.x[i] = t1 + d.x[i];
+          T t2 = nalpha * qi;
+          T ri = t2 + d.r[i];
+          d.r[i] = ri;
+          T zi = d.M ? d.M[i] * ri : ri;
+          d.z[i] = zi;
+          zr2[e] = zi * ri;
+          rr2[e] = ri * ri;
+        }
+      }
+    }
+    lanev[0][k] = zr2[0] + zr2[1];
+    lanev[1][k] = rr2[0] + rr2[1];
+  }
+  split_publish<T, 2>(d, lanev, sh);
+}
+
+// tile partials of <p, q> when the SpMV did not form them
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_split_pq(const SysDesc<T> d) {
+  __shared__ T sh[kSplitTiles * 8];
+  const SolveState *st = d.st;
+  if (st->status != kStatRunning) return;
+  const int t = threadIdx.x;
+  T lanev[1][kSplitTiles];
+#pragma unroll
+  for (int k = 0; k < kSplitTiles; k++) {
+    const int64_t tile = (int64_t)blockIdx.x * kSplitTiles + k;
+    const int64_t R1 = min(d.n, (tile + 1) * kTile);
+    const int64_t i0 = tile * kTile + 2 * t;
+    T p0 = 0, p1 = 0;
+    if (tile < d.ntiles) {
+      if (i0 + 1 < R1) {
+        p0 = d.pa[i0] * d.q[i0];
+        p1 = d.pa[i0 + 1] * d.q[i0 + 1];
+      } else if (i0 < R1) {
+        p0 = d.pa[i0] * d.q[i0];
+      }
+    }
+    lanev[0][k] = p0 + p1;
+  }
+  split_publish<T, 1>(d, lanev, sh);
 }
 
 // ---------------------------------------------------------- BiCGSTAB ----
@@ -1292,9 +1416,15 @@ void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int6
     k_cg_p<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch, stopped);
 }
 
-template <class T>
-void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch) {
-  if (nlaunch > 0) k_cg_pq<T><<<(unsigned)nlaunch, kBlock, 0, c->stream>>>(sys, tiles, nlaunch);
+template <class T> void batch_launch_split(Ctx *c, const SysDesc<T> &d, int stage) {
+  const unsigned grid = (unsigned)((d.ntiles + kSplitTiles - 1) / kSplitTiles);
+  if (grid == 0) return;
+  if (stage == 0)
+    k_split_p<T><<<grid, kBlock, 0, c->stream>>>(d);
+  else if (stage == 1)
+    k_split_pq<T><<<grid, kBlock, 0, c->stream>>>(d);
+  else
+    k_split_update<T><<<grid, kBlock, 0, c->stream>>>(d);
 }
 
 template <class T>
@@ -1335,9 +1465,8 @@ template void batch_launch_cg_p<double>(Ctx *, const SysDesc<double> *, const Ti
                                         int32_t *);
 template void batch_launch_cg_p<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t,
                                        int32_t *);
-template void batch_launch_cg_pq<double>(Ctx *, const SysDesc<double> *, const TileRef *,
-                                         int64_t);
-template void batch_launch_cg_pq<float>(Ctx *, const SysDesc<float> *, const TileRef *, int64_t);
+template void batch_launch_split<double>(Ctx *, const SysDesc<double> &, int);
+template void batch_launch_split<float>(Ctx *, const SysDesc<float> &, int);
 template void batch_launch_cg_finish<double>(Ctx *, const SysDesc<double> *, const int32_t *, int,
                                              int, int32_t *);
 template void batch_launch_cg_finish<float>(Ctx *, const SysDesc<float> *, const int32_t *, int,

@@ -313,17 +313,17 @@ template <class T> void Engine<T>::rebuild_lists() {
   any_fused_ = false;
   any_once_ = false;
   any_bcg_once_ = false;
-  any_pq_unfused_ = false;
   split_slots_.clear();
   for (int s : active_) {
     if (reported_[s]) continue;
     const int m = method_[s];
     if (m == kMethodCG) {
       if (h_sys_[s].split) {
+        // launched one system at a time (batch_launch_split), not listed
         split_slots_.push_back(s);
-        any_pq_unfused_ = any_pq_unfused_ || !h_sys_[s].pq_fused;
-      } else
-        any_fused_ = true;
+        continue;
+      }
+      any_fused_ = true;
       any_once_ = any_once_ || h_sys_[s].once;
     } else {
       any_bcg_once_ = any_bcg_once_ || h_sys_[s].once;
@@ -359,34 +359,34 @@ template <class T> void Engine<T>::rebuild_lists() {
 
 template <class T> void Engine<T>::launch_iters(int iters) {
   for (int i = 0; i < iters; i++) {
-    if (ntiles_[0]) {
-      // p = z + beta p of the split and gather-once systems
-      if (!split_slots_.empty() || any_once_)
-        batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_);
-      if (!split_slots_.empty()) {
-        // split form: q = A p per system, then <p, q>
-        for (int s : split_slots_) {
-          const SysDesc<T> &h = h_sys_[s];
-          if (h.pq_fused) {
-            // the sliced layout was there when the system was added; the
-            // stage-3 kernel skips this system, so the fused SpMV must run
-            MGX_REQUIRE(spmv_sell_pq(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, h.P,
-                                     &d_states_[s].status),
-                        "solver: the matrix's SpMV plan changed during a solve");
-          } else {
-            launch_spmv(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, 1.0, 0.0, false,
-                        &d_states_[s].status);
-          }
+    if (!split_slots_.empty()) {
+      // split form, system by system: p = z + beta p; q = A p with <p, q>
+      for (int s : split_slots_) batch_launch_split<T>(c_, h_sys_[s], 0);
+      for (int s : split_slots_) {
+        const SysDesc<T> &h = h_sys_[s];
+        if (h.pq_fused) {
+          // the sliced layout was there when the system was added, so the
+          // fused SpMV forms the <p, q> partials
+          MGX_REQUIRE(spmv_sell_pq(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, h.P,
+                                   &d_states_[s].status),
+                      "solver: the matrix's SpMV plan changed during a solve");
+        } else {
+          launch_spmv(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, 1.0, 0.0, false,
+                      &d_states_[s].status);
+          batch_launch_split<T>(c_, h, 1);
         }
-        if (any_pq_unfused_) batch_launch_cg_pq<T>(c_, d_sys_, d_tiles_[0], ntiles_[0]);
-        batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 0,
-                                  d_stopped_);
       }
+      batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 0,
+                                d_stopped_);
+      for (int s : split_slots_) batch_launch_split<T>(c_, h_sys_[s], 2);
+      batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 1,
+                                d_stopped_);
+    }
+    if (ntiles_[0]) {
+      // p = z + beta p of the gather-once systems
+      if (any_once_) batch_launch_cg_p<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_);
       batch_launch_cg_iter<T>(c_, d_sys_, d_tiles_[0], ntiles_[0], d_stopped_, lds_entries_,
                               any_fused_);
-      if (!split_slots_.empty())
-        batch_launch_cg_finish<T>(c_, d_sys_, d_split_slots_, (int)split_slots_.size(), 1,
-                                  d_stopped_);
     }
     if (ntiles_[1])
       batch_launch_bicgstab_iter<T>(c_, d_sys_, d_tiles_[1], ntiles_[1], d_stopped_,

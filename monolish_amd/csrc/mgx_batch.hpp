@@ -62,9 +62,9 @@ constexpr int64_t kSplitRows = 1 << 19;
 // distinct 128-byte lines of x (sampled at upload) use the gather-once form
 constexpr double kScatteredLines = 24.0;
 
-// one CG iteration of the listed systems: the fused SpMV stage (when a
-// listed system uses the two-kernel form) and the update stage; systems of
-// the split form need batch_launch_cg_p, their SpMV and batch_launch_cg_pq
+// one CG iteration of the listed systems (split-form systems are not
+// listed): the fused SpMV stage (when a listed system uses the two-kernel
+// form) and the update stage; gather-once systems need batch_launch_cg_p
 // first
 template <class T>
 void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
@@ -72,11 +72,14 @@ void batch_launch_cg_iter(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, i
 template <class T>
 void batch_launch_cg_p(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch,
                        int32_t *stopped);
-template <class T>
-void batch_launch_cg_pq(Ctx *c, const SysDesc<T> *sys, const TileRef *tiles, int64_t nlaunch);
-// split form: stage 2 of <p,q> (stage 0, after batch_launch_cg_pq) or of
-// <z,r>, <r,r> with the convergence test (stage 1, after the update), one
-// block per listed split system
+// split form, one system per launch (descriptor by value, kSplitTiles tiles
+// per block): stage 0 p = z + beta p, 1 the <p,q> tile partials (when its
+// SpMV does not form them), 2 the update with the <z,r>, <r,r> partials
+constexpr int kSplitTiles = 2;
+template <class T> void batch_launch_split(Ctx *c, const SysDesc<T> &d, int stage);
+// split form: stage 2 of <p,q> (stage 0, after the SpMV) or of <z,r>,
+// <r,r> with the convergence test (stage 1, after the update), one block
+// per listed split system
 template <class T>
 void batch_launch_cg_finish(Ctx *c, const SysDesc<T> *sys, const int32_t *slots, int nslots,
                             int stage, int32_t *stopped);
@@ -242,7 +245,6 @@ template <class T> class Engine {
   bool any_fused_ = false;        // a listed CG system uses the two-kernel form
   bool any_once_ = false;         // ... in its gather-once variant
   bool any_bcg_once_ = false;     // a listed BiCGSTAB system in the gather-once form
-  bool any_pq_unfused_ = false;   // a listed split system needs the stage-3 kernel
   std::vector<int> split_slots_;  // listed CG systems of the split form
   int32_t *d_split_slots_ = nullptr;
   int64_t split_cap_ = 0;

@@ -365,13 +365,33 @@ template <class T> constexpr int stream_u(int entries) {
 }
 
 // workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
-// XCD i % 8).  Remapping gives each XCD a contiguous range of logical blocks
-// so neighbouring rows share that XCD's L2.  grid must be a multiple of 8.
+// XCD i % 8).  remap 1 gives each XCD one contiguous range of logical blocks
+// (grid a multiple of 8); remap c > 1 gives each XCD runs of c consecutive
+// logical blocks, the runs dealt to the XCDs in turn (grid a multiple of
+// 8 c), so the rows a run shares with its neighbour runs are fetched into
+// fewer L2s while all XCDs still stream the matrix front to back together.
+// remap_grid() sizes the grid.
 __device__ __forceinline__ int64_t xcd_block(int remap) {
   const int64_t g = gridDim.x, b = blockIdx.x;
   if (!remap) return b;
+  if (remap > 1) {
+    const int64_t c = remap, k = b >> 3;
+    return ((k / c) * 8 + (b & 7)) * c + k % c;
+  }
   const int64_t per = g >> 3;
   return (b & 7) * per + (b >> 3);
+}
+
+// the remap actually used for nb logical blocks (none below 64 blocks) and
+// the grid it needs; blocks past nb return at once
+struct RemapGrid {
+  int remap;
+  unsigned grid;
+};
+inline RemapGrid remap_grid(int remap, int64_t nb) {
+  const int r = nb >= 64 ? remap : 0;
+  const int64_t q = r > 1 ? 8 * (int64_t)r : 8;
+  return {r, (unsigned)(r ? (nb + q - 1) / q * q : nb)};
 }
 
 template <class T> struct vec2;

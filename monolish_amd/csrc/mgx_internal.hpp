@@ -59,6 +59,7 @@ template <class F> int guard(F &&f) {
 // ------------------------------------------------------- device constants --
 constexpr int kBlock = 256;      // threads per block for every kernel
 constexpr int kTile = 512;       // canonical reduction tile (elements)
+constexpr int kMaxRemapRun = 1024;  // longest run of blocks per XCD in a plan
 constexpr int kQuarter = 128;    // rows of one wave's share of a tile
 constexpr int kFusedNnz = 2048;  // entry budget of a fused-SpMV chunk
 constexpr int kStatRunning = 1;  // solver status while iterating
@@ -161,7 +162,8 @@ struct SpmvPlan {
   int32_t nnz_max = 0;  // entry budget of a stream block (LDS products)
   int32_t rows_max = 0;  // row budget of a stream block
   int32_t long_row = 0;  // rows longer than this get a block of their own
-  int32_t remap = 1;     // XCD-aware block order
+  int32_t remap = 1;     // block order: 0 launch order, 1 one range per XCD,
+                         // c > 1 runs of c blocks dealt to the XCDs in turn
   int32_t walk = 0;      // stream blocks: 0 product-staged, 1 row walk
   double mean_row = 0;
   double tuned_us = 0;   // SpMV time measured by the autotuner (0: not tuned)

@@ -320,6 +320,8 @@ int mgx_csr_set_plan(mgx_csr *A, int rows_per_block, int nnz_per_block, int xcd_
     MGX_REQUIRE(rows_per_block >= 1 && rows_per_block <= kBlock,
                 "rows_per_block must be in [1, %d]", kBlock);
     MGX_REQUIRE(row_walk >= 0 && row_walk <= 3, "row_walk must be 0, 1, 2 or 3");
+    MGX_REQUIRE(xcd_remap >= 0 && xcd_remap <= kMaxRemapRun,
+                "xcd_remap must be 0, 1 or a run length in [2, %d]", kMaxRemapRun);
     // the row walk stages 12 bytes per FP64 entry in LDS: 4096 entries fit 64 KiB
     MGX_REQUIRE(nnz_per_block >= 64 && nnz_per_block <= (row_walk == 1 ? 4096 : 8192),
                 "nnz_per_block must be in [64, %d]", row_walk == 1 ? 4096 : 8192);

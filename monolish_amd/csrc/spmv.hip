@@ -353,138 +353,47 @@ __device__ __forceinline__ T sell_row(int64_t r, int32_t len, const int64_t *__r
   return s;
 }
 
-// two rows of the sliced layout at once (their loads interleaved), each
-// summed exactly as sell_row sums it; a row of length 0 gives 0
+// q = A p in the sliced layout for a matrix without long rows, one block of
+// 512 lanes per 512-row tile (a lane per row, as k_spmv_sell), and the tile
+// partial of <p, q> exactly as the CG engine's split-form kernels form it
+// (lane t < 256: p[2t] q[2t] + p[2t+1] q[2t+1], then the block tree over
+// those 256 lanes), so no separate pass is needed.  Two rows per lane (loads
+// of both interleaved) took 72 VGPRs against 56 and measured slower on the
+// 27-point stencil
 template <class T, int W>
-__device__ __forceinline__ void sell_row2(int64_t ra, int32_t la, int64_t rb, int32_t lb,
-                                          const int64_t *__restrict__ soff,
-                                          const int32_t *__restrict__ sci,
-                                          const T *__restrict__ sval, const T *__restrict__ x,
-                                          T &sa, T &sb) {
-  constexpr int QT = 4 / W > 0 ? 4 / W : 1;  // tuples per step and row: 4 entries
-  constexpr int KT = W - 1;  // tail entries loaded with the first tuples
-  typedef T VT __attribute__((ext_vector_type(W)));
-  typedef int32_t VI __attribute__((ext_vector_type(W)));
-  const int64_t sa2 = 2 * (ra / kSlice), sb2 = 2 * (rb / kSlice);
-  const int64_t tupa = la ? soff[sa2] : 0, taila = la ? soff[sa2 + 1] : 0;
-  const int64_t tupb = lb ? soff[sb2] : 0, tailb = lb ? soff[sb2 + 1] : 0;
-  const int32_t na = sell_tupled(la, (int32_t)((taila - tupa) / (kSlice * W)), W) / W;
-  const int32_t nb = sell_tupled(lb, (int32_t)((tailb - tupb) / (kSlice * W)), W) / W;
-  const int32_t rema = la - na * W, remb = lb - nb * W;
-  const int64_t oa = tupa + (ra % kSlice) * W, ob = tupb + (rb % kSlice) * W;
-  const int64_t ta = taila + ra % kSlice, tb = tailb + rb % kSlice;
-  const VT *va = reinterpret_cast<const VT *>(sval + oa);
-  const VI *ca = reinterpret_cast<const VI *>(sci + oa);
-  const VT *vb = reinterpret_cast<const VT *>(sval + ob);
-  const VI *cb = reinterpret_cast<const VI *>(sci + ob);
-  T tva[KT], txa[KT], tvb[KT], txb[KT];
-  int32_t tca[KT], tcb[KT];
-  sell_tail_load<T, KT>(ta, 0, rema, sci, sval, tva, tca);
-  sell_tail_load<T, KT>(tb, 0, remb, sci, sval, tvb, tcb);
-  sa = 0;
-  sb = 0;
-  const int32_t N = na > nb ? na : nb;
-  for (int32_t q0 = 0; q0 < N; q0 += QT) {
-    VT wa[QT], wb[QT];
-    T xa[QT * W], xb[QT * W];
-#pragma unroll
-    for (int q = 0; q < QT; q++) {
-      const int64_t o = (int64_t)(q0 + q) * kSlice;
-      if (q0 + q < na) {
-        wa[q] = __builtin_nontemporal_load(va + o);
-        const VI cc = __builtin_nontemporal_load(ca + o);
-#pragma unroll
-        for (int e = 0; e < W; e++) xa[q * W + e] = x[cc[e]];
-      }
-      if (q0 + q < nb) {
-        wb[q] = __builtin_nontemporal_load(vb + o);
-        const VI cc = __builtin_nontemporal_load(cb + o);
-#pragma unroll
-        for (int e = 0; e < W; e++) xb[q * W + e] = x[cc[e]];
-      }
-    }
-    if (q0 == 0) {
-#pragma unroll
-      for (int k = 0; k < KT; k++) {
-        if (k < rema) txa[k] = x[tca[k]];
-        if (k < remb) txb[k] = x[tcb[k]];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < QT; q++)
-#pragma unroll
-      for (int e = 0; e < W; e++) {
-        if (q0 + q < na) {
-          T p = wa[q][e] * xa[q * W + e];
-          sa = sa + p;
-        }
-        if (q0 + q < nb) {
-          T p = wb[q][e] * xb[q * W + e];
-          sb = sb + p;
-        }
-      }
-  }
-  if (N == 0) {
-#pragma unroll
-    for (int k = 0; k < KT; k++) {
-      if (k < rema) txa[k] = x[tca[k]];
-      if (k < remb) txb[k] = x[tcb[k]];
-    }
-  }
-  sell_tail_add<T, KT>(0, rema, tva, txa, sa);
-  sell_tail_add<T, KT>(0, remb, tvb, txb, sb);
-#pragma unroll 1
-  for (int32_t k0 = KT; k0 < rema || k0 < remb; k0 += KT) {  // longer tails
-    sell_tail_load<T, KT>(ta, k0, rema, sci, sval, tva, tca);
-    sell_tail_load<T, KT>(tb, k0, remb, sci, sval, tvb, tcb);
-#pragma unroll
-    for (int k = 0; k < KT; k++) {
-      if (k0 + k < rema) txa[k] = x[tca[k]];
-      if (k0 + k < remb) txb[k] = x[tcb[k]];
-    }
-    sell_tail_add<T, KT>(k0, rema, tva, txa, sa);
-    sell_tail_add<T, KT>(k0, remb, tvb, txb, sb);
-  }
-}
-
-// q = A p in the sliced layout for a matrix without long rows, one block per
-// 512-row tile (each lane two rows), and the tile partial of <p, q> exactly
-// as the CG engine's stage-3 kernel forms it (lane t: p[2t] q[2t] +
-// p[2t+1] q[2t+1], then the block tree), so that kernel can be skipped
-template <class T, int W>
-__global__ __launch_bounds__(kBlock) void k_spmv_sell_pq(
+__global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
     int64_t rows, int64_t ntiles, const int32_t *__restrict__ rp,
     const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
     const T *__restrict__ sval, const T *__restrict__ p, T *__restrict__ q, T *P, int remap,
     const int32_t *live) {
   __shared__ T qs[kTile];
-  __shared__ T sh[8];
+  __shared__ T sh[kTile / 64];
   const int64_t tile = xcd_block(remap);
   if (tile >= ntiles) return;
   if (live && *live != kStatRunning) return;
   const int64_t r0 = tile * kTile;
-  static_assert(kTile == 2 * kBlock, "two rows per lane");
+  static_assert(kTile == 2 * kBlock, "the partial pairs rows of a 512-row tile");
   {
-    const int64_t ra = r0 + threadIdx.x, rb = ra + kBlock;
-    const int32_t la = ra < rows ? rp[ra + 1] - rp[ra] : 0;
-    const int32_t lb = rb < rows ? rp[rb + 1] - rp[rb] : 0;
-    T sa, sb;
-    sell_row2<T, W>(ra, la, rb, lb, soff, sci, sval, p, sa, sb);
-    if (ra < rows) q[ra] = sa;
-    if (rb < rows) q[rb] = sb;
-    qs[threadIdx.x] = sa;
-    qs[kBlock + threadIdx.x] = sb;
+    const int64_t r = r0 + threadIdx.x;
+    const int32_t len = r < rows ? rp[r + 1] - rp[r] : 0;
+    T s = 0;
+    if (len) s = sell_row<T, W>(r, len, soff, sci, sval, p);
+    if (r < rows) q[r] = s;
+    qs[threadIdx.x] = s;
   }
   __syncthreads();
   const int64_t R1 = rows < r0 + kTile ? rows : r0 + kTile;
   const int64_t i0 = r0 + 2 * threadIdx.x;
   T p0 = 0, p1 = 0;
-  if (i0 + 1 < R1) {
-    p0 = p[i0] * qs[2 * threadIdx.x];
-    p1 = p[i0 + 1] * qs[2 * threadIdx.x + 1];
-  } else if (i0 < R1) {
-    p0 = p[i0] * qs[2 * threadIdx.x];
+  if (threadIdx.x < kBlock) {
+    if (i0 + 1 < R1) {
+      p0 = p[i0] * qs[2 * threadIdx.x];
+      p1 = p[i0 + 1] * qs[2 * threadIdx.x + 1];
+    } else if (i0 < R1) {
+      p0 = p[i0] * qs[2 * threadIdx.x];
+    }
   }
+  // waves 4..7 add zeros; block_tree combines the sums of waves 0..3 only
   const T t = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(P, tile, t);
 }
@@ -699,7 +608,7 @@ void spmv_plan_build_with(Csr *A, const int32_t *rp, int rows_req, int nnz_req, 
   A->plan.nnz_max = nnz_max;
   A->plan.rows_max = rows_max;
   A->plan.long_row = long_t;
-  A->plan.remap = remap_req ? 1 : 0;
+  A->plan.remap = remap_req > 1 ? std::min(remap_req, kMaxRemapRun) : (remap_req ? 1 : 0);
   A->plan.walk = walk ? 1 : 0;
   A->plan.tuned_us = 0;
   if (A->plan.d_blocks) dev_free(A->plan.d_blocks);
@@ -1188,8 +1097,9 @@ static void spmv_panels_dispatch(Csr *A, const T *X, T *Y) {
   for (int k = 0; k < p.npanels; k++) {
     const int64_t nb = p.pblock_off[k + 1] - p.pblock_off[k] - 1;
     if (nb <= 0) continue;
-    const int remap = p.remap && nb >= 64;
-    const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
+    const RemapGrid rg = remap_grid(p.remap, nb);
+    const int remap = rg.remap;
+    const unsigned grid = rg.grid;
     const int2 *B = p.d_pblocks + p.pblock_off[k];
     const int32_t *prp = p.d_prp + (size_t)k * n1;
 #define MGX_PANEL_U(UU)                                                                  \
@@ -1222,8 +1132,9 @@ static void spmv_sell_dispatch(Csr *A, const SpmvPlan::Sliced &l, const T *V, co
   Ctx *c = A->ctx;
   const SpmvPlan &p = A->plan;
   const int64_t nb = (A->rows + kBlock - 1) / kBlock;
-  const int remap = p.remap && nb >= 64;
-  const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
+  const RemapGrid rg = remap_grid(p.remap, nb);
+  const int remap = rg.remap;
+  const unsigned grid = rg.grid;
   constexpr int W = sizeof(T) == 8 ? 2 : 4;
   k_spmv_sell<T, AB, W><<<grid, kBlock, 0, c->stream>>>(
       A->rows, nb, A->rp, l.d_soff, l.d_sci, static_cast<const T *>(l.d_val), X, Y, alpha, beta,
@@ -1269,8 +1180,9 @@ static void spmv_dispatch(Csr *A, const T *V, const T *X, T *Y, T alpha,
     return;
   }
   const int64_t nb = A->plan.nblocks;
-  const int remap = A->plan.remap && nb >= 64;
-  const unsigned grid = (unsigned)(remap ? ((nb + 7) / 8) * 8 : nb);
+  const RemapGrid rg = remap_grid(A->plan.remap, nb);
+  const int remap = rg.remap;
+  const unsigned grid = rg.grid;
   // vector loads per lane: cover nnz_max entries plus the alignment slack
   constexpr int W = 16 / sizeof(T);
   const int u = (A->plan.nnz_max + W - 1 + kBlock * W - 1) / (kBlock * W);  // <= 17
@@ -1519,7 +1431,8 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
       if (spmv_sell_build(A, rp.data())) {
         int sell_remap = best.remap;
         double sell_us = 1e300;
-        for (int rr : {best.remap, 1 - best.remap}) {
+        // launch order, one range per XCD, and runs of 16 row blocks per XCD
+        for (int rr : {best.remap, 1 - best.remap, 16}) {
           A->plan.remap = rr;
           ncand++;
           const double us = timed(reps);
@@ -1596,15 +1509,16 @@ bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q,
   Ctx *c = A->ctx;
   const int64_t nt = (A->rows + kTile - 1) / kTile;
   if (nt == 0) return true;
-  const int remap = A->plan.remap && nt >= 64;
-  const unsigned grid = (unsigned)(remap ? ((nt + 7) / 8) * 8 : nt);
+  const RemapGrid rg = remap_grid(A->plan.remap, nt);
+  const int remap = rg.remap;
+  const unsigned grid = rg.grid;
   if (dt == MGX_F64)
-    k_spmv_sell_pq<double, 2><<<grid, kBlock, 0, c->stream>>>(
+    k_spmv_sell_pq<double, 2><<<grid, kTile, 0, c->stream>>>(
         A->rows, nt, A->rp, l->d_soff, l->d_sci, static_cast<const double *>(l->d_val),
         static_cast<const double *>(p), static_cast<double *>(q), static_cast<double *>(P),
         remap, live);
   else
-    k_spmv_sell_pq<float, 4><<<grid, kBlock, 0, c->stream>>>(
+    k_spmv_sell_pq<float, 4><<<grid, kTile, 0, c->stream>>>(
         A->rows, nt, A->rp, l->d_soff, l->d_sci, static_cast<const float *>(l->d_val),
         static_cast<const float *>(p), static_cast<float *>(q), static_cast<float *>(P), remap,
         live);

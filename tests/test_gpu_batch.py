@@ -129,6 +129,42 @@ def test_batch_identical_to_alone_and_oracle(m, ctx, oracle):
             assert np.array_equal(bits(X.download()), bits(ref[1]))
 
 
+def test_batch_with_split_form_systems(m, ctx, oracle):
+    """Two CG systems of at least 2^19 rows (the split form, launched one
+    system at a time) in one batch with small systems: one with the sliced
+    SpMV forming the <p, q> partials, one with the block-plan SpMV and a
+    separate <p, q> pass, a partial last tile in both; 30 iterations of
+    every system reproduce the oracle's bits."""
+    say("batch split form")
+    syss = []
+    for (nx, ny, nz, shift, seed) in ((96, 96, 64, 0.0, 21), (100, 90, 60, 1e-2, 22)):
+        rp, ci, v = m.stencil3d(nx, ny, nz, 7, shift)
+        syss.append((rp, ci, v, oracle.random_vector(len(rp) - 1, -1.0, 1.0, seed)))
+    assert all(len(b) >= 2 ** 19 for *_, b in syss) and len(syss[1][3]) % 512 != 0
+    for _, rp, ci, v, b in systems(m, oracle)[3:5]:
+        syss.append((rp, ci, v, b))
+    As, Xs, Bs = [], [], []
+    for k, (rp, ci, v, b) in enumerate(syss):
+        n = len(b)
+        A = m.CRS(ctx, n, n, rp, ci, v)
+        if k == 0:
+            A.set_plan(row_walk=2)
+        else:
+            A.set_plan(rows_per_block=256, nnz_per_block=2048)
+        As.append(A)
+        Xs.append(m.Vector(ctx, n=n))
+        Bs.append(m.Vector(ctx, b))
+    res = m.batch_solve(ctx, m.METHOD_CG, As, Xs, Bs, tol=1e-30, maxiter=30, precond=2)
+    for (rp, ci, v, b), r, X in zip(syss, res, Xs):
+        ref = oracle.solve("cg", rp, ci, v, np.zeros(len(b)), b, precond=2, tol=1e-30,
+                           maxiter=30)
+        say(len(b), r.status, r.final_iter, ref[2])
+        assert r.final_iter == ref[2] == 30
+        assert np.array_equal(bits(X.download()), bits(ref[1]))
+    for v in As + Xs + Bs:
+        v.free()
+
+
 def test_server_mix_equals_alone(m, ctx, oracle):
     """Requests of different sizes and methods share fused launches, join and
     leave at different iterations, and queue behind a small HBM budget; each

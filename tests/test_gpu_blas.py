@@ -394,6 +394,9 @@ def test_matvec_plans_bit_identical(m, ctx, oracle, dtype):
                  dict(rows_per_block=256, nnz_per_block=4096, row_walk=1),
                  dict(rows_per_block=7, nnz_per_block=4096, row_walk=1, xcd_remap=False),
                  dict(rows_per_block=64, nnz_per_block=4096),
+                 dict(rows_per_block=128, nnz_per_block=1024, row_walk=1, xcd_remap=16),
+                 dict(rows_per_block=64, nnz_per_block=4096, xcd_remap=3),
+                 dict(rows_per_block=256, nnz_per_block=2048, column_panels=4, xcd_remap=5),
                  dict(rows_per_block=256, nnz_per_block=2048, column_panels=4),
                  dict(rows_per_block=128, nnz_per_block=1024, column_panels=16, xcd_remap=False),
                  dict(row_walk=3, column_panels=1), dict(row_walk=3, column_panels=6),
@@ -402,6 +405,9 @@ def test_matvec_plans_bit_identical(m, ctx, oracle, dtype):
         Y.fill(0.0)
         m.matvec(A, X, Y)
         assert bits_equal(Y.download(), base), plan
+    for bad in (-1, 1025):
+        with pytest.raises(m.MgxError):
+            A.set_plan(rows_per_block=128, nnz_per_block=1024, xcd_remap=bad)
     t = A.autotune(reps=2)
     assert t["candidates"] >= 26 + 3 * (dtype == np.float64)
     # this matrix's row lengths vary too much for the sliced layout

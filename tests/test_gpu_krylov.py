@@ -770,7 +770,9 @@ def _transpose_values(rp, ci, val):
 def test_split_iteration_with_sliced_spmv(m, ctx, oracle, dtype):
     """The split CG iteration with the sliced-layout SpMV, which also forms
     the <p, q> tile partials (the stage-3 kernel then skips the system):
-    bit-identical to the oracle, as with the row-block plan; and the
+    bit-identical to the oracle, as with the row-block plan, in every block
+    order (launch order, a range per XCD, runs of 16 and of 3 blocks per
+    XCD); the SpMV alone gives the same bits in each order; and the
     mixed-precision CG gives the same bits with either plan."""
     say("split sliced", dtype.__name__)
     rp, ci, val = m.stencil3d(96, 96, 64, 27, 0.01)
@@ -780,12 +782,21 @@ def test_split_iteration_with_sliced_spmv(m, ctx, oracle, dtype):
     ref = oracle.solve("cg", rp, ci, val, np.zeros(n, dtype), b, precond=2, tol=1e-30,
                        maxiter=30, history=True)
     A = m.CRS(ctx, n, n, rp, ci, val)
-    A.set_plan(row_walk=2)
     X = m.Vector(ctx, n=n, dtype=dtype)
     B = m.Vector(ctx, b, dtype=dtype)
-    res = m.cg(A, X, B, tol=1e-30, maxiter=30, precond=2, mode=m.MODE_FUSED, history=30)
-    assert res.final_iter == 30 and bits_equal(X.download(), ref[1])
-    assert bits_equal(res.rhistory, ref[4])
+    Y = m.Vector(ctx, n=n, dtype=dtype)
+    ys = []
+    for remap in (1, 0, 16, 3):
+        A.set_plan(row_walk=2, xcd_remap=remap)
+        X.fill(0.0)
+        res = m.cg(A, X, B, tol=1e-30, maxiter=30, precond=2, mode=m.MODE_FUSED, history=30)
+        assert res.final_iter == 30 and bits_equal(X.download(), ref[1]), remap
+        assert bits_equal(res.rhistory, ref[4]), remap
+        m.matvec(A, B, Y)
+        ys.append(Y.download())
+    assert all(bits_equal(y, ys[0]) for y in ys[1:])
+    assert bits_equal(ys[0], oracle.spmv(rp, ci, val, b))
+    Y.free()
     if dtype == np.float64:
         bn = float(np.linalg.norm(b))
         xs = []
