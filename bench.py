@@ -494,8 +494,8 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60
     """Poisson arrivals; 50% 2D/3D Poisson (CG), 50% 2D upwind convection-
     diffusion with Peclet U[1,50] (BiCGSTAB); Jacobi, 1e-10 relative; sizes
     log-uniform in [nmin, nmax].  Requests draw from a pool of systems that
-    are uploaded once (the served matrices and right-hand sides stay resident
-    in HBM); each request is submitted by handle, so the client keeps up with
+    are uploaded and SpMV-autotuned once (the served matrices and right-hand
+    sides stay resident in HBM); each request is submitted by handle, so the client keeps up with
     the arrival rate and queued requests hold no host copies.  Arrivals stop
     after `seconds`; requests still unfinished `drain` seconds later are
     cancelled (a bounded run when the offered load exceeds capacity) and
@@ -507,6 +507,7 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60
     rng = np.random.default_rng(seed)
     systems = []
     nnzs = []
+    sliced = 0
     for k in range(pool):
         n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
         if k % 2 == 0:
@@ -523,7 +524,11 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60
             meth = m.METHOD_BICGSTAB
         nn = len(rp) - 1
         b = m.random_vector(nn, -1.0, 1.0, seed + k)
-        systems.append((meth, m.CRS(ctx, nn, nn, rp, ci, v), m.Vector(ctx, b), nn))
+        A = m.CRS(ctx, nn, nn, rp, ci, v)
+        # resident systems are tuned once at upload: with the sliced layout
+        # the engine's SpMV stages walk it one lane per row
+        sliced += A.autotune(reps=3)["row_walk"] == 2
+        systems.append((meth, A, m.Vector(ctx, b), nn))
         nnzs.append(len(v))
     ctx.sync()
     srv = m.Server(ctx, max_active=1024, iters_per_poll=8)
@@ -618,6 +623,7 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60
            "p99_latency_ms": round(1e3 * st["p99_latency_s"], 2),
            "mean_queue_ms": round(1e3 * st["mean_queue_s"], 2), "wall_s": round(wall, 2),
            "spmv_GB_s_floor": round(spmv_bytes_done / wall / 1e9, 1),
+           "pool_sliced_layout": int(sliced),
            "peak_resident_MB": round(st["peak_bytes"] / 2 ** 20, 1),
            "result_in_mix_equals_alone": same if compared else None,
            "results_compared": compared}

@@ -17,6 +17,7 @@
 
 #include "mgx_batch.hpp"
 #include "mgx_device.hpp"
+#include "mgx_sell.hpp"
 
 namespace mgx {
 
@@ -86,6 +87,24 @@ __device__ __forceinline__ void tile_spmv(const SysDesc<T> &d, int64_t tile,
     }
     __syncthreads();
   }
+}
+
+// the same row sums for a system with a sliced copy (SysDesc::sell): lanes
+// 0..127 walk one row each of the 128-row quarter at R0 (sell_row: the
+// products from 0 in CRS order, as tile_spmv adds them), no LDS staging
+template <class T, class G>
+__device__ __forceinline__ void tile_sell(const SysDesc<T> &d, int64_t R0, int64_t R1, T *qloc,
+                                          T *out, G gv) {
+  constexpr int W = 16 / (int)sizeof(T);
+  const int t = threadIdx.x;
+  if (t < kQuarter && R0 + t < R1) {
+    const int64_t r = R0 + t;
+    const int32_t len = d.rp[r + 1] - d.rp[r];
+    const T s = len ? sell_row<T, W>(r, len, d.soff, d.sci, d.sval, gv) : T(0);
+    qloc[t] = s;
+    out[r] = s;
+  }
+  __syncthreads();
 }
 
 // Solve-state fields are read and written with device-coherent (agent-scope)
@@ -278,7 +297,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const SysDesc<T> *sys,
   const int64_t qi = tile * 4 + (b & 3);
   const int64_t R0 = d.row0 + qi * kQuarter;
   const int64_t R1 = min(d.row0 + d.n, R0 + kQuarter);
-  tile_spmv(d, qi, R0, prod, qloc, d.q, sh, pv);
+  if (d.sell)
+    tile_sell(d, R0, R1, qloc, d.q, pv);
+  else
+    tile_spmv(d, qi, R0, prod, qloc, d.q, sh, pv);
   const int t = threadIdx.x;
   T acc0 = 0, acc1 = 0;
   if (t < 64) {
@@ -900,7 +922,10 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv1(const SysDesc<T> *sys,
   const int64_t qi = tile * 4 + (b & 3);
   const int64_t R0 = d.row0 + qi * kQuarter;
   const int64_t R1 = min(d.row0 + d.n, R0 + kQuarter);
-  tile_spmv(d, qi, R0, prod, qloc, v_new, sh, phv);
+  if (d.sell)
+    tile_sell(d, R0, R1, qloc, v_new, phv);
+  else
+    tile_spmv(d, qi, R0, prod, qloc, v_new, sh, phv);
   const int t = threadIdx.x;
   T acc[2] = {0, 0};
   if (t < 64) {
@@ -967,7 +992,10 @@ __global__ __launch_bounds__(kBlock) void k_bcg_spmv2(const SysDesc<T> *sys,
   const int64_t qi = tile * 4 + (b & 3);
   const int64_t R0 = d.row0 + qi * kQuarter;
   const int64_t R1 = min(d.row0 + d.n, R0 + kQuarter);
-  tile_spmv(d, qi, R0, prod, qloc, d.t, sh, shv);
+  if (d.sell)
+    tile_sell(d, R0, R1, qloc, d.t, shv);
+  else
+    tile_spmv(d, qi, R0, prod, qloc, d.t, sh, shv);
   const int t = threadIdx.x;
   T ts2[2] = {0, 0}, tt2[2] = {0, 0};
   if (t < 64) {

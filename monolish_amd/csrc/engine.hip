@@ -233,6 +233,13 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
                ? 1
                : 0;
   d.pq_fused = d.split && spmv_sell_pq_ok(A, val, DTof<T>::v) ? 1 : 0;
+  // gather-once systems walk the matrix's sliced copy when it has one
+  if (const SpmvPlan::Sliced *sl = d.once ? spmv_sell_layout(A, val, DTof<T>::v) : nullptr) {
+    d.sell = 1;
+    d.soff = sl->d_soff;
+    d.sci = sl->d_sci;
+    d.sval = static_cast<const T *>(sl->d_val);
+  }
   d.lf = cfg.method == kMethodCG && !d.split && nt <= kRedGroups && !c_->use_persistent ? 1 : 0;
   d.vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
   d.method = cfg.method;
@@ -395,6 +402,15 @@ template <class T> void Engine<T>::launch_iters(int iters) {
 }
 
 template <class T> void Engine<T>::step(int iters) {
+  // the kernels read the sliced copies directly: they must still be the
+  // matrices' own (a plan change frees them)
+  for (int s : active_) {
+    const SysDesc<T> &h = h_sys_[s];
+    if (!h.sell || reported_[s]) continue;
+    const SpmvPlan::Sliced *sl = spmv_sell_layout(csr_[s], val_[s], DTof<T>::v);
+    MGX_REQUIRE(sl && sl->d_soff == h.soff && sl->d_sci == h.sci && sl->d_val == h.sval,
+                "solver: the matrix's SpMV plan changed during a solve");
+  }
   flush_pending();
   if (dirty_) {
     rebuild_lists();

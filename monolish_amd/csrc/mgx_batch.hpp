@@ -45,6 +45,14 @@ template <class T> struct SysDesc {
   // finishes it in every block (the canonical order), the system's first
   // block writing the solve state (see k_cg_spmv, k_cg_p, k_cg_update)
   int32_t lf;
+  // 1 = the matrix's sliced layout (soff, sci, sval: the CSR's sliced copy of
+  // these values, no rows above the long-row threshold) is there: the
+  // gather-once SpMV stages walk it one lane per row instead of staging
+  // products in LDS (same bits)
+  int32_t sell;
+  const int64_t *soff;
+  const int32_t *sci;
+  const T *sval;
   double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
 };
 

@@ -417,6 +417,9 @@ void lobpcg_xw(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, const do
 // P (the CG engine's split-form stage 3), for matrices whose values val
 // have a sliced layout and no long rows; false (nothing launched) otherwise
 bool spmv_sell_pq_ok(const Csr *A, const void *val, mgx_dtype dt);
+// the sliced copy of the values val (A's or their FP32 copy) when A has one
+// and no rows above the long-row threshold: nullptr otherwise
+const SpmvPlan::Sliced *spmv_sell_layout(const Csr *A, const void *val, mgx_dtype dt);
 bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
                   const int32_t *live);
 void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,

@@ -129,6 +129,56 @@ def test_batch_identical_to_alone_and_oracle(m, ctx, oracle):
             assert np.array_equal(bits(X.download()), bits(ref[1]))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_sliced_layout_systems(m, ctx, oracle, dtype):
+    """Gather-once batch systems whose matrices hold the sliced layout walk
+    it one lane per row in the SpMV stages (CG and both BiCGSTAB stages),
+    beside systems without it (products staged in LDS): every system
+    reproduces the oracle's solve bit for bit, sizes with a partial last
+    128-row quarter included."""
+    say("batch sliced", dtype.__name__)
+    cases = []
+    for nx, ny, nz in ((33, 29, 17), (40, 40, 40)):
+        rp, ci, v = m.stencil3d(nx, ny, nz, 7, 1e-2)
+        cases.append(("cg", rp, ci, v))
+    rp, ci, v = m.laplacian2d(70, 61)
+    cases.append(("cg", rp, ci, v))
+    for side, pe in ((45, 7.0), (90, 30.0)):
+        rp, ci, v = m.convdiff2d(side, pe)
+        cases.append(("bicgstab", rp, ci, v))
+    for method in ("cg", "bicgstab"):
+        group = [c for c in cases if c[0] == method]
+        As, Xs, Bs, refs = [], [], [], []
+        for k, (_, rp, ci, v) in enumerate(group):
+            n = len(rp) - 1
+            v = v.astype(dtype)
+            b = oracle.random_vector(n, -1.0, 1.0, 700 + k, dtype=dtype)
+            A = m.CRS(ctx, n, n, rp, ci, v)
+            if k != 1:
+                A.set_plan(row_walk=2)  # system 1 keeps a block plan
+            As.append(A)
+            Xs.append(m.Vector(ctx, n=n, dtype=dtype))
+            Bs.append(m.Vector(ctx, b, dtype=dtype))
+            # FP64 to 1e-10 relative; FP32 60 iterations (no tolerance reached)
+            kw = dict(tol=1e-10 * oracle.nrm2(b), maxiter=400) if dtype == np.float64 else \
+                dict(tol=1e-30, maxiter=60)
+            refs.append(oracle.solve(method, rp, ci, v, np.zeros(n, dtype), b, precond=2, **kw))
+        code = m.METHOD_CG if method == "cg" else m.METHOD_BICGSTAB
+        if dtype == np.float64:
+            res = m.batch_solve(ctx, code, As, Xs, Bs, tol=1e-10, tol_is_relative=True, precond=2,
+                                maxiter=400)
+        else:
+            res = m.batch_solve(ctx, code, As, Xs, Bs, tol=1e-30, precond=2, maxiter=60)
+        for r, X, ref in zip(res, Xs, refs):
+            say(method, r.status, r.final_iter, ref[2])
+            assert r.final_iter == ref[2]
+            if dtype == np.float64:
+                assert r.status == ref[0] == 0
+            assert np.array_equal(bits(X.download()), bits(ref[1]))
+        for o in As + Xs + Bs:
+            o.free()
+
+
 def test_batch_with_split_form_systems(m, ctx, oracle):
     """Two CG systems of at least 2^19 rows (the split form, launched one
     system at a time) in one batch with small systems: one with the sliced
