@@ -135,13 +135,15 @@ def bench_krylov(m, ctx, reps_fused, reps_plain):
     rp, ci, val, b = c1_problem(m)
     n = len(rp) - 1
     A = m.CRS(ctx, n, n, rp, ci, val)
-    A.autotune(reps=10)
+    plan = A.autotune(reps=10)
     B = m.Vector(ctx, b)
     X = m.Vector(ctx, n=n)
     bn = m.nrm2(B)
     tol = 1e-10 * bn
     out = {"workload": "Jacobi-PCG, 2D 5-point Poisson 256x256 (n=65536, nnz=326656), "
-                       "b=U[-1,1) seed 1001, x0=0, ||r||<1e-10||b||"}
+                       "b=U[-1,1) seed 1001, x0=0, ||r||<1e-10||b||",
+           "spmv_plan": {k: plan[k] for k in ("row_walk", "rows_per_block", "nnz_per_block",
+                                              "xcd_remap", "column_panels")}}
 
     def run(mode, reps):
         lat = []

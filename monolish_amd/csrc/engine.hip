@@ -233,8 +233,10 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
                ? 1
                : 0;
   d.pq_fused = d.split && spmv_sell_pq_ok(A, val, DTof<T>::v) ? 1 : 0;
-  // gather-once systems walk the matrix's sliced copy when it has one
-  if (const SpmvPlan::Sliced *sl = d.once ? spmv_sell_layout(A, val, DTof<T>::v) : nullptr) {
+  // the fused SpMV stages walk a sliced copy of the matrix when it has one
+  // or qualifies for the engine's own (every row at most kEngineSellRow
+  // entries, slices padded by at most 30 %)
+  if (const SpmvPlan::Sliced *sl = !d.split ? spmv_sell_layout(A, val, DTof<T>::v) : nullptr) {
     d.sell = 1;
     d.soff = sl->d_soff;
     d.sci = sl->d_sci;
@@ -407,7 +409,7 @@ template <class T> void Engine<T>::step(int iters) {
   for (int s : active_) {
     const SysDesc<T> &h = h_sys_[s];
     if (!h.sell || reported_[s]) continue;
-    const SpmvPlan::Sliced *sl = spmv_sell_layout(csr_[s], val_[s], DTof<T>::v);
+    const SpmvPlan::Sliced *sl = spmv_sell_find(csr_[s], val_[s], DTof<T>::v);
     MGX_REQUIRE(sl && sl->d_soff == h.soff && sl->d_sci == h.sci && sl->d_val == h.sval,
                 "solver: the matrix's SpMV plan changed during a solve");
   }

@@ -46,9 +46,9 @@ template <class T> struct SysDesc {
   // block writing the solve state (see k_cg_spmv, k_cg_p, k_cg_update)
   int32_t lf;
   // 1 = the matrix's sliced layout (soff, sci, sval: the CSR's sliced copy of
-  // these values, no rows above the long-row threshold) is there: the
-  // gather-once SpMV stages walk it one lane per row instead of staging
-  // products in LDS (same bits)
+  // these values, no rows above the long-row threshold) is there: the fused
+  // SpMV stages walk it one lane per row instead of staging products in LDS
+  // (same bits)
   int32_t sell;
   const int64_t *soff;
   const int32_t *sci;

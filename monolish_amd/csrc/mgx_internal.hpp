@@ -236,8 +236,16 @@ struct Csr {
   // distinct 128-byte lines of x gathered by 64 consecutive entries
   // (sampled average; 0 when not measured): high for scattered columns
   double gather_lines = 0;
+  // the fused solvers' own sliced copies of the values (esl) and of their
+  // FP32 copy (esl32), apart from the SpMV plan: offsets computed at upload
+  // for square matrices whose rows all hold at most kEngineSellRow entries
+  // and whose slices pad by at most 30 % (empty otherwise), copies built on
+  // a solver's first use while the plan has no sliced copy of its own
+  std::vector<int64_t> esoff, esoff32;
+  SpmvPlan::Sliced esl, esl32;
   size_t elem() const { return dtype == MGX_F64 ? 8 : 4; }
 };
+constexpr int32_t kEngineSellRow = 256;
 
 struct Ctx {
   int device = 0;
@@ -416,10 +424,19 @@ void lobpcg_xw(Ctx *c, mgx_dtype dt, int64_t n, int m, const double *s, const do
 // q = A p in the sliced layout with the tile partials of <p, q> written to
 // P (the CG engine's split-form stage 3), for matrices whose values val
 // have a sliced layout and no long rows; false (nothing launched) otherwise
-bool spmv_sell_pq_ok(const Csr *A, const void *val, mgx_dtype dt);
-// the sliced copy of the values val (A's or their FP32 copy) when A has one
-// and no rows above the long-row threshold: nullptr otherwise
-const SpmvPlan::Sliced *spmv_sell_layout(const Csr *A, const void *val, mgx_dtype dt);
+bool spmv_sell_pq_ok(Csr *A, const void *val, mgx_dtype dt);
+// the sliced copy of the values val (A's or their FP32 copy) for the fused
+// solvers: the plan's when it has one and no rows above the long-row
+// threshold, else the engine's own (built here on first use); nullptr when
+// A has neither
+const SpmvPlan::Sliced *spmv_sell_layout(Csr *A, const void *val, mgx_dtype dt);
+// the same lookup without building anything
+const SpmvPlan::Sliced *spmv_sell_find(const Csr *A, const void *val, mgx_dtype dt);
+// at upload: the engine copies' offsets (Csr::esoff, esoff32)
+void spmv_engine_offsets(Csr *A, const int32_t *rp_host);
+// after new values: rewrite the engine copies that exist
+void spmv_engine_values(Csr *A);
+void spmv_engine_free(Csr *A);
 bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
                   const int32_t *live);
 void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,

@@ -149,6 +149,7 @@ void csr_free(Csr *A) {
   dev_free(A->plan.d_blocks);
   spmv_panels_free(A);
   spmv_sell_free(A);
+  spmv_engine_free(A);
   spmv_lpanel_free(A);
   tri_free(A);
   ilu_free(A);
@@ -551,6 +552,7 @@ int mgx_csr_update_values(mgx_csr *A, const void *val) {
     spmv_panels_free(A);  // their value copies would be stale
     spmv_lpanel_free(A);
     spmv_sell_values(A);
+    spmv_engine_values(A);
     if (A->trans) {
       csr_free(A->trans);
       A->trans = nullptr;

@@ -83,6 +83,7 @@ void fused_plan_build(Csr *A, const int32_t *rp_host) {
   }
   tile_chunk[nq] = (int32_t)chunks.size();
   chunks.push_back((int32_t)n);
+  spmv_engine_offsets(A, rp);
   A->d_chunks = (int32_t *)dev_alloc(chunks.size() * 4);
   A->d_tile_chunk = (int32_t *)dev_alloc(tile_chunk.size() * 4);
   MGX_HIP(hipMemcpy(A->d_chunks, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));

@@ -582,7 +582,7 @@ static bool sliced_offsets(const int32_t *rp, int64_t rows, int long_t, int w,
 // allocates a layout from host offsets and writes its indices and values
 template <class T, int W>
 static void sliced_fill(Csr *A, SpmvPlan::Sliced &l, const std::vector<int64_t> &soff,
-                        const T *val) {
+                        const T *val, int long_t) {
   Ctx *c = A->ctx;
   l.w = W;
   l.entries = soff.back();
@@ -592,17 +592,17 @@ static void sliced_fill(Csr *A, SpmvPlan::Sliced &l, const std::vector<int64_t> 
   MGX_HIP(hipMemcpyAsync(l.d_soff, soff.data(), soff.size() * 8, hipMemcpyHostToDevice,
                          c->stream));
   const unsigned g = (unsigned)std::max<int64_t>((A->rows + kBlock - 1) / kBlock, 1);
-  k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, A->plan.long_row,
+  k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, long_t,
                                                  l.d_soff, l.d_sci, static_cast<T *>(l.d_val));
   MGX_HIP(hipGetLastError());
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
 
 template <class T, int W>
-static void sliced_refill(Csr *A, SpmvPlan::Sliced &l, const T *val) {
+static void sliced_refill(Csr *A, SpmvPlan::Sliced &l, const T *val, int long_t) {
   Ctx *c = A->ctx;
   const unsigned g = (unsigned)std::max<int64_t>((A->rows + kBlock - 1) / kBlock, 1);
-  k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, A->plan.long_row,
+  k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, long_t,
                                                  l.d_soff, nullptr, static_cast<T *>(l.d_val));
   MGX_HIP(hipGetLastError());
   MGX_HIP(hipStreamSynchronize(c->stream));
@@ -613,15 +613,48 @@ static void sliced_refill(Csr *A, SpmvPlan::Sliced &l, const T *val) {
 void spmv_sell_values(Csr *A) {
   SpmvPlan &p = A->plan;
   if (!p.sell) return;
+  const int lt = p.long_row;
   if (A->dtype == MGX_F64) {
-    sliced_refill<double, 2>(A, p.sl, static_cast<const double *>(A->val));
+    sliced_refill<double, 2>(A, p.sl, static_cast<const double *>(A->val), lt);
     if (A->val32 && p.sl32.d_soff)
-      sliced_refill<float, 4>(A, p.sl32, A->val32);
+      sliced_refill<float, 4>(A, p.sl32, A->val32, lt);
     else if (A->val32 && !p.soff32.empty())
-      sliced_fill<float, 4>(A, p.sl32, p.soff32, A->val32);
+      sliced_fill<float, 4>(A, p.sl32, p.soff32, A->val32, lt);
   } else {
-    sliced_refill<float, 4>(A, p.sl, static_cast<const float *>(A->val));
+    sliced_refill<float, 4>(A, p.sl, static_cast<const float *>(A->val), lt);
   }
+}
+
+// The engine copies hold every row (none is above kEngineSellRow entries),
+// so they are filled with that bound as the long-row threshold.
+void spmv_engine_offsets(Csr *A, const int32_t *rp) {
+  A->esoff.clear();
+  A->esoff32.clear();
+  if (A->rows <= 0 || A->nnz <= 0 || A->rows != A->cols) return;
+  for (int64_t r = 0; r < A->rows; r++)
+    if (rp[r + 1] - rp[r] > kEngineSellRow) return;
+  const bool f64 = A->dtype == MGX_F64;
+  if (!sliced_offsets(rp, A->rows, kEngineSellRow, f64 ? 2 : 4, A->esoff, nullptr))
+    A->esoff.clear();
+  if (f64 && !sliced_offsets(rp, A->rows, kEngineSellRow, 4, A->esoff32, nullptr))
+    A->esoff32.clear();
+}
+
+void spmv_engine_free(Csr *A) {
+  if (!A->esl.d_soff && !A->esl32.d_soff) return;
+  if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
+  sliced_free(A->esl);
+  sliced_free(A->esl32);
+}
+
+void spmv_engine_values(Csr *A) {
+  if (A->esl.d_soff) {
+    if (A->dtype == MGX_F64)
+      sliced_refill<double, 2>(A, A->esl, static_cast<const double *>(A->val), kEngineSellRow);
+    else
+      sliced_refill<float, 4>(A, A->esl, static_cast<const float *>(A->val), kEngineSellRow);
+  }
+  if (A->esl32.d_soff && A->val32) sliced_refill<float, 4>(A, A->esl32, A->val32, kEngineSellRow);
 }
 
 // Builds the sliced copy for the plan's long-row threshold; false (nothing
@@ -648,11 +681,13 @@ bool spmv_sell_build(Csr *A, const int32_t *rp) {
       p.d_slrows = (int32_t *)dev_alloc(lrows.size() * 4);
       MGX_HIP(hipMemcpy(p.d_slrows, lrows.data(), lrows.size() * 4, hipMemcpyHostToDevice));
     }
+    const int lt = p.long_row;
     if (f64)
-      sliced_fill<double, 2>(A, p.sl, soff, static_cast<const double *>(A->val));
+      sliced_fill<double, 2>(A, p.sl, soff, static_cast<const double *>(A->val), lt);
     else
-      sliced_fill<float, 4>(A, p.sl, soff, static_cast<const float *>(A->val));
-    if (f64 && A->val32 && !p.soff32.empty()) sliced_fill<float, 4>(A, p.sl32, p.soff32, A->val32);
+      sliced_fill<float, 4>(A, p.sl, soff, static_cast<const float *>(A->val), lt);
+    if (f64 && A->val32 && !p.soff32.empty())
+      sliced_fill<float, 4>(A, p.sl32, p.soff32, A->val32, lt);
   } catch (...) {
     spmv_sell_free(A);
     throw;
@@ -1377,20 +1412,42 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
   }
 }
 
-bool spmv_sell_pq_ok(const Csr *A, const void *val, mgx_dtype dt) {
+bool spmv_sell_pq_ok(Csr *A, const void *val, mgx_dtype dt) {
   return spmv_sell_layout(A, val, dt) != nullptr;
 }
 
-const SpmvPlan::Sliced *spmv_sell_layout(const Csr *A, const void *val, mgx_dtype dt) {
-  if (A->plan.nslong != 0) return nullptr;
-  return sliced_of(A, val, dt == MGX_F64 ? 8 : 4);
+const SpmvPlan::Sliced *spmv_sell_find(const Csr *A, const void *val, mgx_dtype dt) {
+  const size_t es = dt == MGX_F64 ? 8 : 4;
+  if (A->plan.nslong == 0)
+    if (const SpmvPlan::Sliced *l = sliced_of(A, val, es)) return l;
+  if (val == A->val && es == A->elem() && A->esl.d_soff) return &A->esl;
+  if (es == 4 && A->val32 && val == A->val32 && A->esl32.d_soff) return &A->esl32;
+  return nullptr;
+}
+
+const SpmvPlan::Sliced *spmv_sell_layout(Csr *A, const void *val, mgx_dtype dt) {
+  if (const SpmvPlan::Sliced *l = spmv_sell_find(A, val, dt)) return l;
+  const size_t es = dt == MGX_F64 ? 8 : 4;
+  if (val == A->val && es == A->elem() && !A->esoff.empty()) {
+    if (A->dtype == MGX_F64)
+      sliced_fill<double, 2>(A, A->esl, A->esoff, static_cast<const double *>(A->val),
+                             kEngineSellRow);
+    else
+      sliced_fill<float, 4>(A, A->esl, A->esoff, static_cast<const float *>(A->val),
+                            kEngineSellRow);
+    return &A->esl;
+  }
+  if (es == 4 && A->val32 && val == A->val32 && !A->esoff32.empty()) {
+    sliced_fill<float, 4>(A, A->esl32, A->esoff32, A->val32, kEngineSellRow);
+    return &A->esl32;
+  }
+  return nullptr;
 }
 
 bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
                   const int32_t *live) {
-  const size_t es = dt == MGX_F64 ? 8 : 4;
-  const SpmvPlan::Sliced *l = sliced_of(A, val, es);
-  if (!l || A->plan.nslong > 0) return false;
+  const SpmvPlan::Sliced *l = spmv_sell_find(A, val, dt);
+  if (!l) return false;
   Ctx *c = A->ctx;
   const int64_t nt = (A->rows + kTile - 1) / kTile;
   if (nt == 0) return true;

@@ -131,11 +131,13 @@ def test_batch_identical_to_alone_and_oracle(m, ctx, oracle):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batch_sliced_layout_systems(m, ctx, oracle, dtype):
-    """Gather-once batch systems whose matrices hold the sliced layout walk
-    it one lane per row in the SpMV stages (CG and both BiCGSTAB stages),
-    beside systems without it (products staged in LDS): every system
-    reproduces the oracle's solve bit for bit, sizes with a partial last
-    128-row quarter included."""
+    """Gather-once batch systems walk a sliced layout one lane per row in
+    their SpMV stages (CG and both BiCGSTAB stages): the plan's sliced copy
+    (system 0, 2), or the engine's own when the plan has none (system 1);
+    M-matrices, whose 64-row slices would pad by about 50 %, stage products
+    in LDS (the last system of each group).  Every system reproduces the
+    oracle's solve bit for bit, sizes with a partial last 128-row quarter
+    included."""
     say("batch sliced", dtype.__name__)
     cases = []
     for nx, ny, nz in ((33, 29, 17), (40, 40, 40)):
@@ -143,9 +145,11 @@ def test_batch_sliced_layout_systems(m, ctx, oracle, dtype):
         cases.append(("cg", rp, ci, v))
     rp, ci, v = m.laplacian2d(70, 61)
     cases.append(("cg", rp, ci, v))
+    cases.append(("cg",) + tuple(m.mmatrix(3001, 5, 15, 0.5, seed=77)))
     for side, pe in ((45, 7.0), (90, 30.0)):
         rp, ci, v = m.convdiff2d(side, pe)
         cases.append(("bicgstab", rp, ci, v))
+    cases.append(("bicgstab",) + tuple(m.mmatrix(2500, 5, 15, 0.5, seed=78)))
     for method in ("cg", "bicgstab"):
         group = [c for c in cases if c[0] == method]
         As, Xs, Bs, refs = [], [], [], []
@@ -154,7 +158,7 @@ def test_batch_sliced_layout_systems(m, ctx, oracle, dtype):
             v = v.astype(dtype)
             b = oracle.random_vector(n, -1.0, 1.0, 700 + k, dtype=dtype)
             A = m.CRS(ctx, n, n, rp, ci, v)
-            if k != 1:
+            if k != 1 and k != len(group) - 1:
                 A.set_plan(row_walk=2)  # system 1 keeps a block plan
             As.append(A)
             Xs.append(m.Vector(ctx, n=n, dtype=dtype))

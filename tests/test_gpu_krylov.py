@@ -44,9 +44,11 @@ PREC = {"default": 0, "none": 1, "jacobi": 2, "ic": 4, "ilu": 5}
 
 
 def gpu_solve(m, ctx, method, rp, ci, val, x0, b, precond, tol, maxiter=0, miniter=0,
-              mode=None, history=0, omega=None):
+              mode=None, history=0, omega=None, plan=None):
     n = len(rp) - 1
     A = m.CRS(ctx, n, n, rp, ci, val)
+    if plan:
+        A.set_plan(**plan)
     X = m.Vector(ctx, x0, dtype=val.dtype)
     B = m.Vector(ctx, b, dtype=val.dtype)
     kw = dict(tol=tol, maxiter=maxiter, miniter=miniter, precond=precond, history=history)
@@ -97,18 +99,21 @@ def _poisson(oracle, side, dtype=np.float64):
     return oracle.crs_from_coo(rows, cols, r, c, v, dtype=dtype)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_cg_poisson256_bit_exact(m, ctx, oracle, mode):
+@pytest.mark.parametrize("mode,sliced", [(0, False), (1, False), (1, True)])
+def test_cg_poisson256_bit_exact(m, ctx, oracle, mode, sliced):
     """Jacobi-PCG on the 2D 5-point Poisson 256x256, b = U[-1,1) seed 1001,
-    x0 = 0, stop at ||r|| < 1e-10 ||b||."""
-    say("cg poisson256 mode", mode)
+    x0 = 0, stop at ||r|| < 1e-10 ||b||; the fused form also with the
+    matrix in the sliced layout (its SpMV stage then walks the slices one
+    lane per row)."""
+    say("cg poisson256 mode", mode, "sliced", sliced)
     rp, ci, val = _poisson(oracle, 256)
     n = len(rp) - 1
     b = oracle.random_vector(n, -1.0, 1.0, 1001)
     tol = 1e-10 * oracle.nrm2(b)
     ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=tol, history=True)
     assert ref[0] == 0
-    res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, tol, mode=mode, history=5000)
+    res, x = gpu_solve(m, ctx, "cg", rp, ci, val, np.zeros(n), b, 2, tol, mode=mode, history=5000,
+                       plan=dict(row_walk=2) if sliced else None)
     say("iters", res.final_iter, "resid", res.final_resid, "sec", res.seconds)
     assert res.status == 0 and res.final_iter == ref[2]
     assert bits_equal(x, ref[1])
@@ -154,11 +159,43 @@ def test_residual_l2_bit_exact(m, ctx, oracle, case, dtype):
         h.free()
 
 
+def _banded_irregular(side=64):
+    """SPD: the 2D 5-point Laplacian plus symmetric couplings (i, i +- k),
+    k = 2..6, from every 7th row, diagonal = row sum |offdiag| + 0.5; row
+    lengths 5..15 within 64-row slices, so no sliced copy qualifies, and
+    columns near the diagonal, so the fused form gathers z and p."""
+    import scipy.sparse as sp
+    n = side * side
+    A = sp.lil_matrix((n, n))
+    for i in range(n):
+        for j in (i - side, i - 1, i + 1, i + side):
+            if 0 <= j < n:
+                A[i, j] = -1.0
+    for i in range(0, n, 7):
+        for k in range(2, 7):
+            for j in (i - k, i + k):
+                if 0 <= j < n:
+                    A[i, j] = A[j, i] = -0.1
+    A = A.tocsr()
+    A.setdiag(np.asarray(abs(A).sum(axis=1)).ravel() + 0.5)
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("precond", [0, 1, 2])
-def test_cg_fused_equals_reference_mode(m, ctx, oracle, dtype, precond):
-    say("cg fused==reference", dtype.__name__, precond)
-    rp, ci, val = m.stencil3d(19, 17, 23, 27, 1e-2)
+@pytest.mark.parametrize("matrix", ["stencil27", "banded_irregular"])
+def test_cg_fused_equals_reference_mode(m, ctx, oracle, dtype, precond, matrix):
+    """The fused CG (two kernels, reductions finished in the next kernel)
+    against the reference call sequence and the oracle: on a 27-point
+    stencil (the SpMV stage walks the engine's sliced copy) and on a banded
+    matrix whose rows vary too much for one (products staged in LDS)."""
+    say("cg fused==reference", dtype.__name__, precond, matrix)
+    if matrix == "stencil27":
+        rp, ci, val = m.stencil3d(19, 17, 23, 27, 1e-2)
+    else:
+        rp, ci, val = _banded_irregular()
     val = val.astype(dtype)
     n = len(rp) - 1
     b = oracle.random_vector(n, -1.0, 1.0, 3001, dtype=dtype)
