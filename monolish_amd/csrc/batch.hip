@@ -99,7 +99,7 @@ __device__ __forceinline__ void tile_sell(const SysDesc<T> &d, int64_t R0, int64
   const int t = threadIdx.x;
   if (t < kQuarter && R0 + t < R1) {
     const int64_t r = R0 + t;
-    const int32_t len = d.rp[r + 1] - d.rp[r];
+    const int32_t len = sell_len<T>(d.slen, d.rp, r);
     const T s = len ? sell_row<T, W>(r, len, d.soff, d.sci, d.sval, gv) : T(0);
     qloc[t] = s;
     out[r] = s;

@@ -238,6 +238,7 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   // entries, slices padded by at most 30 %)
   if (const SpmvPlan::Sliced *sl = !d.split ? spmv_sell_layout(A, val, DTof<T>::v) : nullptr) {
     d.sell = 1;
+    d.slen = sl->d_len;
     d.soff = sl->d_soff;
     d.sci = sl->d_sci;
     d.sval = static_cast<const T *>(sl->d_val);
@@ -410,7 +411,8 @@ template <class T> void Engine<T>::step(int iters) {
     const SysDesc<T> &h = h_sys_[s];
     if (!h.sell || reported_[s]) continue;
     const SpmvPlan::Sliced *sl = spmv_sell_find(csr_[s], val_[s], DTof<T>::v);
-    MGX_REQUIRE(sl && sl->d_soff == h.soff && sl->d_sci == h.sci && sl->d_val == h.sval,
+    MGX_REQUIRE(sl && sl->d_soff == h.soff && sl->d_sci == h.sci && sl->d_val == h.sval &&
+                    sl->d_len == h.slen,
                 "solver: the matrix's SpMV plan changed during a solve");
   }
   flush_pending();

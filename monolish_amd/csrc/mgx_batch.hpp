@@ -50,6 +50,7 @@ template <class T> struct SysDesc {
   // SpMV stages walk it one lane per row instead of staging products in LDS
   // (same bits)
   int32_t sell;
+  const uint8_t *slen;
   const int64_t *soff;
   const int32_t *sci;
   const T *sval;

@@ -190,6 +190,9 @@ struct SpmvPlan {
     int64_t *d_soff = nullptr;
     int32_t *d_sci = nullptr;
     void *d_val = nullptr;
+    // row lengths, min(len, 255): the kernels read row_ptr only for rows
+    // of 255 entries or more
+    uint8_t *d_len = nullptr;
   };
   int32_t sell = 0;
   Sliced sl, sl32;

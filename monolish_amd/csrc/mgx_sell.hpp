@@ -24,6 +24,21 @@ namespace mgx {
 // addresses.
 constexpr int kSlice = 64;
 
+// a row's length: FP64 layouts read their byte lengths (255: row_ptr
+// decides), one byte per row instead of two row_ptr words (1-2 % on the
+// 7-point FP64 SpMV); FP32 layouts read row_ptr (the byte lengths measured
+// 0.5 % slower there)
+template <class T>
+__device__ __forceinline__ int32_t sell_len(const uint8_t *__restrict__ slen,
+                                            const int32_t *__restrict__ rp, int64_t r) {
+  if constexpr (sizeof(T) == 8) {
+    const int32_t l = slen[r];
+    return l < 255 ? l : rp[r + 1] - rp[r];
+  } else {
+    return rp[r + 1] - rp[r];
+  }
+}
+
 // entries of a row kept in tuples: nt_s tuples at most
 __host__ __device__ constexpr int32_t sell_tupled(int32_t len, int32_t nts, int w) {
   return (len / w < nts ? len / w : nts) * w;

@@ -218,10 +218,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longrows(
 template <class T, int W>
 __global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
                             const int32_t *__restrict__ ci, const T *__restrict__ val,
-                            int long_t, const int64_t *__restrict__ soff, int32_t *sci, T *sval) {
+                            int long_t, const int64_t *__restrict__ soff, int32_t *sci, T *sval,
+                            uint8_t *slen) {
   const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (r >= rows) return;
   const int32_t ra = rp[r], len = rp[r + 1] - ra;
+  if (slen) slen[r] = (uint8_t)(len < 255 ? len : 255);
   if (len > long_t) return;
   const int64_t tup = soff[2 * (r / kSlice)], tail = soff[2 * (r / kSlice) + 1];
   const int32_t t = sell_tupled(len, (int32_t)((tail - tup) / (kSlice * W)), W);
@@ -243,9 +245,9 @@ __global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
 template <class T, int W>
 __global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
     int64_t rows, int64_t ntiles, const int32_t *__restrict__ rp,
-    const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
-    const T *__restrict__ sval, const T *__restrict__ p, T *__restrict__ q, T *P, int remap,
-    const int32_t *live) {
+    const uint8_t *__restrict__ slen, const int64_t *__restrict__ soff,
+    const int32_t *__restrict__ sci, const T *__restrict__ sval, const T *__restrict__ p,
+    T *__restrict__ q, T *P, int remap, const int32_t *live) {
   __shared__ T qs[kTile];
   __shared__ T sh[kTile / 64];
   const int64_t tile = xcd_block(remap);
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
   static_assert(kTile == 2 * kBlock, "the partial pairs rows of a 512-row tile");
   {
     const int64_t r = r0 + threadIdx.x;
-    const int32_t len = r < rows ? rp[r + 1] - rp[r] : 0;
+    const int32_t len = r < rows ? sell_len<T>(slen, rp, r) : 0;
     T s = 0;
     if (len) s = sell_row<T, W>(r, len, soff, sci, sval, [p](int64_t c) { return p[c]; });
     if (r < rows) q[r] = s;
@@ -281,15 +283,15 @@ __global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
 template <class T, bool AB, int W>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell(
     int64_t rows, int64_t nblocks, const int32_t *__restrict__ rp,
-    const int64_t *__restrict__ soff, const int32_t *__restrict__ sci,
-    const T *__restrict__ sval, const T *__restrict__ x, T *__restrict__ y, T alpha, T beta,
-    int long_t, int remap, const int32_t *live) {
+    const uint8_t *__restrict__ slen, const int64_t *__restrict__ soff,
+    const int32_t *__restrict__ sci, const T *__restrict__ sval, const T *__restrict__ x,
+    T *__restrict__ y, T alpha, T beta, int long_t, int remap, const int32_t *live) {
   const int64_t b = xcd_block(remap);
   if (b >= nblocks) return;
   if (live && *live != kStatRunning) return;
   const int64_t r = b * kBlock + threadIdx.x;
   if (r >= rows) return;
-  const int32_t ra = rp[r], len = rp[r + 1] - ra;
+  const int32_t len = sell_len<T>(slen, rp, r);
   if (len > long_t) return;  // a long row: k_spmv_longrows
   const T s = sell_row<T, W>(r, len, soff, sci, sval, [x](int64_t c) { return x[c]; });
   if (AB) {
@@ -502,6 +504,7 @@ static void sliced_free(SpmvPlan::Sliced &l) {
   dev_free(l.d_soff);
   dev_free(l.d_sci);
   dev_free(l.d_val);
+  dev_free(l.d_len);
   l = SpmvPlan::Sliced();
 }
 
@@ -589,11 +592,13 @@ static void sliced_fill(Csr *A, SpmvPlan::Sliced &l, const std::vector<int64_t> 
   l.d_soff = (int64_t *)dev_alloc(soff.size() * 8);
   l.d_sci = (int32_t *)dev_alloc((size_t)(l.entries + W) * 4);
   l.d_val = dev_alloc((size_t)(l.entries + W) * sizeof(T));
+  l.d_len = (uint8_t *)dev_alloc((size_t)A->rows + 16);
   MGX_HIP(hipMemcpyAsync(l.d_soff, soff.data(), soff.size() * 8, hipMemcpyHostToDevice,
                          c->stream));
   const unsigned g = (unsigned)std::max<int64_t>((A->rows + kBlock - 1) / kBlock, 1);
   k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, long_t,
-                                                 l.d_soff, l.d_sci, static_cast<T *>(l.d_val));
+                                                 l.d_soff, l.d_sci, static_cast<T *>(l.d_val),
+                                                 l.d_len);
   MGX_HIP(hipGetLastError());
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
@@ -603,7 +608,8 @@ static void sliced_refill(Csr *A, SpmvPlan::Sliced &l, const T *val, int long_t)
   Ctx *c = A->ctx;
   const unsigned g = (unsigned)std::max<int64_t>((A->rows + kBlock - 1) / kBlock, 1);
   k_sell_fill<T, W><<<g, kBlock, 0, c->stream>>>(A->rows, A->rp, A->ci, val, long_t,
-                                                 l.d_soff, nullptr, static_cast<T *>(l.d_val));
+                                                 l.d_soff, nullptr, static_cast<T *>(l.d_val),
+                                                 nullptr);
   MGX_HIP(hipGetLastError());
   MGX_HIP(hipStreamSynchronize(c->stream));
 }
@@ -1052,8 +1058,8 @@ static void spmv_sell_dispatch(Csr *A, const SpmvPlan::Sliced &l, const T *V, co
   const unsigned grid = rg.grid;
   constexpr int W = sizeof(T) == 8 ? 2 : 4;
   k_spmv_sell<T, AB, W><<<grid, kBlock, 0, c->stream>>>(
-      A->rows, nb, A->rp, l.d_soff, l.d_sci, static_cast<const T *>(l.d_val), X, Y, alpha, beta,
-      p.long_row, remap, live);
+      A->rows, nb, A->rp, l.d_len, l.d_soff, l.d_sci, static_cast<const T *>(l.d_val), X, Y,
+      alpha, beta, p.long_row, remap, live);
   if (p.nslong > 0)
     k_spmv_longrows<T, AB><<<(unsigned)p.nslong, kBlock, 0, c->stream>>>(
         p.d_slrows, p.nslong, A->rp, A->ci, V, X, Y, alpha, beta, live);
@@ -1456,12 +1462,13 @@ bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q,
   const unsigned grid = rg.grid;
   if (dt == MGX_F64)
     k_spmv_sell_pq<double, 2><<<grid, kTile, 0, c->stream>>>(
-        A->rows, nt, A->rp, l->d_soff, l->d_sci, static_cast<const double *>(l->d_val),
+        A->rows, nt, A->rp, l->d_len, l->d_soff, l->d_sci,
+        static_cast<const double *>(l->d_val),
         static_cast<const double *>(p), static_cast<double *>(q), static_cast<double *>(P),
         remap, live);
   else
     k_spmv_sell_pq<float, 4><<<grid, kTile, 0, c->stream>>>(
-        A->rows, nt, A->rp, l->d_soff, l->d_sci, static_cast<const float *>(l->d_val),
+        A->rows, nt, A->rp, l->d_len, l->d_soff, l->d_sci, static_cast<const float *>(l->d_val),
         static_cast<const float *>(p), static_cast<float *>(q), static_cast<float *>(P), remap,
         live);
   MGX_HIP(hipGetLastError());

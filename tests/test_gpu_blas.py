@@ -504,13 +504,16 @@ def test_split_cg_with_l2_panel_spmv(m, ctx, oracle):
 def test_matvec_sliced_layout(m, ctx, oracle, dtype):
     """The sliced layout (64-row slices, one lane per row) computes the bits
     of every other plan: 27-point and 7-point stencils, a matrix with rows
-    above the long-row threshold, y = a A x + b y, after update_values, and
-    inside the mixed-precision solver (its FP32 copy of the values)."""
+    above the long-row threshold and short rows of 255 and 256 entries (the
+    layout's byte lengths send those to row_ptr), y = a A x + b y, after
+    update_values, and inside the mixed-precision solver (its FP32 copy of
+    the values)."""
     say("matvec sliced", dtype.__name__)
     rng = np.random.default_rng(5)
     mats = [m.stencil3d(24, 20, 17, 27, 1e-2), m.stencil3d(40, 33, 9, 7)]
     lens = rng.integers(10, 13, 9000)  # 64-row slices padded by under 10 %
     lens[[17, 4000, 8999]] = [600, 300, 1000]  # long rows: the block-tree path
+    lens[[5000, 5001]] = [255, 256]  # short rows whose length row_ptr gives
     rp = np.zeros(len(lens) + 1, np.int32)
     rp[1:] = np.cumsum(lens)
     cols = 12000
