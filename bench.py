@@ -532,28 +532,32 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60
         sliced += A.autotune(reps=3)["row_walk"] == 2
         systems.append((meth, A, m.Vector(ctx, b), nn))
         nnzs.append(len(v))
-    ctx.sync()
-    srv = m.Server(ctx, max_active=1024, iters_per_poll=8)
-    # the first request of a few systems keeps its solution, for the check
-    # that a result in the mix equals the same system solved alone
-    keep = {}
-    sys_of = []
-    t0 = time.perf_counter()
+    # the arrival schedule (times, systems) first; the first request of a
+    # few systems keeps its solution, for the check that a result in the mix
+    # equals the same system solved alone: those vectors are made before the
+    # server exists (its context takes no other calls meanwhile)
+    sched = []
     t = 0.0
-    tickets = []
     while True:
         t += rng.exponential(1.0 / rate)
         if t > seconds:
             break
+        sched.append((t, int(rng.integers(pool))))
+    keep = {}
+    for i, (_, k) in enumerate(sched):
+        if k not in keep and len(keep) < 4:
+            keep[k] = (m.Vector(ctx, n=systems[k][3]), i)
+    ctx.sync()
+    srv = m.Server(ctx, max_active=1024, iters_per_poll=8)
+    sys_of = []
+    t0 = time.perf_counter()
+    tickets = []
+    for i, (t, k) in enumerate(sched):
         now = time.perf_counter() - t0
         if t > now:
             time.sleep(t - now)
-        k = int(rng.integers(pool))
         meth, A, B, nn = systems[k]
-        X = None
-        if k not in keep and len(keep) < 4:
-            X = m.Vector(ctx, n=nn)
-            keep[k] = (X, len(tickets))
+        X = keep[k][0] if k in keep and keep[k][1] == i else None
         mode = cg_mode if meth == m.METHOD_CG else m.MODE_FUSED
         tickets.append(srv.submit_resident(meth, A, B, X, tol=1e-10, tol_is_relative=True,
                                            mode=mode))
