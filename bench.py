@@ -814,8 +814,8 @@ def main():
                          "autotuned plan), for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "runs on one GPU only: the row-sharded multi-GPU form is "
+                                   "designed but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import monolish_amd as m
