@@ -372,10 +372,10 @@ __global__ __launch_bounds__(kSweepWgThreads) void k_tri_staged(
 // signalling NaN that no arithmetic produces, and a row publishes its value
 // with one device-coherent atomic exchange, so a reader that finds
 // something else has the final value.  A lane loads its entries eight at a
-// time and consumes them strictly in the row's order; at the first one
-// still kEmpty it polls that one entry with a device-coherent atomic read
-// (a plain load may keep seeing a cached copy) until it is written, then
-// goes on, so the arithmetic is the serial sweep's.  Rows only read rows of earlier
+// time (plain loads) and consumes them strictly in the row's order; at the
+// first one still kEmpty it polls that one entry with a device-coherent
+// atomic read (a plain load may keep seeing a cached copy) until it is
+// written, then goes on, so the arithmetic is the serial sweep's.  Rows only read rows of earlier
 // levels, which sit at earlier positions, so their tickets were taken by
 // waves that are already running: the sweep cannot deadlock.  A lane that
 // has consumed none of its row's entries for kFlowSpinTicks of the 100 MHz
@@ -474,10 +474,11 @@ __global__ __launch_bounds__(kFlowThreads) void k_tri_flow(
             a[u] = dcoef[e];
           }
 #pragma unroll
-          for (int u = 0; u < kTriChunk; u++)
-            zb[u] = d + u < d1 ? __hip_atomic_load(ob + c[u], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT)
-                               : (U)0;
+          // plain loads: out held kEmpty when this kernel started (k_flow_init,
+          // an earlier kernel of the stream) and changes once, to the final
+          // value, so a cached copy is either that value or a stale kEmpty,
+          // which the device-coherent poll below resolves
+          for (int u = 0; u < kTriChunk; u++) zb[u] = d + u < d1 ? ob[c[u]] : (U)0;
           bool stop = false;
           int32_t taken = 0;
 #pragma unroll
