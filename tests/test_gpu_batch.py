@@ -219,6 +219,40 @@ def test_batch_with_split_form_systems(m, ctx, oracle):
         v.free()
 
 
+def test_c4_full_size_batch(m, ctx, oracle):
+    """The full C4 batch: 4096 SPD M-matrices, sizes log-uniform in
+    [500, 20000] (seed 4000), 5-15 random off-diagonals per row symmetrised,
+    diagonal = row sum |offdiag| + 0.5, b = U[-1,1) per system; all solved in
+    one batched Jacobi-PCG to 1e-10 relative.  Every system converges, and
+    64 systems spread over the batch reproduce the oracle's lone solve bit
+    for bit (iterations and x)."""
+    import concurrent.futures as cf
+    say("c4 full size")
+    nsys, seed0 = 4096, 4000
+    rng = np.random.default_rng(seed0)
+    sizes = np.floor(np.exp(rng.uniform(np.log(500), np.log(20000), nsys))).astype(int)
+
+    def gen(s):
+        rp, ci, v = m.mmatrix(int(sizes[s]), 5, 15, 0.5, seed=seed0 + s)
+        return rp, ci, v, m.random_vector(int(sizes[s]), -1.0, 1.0, seed0 + s)
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        mats = list(ex.map(gen, range(nsys)))
+    As = [m.CRS(ctx, len(b), len(b), rp, ci, v) for rp, ci, v, b in mats]
+    Bs = [m.Vector(ctx, b) for *_, b in mats]
+    Xs = [m.Vector(ctx, n=len(b)) for *_, b in mats]
+    res = m.batch_solve(ctx, m.METHOD_CG, As, Xs, Bs, tol=1e-10, tol_is_relative=True,
+                        precond=m.PRECOND_JACOBI)
+    assert all(r.status == 0 for r in res)
+    for s in range(0, nsys, nsys // 64):
+        rp, ci, v, b = mats[s]
+        ref = oracle_solve(oracle, "cg", rp, ci, v, b)
+        assert res[s].final_iter == ref[2], s
+        assert np.array_equal(bits(Xs[s].download()), bits(ref[1])), s
+    for o in As + Bs + Xs:
+        o.free()
+
+
 def test_server_mix_equals_alone(m, ctx, oracle):
     """Requests of different sizes and methods share fused launches, join and
     leave at different iterations, and queue behind a small HBM budget; each

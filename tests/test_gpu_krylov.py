@@ -803,6 +803,44 @@ def _transpose_values(rp, ci, val):
     return out
 
 
+def test_c3_full_size_bit_exact(m, ctx, oracle):
+    """The 27-point 128^3 system at full size (diagonal shift 1e-2, b =
+    U[-1,1) seed 3001, Jacobi-PCG to 1e-10 ||b||): the fused FP64 solve
+    (split form: per-system streaming kernels, sliced SpMV forming the <p,q>
+    partials) and the plain one reproduce the oracle's 273 iterations bit
+    for bit, residual history included; the mixed-precision solve (FP32
+    iterations, FP64 residual replacement) ends below 1e-10 ||b|| in true
+    residual, within 1e-7 of that solution."""
+    say("c3 full size")
+    rp, ci, val = m.stencil3d(128, 128, 128, 27, 1e-2)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 3001)
+    bn = oracle.nrm2(b)
+    tol = 1e-10 * bn
+    ref = oracle.solve("cg", rp, ci, val, np.zeros(n), b, precond=2, tol=tol, history=True)
+    assert ref[0] == 0 and ref[2] == 273
+    A = m.CRS(ctx, n, n, rp, ci, val)
+    A.autotune(reps=3)
+    X = m.Vector(ctx, n=n)
+    B = m.Vector(ctx, b)
+    for mode in (m.MODE_FUSED, m.MODE_REFERENCE):
+        X.fill(0.0)
+        res = m.cg(A, X, B, tol=tol, precond=2, mode=mode, history=400)
+        say("mode", mode, res.final_iter)
+        assert res.status == 0 and res.final_iter == ref[2], mode
+        assert bits_equal(X.download(), ref[1]), mode
+        assert bits_equal(res.rhistory, ref[4]), mode
+    X.fill(0.0)
+    res = m.mixed_cg(A, X, B, tol=1e-10, tol_is_relative=True, delta=1e-3)
+    x = X.download()
+    say("mixed", res.status, res.final_iter, res.outer_iter)
+    assert res.status == 0
+    assert oracle.residual_l2(rp, ci, val, x, b) < 1e-10 * bn
+    assert np.linalg.norm(x - ref[1]) < 1e-7 * np.linalg.norm(ref[1])
+    for h in (A, X, B):
+        h.free()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_split_iteration_with_sliced_spmv(m, ctx, oracle, dtype):
     """The split CG iteration with the sliced-layout SpMV, which also forms
