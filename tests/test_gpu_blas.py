@@ -258,6 +258,40 @@ def test_matvec_c2a_full_size(m, ctx, oracle):
     assert np.array_equal(ones, 7.0 - np.diff(rp))
 
 
+def test_matvec_c2b_full_size(m, ctx, oracle):
+    """The random CRS of 4M rows at full size (row lengths lognormal, mean
+    32, sigma 1, clipped to [1, 2048]; uniform columns without repeats;
+    seed 2002), FP64 and FP32, under the autotuned plan (column panels):
+    every row of at most 256 entries carries the CPU row loop's bits, the
+    longer ones (the block tree) are within 1e-12 (FP64) / 1e-5 (FP32) of
+    it; and the bits do not depend on the plan (the block plan gives the
+    same y)."""
+    say("matvec c2b full size")
+    rp, ci, val = m.random_lognormal(4_000_000, 4_000_000, 32.0, 1.0, 2048, seed=2002)
+    short = np.diff(rp) <= 256
+    assert short.sum() < len(short)
+    for dtype, tol in ((np.float64, 1e-12), (np.float32, 1e-5)):
+        v = val.astype(dtype)
+        x = oracle.random_vector(len(rp) - 1, -1.0, 1.0, 2003, dtype=dtype)
+        A = m.CRS(ctx, len(rp) - 1, len(x), rp, ci, v)
+        X = m.Vector(ctx, x, dtype=dtype)
+        Y = m.Vector(ctx, n=len(rp) - 1, dtype=dtype)
+        t = A.autotune(reps=3)
+        say(dtype.__name__, t)
+        m.matvec(A, X, Y)
+        y = Y.download()
+        ref = oracle.spmv(rp, ci, v, x)
+        assert bits_equal(y[short], ref[short])
+        d = (y - ref)[~short].astype(np.float64)
+        assert np.linalg.norm(d) <= tol * np.linalg.norm(ref[~short].astype(np.float64))
+        A.set_plan(rows_per_block=256, nnz_per_block=2048)
+        Y.fill(0.0)
+        m.matvec(A, X, Y)
+        assert bits_equal(Y.download(), y)
+        for h in (A, X, Y):
+            h.free()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("n", [0, 1, 2, 511, 512, 513, 4099, 300001, 3000000])
 def test_dot_nrm2_bit_exact(m, ctx, oracle, dtype, n):
