@@ -492,7 +492,7 @@ def bench_batched(m, ctx, nsys, seed0=4000):
 
 
 def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60.0,
-                 cg_mode=None):
+                 cg_mode=None, one_at_a_time=False):
     """Poisson arrivals; 50% 2D/3D Poisson (CG), 50% 2D upwind convection-
     diffusion with Peclet U[1,50] (BiCGSTAB); Jacobi, 1e-10 relative; sizes
     log-uniform in [nmin, nmax].  Requests draw from a pool of systems that
@@ -633,6 +633,26 @@ def bench_served(m, ctx, rate, seconds, nmin, nmax, seed=5001, pool=48, drain=60
            "peak_resident_MB": round(st["peak_bytes"] / 2 ** 20, 1),
            "result_in_mix_equals_alone": same if compared else None,
            "results_compared": compared}
+    if one_at_a_time:
+        # the same request mix (uniform over the pool) served one request at a
+        # time: each pool system solved alone by the plain FP64 GPU CG /
+        # BiCGSTAB (the reference's call sequence) and by the fused solver
+        per = {}
+        for mode, key in ((m.MODE_REFERENCE, "plain_fp64_gpu"), (m.MODE_FUSED, "fused")):
+            ts = []
+            for meth, A, B, nn in systems:
+                X1 = m.Vector(ctx, n=nn)
+                fn = m.cg if meth == m.METHOD_CG else m.bicgstab
+                ctx.sync()
+                t1 = time.perf_counter()
+                fn(A, X1, B, tol=1e-10, tol_is_relative=True, precond=m.PRECOND_JACOBI, mode=mode)
+                ctx.sync()
+                ts.append(time.perf_counter() - t1)
+                X1.free()
+            per[key] = len(ts) / sum(ts)
+        out["plain_fp64_gpu_one_at_a_time_solves_per_s"] = round(per["plain_fp64_gpu"], 2)
+        out["fused_one_at_a_time_solves_per_s"] = round(per["fused"], 2)
+        out["served_vs_plain_one_at_a_time"] = round(out["solves_per_s"] / per["plain_fp64_gpu"], 2)
     log(f"served C5: {out}")
     for _, A, B, _ in systems:
         A.free()
@@ -929,6 +949,13 @@ def main():
                                                         args.serve_seconds, 1000,
                                                         args.serve_nmax, seed=5002,
                                                         drain=args.serve_drain)
+        # a half-second window of the same stream, drained: every request
+        # completes, so its solves/s is the capacity for the uniform mix,
+        # set against the same mix solved one request at a time
+        line["served"]["drained_window"] = bench_served(m, ctx, 2000.0,
+                                                        min(args.serve_seconds, 0.5), 1000,
+                                                        args.serve_nmax, drain=args.serve_drain,
+                                                        one_at_a_time=True)
         # the same stream with the CG requests refined in FP32 (a half-second
         # window keeps the default run short)
         line["served"]["fp32_refined_cg"] = bench_served(m, ctx, 2000.0,
