@@ -803,6 +803,45 @@ def _transpose_values(rp, ci, val):
     return out
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_engine_sliced_copy_follows_values(m, ctx, oracle, dtype):
+    """The fused solver's own sliced copy (a matrix whose SpMV plan is a
+    block plan): built on the first fused solve, rewritten by update_values,
+    so the next fused solve walks the new values; both solves reproduce the
+    oracle bit for bit, and so does a mixed-precision solve (its FP32 copy
+    gets a sliced copy of its own) for FP64."""
+    say("engine sliced copy", dtype.__name__)
+    rp, ci, val = m.stencil3d(30, 27, 21, 7, 0.5)
+    n = len(rp) - 1
+    b = oracle.random_vector(n, -1.0, 1.0, 4242, dtype=dtype)
+    A = m.CRS(ctx, n, n, rp, ci, val.astype(dtype))
+    A.set_plan(rows_per_block=256, nnz_per_block=2048)
+    X = m.Vector(ctx, n=n, dtype=dtype)
+    B = m.Vector(ctx, b, dtype=dtype)
+    for scale in (1.0, 3.0):
+        v = (val * scale).astype(dtype)
+        if scale != 1.0:
+            A.update_values(v)
+        ref = oracle.solve("cg", rp, ci, v, np.zeros(n, dtype), b, precond=2, tol=1e-30,
+                           maxiter=40)
+        X.fill(0.0)
+        res = m.cg(A, X, B, tol=1e-30, maxiter=40, precond=2, mode=m.MODE_FUSED)
+        assert res.final_iter == ref[2] == 40, scale
+        assert bits_equal(X.download(), ref[1]), scale
+    if dtype == np.float64:
+        bn = float(np.linalg.norm(b))
+        X.fill(0.0)
+        r = m.mixed_cg(A, X, B, tol=1e-9 * bn, tol_is_relative=False, delta=1e-3)
+        x1 = X.download()
+        A.set_plan(row_walk=2)  # the plan's sliced copy from now on
+        X.fill(0.0)
+        r2 = m.mixed_cg(A, X, B, tol=1e-9 * bn, tol_is_relative=False, delta=1e-3)
+        assert r.status == r2.status == 0 and r.final_iter == r2.final_iter
+        assert bits_equal(X.download(), x1)
+    for h in (A, X, B):
+        h.free()
+
+
 def test_c3_full_size_bit_exact(m, ctx, oracle):
     """The 27-point 128^3 system at full size (diagonal shift 1e-2, b =
     U[-1,1) seed 3001, Jacobi-PCG to 1e-10 ||b||): the fused FP64 solve
