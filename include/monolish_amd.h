@@ -78,7 +78,7 @@ int mgx_set_spmv_tuning(mgx_ctx *ctx, int rows_per_block, int nnz_per_block,
  * order on/off with products staged in LDS in entry order; rows per block
  * {128, 256} x entries {1024, 2048} x XCD order with the row walk -- then,
  * for matrices whose x does not fit an XCD's L2 and whose rows list columns
- * in ascending order, the best of these with 4, 8 or 16 column panels (the
+ * in ascending order, the best of these with 4, 8, 12 or 16 column panels (the
  * short rows' entries stored a second time, panel-major, so each pass
  * gathers x from an L2-sized slice), and, where 64-row slices pad the
  * entries by at most 30 %, the sliced layout (entries stored a second time

@@ -1252,7 +1252,7 @@ void spmv_autotune(Csr *A, int reps, mgx_spmv_tuning *out) {
       MGX_HIP(hipMemsetAsync(x, 0, (size_t)cols * es, c->stream));
       MGX_HIP(hipEventCreate(&e0));
       MGX_HIP(hipEventCreate(&e1));
-      for (int np : {4, 8, 16}) {
+      for (int np : {4, 8, 12, 16}) {
         if (!spmv_panels_build(A, np)) break;
         ncand++;
         for (int i = 0; i < 3; i++) launch_spmv(A, A->val, A->dtype, x, y, 1, 0, false);
