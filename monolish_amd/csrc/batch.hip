@@ -119,13 +119,14 @@ template <class V> __device__ __forceinline__ void sst(V *p, V v) {
 
 // end of a CG iteration (cg.cpp:64-84): beta = <z,r>_new / <z,r>_old,
 // residual nrm2(r), history, then the stop tests in the reference's order;
-// called by one thread of the block that completed the reductions
-template <class T>
-__device__ __forceinline__ void cg_finish(const SysDesc<T> &d, SolveState *st, T zr_new, T rr,
+// called by one thread of the block that completed the reductions; S: the
+// scalars' type (T, or double for the mixed-precision solver's FP32 systems)
+template <class T, class S = T>
+__device__ __forceinline__ void cg_finish(const SysDesc<T> &d, SolveState *st, S zr_new, S rr,
                                           int32_t *stopped) {
-  const T zr_old = (T)sld(&st->zr);
-  const T beta = zr_new / zr_old;
-  const T resid = dsqrt(rr);
+  const S zr_old = (S)sld(&st->zr);
+  const S beta = zr_new / zr_old;
+  const S resid = dsqrt(rr);
   const int64_t it = sld(&st->iter);
   if (d.rhist && it < d.rhist_len) d.rhist[it] = (double)resid;
   sst(&st->beta, (double)beta);
@@ -457,6 +458,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_finish(const SysDesc<T> *sys,
   const SysDesc<T> &d = sys[slots[blockIdx.x]];
   SolveState *st = d.st;
   if (st->status != kStatRunning) return;
+  if (d.acc64) {
+    // FP64 partials and scalars (mixed-precision inner CG)
+    __shared__ double shd[8];
+    const double *Pd = reinterpret_cast<const double *>(d.P);
+    if (stage == 0) {
+      const double pq = reduce_partials(Pd, d.ntiles, shd);
+      if (threadIdx.x == 0) {
+        st->pq = pq;
+        st->alpha = st->zr / pq;
+      }
+    } else {
+      const double zr_new = reduce_partials(Pd, d.ntiles, shd);
+      __syncthreads();
+      const double rr = reduce_partials(Pd + d.ntiles, d.ntiles, shd);
+      if (threadIdx.x == 0) cg_finish<T, double>(d, st, zr_new, rr, stopped);
+    }
+    return;
+  }
   if (stage == 0) {
     const T pq = reduce_partials(d.P, d.ntiles, sh);
     if (threadIdx.x == 0) {
@@ -753,9 +772,9 @@ __global__ __launch_bounds__(kBlock) void k_split_p(const SysDesc<T> d) {
 // the per-wave sums of kSplitTiles tiles' lane values (a: slots 0..3, b:
 // slots 4..7 of each tile), then the tiles' block sums published by threads
 // 0..kSplitTiles-1 in block_tree2's order
-template <class T, int NR>
-__device__ __forceinline__ void split_publish(const SysDesc<T> &d, const T (*lanev)[kSplitTiles],
-                                              T *sh) {
+template <class T, int NR, class R = T>
+__device__ __forceinline__ void split_publish(const SysDesc<T> &d, const R (*lanev)[kSplitTiles],
+                                              R *sh) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < kSplitTiles; k++)
@@ -767,20 +786,21 @@ __device__ __forceinline__ void split_publish(const SysDesc<T> &d, const T (*lan
   __syncthreads();
   const int64_t tile = (int64_t)blockIdx.x * kSplitTiles + threadIdx.x;
   if ((int)threadIdx.x < kSplitTiles && tile < d.ntiles) {
-    const T *w = sh + threadIdx.x * 8;
+    const R *w = sh + threadIdx.x * 8;
+    R *P = reinterpret_cast<R *>(d.P);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      const T lo = w[4 * r] + w[4 * r + 1], hi = w[4 * r + 2] + w[4 * r + 3];
-      publish_partial(d.P, r * d.ntiles + tile, lo + hi);
+      const R lo = w[4 * r] + w[4 * r + 1], hi = w[4 * r + 2] + w[4 * r + 3];
+      publish_partial(P, r * d.ntiles + tile, lo + hi);
     }
   }
 }
 
 // x += alpha p, r -= alpha q, z = M r; tile partials of <z, r> and <r, r>
-// (cg.cpp:60-73), as k_cg_update for a split system
-template <class T>
+// (cg.cpp:60-73), as k_cg_update for a split system; R: the partials' type
+template <class T, class R>
 __global__ __launch_bounds__(kBlock) void k_split_update(const SysDesc<T> d) {
-  __shared__ T sh[kSplitTiles * 8];
+  __shared__ R sh[kSplitTiles * 8];
   const SolveState *st = d.st;
   if (st->status != kStatRunning) return;
   const T alpha = (T)st->alpha;
@@ -802,12 +822,12 @@ __global__ __launch_bounds__(kBlock) void k_split_update(const SysDesc<T> d) {
       if (d.M) mv[k] = *reinterpret_cast<const V2 *>(d.M + i0);
     }
   }
-  T lanev[2][kSplitTiles];
+  R lanev[2][kSplitTiles];
 #pragma unroll
   for (int k = 0; k < kSplitTiles; k++) {
     const int64_t tile = (int64_t)blockIdx.x * kSplitTiles + k;
     const int64_t i0 = tile * kTile + 2 * t;
-    T zr2[2] = {0, 0}, rr2[2] = {0, 0};
+    R zr2[2] = {0, 0}, rr2[2] = {0, 0};
     if (full[k]) {
       V2 zv;
 #pragma unroll
@@ -819,8 +839,8 @@ __global__ __launch_bounds__(kBlock) void k_split_update(const SysDesc<T> d) {
         rv[k][e] = r_;
         T z_ = d.M ? mv[k][e] * r_ : r_;
         zv[e] = z_;
-        zr2[e] = z_ * r_;
-        rr2[e] = r_ * r_;
+        zr2[e] = (R)z_ * (R)r_;
+        rr2[e] = (R)r_ * (R)r_;
       }
       *reinterpret_cast<V2 *>(d.x + i0) = xv[k];
       *reinterpret_cast<V2 *>(d.r + i0) = rv[k];
@@ -839,15 +859,15 @@ __global__ __launch_bounds__(kBlock) void k_split_update(const SysDesc<T> d) {
           d.r[i] = ri;
           T zi = d.M ? d.M[i] * ri : ri;
           d.z[i] = zi;
-          zr2[e] = zi * ri;
-          rr2[e] = ri * ri;
+          zr2[e] = (R)zi * (R)ri;
+          rr2[e] = (R)ri * (R)ri;
         }
       }
     }
     lanev[0][k] = zr2[0] + zr2[1];
     lanev[1][k] = rr2[0] + rr2[1];
   }
-  split_publish<T, 2>(d, lanev, sh);
+  split_publish<T, 2, R>(d, lanev, sh);
 }
 
 // tile partials of <p, q> when the SpMV did not form them
@@ -1451,8 +1471,10 @@ template <class T> void batch_launch_split(Ctx *c, const SysDesc<T> &d, int stag
     k_split_p<T><<<grid, kBlock, 0, c->stream>>>(d);
   else if (stage == 1)
     k_split_pq<T><<<grid, kBlock, 0, c->stream>>>(d);
+  else if (sizeof(T) == 4 && d.acc64)
+    k_split_update<T, double><<<grid, kBlock, 0, c->stream>>>(d);
   else
-    k_split_update<T><<<grid, kBlock, 0, c->stream>>>(d);
+    k_split_update<T, T><<<grid, kBlock, 0, c->stream>>>(d);
 }
 
 template <class T>

@@ -233,6 +233,8 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
                ? 1
                : 0;
   d.pq_fused = d.split && spmv_sell_pq_ok(A, val, DTof<T>::v) ? 1 : 0;
+  d.acc64 = cfg.acc64 && d.pq_fused && sizeof(T) == 4 ? 1 : 0;
+  d.dcorr = d.acc64 ? static_cast<const T *>(cfg.dcorr) : nullptr;
   // the fused SpMV stages walk a sliced copy of the matrix when it has one
   // or qualifies for the engine's own (every row at most kEngineSellRow
   // entries, slices padded by at most 30 %)
@@ -378,7 +380,8 @@ template <class T> void Engine<T>::launch_iters(int iters) {
           // the sliced layout was there when the system was added, so the
           // fused SpMV forms the <p, q> partials
           MGX_REQUIRE(spmv_sell_pq(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, h.P,
-                                   &d_states_[s].status),
+                                   &d_states_[s].status, h.acc64 != 0,
+                                   reinterpret_cast<const float *>(h.dcorr)),
                       "solver: the matrix's SpMV plan changed during a solve");
         } else {
           launch_spmv(csr_[s], val_[s], DTof<T>::v, h.pa, h.q, 1.0, 0.0, false,

@@ -55,6 +55,12 @@ template <class T> struct SysDesc {
   const int32_t *sci;
   const T *sval;
   double tol_scale;  // set-up: tol = tol_scale * nrm2(b) when nonzero
+  // FP32 split-form CG inside the mixed-precision solver only (pq_fused):
+  // 1 = the iteration's reductions multiply and add in FP64 (P holds double
+  // partials), and q = A p adds dcorr[i] * p[i] per row (dcorr: the FP64
+  // diagonal minus its FP32 rounding)
+  int32_t acc64;
+  const T *dcorr;
 };
 
 struct TileRef {
@@ -163,6 +169,11 @@ struct SolveCfg {
   // extra p pass costs less than the second gather); otherwise only for
   // matrices with scattered columns.  BiCGSTAB always uses it.
   int gather_once = 0;
+  // mixed-precision inner CG (FP32 systems in the split form with the fused
+  // <p, q>; ignored otherwise): FP64 reductions, and the FP32 diagonal
+  // corrected by dcorr (float per row) in q = A p
+  int acc64 = 0;
+  const void *dcorr = nullptr;
 };
 
 // FP32-refined solves: one system's step of iterative refinement

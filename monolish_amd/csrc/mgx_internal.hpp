@@ -233,6 +233,9 @@ struct Csr {
   int32_t *d_tile_chunk = nullptr;
   int32_t fused_nnz = 0;     // entry budget of a chunk
   float *val32 = nullptr;    // FP32 shadow values (mixed precision)
+  // per row: the FP64 diagonal entries minus their FP32 roundings (FP64
+  // matrices, built with val32)
+  float *dcorr32 = nullptr;
   Csr *trans = nullptr;      // A^T, built on first matvec_T
   TriPlan tri[4];            // triangular sweep plans (kTriLower ... kTriLt)
   IluStruct ilu;             // IC / ILU structure
@@ -440,13 +443,18 @@ void spmv_engine_offsets(Csr *A, const int32_t *rp_host);
 // after new values: rewrite the engine copies that exist
 void spmv_engine_values(Csr *A);
 void spmv_engine_free(Csr *A);
+// q = A p in the sliced layout with the <p, q> tile partials into P; FP32
+// with acc64: partials in FP64 (P of doubles) and, when dcorr is given,
+// q[i] += dcorr[i] * p[i]
 bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
-                  const int32_t *live);
+                  const int32_t *live, bool acc64 = false, const float *dcorr = nullptr);
 void lobpcg_combine(Ctx *c, mgx_dtype dt, int64_t n, int m, int stride, bool has_p,
                     const void *coef, const void *wxp_p, const void *WXP_p, void *wxp,
                     void *WXP);
 // FP32 shadow of an FP64 matrix's values (mixed precision), made once
 void ensure_val32(Csr *A);
+// (re)computes A->dcorr32 from A's FP64 values
+void diag_corr_build(Csr *A);
 
 double seconds_now();
 

@@ -146,6 +146,7 @@ void csr_free(Csr *A) {
   dev_free(A->ci);
   dev_free(A->val);
   dev_free(A->val32);
+  dev_free(A->dcorr32);
   dev_free(A->plan.d_blocks);
   spmv_panels_free(A);
   spmv_sell_free(A);
@@ -548,7 +549,10 @@ int mgx_csr_update_values(mgx_csr *A, const void *val) {
     activate(A->ctx);
     MGX_HIP(hipStreamSynchronize(A->ctx->stream));
     if (A->nnz) MGX_HIP(hipMemcpy(A->val, val, (size_t)A->nnz * A->elem(), hipMemcpyHostToDevice));
-    if (A->val32) launch_convert(A->ctx, A->dtype, MGX_F32, A->nnz, A->val, A->val32);
+    if (A->val32) {
+      launch_convert(A->ctx, A->dtype, MGX_F32, A->nnz, A->val, A->val32);
+      diag_corr_build(A);
+    }
     spmv_panels_free(A);  // their value copies would be stale
     spmv_lpanel_free(A);
     spmv_sell_values(A);

@@ -241,15 +241,17 @@ __global__ void k_sell_fill(int64_t rows, const int32_t *__restrict__ rp,
 // (lane t < 256: p[2t] q[2t] + p[2t+1] q[2t+1], then the block tree over
 // those 256 lanes), so no separate pass is needed.  Two rows per lane (loads
 // of both interleaved) took 72 VGPRs against 56 and measured slower on the
-// 27-point stencil
-template <class T, int W>
+// 27-point stencil.  R: the partials' type (double with FP32 values for the
+// mixed-precision solver, whose products and sums are then exact-ish);
+// dcorr (or nullptr) adds a per-row diagonal correction after the row sum.
+template <class T, int W, class R>
 __global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
     int64_t rows, int64_t ntiles, const int32_t *__restrict__ rp,
     const uint8_t *__restrict__ slen, const int64_t *__restrict__ soff,
     const int32_t *__restrict__ sci, const T *__restrict__ sval, const T *__restrict__ p,
-    T *__restrict__ q, T *P, int remap, const int32_t *live) {
+    T *__restrict__ q, R *P, const T *__restrict__ dcorr, int remap, const int32_t *live) {
   __shared__ T qs[kTile];
-  __shared__ T sh[kTile / 64];
+  __shared__ R sh[kTile / 64];
   const int64_t tile = xcd_block(remap);
   if (tile >= ntiles) return;
   if (live && *live != kStatRunning) return;
@@ -260,23 +262,24 @@ __global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
     const int32_t len = r < rows ? sell_len<T>(slen, rp, r) : 0;
     T s = 0;
     if (len) s = sell_row<T, W>(r, len, soff, sci, sval, [p](int64_t c) { return p[c]; });
+    if (dcorr && r < rows) s = s + dcorr[r] * p[r];
     if (r < rows) q[r] = s;
     qs[threadIdx.x] = s;
   }
   __syncthreads();
   const int64_t R1 = rows < r0 + kTile ? rows : r0 + kTile;
   const int64_t i0 = r0 + 2 * threadIdx.x;
-  T p0 = 0, p1 = 0;
+  R p0 = 0, p1 = 0;
   if (threadIdx.x < kBlock) {
     if (i0 + 1 < R1) {
-      p0 = p[i0] * qs[2 * threadIdx.x];
-      p1 = p[i0 + 1] * qs[2 * threadIdx.x + 1];
+      p0 = (R)p[i0] * (R)qs[2 * threadIdx.x];
+      p1 = (R)p[i0 + 1] * (R)qs[2 * threadIdx.x + 1];
     } else if (i0 < R1) {
-      p0 = p[i0] * qs[2 * threadIdx.x];
+      p0 = (R)p[i0] * (R)qs[2 * threadIdx.x];
     }
   }
   // waves 4..7 add zeros; block_tree combines the sums of waves 0..3 only
-  const T t = block_tree(p0 + p1, sh);
+  const R t = block_tree(p0 + p1, sh);
   if (threadIdx.x == 0) publish_partial(P, tile, t);
 }
 
@@ -1451,7 +1454,7 @@ const SpmvPlan::Sliced *spmv_sell_layout(Csr *A, const void *val, mgx_dtype dt) 
 }
 
 bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q, void *P,
-                  const int32_t *live) {
+                  const int32_t *live, bool acc64, const float *dcorr) {
   const SpmvPlan::Sliced *l = spmv_sell_find(A, val, dt);
   if (!l) return false;
   Ctx *c = A->ctx;
@@ -1461,16 +1464,21 @@ bool spmv_sell_pq(Csr *A, const void *val, mgx_dtype dt, const void *p, void *q,
   const int remap = rg.remap;
   const unsigned grid = rg.grid;
   if (dt == MGX_F64)
-    k_spmv_sell_pq<double, 2><<<grid, kTile, 0, c->stream>>>(
+    k_spmv_sell_pq<double, 2, double><<<grid, kTile, 0, c->stream>>>(
         A->rows, nt, A->rp, l->d_len, l->d_soff, l->d_sci,
         static_cast<const double *>(l->d_val),
         static_cast<const double *>(p), static_cast<double *>(q), static_cast<double *>(P),
+        nullptr, remap, live);
+  else if (acc64)
+    k_spmv_sell_pq<float, 4, double><<<grid, kTile, 0, c->stream>>>(
+        A->rows, nt, A->rp, l->d_len, l->d_soff, l->d_sci, static_cast<const float *>(l->d_val),
+        static_cast<const float *>(p), static_cast<float *>(q), static_cast<double *>(P), dcorr,
         remap, live);
   else
-    k_spmv_sell_pq<float, 4><<<grid, kTile, 0, c->stream>>>(
+    k_spmv_sell_pq<float, 4, float><<<grid, kTile, 0, c->stream>>>(
         A->rows, nt, A->rp, l->d_len, l->d_soff, l->d_sci, static_cast<const float *>(l->d_val),
-        static_cast<const float *>(p), static_cast<float *>(q), static_cast<float *>(P), remap,
-        live);
+        static_cast<const float *>(p), static_cast<float *>(q), static_cast<float *>(P), nullptr,
+        remap, live);
   MGX_HIP(hipGetLastError());
   return true;
 }
