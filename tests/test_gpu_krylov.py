@@ -375,6 +375,39 @@ def test_mixed_cg_residual_replacement(m, ctx, oracle, grid, delta):
         v.free()
 
 
+def test_mixed_cg_split_form_keeps_fp64_iteration_count(m, ctx, oracle):
+    """The split-form FP32 iteration of the mixed solver reduces in FP64 and
+    corrects the FP32-rounded diagonal, so on the 27-point stencil (81^3,
+    531,441 rows) it needs no more iterations than the FP64 CG: with FP32
+    dot products or the rounded diagonal (every eigenvalue shifted by the
+    same rounding) it needs several percent more.  After new values (another
+    diagonal shift) the correction follows them."""
+    say("mixed-cg split iteration count")
+    grid = 81
+    for shift in (1e-2, 3e-2):
+        rp, ci, val = m.stencil3d(grid, grid, grid, 27, shift)
+        n = len(rp) - 1
+        if shift == 1e-2:
+            A = m.CRS(ctx, n, n, rp, ci, val)
+            b = oracle.random_vector(n, -1.0, 1.0, 3001)
+            bn = oracle.nrm2(b)
+            B = m.Vector(ctx, b)
+        else:
+            A.update_values(val)
+        X64 = m.Vector(ctx, np.zeros(n))
+        ref = m.cg(A, X64, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI)
+        X = m.Vector(ctx, np.zeros(n))
+        res = m.mixed_cg(A, X, B, tol=1e-10, tol_is_relative=True, delta=1e-3)
+        say("shift", shift, "mixed", res, "fp64 iterations", ref.final_iter)
+        assert ref.status == 0 and res.status == 0
+        assert oracle.residual_l2(rp, ci, val, X.download(), b) < 1e-10 * bn
+        assert res.inner_iter <= ref.final_iter + 2
+        X64.free()
+        X.free()
+    A.free()
+    B.free()
+
+
 def test_mixed_cg_edge_cases(m, ctx, oracle):
     """x0 already a solution: no FP32 iteration; maxiter reached: status
     MAXITER with the FP64 residual reported; delta outside (0, 1) rejected."""
