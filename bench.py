@@ -341,7 +341,7 @@ def bench_mixed(m, ctx, side, reps):
     # stagnation; residual replacement keeps the search direction and
     # replaces the FP32 residual by the FP64 one after each drop by delta
     inner_tols = (1e-4, 1e-5, 1e-6)
-    deltas = (1e-1, 1e-2, 1e-3)
+    deltas = (1e-1, 1e-2, 1e-3, 1e-4)
     legs = [("fp64_fused_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI)),
             ("fp64_plain_cg", lambda: m.cg(A, X, B, tol=1e-10 * bn, precond=m.PRECOND_JACOBI,
                                            mode=m.MODE_REFERENCE))]

@@ -267,6 +267,7 @@ int Engine<T>::add(Csr *A, const void *val, T *x, const T *b, const SolveCfg &cf
   active_.push_back(slot);
   host_running_++;
   dirty_ = true;
+  listed_ok_ = false;
   return slot;
 }
 
@@ -420,8 +421,19 @@ template <class T> void Engine<T>::step(int iters) {
   }
   flush_pending();
   if (dirty_) {
-    rebuild_lists();
-    gstale_ = true;
+    // a slot that stopped and was resumed before the next step leaves the
+    // lists as they were
+    std::vector<int> run;
+    for (int s : active_)
+      if (!reported_[s]) run.push_back(s);
+    if (listed_ok_ && run == listed_) {
+      dirty_ = false;
+    } else {
+      rebuild_lists();
+      listed_ = run;
+      listed_ok_ = true;
+      gstale_ = true;
+    }
   }
   if (!c_->use_graphs) {
     launch_iters(iters);
@@ -524,6 +536,15 @@ template <class T> void Engine<T>::resume(int slot, const SolveState &s) {
   dirty_ = true;
 }
 
+template <class T> void Engine<T>::resume_device(int slot) {
+  MGX_REQUIRE(slot >= 0 && slot < cap_ && reported_[slot] &&
+                  std::find(active_.begin(), active_.end(), slot) != active_.end(),
+              "engine: resume needs a stopped, unreleased slot");
+  reported_[slot] = 0;
+  host_running_++;
+  dirty_ = true;
+}
+
 template <class T> void Engine<T>::fetch_states() {
   MGX_HIP(hipStreamSynchronize(c_->stream));
   MGX_HIP(hipMemcpy(h_states_.data(), d_states_, sizeof(SolveState) * cap_,
@@ -541,6 +562,7 @@ template <class T> void Engine<T>::release(int slot) {
   bufs_[slot].clear();
   free_slots_.push_back(slot);
   dirty_ = true;
+  listed_ok_ = false;
 }
 
 template class Engine<double>;

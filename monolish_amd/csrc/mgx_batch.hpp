@@ -213,6 +213,12 @@ template <class T> class Engine {
   SolveState state(int slot);  // valid after poll() reported the slot
   // continue a stopped slot from the state s (its vectors as they are now)
   void resume(int slot, const SolveState &s);
+  // the same, with the slot's new state already written to its device
+  // state (status running, fresh and ia as resume sets them) by work
+  // queued on the context's stream
+  void resume_device(int slot);
+  // the slot's device state (for kernels that write it)
+  SolveState *device_state(int slot) { return d_states_ + slot; }
   void release(int slot);
   int active() const { return (int)active_.size(); }
   // run the set-up of the systems added since the last launch now
@@ -244,6 +250,10 @@ template <class T> class Engine {
   int64_t tiles_cap_[2] = {0, 0};
   int64_t ntiles_[2] = {0, 0};
   bool dirty_ = true;
+  // the running slots the tile lists were last built for; listed_ok_ false
+  // after an add() or release() (a slot's descriptor may differ)
+  std::vector<int> listed_;
+  bool listed_ok_ = false;
   std::vector<SysDesc<T>> h_sys_;
   std::vector<int> method_;
   std::vector<int> free_slots_;

@@ -626,6 +626,24 @@ __global__ void k_mp_replace(int64_t n, const double *r, const float *M, float *
   }
 }
 
+// the FP32 CG's state after a residual replacement, written on the device:
+// beta = zr_k / zr_{k-1} and zr = zr_k were of the replaced residual, so
+// beta is rescaled to the new <z, r>; zr, rr and the residual of the new
+// residual; the next tolerance; running, resumed with beta and zr given
+__global__ void k_mp_resume(SolveState *st, const double *zr_new, const double *rr_new,
+                            double tol) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double zr = *zr_new, rr = *rr_new;
+  st->beta = st->beta * (zr / st->zr);
+  st->zr = zr;
+  st->rr = rr;
+  st->resid = __builtin_sqrt(rr);
+  st->tol = tol;
+  st->status = kStatRunning;
+  st->fresh = 2;
+  st->ia = st->iter;
+}
+
 // Mixed-precision PCG with FP64 residual replacement ("reliable updates",
 // Sleijpen & van der Vorst, Computing 56 (1996); the FP32/FP64 form of
 // Clark et al., Comput. Phys. Commun. 181 (2010), alg. 3).  The FP32 PCG
@@ -713,22 +731,18 @@ SolveOut mp_cg(Csr *A, double *x, const double *b, int precond, double thr,
         o.status = MGX_SOLVER_MAXITER;
         break;
       }
-      // replace the FP32 residual; d32 restarts from 0, p is kept
+      // replace the FP32 residual; d32 restarts from 0, p is kept.  The new
+      // state is formed on the device (no host round trip)
       const SysDesc<float> &d = eng.desc(slot);
       launch_fill(c, MGX_F32, d32, n, 0.0);
       k_mp_replace<<<g, kBlock, 0, c->stream>>>(n, r, d.M, d.r, d.z);
       MGX_HIP(hipGetLastError());
-      const double zr = dot_host(c, MGX_F32, n, d.z, d.r);
-      const double rr = dot_host(c, MGX_F32, n, d.r, d.r);
-      SolveState s = hs;
-      // hs.beta = zr_k / zr_{k-1} and hs.zr = zr_k, both of the replaced residual
-      s.beta = hs.beta * (zr / hs.zr);
-      s.zr = zr;
-      s.rr = rr;
-      s.resid = std::sqrt(rr);
-      s.tol = std::max(delta * rn, thr);
-      s.status = kStatRunning;
-      eng.resume(slot, s);
+      launch_dot(c, MGX_F32, n, d.z, d.r, &c->d_state->zr, c->d_tick);
+      launch_dot(c, MGX_F32, n, d.r, d.r, &c->d_state->rr, c->d_tick);
+      k_mp_resume<<<1, 64, 0, c->stream>>>(eng.device_state(slot), &c->d_state->zr,
+                                            &c->d_state->rr, std::max(delta * rn, thr));
+      MGX_HIP(hipGetLastError());
+      eng.resume_device(slot);
     }
   } catch (...) {
     eng.release(slot);
