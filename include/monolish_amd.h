@@ -356,7 +356,10 @@ int mgx_ir_cg_solve(mgx_csr *A, mgx_vec *x, mgx_vec *b,
                     int64_t max_outer, mgx_solve_result *res);
 
 /* Mixed-precision Jacobi-PCG with FP64 residual replacement: the CG
- * iterations run on FP32 values and vectors with FP32 accumulation; each
+ * iterations run on FP32 values and vectors with FP32 accumulation in the
+ * SpMV; for systems of at least 2^19 rows with a sliced layout, their dot
+ * products multiply and add in FP64 and q = A p adds a per-row correction
+ * of the FP32-rounded diagonal (FP32 accumulation otherwise); each
  * time the FP32 residual has dropped by the factor delta since the last
  * replacement, the FP32 correction is added to the FP64 x and the FP64
  * residual r = b - A x replaces the FP32 one, keeping the search direction.
