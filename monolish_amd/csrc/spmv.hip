@@ -260,9 +260,15 @@ __global__ __launch_bounds__(kTile) void k_spmv_sell_pq(
   {
     const int64_t r = r0 + threadIdx.x;
     const int32_t len = r < rows ? sell_len<T>(slen, rp, r) : 0;
+    // the correction's two loads are issued before the row's gathers
+    T dc = 0, pr = 0;
+    if (dcorr && r < rows) {
+      dc = dcorr[r];
+      pr = p[r];
+    }
     T s = 0;
     if (len) s = sell_row<T, W>(r, len, soff, sci, sval, [p](int64_t c) { return p[c]; });
-    if (dcorr && r < rows) s = s + dcorr[r] * p[r];
+    if (dcorr && r < rows) s = s + dc * pr;
     if (r < rows) q[r] = s;
     qs[threadIdx.x] = s;
   }
